@@ -1,0 +1,290 @@
+/*
+ * ngspsr C ABI: MI355X (gfx950) kernels of the jspsr screened-Poisson reconstruction path.
+ *
+ * Conventions
+ *  - All pointers are device pointers owned by the caller unless stated otherwise; sizes are
+ *    element counts. Every call takes the HIP stream to launch on (a hipStream_t, passed as
+ *    ngs_stream_t) and returns an ngs_status; ngs_last_error() gives the text of the last error
+ *    on the calling thread.
+ *  - Voxel coordinates are int32 triples in finest-voxel units (multiples of the level stride),
+ *    the layout of the reference's HashTree coords (jspsr/core/_hash_backend.py:121-147).
+ *  - Calls do not synchronise the stream unless documented (table build retries, PCG solve,
+ *    count reads). No call allocates device memory except ngs_pcg_solve's graph objects.
+ *
+ * The reference's operator layer is the pybind11 modules built by setup.py:17-54; each entry
+ * point below names the reference function whose role it takes over.
+ */
+#ifndef NGSPSR_H
+#define NGSPSR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *ngs_stream_t; /* hipStream_t */
+
+typedef enum {
+    NGS_OK = 0,
+    NGS_ERR_ARG = 1,             /* bad argument (shape, size, null pointer) */
+    NGS_ERR_HIP = 2,             /* HIP runtime or launch error */
+    NGS_ERR_HASH_OVERFLOW = 3,   /* a cuckoo bucket received more keys than it holds */
+    NGS_ERR_REHASH = 4,          /* cuckoo insertion failed for every attempted hash family */
+    NGS_ERR_NOT_CONVERGED = 5,   /* PCG hit its iteration cap before the tolerance */
+    NGS_ERR_CAPACITY = 6         /* a bounded structure (bitmap box, table) would be too large */
+} ngs_status;
+
+/* Ordered bitmap set of lattice points ("grid"): a box origin/dims in lattice units
+ * (coordinate >> log2_stride), one bit per lattice point in x-major order, and the exclusive
+ * popcount prefix per 32-bit word. Member rank = lexicographic (x, y, z) position. */
+typedef struct {
+    int32_t origin[3];
+    int32_t dims[3];
+    int32_t log2_stride;
+    int32_t pad_;
+    int64_t nwords;
+    uint32_t *bits;  /* nwords */
+    int32_t *rank;   /* nwords */
+} ngs_grid;
+
+/* Cuckoo hash table over 60-bit voxel keys: three sub-tables of table_size slots split into
+ * 512-slot buckets (bucket = key % num_buckets), multiply-shift slot hash per sub-table. */
+typedef struct {
+    int64_t table_size;
+    int32_t num_buckets;
+    int32_t evict_bound;
+    uint64_t *keys;      /* 3 * table_size, empty = UINT64_MAX */
+    int64_t *vals;       /* 3 * table_size */
+    uint64_t mult[3];    /* odd multipliers of the slot hash of each sub-table */
+    int64_t inserted;
+    int32_t attempts;    /* hash families tried by the last build */
+    int32_t pad_;
+} ngs_hash_table;
+
+const char *ngs_last_error(void);
+int ngs_abi_version(void);
+
+/* ------------------------------------------------------------------------------- keys, hash */
+
+/* Replaces _common.hash_cuda (csrc/common/hashmap_cuda.cu:269-305): keys[i] = fold60 FNV-1a-64
+ * of the dim (2..4) uint32 words of row i. */
+int ngs_hash_coords(const int32_t *coords, int64_t n, int32_t dim, uint64_t *keys, ngs_stream_t s);
+
+/* Replaces _common.kernel_hash_cuda (hashmap_cuda.cu:307-339): out[k*n + i] = key of
+ * coords[i] + offsets[k] (first three words offset, a 4th word copied). */
+int ngs_kernel_hash(const int32_t *coords, int64_t n, int32_t dim, const int32_t *offsets, int32_t k,
+                    uint64_t *out, ngs_stream_t s);
+
+/* Table sizing of build_hash_table (hashmap_cuda.cu:341-360). */
+int ngs_hash_sizing(int64_t n, int32_t enlarge, int64_t *table_size, int32_t *num_buckets,
+                    int32_t *evict_bound);
+/* Device workspace bytes for ngs_hash_build. */
+int64_t ngs_hash_workspace_bytes(int64_t table_size);
+
+/* Replaces _common.build_hash_table (hashmap_cuda.cu:341-379, insert kernels :13-107). Inserts
+ * (keys[i], vals ? vals[i] : i). t->table_size/num_buckets/evict_bound/keys/vals must be set by
+ * the caller (ngs_hash_sizing). Synchronises the stream to check for bucket overflow and
+ * eviction failure; retries with new hash families derived from `seed`. */
+int ngs_hash_build(ngs_hash_table *t, const uint64_t *keys, const int64_t *vals, int64_t n,
+                   uint64_t seed, void *workspace, ngs_stream_t s);
+
+/* Replaces _common.hash_table_query (hashmap_cuda.cu:381-398) with the CuckooHashTable.query
+ * result convention (jspsr/ext.py:74-84): out[i] = value, or -1 when the key is absent. */
+int ngs_hash_query(const ngs_hash_table *t, const uint64_t *queries, int64_t n, int64_t *out,
+                   ngs_stream_t s);
+
+/* ------------------------------------------------------------------- ordered bitmap set, scan */
+
+/* Words needed by a grid with these dims (0 if the box exceeds the capacity limit). */
+int64_t ngs_grid_words(const int32_t dims[3]);
+/* Device workspace bytes of the exclusive scans used below (n = words or rows). */
+int64_t ngs_scan_workspace_bytes(int64_t n);
+
+int ngs_grid_clear(ngs_grid g, ngs_stream_t s);
+/* Mark int32 coordinate rows (multiples of the grid stride). */
+int ngs_grid_mark_coords(ngs_grid g, const int32_t *coords, int64_t n, ngs_stream_t s);
+/* Mark the 8 trilinear corners at the grid stride of every point whose level <= max_level
+ * (levels may be NULL: all points). Replaces _quantize_coords (_hash_backend.py:457-462, with
+ * _trilinear_weights :469-510) up to the sort, which the bitmap order provides. */
+int ngs_grid_mark_corners(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size,
+                          const int32_t *levels, int32_t max_level, ngs_stream_t s);
+/* Fill rank[] and write the member count to *count (device int64). */
+int ngs_grid_finalize(ngs_grid g, int64_t *count, void *workspace, ngs_stream_t s);
+/* Write the members in rank order (the reference's torch.unique(dim=0) order). */
+int ngs_grid_emit(ngs_grid g, int32_t *coords_out, ngs_stream_t s);
+/* out[k*n + i] = rank of coords[i] + offsets[k], or -1 (offsets may be NULL, k = 1). Offsets are
+ * added as given (pre-multiplied by the stride, as the reference's callers do).
+ * The query of CuckooHashTable.query(coords, offsets) (ext.py:74-84) over a level's voxels. */
+int ngs_grid_query(ngs_grid g, const int32_t *coords, int64_t n, const int32_t *offsets, int32_t k,
+                   int64_t *out, ngs_stream_t s);
+/* min (3) and max (3) of q = xyz * inv_voxel_size over all points, into minmax[6] (device float). */
+int ngs_point_bounds(const float *xyz, int64_t n, float inv_voxel_size, float *minmax, ngs_stream_t s);
+/* Exclusive scan of int32 counts into int32 offsets (n + 1 entries; offsets[n] = total). */
+int ngs_scan_i32(const int32_t *counts, int64_t n, int32_t *offsets, void *workspace, ngs_stream_t s);
+
+/* --------------------------------------------------------------------- density, levels, splat */
+
+/* Trilinear splat (_hash_backend.py:332-356, splat_data with data = 1 or point rows):
+ * acc[id * ncomp + c] += w * (data ? data[p*ncomp + c] * scale[p] : 1) * post, for points with
+ * levels[p] == level (levels NULL: all), ids looked up in g. acc is float64. */
+int ngs_splat(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size, const float *data,
+              int32_t ncomp, const float *scale, const int32_t *levels, int32_t level, double post,
+              double *acc, ngs_stream_t s);
+/* Basis-interpolated field at points (hashtree.py:157-192, evaluate_interpolated with the Bezier
+ * basis): out[p] = (accumulate ? out[p] : 0) + scale * sum_i field[i] B((q_p - c_i)/s) over the 27
+ * voxels of g around the cell of q_p = xyz_p * inv_voxel_size. grad_out (nullable, n x 3) receives
+ * the derivative with respect to q (finest-voxel units), as evaluate_derivative returns it. */
+int ngs_eval_field(ngs_grid g, const float *field, const float *xyz, int64_t n, float inv_voxel_size,
+                   float scale, int32_t accumulate, float *out, float *grad_out, ngs_stream_t s);
+/* xyz_depth of build_hierarchy_adaptive (_hash_backend.py:428-429) from densities. */
+int ngs_point_levels(const float *density, int64_t n, float min_density, float log_base,
+                     int32_t depth, int32_t *levels, ngs_stream_t s);
+
+/* ------------------------------------------------------------------------ point binning (screening) */
+
+/* Per point: the cell (at the grid stride) containing it and the 3x3 per-axis basis values of
+ * the 27 voxels around it (reconstructor.py:111-120 pre-evaluation): cell[3p..], b[9p..]
+ * ordered (x: k=-1,0,1)(y)(z). */
+int ngs_point_basis(const float *xyz, int64_t n, float inv_voxel_size, int32_t stride, int32_t *cell,
+                    float *b, ngs_stream_t s);
+/* Rank of each point's cell in the cell grid (int32). */
+int ngs_grid_rank_rows(ngs_grid g, const int32_t *coords, int64_t n, int32_t *out, ngs_stream_t s);
+/* Stable bucketing of points by cell rank: perm (points ordered by cell, ties by index) and
+ * cell_start (ncells + 1). */
+int64_t ngs_bucket_workspace_bytes(int64_t n, int64_t ncells);
+int ngs_bucket_points(const int32_t *cell_rank, int64_t n, int64_t ncells, int32_t *perm,
+                      int32_t *cell_start, void *workspace, int64_t workspace_bytes, ngs_stream_t s);
+
+/* ----------------------------------------------------------------------------------- assembly */
+
+/* Registers the device copy of the integral tables (jspsr/bases/bezier_tensor.py:27-34, the
+ * tables the reference loads from bezier_integrals.pkl): for k = 0..levels-1 (m = 2^k) six float
+ * arrays F, G (3m+2), P, Q (3m), Pinv, Qinv (m+2; unused for k = 0) at element offsets
+ * offsets[6k + j] of `tables`. The buffer must outlive every assembly call. */
+int ngs_set_tables(const float *tables, const int64_t *offsets, int32_t levels);
+
+/* Everything the level-d row kernels need about one tree level. */
+typedef struct {
+    ngs_grid grid;           /* voxels of the level */
+    const int32_t *coords;   /* M x 3 sorted */
+    int64_t M;
+    int32_t stride;
+    int32_t pad_;
+} ngs_level;
+
+/* Points binned by their level-d cell, with per-point basis values at level d. */
+typedef struct {
+    ngs_grid cells;          /* cells containing >= 1 point */
+    const int32_t *cell_start; /* ncells + 1 */
+    const int32_t *perm;     /* n: point ids ordered by cell */
+    const float *b;          /* n x 9 basis values at level d (original point order) */
+    const float *alpha;      /* n screening weights, or NULL with alpha_scalar */
+    float alpha_scalar;
+    float pad_;
+    int64_t n;
+} ngs_point_bins;
+
+/* Entries per row of the level's 125-stencil (existing neighbours), rounded up to a multiple of
+ * 4 (padding entries are explicit zeros on the diagonal column). */
+int ngs_lhs_row_counts(ngs_level lv, int32_t *row_nnz, ngs_stream_t s);
+
+/* Same-level system of level d (reconstructor.py:170-183 LHS; :143-151 screening RHS;
+ * :153-168 screening part of the cross-level correction):
+ *   A_ij = K(j - i) + sf * sum_p alpha_p B_i(p) B_j(p)  written as CSR in column order,
+ *   inv_diag_i = 1 / A_ii,
+ *   rhs_i += sf * delta * sum_p alpha_p B_i(p) - sf * sum_p alpha_p B_i(p) coarse_p
+ * (coarse may be NULL: the field of coarser levels at the points). K uses the m = 1 tables. */
+int ngs_lhs_assemble(ngs_level lv, ngs_point_bins pts, const float *coarse, float sf, float delta,
+                     const int32_t *rowptr, int32_t *cols, float *vals, float *inv_diag, double *rhs,
+                     ngs_stream_t s);
+
+/* Stiffness part of the cross-level correction (reconstructor.py:153-168, COO of
+ * get_neighbours(d, dd, 5), integrate_deriv_deriv_product bezier_tensor.py:75-96):
+ * rhs_i -= sum_j K_{d,dd}(i, j) x_dd[j] over the 125 dd-voxels around the dd-cell of i. */
+int ngs_rhs_cross_stiffness(ngs_level fine, ngs_level coarse, const float *x_coarse, double *rhs,
+                            ngs_stream_t s);
+
+/* Normal-field RHS (reconstructor.py:125-141, integrate_const_deriv_product
+ * bezier_tensor.py:98-121): rhs_i += int grad B_i . (-N) over the data voxels of a data level.
+ * Data at a finer or equal level is scattered (27 targets per data voxel); data at a coarser
+ * level is gathered with the reference's |rel| <= 1.5 filter (_hash_backend.py:254-262). */
+int ngs_rhs_normals(ngs_level target, ngs_level data, const float *normals, double *rhs, ngs_stream_t s);
+
+/* ----------------------------------------------------------------------------------- solver */
+
+/* Replaces _sparse_solve.solve_pcg_diag (csrc/sparse_solve/solve_pcg.cu:15-136): Jacobi-PCG from
+ * x = 0 with FP32 vectors and FP64 dot products; stops when ||r|| <= tol ||b|| (recurrence
+ * residual) or after max_iter iterations (< 0: 10^7). Iterations run in captured hipGraph chunks
+ * of `chunk` iterations; the stream is synchronised between chunks. rowptr/cols/vals: CSR as
+ * written by ngs_lhs_assemble (rows padded to multiples of 4). On return *iters holds the
+ * iteration count and *rel_resid the final recurrence residual ||r||/||b||. */
+int64_t ngs_pcg_workspace_bytes(int64_t M);
+int ngs_pcg_solve(const int32_t *rowptr, const int32_t *cols, const float *vals, const float *inv_diag,
+                  const float *b, float *x, int64_t M, double tol, int32_t max_iter, int32_t chunk,
+                  void *workspace, ngs_stream_t s, int32_t *iters, double *rel_resid);
+
+/* Solver knobs: use_graph (capture iteration chunks in a hipGraph; default 1) and lanes per CSR
+ * row in the SpMV (8, 16, 32; 0 = automatic). */
+int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row);
+
+/* y = A x (FP32), the SpMV of the solver, exposed for measurement. */
+int ngs_spmv(const int32_t *rowptr, const int32_t *cols, const float *vals, const float *x, float *y,
+             int64_t M, ngs_stream_t s);
+
+/* r = b - A x in FP64 (x FP64), and ||r||^2, ||b||^2 into norms[0..1] (device double). */
+int ngs_residual_f64(const int32_t *rowptr, const int32_t *cols, const float *vals, const double *b,
+                     const double *x, double *r, int64_t M, double *norms, void *workspace, ngs_stream_t s);
+
+/* ------------------------------------------------------------------------------------- mesh */
+
+/* Conformal leaf construction (reconstructor.py:314-336 with get_coords(conforming=True),
+ * _hash_backend.py:139-145): the 8 children (CONFORM order) of each sorted parent, and keep[] = 1
+ * unless the child's coordinate is a parent of the next finer level (finer_parents, tested only
+ * when has_finer). */
+int ngs_conformal_children(const int32_t *parents, int64_t np, int32_t stride, ngs_grid finer_parents,
+                           int32_t has_finer, int32_t *children, int32_t *keep, ngs_stream_t s);
+/* keep[i] = 1 unless coords[i] is a parent of the next finer level (leaf test of the coarsest level). */
+int ngs_unrefined_flags(const int32_t *coords, int64_t n, ngs_grid finer_parents, int32_t *keep, ngs_stream_t s);
+/* Ordered compaction of int32 rows of `width` columns with keep[] flags; offsets (n + 1) receives
+ * the exclusive scan of keep, offsets[n] = kept count. */
+int ngs_compact_rows(const int32_t *rows, int32_t width, const int32_t *keep, int64_t n, int32_t *out,
+                     int32_t *offsets, void *workspace, ngs_stream_t s);
+/* Sub-cell expansion (reconstructor.py:338-348): out[(l*n^3 + e)] = leaf*n + e*stride. */
+int ngs_expand_primal(const int32_t *leaves, int64_t n, int32_t n_upsample, int32_t stride, int32_t *out,
+                      ngs_stream_t s);
+/* Mark the 8 corners of every primal sub-cell (reconstructor.py:350-358). */
+int ngs_mark_dual_centres(ngs_grid g, const int32_t *primal, int64_t n, int32_t stride, ngs_stream_t s);
+/* Dual-cube assembly (csrc/marching_cubes/dual_mc_inds.cu:45-182): for each primal sub-cell and
+ * each lattice point on its boundary that is a dual centre, register (id_base + cell, level) in
+ * the octant slots the cell covers; acc (DC x 8 x 2) int32, count (DC) int32. */
+int ngs_dual_accumulate(ngs_grid centres, const int32_t *primal, int64_t n, int32_t stride, int32_t level,
+                        int32_t id_base, int32_t *acc, int32_t *count, ngs_stream_t s);
+/* Keep complete dual cubes (count == 8) in order: cubes (K x 8) corner ids; flags (dc) and
+ * offsets (dc + 1, offsets[dc] = K) are written on the way. */
+int ngs_dual_compact(const int32_t *acc, const int32_t *count, int64_t dc, int32_t *cubes, int32_t *flags,
+                     int32_t *offsets, void *workspace, ngs_stream_t s);
+/* Dual-corner positions (primal + stride) * voxel_size / n_upsample (reconstructor.py:368-373). */
+int ngs_dual_corner_positions(const int32_t *primal, int64_t n, int32_t stride, float scale, float *pos,
+                              ngs_stream_t s);
+/* Dual marching cubes (csrc/marching_cubes/dual_mc.cu:33-189): per-cube vertex counts, then
+ * triangles (T x 3 x 3 float) and vertex keys (T x 3, uint64 = larger id << 32 | smaller id). */
+int ngs_mc_count(const int32_t *cubes, int64_t k, const float *values, int32_t *vcount, ngs_stream_t s);
+int ngs_mc_emit(const int32_t *cubes, int64_t k, const float *values, const float *pos, const int32_t *voffset,
+                float *tris, uint64_t *vkeys, ngs_stream_t s);
+/* Vertex de-duplication (reconstructor.py:379-383): sorted unique keys, faces (T*3) = rank of
+ * each key, vertex positions from the first occurrence. *nverts (device int64). */
+int64_t ngs_dedup_workspace_bytes(int64_t nkeys);
+int ngs_dedup_vertices(const uint64_t *vkeys, const float *tris, int64_t nkeys, int64_t *faces, float *verts,
+                       int64_t *nverts, void *workspace, int64_t workspace_bytes, ngs_stream_t s);
+
+/* ------------------------------------------------------------------------------------ misc */
+int ngs_f64_to_f32(const double *in, float *out, int64_t n, ngs_stream_t s);
+/* Weighted sum of field values (FP64) for the level set (reconstructor.py:228-236):
+ * sums[0] = sum w_p v_p, sums[1] = sum w_p. */
+int ngs_weighted_sum(const float *v, const float *w, int64_t n, double *sums, ngs_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NGSPSR_H */

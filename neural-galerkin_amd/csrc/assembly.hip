@@ -1,0 +1,341 @@
+// Galerkin assembly of the screened Poisson system on the sparse voxel hierarchy.
+//
+// Row-oriented, race-free design: every row of level d is built by one thread from the points in
+// the 27 level-d cells that support its basis function (points are pre-binned by cell), with the
+// 125-entry stencil row held in registers. This replaces the reference's per-point 27x27 pair
+// loop with a hash lookup and an FP32 atomic per matrix entry (csrc/sparse_op/sparse_product.cu:100-137)
+// and the enlarged pair hash table (jspsr/core/reconstructor.py:71).
+#include "common.h"
+
+namespace ngs {
+
+// Integral tables of one multiple m (device pointers, layout of the reference tables,
+// jspsr/bases/bezier_tensor.py:75-121): F/G length 3m+2, P/Q length 3m, Pinv/Qinv length m+2.
+struct Tables {
+    const float* F;
+    const float* G;
+    const float* P;
+    const float* Q;
+    const float* Pinv;
+    const float* Qinv;
+    int m;
+};
+
+__global__ void lhs_count_kernel(ngs_level lv, int32_t* __restrict__ row_nnz) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= lv.M) return;
+    const int s = lv.stride;
+    const int cx = lv.coords[3 * i], cy = lv.coords[3 * i + 1], cz = lv.coords[3 * i + 2];
+    int cnt = 0;
+    for (int dx = -2; dx <= 2; ++dx)
+        for (int dy = -2; dy <= 2; ++dy)
+            for (int dz = -2; dz <= 2; ++dz) cnt += grid_test(lv.grid, cx + dx * s, cy + dy * s, cz + dz * s) ? 1 : 0;
+    row_nnz[i] = (cnt + 3) & ~3;
+}
+
+template <bool HAS_ALPHA, bool HAS_COARSE>
+__global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_point_bins pts,
+                                                          const float* __restrict__ coarse, float sf, float sf_delta,
+                                                          const float* __restrict__ F1, const float* __restrict__ G1,
+                                                          const int32_t* __restrict__ rowptr, int32_t* __restrict__ cols,
+                                                          float* __restrict__ vals, float* __restrict__ inv_diag,
+                                                          double* __restrict__ rhs) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= lv.M) return;
+    const int s = lv.stride;
+    const int cx = lv.coords[3 * i], cy = lv.coords[3 * i + 1], cz = lv.coords[3 * i + 2];
+    float acc[125];
+#pragma unroll
+    for (int k = 0; k < 125; ++k) acc[k] = 0.f;
+    float rs = 0.f, rc = 0.f;
+#pragma unroll
+    for (int ox = -1; ox <= 1; ++ox) {
+#pragma unroll
+        for (int oy = -1; oy <= 1; ++oy) {
+#pragma unroll
+            for (int oz = -1; oz <= 1; ++oz) {
+                int cid = grid_find(pts.cells, cx + ox * s, cy + oy * s, cz + oz * s);
+                if (cid < 0) continue;
+                const int k0 = pts.cell_start[cid], k1 = pts.cell_start[cid + 1];
+                for (int k = k0; k < k1; ++k) {
+                    const int p = pts.perm[k];
+                    const float* bp = pts.b + 9 * (int64_t)p;
+                    float bx[3] = {bp[0], bp[1], bp[2]};
+                    float by[3] = {bp[3], bp[4], bp[5]};
+                    float bz[3] = {bp[6], bp[7], bp[8]};
+                    const float a = HAS_ALPHA ? pts.alpha[p] : pts.alpha_scalar;
+                    const float w = a * (bx[1 - ox] * by[1 - oy] * bz[1 - oz]);
+                    rs += w;
+                    if (HAS_COARSE) rc += w * coarse[p];
+#pragma unroll
+                    for (int vx = -1; vx <= 1; ++vx) {
+#pragma unroll
+                        for (int vy = -1; vy <= 1; ++vy) {
+                            const float wxy = w * (bx[vx + 1] * by[vy + 1]);
+#pragma unroll
+                            for (int vz = -1; vz <= 1; ++vz)
+                                acc[(ox + vx + 2) * 25 + (oy + vy + 2) * 5 + (oz + vz + 2)] += wxy * bz[vz + 1];
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // Emit the row in increasing column order (lexicographic offsets over sorted voxels).
+    int32_t base = rowptr[i];
+    int cnt = 0;
+    const float fs = (float)s, inv_s = 1.0f / (float)s;
+#pragma unroll
+    for (int dx = -2; dx <= 2; ++dx) {
+#pragma unroll
+        for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+            for (int dz = -2; dz <= 2; ++dz) {
+                int j = grid_find(lv.grid, cx + dx * s, cy + dy * s, cz + dz * s);
+                if (j < 0) continue;
+                float v;
+                {
+#pragma clang fp contract(off)
+                    const float c0 = F1[dx + 2] * fs, c1 = F1[dy + 2] * fs, c2 = F1[dz + 2] * fs;
+                    const float g0 = G1[dx + 2] * inv_s, g1 = G1[dy + 2] * inv_s, g2 = G1[dz + 2] * inv_s;
+                    const float K = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
+                    v = K + sf * acc[(dx + 2) * 25 + (dy + 2) * 5 + (dz + 2)];
+                }
+                cols[base + cnt] = j;
+                vals[base + cnt] = v;
+                ++cnt;
+                if (dx == 0 && dy == 0 && dz == 0) inv_diag[i] = 1.0f / v;
+            }
+        }
+    }
+    while (cnt & 3) {
+        cols[base + cnt] = (int32_t)i;
+        vals[base + cnt] = 0.f;
+        ++cnt;
+    }
+    rhs[i] += (double)(sf_delta * rs) - (double)(sf * rc);
+}
+
+// rel (in units of the fine stride) and table row for the cross-level pair (fine voxel at c,
+// coarse voxel at Q + k s_c): rel = k m + (Q - c)/s_f + (m - 1)/2, row = rel + 1.5 m + 0.5.
+__global__ void cross_stiffness_kernel(ngs_level fine, ngs_level coarse, const float* __restrict__ xc, Tables T,
+                                       double* __restrict__ rhs) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fine.M) return;
+    const int sf_ = fine.stride, sc = coarse.stride, m = T.m;
+    int c[3], Q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c[a] = fine.coords[3 * i + a];
+        // floor((c + s_f/2) / s_c) * s_c with exact integer arithmetic
+        int num = 2 * c[a] + sf_;
+        int den = 2 * sc;
+        int q = num >= 0 ? num / den : -((-num + den - 1) / den);
+        Q[a] = q * sc;
+    }
+    const int nrow = 3 * m + 2;
+    const float fs = (float)sf_, inv_sc = 1.0f / (float)sc;
+    float acc = 0.f;
+    for (int kx = -2; kx <= 2; ++kx) {
+        int ix = kx * m + (Q[0] - c[0]) / sf_ + 2 * m;
+        for (int ky = -2; ky <= 2; ++ky) {
+            int iy = ky * m + (Q[1] - c[1]) / sf_ + 2 * m;
+            for (int kz = -2; kz <= 2; ++kz) {
+                int iz = kz * m + (Q[2] - c[2]) / sf_ + 2 * m;
+                int j = grid_find(coarse.grid, Q[0] + kx * sc, Q[1] + ky * sc, Q[2] + kz * sc);
+                if (j < 0) continue;
+                if (ix < 0 || ix >= nrow || iy < 0 || iy >= nrow || iz < 0 || iz >= nrow) continue;
+                float K;
+                {
+#pragma clang fp contract(off)
+                    const float c0 = T.F[ix] * fs, c1 = T.F[iy] * fs, c2 = T.F[iz] * fs;
+                    const float g0 = T.G[ix] * inv_sc, g1 = T.G[iy] * inv_sc, g2 = T.G[iz] * inv_sc;
+                    K = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
+                }
+                acc += K * xc[j];
+            }
+        }
+    }
+    rhs[i] -= (double)acc;
+}
+
+// Data at a finer or equal level: scatter each data voxel to the 27 target voxels around the
+// target cell containing its centre (get_neighbours(dd, d, 3), _hash_backend.py:176-228).
+__global__ void rhs_normals_scatter_kernel(ngs_level tgt, ngs_level dat, const float* __restrict__ nrm, Tables T,
+                                           double* __restrict__ rhs) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= dat.M) return;
+    const int sd = dat.stride, st = tgt.stride, m = T.m;
+    int c[3], Q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c[a] = dat.coords[3 * v + a];
+        int num = 2 * c[a] + sd, den = 2 * st;
+        int q = num >= 0 ? num / den : -((-num + den - 1) / den);
+        Q[a] = q * st;
+    }
+    const float n0 = -nrm[3 * v], n1 = -nrm[3 * v + 1], n2 = -nrm[3 * v + 2];
+    const float fsd = (float)sd, imult = (float)sd / (float)st;
+    for (int kx = -1; kx <= 1; ++kx) {
+        int ix = kx * m + (Q[0] - c[0]) / sd + 2 * m - 1;
+        for (int ky = -1; ky <= 1; ++ky) {
+            int iy = ky * m + (Q[1] - c[1]) / sd + 2 * m - 1;
+            for (int kz = -1; kz <= 1; ++kz) {
+                int iz = kz * m + (Q[2] - c[2]) / sd + 2 * m - 1;
+                if ((unsigned)ix >= (unsigned)(3 * m) || (unsigned)iy >= (unsigned)(3 * m) ||
+                    (unsigned)iz >= (unsigned)(3 * m))
+                    continue;
+                int t = grid_find(tgt.grid, Q[0] + kx * st, Q[1] + ky * st, Q[2] + kz * st);
+                if (t < 0) continue;
+                float r;
+                {
+#pragma clang fp contract(off)
+                    const float c0 = T.P[ix] * fsd, c1 = T.P[iy] * fsd, c2 = T.P[iz] * fsd;
+                    const float g0 = T.Q[ix] * imult, g1 = T.Q[iy] * imult, g2 = T.Q[iz] * imult;
+                    r = n0 * g0 * c1 * c2 + n1 * c0 * g1 * c2 + n2 * c0 * c1 * g2;
+                }
+                atomicAdd(&rhs[t], (double)r);
+            }
+        }
+    }
+}
+
+// Data at a coarser level: gather, per target voxel, the data voxels around the data cell
+// containing its centre, kept when |rel| <= 1.5 target strides (_hash_backend.py:254-262).
+__global__ void rhs_normals_gather_kernel(ngs_level tgt, ngs_level dat, const float* __restrict__ nrm, Tables T,
+                                          double* __restrict__ rhs) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tgt.M) return;
+    const int st = tgt.stride, sd = dat.stride, m = T.m;  // m = sd / st >= 2
+    int c[3], Q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c[a] = tgt.coords[3 * i + a];
+        int num = 2 * c[a] + st, den = 2 * sd;
+        int q = num >= 0 ? num / den : -((-num + den - 1) / den);
+        Q[a] = q * sd;
+    }
+    const float fst = (float)st;
+    float acc = 0.f;
+    for (int kx = -1; kx <= 1; ++kx) {
+        // twice the offset of the data centre from the target centre, in target strides
+        int r2x = 2 * (kx * m + (Q[0] - c[0]) / st) + (m - 1);
+        if (r2x > 3 || r2x < -3) continue;
+        for (int ky = -1; ky <= 1; ++ky) {
+            int r2y = 2 * (ky * m + (Q[1] - c[1]) / st) + (m - 1);
+            if (r2y > 3 || r2y < -3) continue;
+            for (int kz = -1; kz <= 1; ++kz) {
+                int r2z = 2 * (kz * m + (Q[2] - c[2]) / st) + (m - 1);
+                if (r2z > 3 || r2z < -3) continue;
+                int v = grid_find(dat.grid, Q[0] + kx * sd, Q[1] + ky * sd, Q[2] + kz * sd);
+                if (v < 0) continue;
+                // row = round(-rel_inv + m/2 + 1/2) with rel_inv = r2/2
+                int ix = (-r2x + m + 1) / 2, iy = (-r2y + m + 1) / 2, iz = (-r2z + m + 1) / 2;
+                if ((unsigned)ix > (unsigned)(m + 1) || (unsigned)iy > (unsigned)(m + 1) || (unsigned)iz > (unsigned)(m + 1))
+                    continue;
+                float r;
+                {
+#pragma clang fp contract(off)
+                    const float c0 = T.Pinv[ix] * fst, c1 = T.Pinv[iy] * fst, c2 = T.Pinv[iz] * fst;
+                    const float g0 = T.Qinv[ix], g1 = T.Qinv[iy], g2 = T.Qinv[iz];
+                    r = -nrm[3 * v] * g0 * c1 * c2 + -nrm[3 * v + 1] * c0 * g1 * c2 + -nrm[3 * v + 2] * c0 * c1 * g2;
+                }
+                acc += r;
+            }
+        }
+    }
+    rhs[i] += (double)acc;
+}
+
+}  // namespace ngs
+
+using namespace ngs;
+
+// Tables are uploaded once per process by the host layer.
+static const float* g_tab_ptr = nullptr;
+static int64_t g_tab_off[8][6];
+static int g_tab_levels = 0;
+
+static bool tables_for(int m, Tables* T) {
+    int k = 0;
+    while ((1 << k) < m) ++k;
+    if ((1 << k) != m || k >= g_tab_levels || !g_tab_ptr) return false;
+    T->m = m;
+    T->F = g_tab_ptr + g_tab_off[k][0];
+    T->G = g_tab_ptr + g_tab_off[k][1];
+    T->P = g_tab_ptr + g_tab_off[k][2];
+    T->Q = g_tab_ptr + g_tab_off[k][3];
+    T->Pinv = k > 0 ? g_tab_ptr + g_tab_off[k][4] : nullptr;
+    T->Qinv = k > 0 ? g_tab_ptr + g_tab_off[k][5] : nullptr;
+    return true;
+}
+
+extern "C" {
+
+// Registers the device copy of the integral tables: for k = 0..levels-1 (m = 2^k), six float
+// arrays F, G (3m+2), P, Q (3m), Pinv, Qinv (m+2; unused for k = 0) at the given element offsets
+// of `tables`. The buffer must stay alive while kernels use it.
+int ngs_set_tables(const float* tables, const int64_t* offsets, int32_t levels) {
+    NGS_REQUIRE(tables && offsets && levels >= 1 && levels <= 8, "ngs_set_tables: bad arguments");
+    g_tab_ptr = tables;
+    g_tab_levels = levels;
+    for (int k = 0; k < levels; ++k)
+        for (int j = 0; j < 6; ++j) g_tab_off[k][j] = offsets[6 * k + j];
+    return NGS_OK;
+}
+
+int ngs_lhs_row_counts(ngs_level lv, int32_t* row_nnz, ngs_stream_t s) {
+    if (lv.M <= 0) return NGS_OK;
+    lhs_count_kernel<<<grid1d(lv.M, 256), 256, 0, S(s)>>>(lv, row_nnz);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, float sf, float delta,
+                     const int32_t* rowptr, int32_t* cols, float* vals, float* inv_diag, double* rhs, ngs_stream_t s) {
+    if (lv.M <= 0) return NGS_OK;
+    Tables T;
+    NGS_REQUIRE(tables_for(1, &T), "ngs_lhs_assemble: integral tables not registered");
+    const float sfd = sf * delta;
+    dim3 grid(grid1d(lv.M, 64)), block(64);
+    hipStream_t st = S(s);
+    if (pts.alpha) {
+        if (coarse)
+            lhs_assemble_kernel<true, true><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, rowptr, cols, vals, inv_diag, rhs);
+        else
+            lhs_assemble_kernel<true, false><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, rowptr, cols, vals, inv_diag, rhs);
+    } else {
+        if (coarse)
+            lhs_assemble_kernel<false, true><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, rowptr, cols, vals, inv_diag, rhs);
+        else
+            lhs_assemble_kernel<false, false><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, rowptr, cols, vals, inv_diag, rhs);
+    }
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_rhs_cross_stiffness(ngs_level fine, ngs_level coarse, const float* x_coarse, double* rhs, ngs_stream_t s) {
+    if (fine.M <= 0 || coarse.M <= 0) return NGS_OK;
+    NGS_REQUIRE(coarse.stride > fine.stride, "ngs_rhs_cross_stiffness: coarse stride must exceed fine stride");
+    Tables T;
+    NGS_REQUIRE(tables_for(coarse.stride / fine.stride, &T), "ngs_rhs_cross_stiffness: table multiple unavailable");
+    cross_stiffness_kernel<<<grid1d(fine.M, 128), 128, 0, S(s)>>>(fine, coarse, x_coarse, T, rhs);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_rhs_normals(ngs_level target, ngs_level data, const float* normals, double* rhs, ngs_stream_t s) {
+    if (target.M <= 0 || data.M <= 0) return NGS_OK;
+    Tables T;
+    if (target.stride >= data.stride) {
+        NGS_REQUIRE(tables_for(target.stride / data.stride, &T), "ngs_rhs_normals: table multiple unavailable");
+        rhs_normals_scatter_kernel<<<grid1d(data.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);
+    } else {
+        NGS_REQUIRE(tables_for(data.stride / target.stride, &T), "ngs_rhs_normals: table multiple unavailable");
+        rhs_normals_gather_kernel<<<grid1d(target.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);
+    }
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+}  // extern "C"

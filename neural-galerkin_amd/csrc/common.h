@@ -1,0 +1,176 @@
+// Shared device/host helpers of libngspsr (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ngspsr.h"
+
+namespace ngs {
+
+// ----------------------------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+
+#define NGS_CHECK_HIP(expr)                                                                   \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            ::ngs::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));              \
+            return NGS_ERR_HIP;                                                               \
+        }                                                                                     \
+    } while (0)
+
+#define NGS_CHECK_LAUNCH()                                                                    \
+    do {                                                                                      \
+        hipError_t _e = hipGetLastError();                                                    \
+        if (_e != hipSuccess) {                                                               \
+            ::ngs::set_error(std::string("kernel launch: ") + hipGetErrorString(_e));         \
+            return NGS_ERR_HIP;                                                               \
+        }                                                                                     \
+    } while (0)
+
+#define NGS_REQUIRE(cond, msg)                                                                \
+    do {                                                                                      \
+        if (!(cond)) {                                                                        \
+            ::ngs::set_error(msg);                                                            \
+            return NGS_ERR_ARG;                                                               \
+        }                                                                                     \
+    } while (0)
+
+inline hipStream_t S(ngs_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned grid1d(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ------------------------------------------------------------------------- ordered bitmap set
+// A set of integer lattice points inside a box, stored as a bitmap in x-major (x, y, z) order
+// plus the exclusive popcount prefix of every 32-bit word. The rank of a member is its position
+// in lexicographic (x, y, z) order, so emitting members in word order yields the sorted unique
+// coordinate list of the reference's torch.unique(dim=0), and the rank lookup is a perfect hash.
+__device__ __forceinline__ int64_t grid_linear(const ngs_grid& g, int x, int y, int z) {
+    // x, y, z are lattice coordinates relative to origin; caller checked bounds.
+    return ((int64_t)x * g.dims[1] + y) * (int64_t)g.dims[2] + z;
+}
+
+// coordinate (in finest units, multiple of g.stride) -> lattice position; false if outside.
+__device__ __forceinline__ bool grid_cell(const ngs_grid& g, int cx, int cy, int cz, int64_t* lin) {
+    // floor division by the (power of two) stride
+    int lx = (cx >> g.log2_stride) - g.origin[0];
+    int ly = (cy >> g.log2_stride) - g.origin[1];
+    int lz = (cz >> g.log2_stride) - g.origin[2];
+    if ((unsigned)lx >= (unsigned)g.dims[0] || (unsigned)ly >= (unsigned)g.dims[1] ||
+        (unsigned)lz >= (unsigned)g.dims[2])
+        return false;
+    *lin = grid_linear(g, lx, ly, lz);
+    return true;
+}
+
+// Rank of coordinate c in the set, or -1. Coordinates that are not multiples of the stride are
+// absent.
+__device__ __forceinline__ int32_t grid_find(const ngs_grid& g, int cx, int cy, int cz) {
+    const int mask = (1 << g.log2_stride) - 1;
+    if (((cx | cy | cz) & mask) != 0) return -1;
+    int64_t lin;
+    if (!grid_cell(g, cx, cy, cz, &lin)) return -1;
+    uint32_t w = g.bits[lin >> 5];
+    uint32_t b = (uint32_t)(lin & 31);
+    if (!((w >> b) & 1u)) return -1;
+    return g.rank[lin >> 5] + __popc(w & ((1u << b) - 1u));
+}
+
+__device__ __forceinline__ bool grid_test(const ngs_grid& g, int cx, int cy, int cz) {
+    const int mask = (1 << g.log2_stride) - 1;
+    if (((cx | cy | cz) & mask) != 0) return false;
+    int64_t lin;
+    if (!grid_cell(g, cx, cy, cz, &lin)) return false;
+    return (g.bits[lin >> 5] >> (lin & 31)) & 1u;
+}
+
+__device__ __forceinline__ void grid_mark(const ngs_grid& g, int cx, int cy, int cz) {
+    int64_t lin;
+    if (grid_cell(g, cx, cy, cz, &lin)) atomicOr(&g.bits[lin >> 5], 1u << (lin & 31));
+}
+
+// ----------------------------------------------------------------------------------- FNV keys
+// fold60(FNV-1a-64 over the uint32 words of a coordinate row); csrc/common/hashmap_cuda.cu:248-261.
+__device__ __forceinline__ uint64_t fnv_fold60(const int32_t* c, int dim) {
+    uint64_t h = 14695981039346656037ull;
+    for (int j = 0; j < dim; ++j) {
+        h ^= (uint32_t)c[j];
+        h *= 1099511628211ull;
+    }
+    return (h >> 60) ^ (h & 0x0FFFFFFFFFFFFFFFull);
+}
+
+// ---------------------------------------------------------------------------- B-spline basis
+// B(x) = 2 N2(x): (x+1.5)^2 | -2x^2+1.5 | (x-1.5)^2 on [-1.5,-.5) | [-.5,.5) | [.5,1.5);
+// jspsr/bases/bezier_tensor.py:37-44. Evaluated with the reference's operation order.
+__device__ __forceinline__ float bspline(float x) {
+#pragma clang fp contract(off)
+    if (x >= -1.5f && x < -0.5f) { float t = x + 1.5f; return t * t; }
+    if (x >= -0.5f && x < 0.5f) return -2.0f * (x * x) + 1.5f;
+    if (x >= 0.5f && x < 1.5f) { float t = x - 1.5f; return t * t; }
+    return 0.0f;
+}
+
+__device__ __forceinline__ float bspline_deriv(float x) {
+#pragma clang fp contract(off)
+    if (x >= -1.5f && x < -0.5f) return 2.0f * x + 3.0f;
+    if (x >= -0.5f && x < 0.5f) return -4.0f * x;
+    if (x >= 0.5f && x < 1.5f) return 2.0f * x - 3.0f;
+    return 0.0f;
+}
+
+// Per-axis basis values of a query position around its containing cell at stride s, following
+// the reference's neighbour query for float sources (hashtree.py:169-181,
+// _hash_backend.py:196-226): base = q - 0.5; qc = floor((base + 0.5) / s) * s;
+// c_off = (qc - base) + (s - 1) / 2; rel_k = k s + c_off; arg = -rel_k / s for k = -1, 0, 1.
+// Writes the cell coordinate and b[k+1] = B(arg_k).
+__device__ __forceinline__ void axis_basis(float q, int s, float inv_s, int* cell, float b[3]) {
+#pragma clang fp contract(off)
+    float base = q - 0.5f;
+    float t = base + 0.5f;
+    int qc = (int)(floorf(t * inv_s) * (float)s);
+    float coff = ((float)qc - base) + (float)(s - 1) * 0.5f;
+    *cell = qc;
+#pragma unroll
+    for (int k = -1; k <= 1; ++k) {
+        float rel = (float)(k * s) + coff;
+        b[k + 1] = bspline(-rel * inv_s);
+    }
+}
+
+// Same, also returning the derivative of the basis (per unit of q) for gradient queries.
+__device__ __forceinline__ void axis_basis_grad(float q, int s, float inv_s, int* cell, float b[3], float db[3]) {
+#pragma clang fp contract(off)
+    float base = q - 0.5f;
+    float t = base + 0.5f;
+    int qc = (int)(floorf(t * inv_s) * (float)s);
+    float coff = ((float)qc - base) + (float)(s - 1) * 0.5f;
+    *cell = qc;
+#pragma unroll
+    for (int k = -1; k <= 1; ++k) {
+        float rel = (float)(k * s) + coff;
+        b[k + 1] = bspline(-rel * inv_s);
+        db[k + 1] = bspline_deriv(-rel * inv_s) * inv_s;
+    }
+}
+
+// Trilinear corner selection of a point at stride s (_hash_backend.py:469-510):
+// dc = floor(q / s) * s, rel = q - dc - s/2; corner choice per axis is dc or dc + sign(rel) s,
+// weight of the moved corner |rel|/s. Returns base corner, per-axis sign, per-axis local weight.
+__device__ __forceinline__ void trilinear_axis(float q, int s, float inv_s, int* dc, int* sgn, float* loc) {
+#pragma clang fp contract(off)
+    float d = floorf(q * inv_s) * (float)s;
+    float rel = (q - d) - (float)s * 0.5f;
+    *dc = (int)d;
+    *sgn = (rel > 0.f) ? 1 : ((rel < 0.f) ? -1 : 0);
+    *loc = fabsf(rel) * inv_s;
+}
+
+}  // namespace ngs

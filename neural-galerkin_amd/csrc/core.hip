@@ -1,0 +1,821 @@
+// Keys, cuckoo table, ordered bitmap sets, scans, density/levels, splat and field evaluation.
+#include "common.h"
+
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+namespace ngs {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+// ========================================================================================
+// Scan (exclusive prefix sums) — three-phase: block totals, scan of totals, block downsweep.
+// ========================================================================================
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+struct LoadPopc {
+    const uint32_t* w;
+    __device__ int64_t operator()(int64_t i) const { return __popc(w[i]); }
+};
+struct LoadI32 {
+    const int32_t* v;
+    __device__ int64_t operator()(int64_t i) const { return v[i]; }
+};
+
+// Block-wide exclusive scan of one value per thread (256 threads). Returns the exclusive
+// prefix and writes the block total to *total.
+__device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
+    __shared__ int64_t wave_sums[SCAN_THREADS / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wave_sums[wave] = x;
+    __syncthreads();
+    int64_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+        if (w < wave) off += wave_sums[w];
+        tot += wave_sums[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+}
+
+template <class Load>
+__global__ void __launch_bounds__(SCAN_THREADS) scan_tile_totals(Load ld, int64_t n, int64_t* tile_sums) {
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j)
+        if (base + j < n) s += ld(base + j);
+    int64_t tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// Single block: in-place exclusive scan of nt tile totals; tile_sums[nt] = grand total.
+__global__ void __launch_bounds__(SCAN_THREADS) scan_tile_prefix(int64_t* tile_sums, int64_t nt) {
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nt; base += SCAN_THREADS) {
+        int64_t i = base + threadIdx.x;
+        int64_t v = i < nt ? tile_sums[i] : 0;
+        int64_t tot;
+        int64_t ex = block_exclusive_scan(v, &tot);
+        if (i < nt) tile_sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) tile_sums[nt] = carry;
+}
+
+template <class Load>
+__global__ void __launch_bounds__(SCAN_THREADS) scan_downsweep(Load ld, int64_t n, const int64_t* tile_sums,
+                                                               int32_t* out) {
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t vals[SCAN_ITEMS];
+    int64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        vals[j] = (base + j < n) ? ld(base + j) : 0;
+        s += vals[j];
+    }
+    int64_t tot;
+    int64_t run = block_exclusive_scan(s, &tot) + tile_sums[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        if (base + j < n) out[base + j] = (int32_t)run;
+        run += vals[j];
+    }
+}
+
+__global__ void write_total_i32(const int64_t* tile_sums, int64_t nt, int32_t* dst) { *dst = (int32_t)tile_sums[nt]; }
+__global__ void write_total_i64(const int64_t* tile_sums, int64_t nt, int64_t* dst) { *dst = tile_sums[nt]; }
+
+static int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+template <class Load>
+static int run_scan(Load ld, int64_t n, int32_t* out, int64_t* total64, int32_t* total32, void* ws, hipStream_t st) {
+    int64_t nt = scan_tiles(n);
+    int64_t* tiles = reinterpret_cast<int64_t*>(ws);
+    if (nt > 0) {
+        scan_tile_totals<<<(unsigned)nt, SCAN_THREADS, 0, st>>>(ld, n, tiles);
+        NGS_CHECK_LAUNCH();
+    }
+    scan_tile_prefix<<<1, SCAN_THREADS, 0, st>>>(tiles, nt);
+    NGS_CHECK_LAUNCH();
+    if (nt > 0) {
+        scan_downsweep<<<(unsigned)nt, SCAN_THREADS, 0, st>>>(ld, n, tiles, out);
+        NGS_CHECK_LAUNCH();
+    }
+    if (total64) write_total_i64<<<1, 1, 0, st>>>(tiles, nt, total64);
+    if (total32) write_total_i32<<<1, 1, 0, st>>>(tiles, nt, total32);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int scan_i32_internal(const int32_t* counts, int64_t n, int32_t* offsets, void* ws, hipStream_t st) {
+    return run_scan(LoadI32{counts}, n, offsets, nullptr, offsets + n, ws, st);
+}
+
+// ========================================================================================
+// FNV keys
+// ========================================================================================
+__global__ void hash_rows_kernel(const int32_t* __restrict__ c, int64_t n, int dim, uint64_t* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t row[4];
+    for (int j = 0; j < dim; ++j) row[j] = c[i * dim + j];
+    out[i] = fnv_fold60(row, dim);
+}
+
+__global__ void kernel_hash_kernel(const int32_t* __restrict__ c, int64_t n, int dim, const int32_t* __restrict__ off,
+                                   int k, uint64_t* __restrict__ out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    int64_t kk = t / n, i = t % n;
+    int32_t row[4];
+    for (int j = 0; j < dim; ++j) row[j] = c[i * dim + j];
+    for (int j = 0; j < 3; ++j) row[j] += off[kk * 3 + j];
+    out[t] = fnv_fold60(row, dim);
+}
+
+// ========================================================================================
+// Cuckoo table
+// ========================================================================================
+constexpr int BUCKET = 512;
+constexpr uint64_t EMPTY_KEY = ~0ull;
+
+__device__ __forceinline__ int slot_hash(uint64_t key, uint64_t mult) { return (int)((key * mult) >> 55) & (BUCKET - 1); }
+
+struct HashWs {
+    int32_t* bucket_count;  // num_buckets
+    int32_t* staged;        // num_buckets * BUCKET key indices
+    int32_t* flags;         // [0] overflow, [1] eviction failures
+};
+
+static HashWs hash_ws(void* ws, int64_t table_size) {
+    int64_t nb = (table_size + BUCKET - 1) / BUCKET;
+    char* p = reinterpret_cast<char*>(ws);
+    HashWs h;
+    h.bucket_count = reinterpret_cast<int32_t*>(p);
+    h.staged = reinterpret_cast<int32_t*>(p + ((nb * 4 + 255) / 256) * 256);
+    h.flags = reinterpret_cast<int32_t*>(p + ((nb * 4 + 255) / 256) * 256 + nb * BUCKET * 4);
+    return h;
+}
+
+__global__ void zero_i32(int32_t* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Phase 1: distribute key indices into buckets. Lanes of a wavefront that target the same
+// bucket are served by one atomic: the leader reserves popcount(match) slots and every lane
+// takes its rank inside the ballot mask.
+__global__ void bucket_fill_kernel(const uint64_t* __restrict__ keys, int64_t n, int nb, int32_t* bucket_count,
+                                   int32_t* staged, int32_t* flags, int aggregate) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = i < n;
+    int b = valid ? (int)(keys[i] % (uint64_t)nb) : -1;
+    int slot = -1;
+    if (aggregate) {
+        const int lane = threadIdx.x & 63;
+        uint64_t pending = __ballot(valid);
+        while (pending) {
+            int leader = __ffsll((unsigned long long)pending) - 1;
+            int lb = __shfl(b, leader, 64);
+            uint64_t same = __ballot(valid && b == lb) & pending;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&bucket_count[lb], __popcll(same));
+            base = __shfl(base, leader, 64);
+            if (valid && b == lb) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+            pending &= ~same;
+        }
+    } else if (valid) {
+        slot = atomicAdd(&bucket_count[b], 1);
+    }
+    if (!valid) return;
+    if (slot < BUCKET)
+        staged[(int64_t)b * BUCKET + slot] = (int32_t)i;
+    else
+        atomicAdd(&flags[0], 1);
+}
+
+// Phase 2: one workgroup per bucket runs the three-table cuckoo insertion in LDS.
+__global__ void __launch_bounds__(BUCKET) bucket_cuckoo_kernel(const uint64_t* __restrict__ keys,
+                                                               const int64_t* __restrict__ vals,
+                                                               const int32_t* __restrict__ bucket_count,
+                                                               const int32_t* __restrict__ staged, int64_t table_size,
+                                                               uint64_t m0, uint64_t m1, uint64_t m2, int evict_bound,
+                                                               uint64_t* __restrict__ tkeys, int64_t* __restrict__ tvals,
+                                                               int32_t* flags) {
+    __shared__ int32_t tab[3][BUCKET];
+    const int t = threadIdx.x;
+    const int b = blockIdx.x;
+    tab[0][t] = -1;
+    tab[1][t] = -1;
+    tab[2][t] = -1;
+    __syncthreads();
+    const uint64_t mult[3] = {m0, m1, m2};
+    int cnt = bucket_count[b];
+    if (cnt > BUCKET) cnt = BUCKET;
+    if (t < cnt) {
+        int cur = staged[(int64_t)b * BUCKET + t];
+        int f = 0;
+        int evictions = 0;
+        const int limit = 3 * evict_bound;
+        while (true) {
+            int pos = slot_hash(keys[cur], mult[f]);
+            int old = atomicExch(&tab[f][pos], cur);
+            if (old < 0) break;
+            cur = old;
+            f = (f + 1) % 3;
+            if (++evictions >= limit) {
+                atomicAdd(&flags[1], 1);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        int idx = tab[f][t];
+        int64_t o = (int64_t)f * table_size + (int64_t)b * BUCKET + t;
+        tkeys[o] = idx >= 0 ? keys[idx] : EMPTY_KEY;
+        tvals[o] = idx >= 0 ? (vals ? vals[idx] : (int64_t)idx) : -1;
+    }
+}
+
+__global__ void hash_query_kernel(const uint64_t* __restrict__ tkeys, const int64_t* __restrict__ tvals,
+                                  int64_t table_size, int nb, uint64_t m0, uint64_t m1, uint64_t m2,
+                                  const uint64_t* __restrict__ q, int64_t n, int64_t* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t key = q[i];
+    int64_t base = (int64_t)(key % (uint64_t)nb) * BUCKET;
+    int64_t res = -1;
+    const uint64_t mult[3] = {m0, m1, m2};
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        int64_t o = (int64_t)f * table_size + base + slot_hash(key, mult[f]);
+        if (tkeys[o] == key) { res = tvals[o]; break; }
+    }
+    out[i] = res;
+}
+
+static uint64_t splitmix64(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// ========================================================================================
+// Ordered bitmap sets
+// ========================================================================================
+__global__ void clear_words(uint32_t* w, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = 0u;
+}
+
+__global__ void mark_coords_kernel(ngs_grid g, const int32_t* __restrict__ c, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    grid_mark(g, c[3 * i], c[3 * i + 1], c[3 * i + 2]);
+}
+
+__global__ void mark_corners_kernel(ngs_grid g, const float* __restrict__ xyz, int64_t n, float inv_vs,
+                                    const int32_t* __restrict__ levels, int max_level) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (levels && levels[i] > max_level) return;
+    const int s = 1 << g.log2_stride;
+    const float inv_s = 1.0f / (float)s;
+    int dc[3], sg[3];
+    float loc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) trilinear_axis(xyz[3 * i + a] * inv_vs, s, inv_s, &dc[a], &sg[a], &loc[a]);
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        int nx = (o >> 2) & 1, ny = (o >> 1) & 1, nz = o & 1;
+        grid_mark(g, dc[0] + nx * sg[0] * s, dc[1] + ny * sg[1] * s, dc[2] + nz * sg[2] * s);
+    }
+}
+
+__global__ void emit_kernel(ngs_grid g, int32_t* __restrict__ out) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= g.nwords) return;
+    uint32_t bits = g.bits[w];
+    if (!bits) return;
+    int32_t r = g.rank[w];
+    const int64_t dyz = (int64_t)g.dims[1] * g.dims[2];
+    while (bits) {
+        int b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        int64_t lin = w * 32 + b;
+        int x = (int)(lin / dyz);
+        int64_t rem = lin - (int64_t)x * dyz;
+        int y = (int)(rem / g.dims[2]);
+        int z = (int)(rem - (int64_t)y * g.dims[2]);
+        out[3 * (int64_t)r] = (x + g.origin[0]) << g.log2_stride;
+        out[3 * (int64_t)r + 1] = (y + g.origin[1]) << g.log2_stride;
+        out[3 * (int64_t)r + 2] = (z + g.origin[2]) << g.log2_stride;
+        ++r;
+    }
+}
+
+__global__ void grid_query_kernel(ngs_grid g, const int32_t* __restrict__ c, int64_t n, const int32_t* __restrict__ off,
+                                  int k, int64_t* __restrict__ out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    int64_t kk = t / n, i = t % n;
+    int x = c[3 * i], y = c[3 * i + 1], z = c[3 * i + 2];
+    if (off) { x += off[3 * kk]; y += off[3 * kk + 1]; z += off[3 * kk + 2]; }
+    out[t] = grid_find(g, x, y, z);
+}
+
+__global__ void grid_rank_rows_kernel(ngs_grid g, const int32_t* __restrict__ c, int64_t n, int32_t* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = grid_find(g, c[3 * i], c[3 * i + 1], c[3 * i + 2]);
+}
+
+// Float ordering as signed ints for atomic min/max.
+__device__ __forceinline__ int f2key(float f) {
+    int b = __float_as_int(f);
+    return b >= 0 ? b : (b ^ 0x7FFFFFFF);
+}
+__device__ __forceinline__ float key2f(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7FFFFFFF)); }
+
+__global__ void bounds_init(int* mm) {
+    if (threadIdx.x < 3) { mm[threadIdx.x] = 0x7FFFFFFF; mm[3 + threadIdx.x] = (int)0x80000000; }
+}
+
+__global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ xyz, int64_t n, float inv_vs, int* mm) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float q = xyz[3 * i + a] * inv_vs;
+            lo[a] = fminf(lo[a], q);
+            hi[a] = fmaxf(hi[a], q);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        for (int d = 32; d > 0; d >>= 1) {
+            lo[a] = fminf(lo[a], __shfl_xor(lo[a], d, 64));
+            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], d, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&mm[a], f2key(lo[a]));
+            atomicMax(&mm[3 + a], f2key(hi[a]));
+        }
+    }
+}
+
+__global__ void bounds_decode(int* mm) {
+    if (threadIdx.x < 6) {
+        float f = key2f(mm[threadIdx.x]);
+        mm[threadIdx.x] = __float_as_int(f);
+    }
+}
+
+// ========================================================================================
+// Splat, field evaluation, levels, point basis
+// ========================================================================================
+__global__ void splat_kernel(ngs_grid g, const float* __restrict__ xyz, int64_t n, float inv_vs,
+                             const float* __restrict__ data, int ncomp, const float* __restrict__ scale,
+                             const int32_t* __restrict__ levels, int level, double post, double* __restrict__ acc) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (levels && levels[i] != level) return;
+    const int s = 1 << g.log2_stride;
+    const float inv_s = 1.0f / (float)s;
+    int dc[3], sg[3];
+    float loc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) trilinear_axis(xyz[3 * i + a] * inv_vs, s, inv_s, &dc[a], &sg[a], &loc[a]);
+    float row[3] = {1.f, 1.f, 1.f};
+    if (data) {
+        float sc = scale ? scale[i] : 1.0f;
+        for (int c = 0; c < ncomp; ++c) row[c] = data[i * ncomp + c] * sc;
+    }
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        int nx = (o >> 2) & 1, ny = (o >> 1) & 1, nz = o & 1;
+        int id = grid_find(g, dc[0] + nx * sg[0] * s, dc[1] + ny * sg[1] * s, dc[2] + nz * sg[2] * s);
+        if (id < 0) continue;
+        float ax = nx ? loc[0] : 1.0f - loc[0];
+        float ay = ny ? loc[1] : 1.0f - loc[1];
+        float az = nz ? loc[2] : 1.0f - loc[2];
+        float w = ax * ay * az;
+        if (data) {
+            for (int c = 0; c < ncomp; ++c) atomicAdd(&acc[(int64_t)id * ncomp + c], (double)(w * row[c]) * post);
+        } else {
+            atomicAdd(&acc[id], (double)w * post);
+        }
+    }
+}
+
+__global__ void eval_field_kernel(ngs_grid g, const float* __restrict__ field, const float* __restrict__ xyz, int64_t n,
+                                  float inv_vs, float scale, int accumulate, float* __restrict__ out,
+                                  float* __restrict__ grad) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = 1 << g.log2_stride;
+    const float inv_s = 1.0f / (float)s;
+    int cell[3];
+    float b[3][3], db[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) axis_basis_grad(xyz[3 * i + a] * inv_vs, s, inv_s, &cell[a], b[a], db[a]);
+    float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    for (int kx = 0; kx < 3; ++kx)
+        for (int ky = 0; ky < 3; ++ky)
+            for (int kz = 0; kz < 3; ++kz) {
+                int id = grid_find(g, cell[0] + (kx - 1) * s, cell[1] + (ky - 1) * s, cell[2] + (kz - 1) * s);
+                if (id < 0) continue;
+                float f = field[id];
+                v += (b[0][kx] * b[1][ky] * b[2][kz]) * f;
+                if (grad) {
+                    gx += (db[0][kx] * b[1][ky] * b[2][kz]) * f;
+                    gy += (b[0][kx] * db[1][ky] * b[2][kz]) * f;
+                    gz += (b[0][kx] * b[1][ky] * db[2][kz]) * f;
+                }
+            }
+    out[i] = (accumulate ? out[i] : 0.f) + v * scale;
+    if (grad) {
+        float* gp = grad + 3 * i;
+        gp[0] = (accumulate ? gp[0] : 0.f) + gx * scale;
+        gp[1] = (accumulate ? gp[1] : 0.f) + gy * scale;
+        gp[2] = (accumulate ? gp[2] : 0.f) + gz * scale;
+    }
+}
+
+__global__ void levels_kernel(const float* __restrict__ rho, int64_t n, float inv_min, float inv_ln, int depth,
+                              int32_t* __restrict__ lv) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float t = floorf(logf(rho[i] * inv_min) * inv_ln);
+    int k = (int)t;
+    if (k > 0) k = 0;
+    k = -k;
+    if (k > depth - 1) k = depth - 1;
+    lv[i] = k;
+}
+
+__global__ void point_basis_kernel(const float* __restrict__ xyz, int64_t n, float inv_vs, int s,
+                                   int32_t* __restrict__ cell, float* __restrict__ b) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float inv_s = 1.0f / (float)s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int c;
+        float bb[3];
+        axis_basis(xyz[3 * i + a] * inv_vs, s, inv_s, &c, bb);
+        cell[3 * i + a] = c;
+        b[9 * i + 3 * a] = bb[0];
+        b[9 * i + 3 * a + 1] = bb[1];
+        b[9 * i + 3 * a + 2] = bb[2];
+    }
+}
+
+__global__ void count_cells_kernel(const int32_t* __restrict__ rank, int64_t n, int32_t* __restrict__ counts) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && rank[i] >= 0) atomicAdd(&counts[rank[i]], 1);
+}
+
+__global__ void iota_i32(int32_t* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
+__global__ void f64_to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void __launch_bounds__(256) weighted_sum_kernel(const float* __restrict__ v, const float* __restrict__ w,
+                                                           int64_t n, double* sums) {
+    double a = 0.0, c = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double wi = w ? (double)w[i] : 1.0;
+        a += wi * (double)v[i];
+        c += wi;
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        c += __shfl_xor(c, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&sums[0], a);
+        atomicAdd(&sums[1], c);
+    }
+}
+
+__global__ void zero_f64(double* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0.0;
+}
+
+// rocPRIM stable radix sort wrapper (sort.hip).
+int sort_pairs_u32(const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout, int64_t n, int bits,
+                   void* tmp, size_t* tmp_bytes, hipStream_t st);
+
+}  // namespace ngs
+
+using namespace ngs;
+
+// ========================================================================================
+// C ABI
+// ========================================================================================
+extern "C" {
+
+const char* ngs_last_error(void) { return g_last_error.c_str(); }
+int ngs_abi_version(void) { return 1; }
+
+int ngs_hash_coords(const int32_t* coords, int64_t n, int32_t dim, uint64_t* keys, ngs_stream_t s) {
+    NGS_REQUIRE(dim >= 1 && dim <= 4, "ngs_hash_coords: dim must be 1..4");
+    NGS_REQUIRE(n >= 0, "ngs_hash_coords: n < 0");
+    if (n == 0) return NGS_OK;
+    hash_rows_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(coords, n, dim, keys);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_kernel_hash(const int32_t* coords, int64_t n, int32_t dim, const int32_t* offsets, int32_t k, uint64_t* out,
+                    ngs_stream_t s) {
+    NGS_REQUIRE(dim == 3 || dim == 4, "ngs_kernel_hash: dim must be 3 or 4");
+    NGS_REQUIRE(n >= 0 && k >= 0, "ngs_kernel_hash: bad sizes");
+    if (n == 0 || k == 0) return NGS_OK;
+    kernel_hash_kernel<<<grid1d(n * k, 256), 256, 0, S(s)>>>(coords, n, dim, offsets, k, out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_hash_sizing(int64_t n, int32_t enlarge, int64_t* table_size, int32_t* num_buckets, int32_t* evict_bound) {
+    NGS_REQUIRE(n > 0, "ngs_hash_sizing: n must be positive");
+    NGS_REQUIRE(n <= (int64_t)1 << 30, "ngs_hash_sizing: n too large");
+    double lg = std::ceil(std::log2((double)n));
+    int64_t next_pow2 = (int64_t)std::pow(2.0, lg);
+    int64_t ts = (n < 2048) ? 4 * next_pow2 : 2 * next_pow2;
+    if (enlarge) ts = 4 * next_pow2;
+    if (ts < 512) ts = 512;
+    *table_size = ts;
+    *num_buckets = (int32_t)((ts + BUCKET - 1) / BUCKET);
+    *evict_bound = (int32_t)(8 * lg);
+    return NGS_OK;
+}
+
+int64_t ngs_hash_workspace_bytes(int64_t table_size) {
+    int64_t nb = (table_size + BUCKET - 1) / BUCKET;
+    return ((nb * 4 + 255) / 256) * 256 + nb * BUCKET * 4 + 256;
+}
+
+int ngs_hash_build(ngs_hash_table* t, const uint64_t* keys, const int64_t* vals, int64_t n, uint64_t seed,
+                   void* workspace, ngs_stream_t s) {
+    NGS_REQUIRE(t && t->keys && t->vals && workspace, "ngs_hash_build: null table or workspace");
+    NGS_REQUIRE(n > 0, "ngs_hash_build: empty key set");
+    NGS_REQUIRE(t->num_buckets == (t->table_size + BUCKET - 1) / BUCKET, "ngs_hash_build: inconsistent sizing");
+    hipStream_t st = S(s);
+    HashWs ws = hash_ws(workspace, t->table_size);
+    const int nb = t->num_buckets;
+    zero_i32<<<grid1d(nb, 256), 256, 0, st>>>(ws.bucket_count, nb);
+    zero_i32<<<1, 64, 0, st>>>(ws.flags, 2);
+    NGS_CHECK_LAUNCH();
+    bucket_fill_kernel<<<grid1d(n, 256), 256, 0, st>>>(keys, n, nb, ws.bucket_count, ws.staged, ws.flags, nb <= 64 ? 1 : 0);
+    NGS_CHECK_LAUNCH();
+    int32_t flags[2];
+    NGS_CHECK_HIP(hipMemcpyAsync(flags, ws.flags, sizeof(flags), hipMemcpyDeviceToHost, st));
+    NGS_CHECK_HIP(hipStreamSynchronize(st));
+    if (flags[0] != 0) {
+        set_error("ngs_hash_build: bucket overflow (" + std::to_string(flags[0]) + " keys beyond bucket capacity)");
+        return NGS_ERR_HASH_OVERFLOW;
+    }
+    uint64_t state = seed;
+    for (int attempt = 0; attempt < 100; ++attempt) {
+        for (int f = 0; f < 3; ++f) t->mult[f] = splitmix64(state) | 1ull;
+        zero_i32<<<1, 64, 0, st>>>(ws.flags + 1, 1);
+        bucket_cuckoo_kernel<<<nb, BUCKET, 0, st>>>(keys, vals, ws.bucket_count, ws.staged, t->table_size, t->mult[0],
+                                                    t->mult[1], t->mult[2], t->evict_bound, t->keys, t->vals, ws.flags);
+        NGS_CHECK_LAUNCH();
+        int32_t fail = 0;
+        NGS_CHECK_HIP(hipMemcpyAsync(&fail, ws.flags + 1, sizeof(fail), hipMemcpyDeviceToHost, st));
+        NGS_CHECK_HIP(hipStreamSynchronize(st));
+        t->attempts = attempt + 1;
+        if (fail == 0) {
+            t->inserted = n;
+            return NGS_OK;
+        }
+    }
+    set_error("ngs_hash_build: cuckoo insertion failed for 100 hash families");
+    return NGS_ERR_REHASH;
+}
+
+int ngs_hash_query(const ngs_hash_table* t, const uint64_t* queries, int64_t n, int64_t* out, ngs_stream_t s) {
+    NGS_REQUIRE(t && t->keys, "ngs_hash_query: null table");
+    if (n <= 0) return NGS_OK;
+    hash_query_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(t->keys, t->vals, t->table_size, t->num_buckets, t->mult[0],
+                                                        t->mult[1], t->mult[2], queries, n, out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+// Bitmap boxes are limited to 2^36 lattice points (8 GiB of bits).
+int64_t ngs_grid_words(const int32_t dims[3]) {
+    if (dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return 0;
+    double v = (double)dims[0] * dims[1] * dims[2];
+    if (v > 68719476736.0) return 0;
+    int64_t bits = (int64_t)dims[0] * dims[1] * dims[2];
+    return (bits + 31) / 32;
+}
+
+int64_t ngs_scan_workspace_bytes(int64_t n) { return (scan_tiles(n) + 2) * 8; }
+
+int ngs_grid_clear(ngs_grid g, ngs_stream_t s) {
+    NGS_REQUIRE(g.bits, "ngs_grid_clear: null bits");
+    if (g.nwords == 0) return NGS_OK;
+    clear_words<<<grid1d(g.nwords, 256), 256, 0, S(s)>>>(g.bits, g.nwords);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_grid_mark_coords(ngs_grid g, const int32_t* coords, int64_t n, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    mark_coords_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, coords, n);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_grid_mark_corners(ngs_grid g, const float* xyz, int64_t n, float inv_voxel_size, const int32_t* levels,
+                          int32_t max_level, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    mark_corners_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, xyz, n, inv_voxel_size, levels, max_level);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_grid_finalize(ngs_grid g, int64_t* count, void* workspace, ngs_stream_t s) {
+    NGS_REQUIRE(g.bits && g.rank && workspace, "ngs_grid_finalize: null pointer");
+    return run_scan(LoadPopc{g.bits}, g.nwords, g.rank, count, nullptr, workspace, S(s));
+}
+
+int ngs_grid_emit(ngs_grid g, int32_t* coords_out, ngs_stream_t s) {
+    if (g.nwords == 0) return NGS_OK;
+    emit_kernel<<<grid1d(g.nwords, 256), 256, 0, S(s)>>>(g, coords_out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_grid_query(ngs_grid g, const int32_t* coords, int64_t n, const int32_t* offsets, int32_t k, int64_t* out,
+                   ngs_stream_t s) {
+    if (!offsets) k = 1;
+    if (n <= 0 || k <= 0) return NGS_OK;
+    grid_query_kernel<<<grid1d(n * k, 256), 256, 0, S(s)>>>(g, coords, n, offsets, k, out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_grid_rank_rows(ngs_grid g, const int32_t* coords, int64_t n, int32_t* out, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    grid_rank_rows_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, coords, n, out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_point_bounds(const float* xyz, int64_t n, float inv_voxel_size, float* minmax, ngs_stream_t s) {
+    NGS_REQUIRE(n > 0, "ngs_point_bounds: no points");
+    hipStream_t st = S(s);
+    int* mm = reinterpret_cast<int*>(minmax);
+    bounds_init<<<1, 64, 0, st>>>(mm);
+    unsigned blocks = grid1d(n, 256);
+    if (blocks > 1024) blocks = 1024;
+    bounds_kernel<<<blocks, 256, 0, st>>>(xyz, n, inv_voxel_size, mm);
+    bounds_decode<<<1, 64, 0, st>>>(mm);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_scan_i32(const int32_t* counts, int64_t n, int32_t* offsets, void* workspace, ngs_stream_t s) {
+    NGS_REQUIRE(workspace && offsets, "ngs_scan_i32: null pointer");
+    return scan_i32_internal(counts, n, offsets, workspace, S(s));
+}
+
+int ngs_splat(ngs_grid g, const float* xyz, int64_t n, float inv_voxel_size, const float* data, int32_t ncomp,
+              const float* scale, const int32_t* levels, int32_t level, double post, double* acc, ngs_stream_t s) {
+    NGS_REQUIRE(!data || (ncomp >= 1 && ncomp <= 3), "ngs_splat: ncomp must be 1..3");
+    if (n <= 0) return NGS_OK;
+    splat_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, xyz, n, inv_voxel_size, data, ncomp, scale, levels, level, post, acc);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_eval_field(ngs_grid g, const float* field, const float* xyz, int64_t n, float inv_voxel_size, float scale,
+                   int32_t accumulate, float* out, float* grad_out, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    eval_field_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, field, xyz, n, inv_voxel_size, scale, accumulate, out, grad_out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_point_levels(const float* density, int64_t n, float min_density, float log_base, int32_t depth, int32_t* levels,
+                     ngs_stream_t s) {
+    NGS_REQUIRE(min_density > 0 && log_base > 1 && depth >= 1, "ngs_point_levels: bad parameters");
+    if (n <= 0) return NGS_OK;
+    // PyTorch divides a float tensor by a CPU scalar as a multiplication by the float reciprocal.
+    float inv_min = 1.0f / min_density;
+    float inv_ln = 1.0f / (float)std::log((double)log_base);
+    levels_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(density, n, inv_min, inv_ln, depth, levels);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_point_basis(const float* xyz, int64_t n, float inv_voxel_size, int32_t stride, int32_t* cell, float* b,
+                    ngs_stream_t s) {
+    NGS_REQUIRE(stride > 0 && (stride & (stride - 1)) == 0, "ngs_point_basis: stride must be a power of two");
+    if (n <= 0) return NGS_OK;
+    point_basis_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(xyz, n, inv_voxel_size, stride, cell, b);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+static int bucket_layout(int64_t n, int64_t ncells, size_t* sort_tmp, int64_t* off_keys, int64_t* off_vin,
+                         int64_t* off_scan, int64_t* off_counts, int64_t* total) {
+    size_t tmp = 0;
+    int r = sort_pairs_u32(nullptr, nullptr, nullptr, nullptr, n, 32, nullptr, &tmp, nullptr);
+    if (r != NGS_OK) return r;
+    auto al = [](int64_t x) { return (x + 255) / 256 * 256; };
+    *sort_tmp = tmp;
+    *off_keys = al((int64_t)tmp);
+    *off_vin = *off_keys + al(n * 4);
+    *off_scan = *off_vin + al(n * 4);
+    *off_counts = *off_scan + al(ngs_scan_workspace_bytes(ncells));
+    *total = *off_counts + al(ncells * 4);
+    return NGS_OK;
+}
+
+int64_t ngs_bucket_workspace_bytes(int64_t n, int64_t ncells) {
+    size_t tmp;
+    int64_t a, b, c, d, total;
+    if (bucket_layout(n, ncells, &tmp, &a, &b, &c, &d, &total) != NGS_OK) return -1;
+    return total;
+}
+
+int ngs_bucket_points(const int32_t* cell_rank, int64_t n, int64_t ncells, int32_t* perm, int32_t* cell_start,
+                      void* workspace, int64_t workspace_bytes, ngs_stream_t s) {
+    NGS_REQUIRE(n > 0 && ncells > 0, "ngs_bucket_points: empty input");
+    hipStream_t st = S(s);
+    size_t tmp;
+    int64_t ok, ov, osc, oc, total;
+    int r = bucket_layout(n, ncells, &tmp, &ok, &ov, &osc, &oc, &total);
+    if (r != NGS_OK) return r;
+    NGS_REQUIRE(workspace_bytes >= total, "ngs_bucket_points: workspace too small");
+    char* w = reinterpret_cast<char*>(workspace);
+    uint32_t* kout = reinterpret_cast<uint32_t*>(w + ok);
+    int32_t* vin = reinterpret_cast<int32_t*>(w + ov);
+    int32_t* counts = reinterpret_cast<int32_t*>(w + oc);
+    zero_i32<<<grid1d(ncells, 256), 256, 0, st>>>(counts, ncells);
+    count_cells_kernel<<<grid1d(n, 256), 256, 0, st>>>(cell_rank, n, counts);
+    iota_i32<<<grid1d(n, 256), 256, 0, st>>>(vin, n);
+    NGS_CHECK_LAUNCH();
+    r = scan_i32_internal(counts, ncells, cell_start, w + osc, st);
+    if (r != NGS_OK) return r;
+    int bits = 1;
+    while (((int64_t)1 << bits) < ncells + 1 && bits < 32) ++bits;
+    size_t tb = tmp;
+    return sort_pairs_u32(reinterpret_cast<const uint32_t*>(cell_rank), kout, vin, perm, n, bits, w, &tb, st);
+}
+
+int ngs_f64_to_f32(const double* in, float* out, int64_t n, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    f64_to_f32_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(in, out, n);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_weighted_sum(const float* v, const float* w, int64_t n, double* sums, ngs_stream_t s) {
+    hipStream_t st = S(s);
+    zero_f64<<<1, 64, 0, st>>>(sums, 2);
+    if (n > 0) {
+        unsigned blocks = grid1d(n, 256);
+        if (blocks > 1024) blocks = 1024;
+        weighted_sum_kernel<<<blocks, 256, 0, st>>>(v, w, n, sums);
+    }
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,460 @@
+// Sparse dual marching cubes over the conformal leaves of the voxel hierarchy.
+#include "common.h"
+
+namespace ngs {
+
+int scan_i32_internal(const int32_t* counts, int64_t n, int32_t* offsets, void* ws, hipStream_t st);
+int sort_pairs_u64(const uint64_t* kin, uint64_t* kout, const int32_t* vin, int32_t* vout, int64_t n, int bits,
+                   void* tmp, size_t* tmp_bytes, hipStream_t st);
+
+// Standard 256-case marching-cubes triangle table (Lorensen-Cline / Bourke numbering), one case per
+// 60-bit word: triangle edge ids as nibbles from the low end, 0xF terminates.
+__constant__ uint64_t c_mc_cases[256] = {
+    0xfffffffffffffff, 0xffffffffffff380, 0xffffffffffff910, 0xfffffffff189381,
+    0xffffffffffffa21, 0xfffffffffa21380, 0xfffffffff920a29, 0xffffff89a8a2382,
+    0xffffffffffff2b3, 0xfffffffff0b82b0, 0xfffffffffb32091, 0xffffffb89b912b1,
+    0xfffffffff3ab1a3, 0xffffffab8a801a0, 0xffffff9ab9b3093, 0xfffffffffb8aa89,
+    0xffffffffffff874, 0xfffffffff437034, 0xfffffffff748910, 0xffffff137174914,
+    0xfffffffff748a21, 0xffffffa21403743, 0xffffff748209a29, 0xfff4973727929a2,
+    0xfffffffff2b3748, 0xffffff40242b74b, 0xffffffb32748109, 0xfff1292b9b49b74,
+    0xffffff487ab31a3, 0xfff4b7401b41ab1, 0xfff30bab9b09874, 0xffffffab99b4b74,
+    0xffffffffffff459, 0xfffffffff380459, 0xfffffffff051450, 0xffffff513538458,
+    0xfffffffff459a21, 0xffffff594a21803, 0xffffff204245a25, 0xfff8434535235a2,
+    0xfffffffffb32459, 0xffffff594b802b0, 0xffffffb32510450, 0xfff584b82852512,
+    0xffffff45931ab3a, 0xfffab81a8180594, 0xfff30bab5b05045, 0xffffffb8aa85845,
+    0xfffffffff975879, 0xffffff375359039, 0xffffff751710870, 0xfffffffff753351,
+    0xffffff21a759879, 0xfff37503505921a, 0xfff25a758528208, 0xffffff7533525a2,
+    0xffffff2b3987597, 0xfffb72029279759, 0xfff751871810b32, 0xffffff51771b12b,
+    0xfffb3a31a758859, 0x0aba010b7905075, 0x07570805a30b0ab, 0xfffffffff5b75ab,
+    0xffffffffffff56a, 0xfffffffff6a5380, 0xfffffffff6a5109, 0xffffff6a5891381,
+    0xfffffffff162561, 0xffffff803621561, 0xffffff620609569, 0xfff823625285895,
+    0xfffffffff56ab32, 0xffffff56a02b80b, 0xffffff6a5b32910, 0xfffb892b92916a5,
+    0xffffff315356b36, 0xfff6b51505b0b80, 0xfff9505606306b3, 0xffffff89bb96956,
+    0xfffffffff8746a5, 0xffffffa56374034, 0xffffff7486a5091, 0xfff49737179156a,
+    0xffffff874156216, 0xfff743403625521, 0xfff620560509748, 0x962695923497937,
+    0xffffff56a4872b3, 0xfffb720242746a5, 0xfff6a5b32874910, 0x6a54b7b492b9129,
+    0xfff6b51535b3748, 0xb404b7b016b5b15, 0x74836b630560950, 0xfff9b7974b96956,
+    0xfffffffffa4694a, 0xffffff380a946a4, 0xffffff04606a10a, 0xfffa16468618138,
+    0xffffff462421941, 0xfff462942921803, 0xfffffffff624420, 0xffffff624428238,
+    0xffffff32b46a94a, 0xfff6a4a94b82280, 0xfffa164606102b3, 0x1b8b12184a16146,
+    0xfff36b319639469, 0x14641916b0181b8, 0xffffff4600636b3, 0xfffffffff86b846,
+    0xffffffa98a876a7, 0xfffa76a907a0370, 0xfff0818717a176a, 0xffffff37117a76a,
+    0xfff768981861621, 0x937390976192962, 0xffffff206607087, 0xfffffffff276237,
+    0xfff76898a86ab32, 0x7a9a76790b72702, 0xb32a767a1871081, 0xfff17616a71b12b,
+    0x63136b619768698, 0xfffffffff76b190, 0xfff06b0b3607087, 0xffffffffffff6b7,
+    0xffffffffffffb67, 0xfffffffff67b803, 0xfffffffff67b910, 0xffffff67b138918,
+    0xfffffffff7b621a, 0xffffff7b6803a21, 0xffffff7b69a2092, 0xfff89a38a3a27b6,
+    0xfffffffff726327, 0xffffff026067807, 0xffffff910732672, 0xfff678891681261,
+    0xffffff73171a67a, 0xfff801781a7167a, 0xfff7a69a0a70730, 0xffffff9a88a7a67,
+    0xfffffffff68b486, 0xffffff640603b63, 0xffffff109648b68, 0xfff63b139369649,
+    0xffffff1a28b6486, 0xfff640b60b03a21, 0xfff9a2920b648b4, 0x36463b34923a39a,
+    0xffffff264248328, 0xfffffffff264240, 0xfff834642432091, 0xffffff642241491,
+    0xfff1a6648168318, 0xffffff40660a01a, 0x39a9303a6834364, 0xfffffffff4a649a,
+    0xfffffffffb67594, 0xffffff67b594380, 0xffffffb67045105, 0xfff51345343867b,
+    0xffffffb6721a459, 0xfff594380a217b6, 0xfff204a24a45b67, 0x67b25a523453843,
+    0xffffff945267327, 0xfff786260680459, 0xfff045051673263, 0x851584812786826,
+    0xfff73167161a459, 0x459078701671a61, 0xa737a6a305a4a04, 0xfffa84a458a7a67,
+    0xffffff98b9b6596, 0xfff590650360b63, 0xfffb65510b508b0, 0xffffff1355363b6,
+    0xfff65b8b9b59a21, 0xa21965690b603b0, 0x52025a50865b58b, 0xfff35a3a25363b6,
+    0xfff283265825985, 0xffffff260069659, 0x826283865081851, 0xfffffffff612651,
+    0x698965683a61631, 0xfff06505960a01a, 0xfffffffffa65830, 0xffffffffffff65a,
+    0xfffffffffb57a5b, 0xffffff03857ba5b, 0xffffff091ba57b5, 0xfff1381897ba57a,
+    0xffffff15717b21b, 0xfffb27571721380, 0xfff7b2209729579, 0x289823295b27257,
+    0xffffff573532a52, 0xfff52a578258028, 0xfff2a37353a5109, 0x25752a278129289,
+    0xfffffffff573531, 0xffffff571170780, 0xffffff735539309, 0xfffffffff795789,
+    0xffffff8ba8a5485, 0xfff03bba50b5405, 0xfff54aba8a48910, 0x41314943b54a4ba,
+    0xfff8548b2582152, 0xb151b2b543b0b40, 0x58b8545b2950520, 0xfffffffff3b2549,
+    0xfff483543253a52, 0xffffff0244252a5, 0x910854583a532a3, 0xfff2492914252a5,
+    0xffffff153358548, 0xfffffffff501540, 0xfff530509358548, 0xffffffffffff549,
+    0xffffffba9b947b4, 0xfffba97b9794380, 0xfffb470414b1ba1, 0x4bab474a1843413,
+    0xfff219b294b97b4, 0x3801b2b197b9479, 0xffffff04224b47b, 0xfff42343824b47b,
+    0xfff947732972a92, 0x70207872a4797a9, 0xa040a1a472a3a73, 0xfffffffff4782a1,
+    0xffffff317714194, 0xfff178180714194, 0xfffffffff347304, 0xffffffffffff784,
+    0xfffffffff8ba8a9, 0xffffffa9bb93903, 0xffffffba88a0a10, 0xfffffffffa3ba13,
+    0xffffff8b99b1b21, 0xfff9b2921b93903, 0xfffffffffb08b20, 0xffffffffffffb23,
+    0xffffff98aa82832, 0xfffffffff2902a9, 0xfff8a1810a82832, 0xffffffffffff2a1,
+    0xfffffffff819831, 0xffffffffffff190, 0xffffffffffff830, 0xfffffffffffffff,
+};
+
+// Edge e interpolates from corner c_edge[e][0] to c_edge[e][1] (argument order of the reference's
+// sdf_interp calls); corner k of a dual cube sits at octant (k&1 ^ k>>1&1, k>>1&1, k>>2&1).
+__constant__ int8_t c_edge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                     {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+// MC corner index of the cell lying on side (sx, sy, sz) (1 = positive) of a dual centre.
+__device__ __forceinline__ int octant_slot(int sx, int sy, int sz) {
+    return sz * 4 + (sy ? (sx ? 2 : 3) : (sx ? 1 : 0));
+}
+
+__device__ __forceinline__ int case_count(int c) {
+    uint64_t w = c_mc_cases[c];
+    int n = 0;
+    while (n < 15 && ((w >> (4 * n)) & 0xF) != 0xF) ++n;
+    return n;
+}
+
+__global__ void conformal_children_kernel(const int32_t* __restrict__ par, int64_t np, int s, ngs_grid finer,
+                                          int has_finer, int32_t* __restrict__ ch, int32_t* __restrict__ keep) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np * 8) return;
+    int64_t p = t >> 3;
+    int o = (int)(t & 7);
+    int x = par[3 * p] + ((o >> 2) & 1) * s, y = par[3 * p + 1] + ((o >> 1) & 1) * s, z = par[3 * p + 2] + (o & 1) * s;
+    ch[3 * t] = x;
+    ch[3 * t + 1] = y;
+    ch[3 * t + 2] = z;
+    keep[t] = (has_finer && grid_test(finer, x, y, z)) ? 0 : 1;
+}
+
+__global__ void unrefined_kernel(const int32_t* __restrict__ c, int64_t n, ngs_grid finer, int32_t* __restrict__ keep) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keep[i] = grid_test(finer, c[3 * i], c[3 * i + 1], c[3 * i + 2]) ? 0 : 1;
+}
+
+__global__ void compact_rows_kernel(const int32_t* __restrict__ rows, int width, const int32_t* __restrict__ keep,
+                                    const int32_t* __restrict__ off, int64_t n, int32_t* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    int64_t o = off[i];
+    for (int j = 0; j < width; ++j) out[o * width + j] = rows[i * width + j];
+}
+
+__global__ void expand_kernel(const int32_t* __restrict__ leaves, int64_t n, int nu, int s, int32_t* __restrict__ out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu3 = nu * nu * nu;
+    if (t >= n * nu3) return;
+    int64_t l = t / nu3;
+    int e = (int)(t % nu3);
+    int ex = e / (nu * nu), ey = (e / nu) % nu, ez = e % nu;
+    out[3 * t] = leaves[3 * l] * nu + ex * s;
+    out[3 * t + 1] = leaves[3 * l + 1] * nu + ey * s;
+    out[3 * t + 2] = leaves[3 * l + 2] * nu + ez * s;
+}
+
+__global__ void mark_dual_kernel(ngs_grid g, const int32_t* __restrict__ prim, int64_t n, int s) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 8) return;
+    int64_t p = t >> 3;
+    int o = (int)(t & 7);
+    grid_mark(g, prim[3 * p] + ((o >> 2) & 1) * s, prim[3 * p + 1] + ((o >> 1) & 1) * s, prim[3 * p + 2] + (o & 1) * s);
+}
+
+// Boundary lattice point t of a side-s cell: 8 corners, then 12 (s-1) edge interiors (edges
+// along axis e/4 at the (e&1, e>>1&1) corner of the other two axes), then 6 (s-1)^2 face
+// interiors (face f normal to axis f/2 on side f&1).
+__device__ __forceinline__ void boundary_point(int t, int s, int loc[3]) {
+    if (t < 8) {
+        loc[0] = ((t >> 2) & 1) * s;
+        loc[1] = ((t >> 1) & 1) * s;
+        loc[2] = (t & 1) * s;
+        return;
+    }
+    t -= 8;
+    const int e_n = 12 * (s - 1);
+    if (t < e_n) {
+        int e = t / (s - 1), u = t % (s - 1) + 1;
+        int ax = e / 4, a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+        loc[ax] = u;
+        loc[a1] = (e & 1) * s;
+        loc[a2] = ((e >> 1) & 1) * s;
+        return;
+    }
+    t -= e_n;
+    const int fs = (s - 1) * (s - 1);
+    int f = t / fs, u = t % fs;
+    int ax = f / 2, a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+    loc[ax] = (f & 1) * s;
+    loc[a1] = u % (s - 1) + 1;
+    loc[a2] = u / (s - 1) + 1;
+}
+
+__global__ void dual_accumulate_kernel(ngs_grid cen, const int32_t* __restrict__ prim, int64_t n, int s, int level,
+                                       int id_base, int32_t* __restrict__ acc, int32_t* __restrict__ cnt) {
+    const int per = 6 * s * s + 2;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * per) return;
+    int64_t p = t / per;
+    int loc[3];
+    boundary_point((int)(t % per), s, loc);
+    int dc = grid_find(cen, prim[3 * p] + loc[0], prim[3 * p + 1] + loc[1], prim[3 * p + 2] + loc[2]);
+    if (dc < 0) return;
+    // sides of the dual centre covered by the cell, per axis: bit0 negative side, bit1 positive
+    int side[3];
+    int nsl = 1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        side[a] = loc[a] == 0 ? 2 : (loc[a] == s ? 1 : 3);
+        nsl *= (side[a] == 3) ? 2 : 1;
+    }
+    int old = atomicAdd(&cnt[dc], nsl);
+    if (old >= 8) return;
+    const int pid = id_base + (int)p;
+    for (int sx = 0; sx < 2; ++sx) {
+        if (!((side[0] >> sx) & 1)) continue;
+        for (int sy = 0; sy < 2; ++sy) {
+            if (!((side[1] >> sy) & 1)) continue;
+            for (int sz = 0; sz < 2; ++sz) {
+                if (!((side[2] >> sz) & 1)) continue;
+                int sl = octant_slot(sx, sy, sz);
+                acc[((int64_t)dc * 8 + sl) * 2] = pid;
+                acc[((int64_t)dc * 8 + sl) * 2 + 1] = level;
+            }
+        }
+    }
+}
+
+__global__ void complete_flags_kernel(const int32_t* __restrict__ cnt, int64_t n, int32_t* __restrict__ flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = cnt[i] == 8 ? 1 : 0;
+}
+
+__global__ void compact_cubes_kernel(const int32_t* __restrict__ acc, const int32_t* __restrict__ flag,
+                                     const int32_t* __restrict__ off, int64_t n, int32_t* __restrict__ cubes) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    int64_t o = off[i];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cubes[o * 8 + k] = acc[(i * 8 + k) * 2];
+}
+
+__global__ void corner_pos_kernel(const int32_t* __restrict__ prim, int64_t n, int s, float scale, float* __restrict__ pos) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    pos[i] = (float)(prim[i] + s) * scale;
+}
+
+__device__ __forceinline__ int cube_case(const int32_t* ids, const float* __restrict__ val) {
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (val[ids[k]] < 0.f) c |= 1 << k;
+    return c;
+}
+
+__global__ void mc_count_kernel(const int32_t* __restrict__ cubes, int64_t k, const float* __restrict__ val,
+                                int32_t* __restrict__ vc) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    vc[i] = case_count(cube_case(cubes + 8 * i, val));
+}
+
+// dual_mc.cu:20-31 semantics: snap to an endpoint whose value is within 1e-5 of zero, or to p1
+// when the two values are within 1e-5; otherwise linear interpolation of the zero crossing.
+__device__ __forceinline__ void sdf_interp(const float* p1, const float* p2, float v1, float v2, float* out) {
+#pragma clang fp contract(off)
+    if (fabsf(0.0f - v1) < 1.0e-5f) { out[0] = p1[0]; out[1] = p1[1]; out[2] = p1[2]; return; }
+    if (fabsf(0.0f - v2) < 1.0e-5f) { out[0] = p2[0]; out[1] = p2[1]; out[2] = p2[2]; return; }
+    if (fabsf(v1 - v2) < 1.0e-5f) { out[0] = p1[0]; out[1] = p1[1]; out[2] = p1[2]; return; }
+    float w2 = (0.0f - v1) / (v2 - v1);
+    float w1 = 1.0f - w2;
+    out[0] = p1[0] * w1 + p2[0] * w2;
+    out[1] = p1[1] * w1 + p2[1] * w2;
+    out[2] = p1[2] * w1 + p2[2] * w2;
+}
+
+__global__ void mc_emit_kernel(const int32_t* __restrict__ cubes, int64_t k, const float* __restrict__ val,
+                               const float* __restrict__ pos, const int32_t* __restrict__ voff, float* __restrict__ tris,
+                               uint64_t* __restrict__ keys) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const int32_t* ids = cubes + 8 * i;
+    const int c = cube_case(ids, val);
+    uint64_t w = c_mc_cases[c];
+    int64_t v = voff[i];
+    for (int n = 0; n < 15; ++n, ++v) {
+        int e = (int)((w >> (4 * n)) & 0xF);
+        if (e == 0xF) break;
+        int a = c_edge[e][0], b = c_edge[e][1];
+        int ia = ids[a], ib = ids[b];
+        float p1[3] = {pos[3 * (int64_t)ia], pos[3 * (int64_t)ia + 1], pos[3 * (int64_t)ia + 2]};
+        float p2[3] = {pos[3 * (int64_t)ib], pos[3 * (int64_t)ib + 1], pos[3 * (int64_t)ib + 2]};
+        float out[3];
+        sdf_interp(p1, p2, val[ia], val[ib], out);
+        tris[3 * v] = out[0];
+        tris[3 * v + 1] = out[1];
+        tris[3 * v + 2] = out[2];
+        uint32_t hi = (uint32_t)max(ia, ib), lo = (uint32_t)min(ia, ib);
+        keys[v] = ((uint64_t)hi << 32) | lo;
+    }
+}
+
+__global__ void iota_kernel(int32_t* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
+__global__ void run_heads_kernel(const uint64_t* __restrict__ k, int64_t n, int32_t* __restrict__ head) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) head[i] = (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
+}
+
+__global__ void dedup_scatter_kernel(const int32_t* __restrict__ head, const int32_t* __restrict__ off,
+                                     const int32_t* __restrict__ idx, const float* __restrict__ tris, int64_t n,
+                                     int64_t* __restrict__ faces, float* __restrict__ verts) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // off[i] = number of run heads strictly before i; the run of element i has rank off[i+1]-1
+    int32_t rank = off[i] + head[i] - 1;
+    int32_t src = idx[i];
+    faces[src] = rank;
+    if (head[i]) {
+        verts[3 * (int64_t)rank] = tris[3 * (int64_t)src];
+        verts[3 * (int64_t)rank + 1] = tris[3 * (int64_t)src + 1];
+        verts[3 * (int64_t)rank + 2] = tris[3 * (int64_t)src + 2];
+    }
+}
+
+__global__ void widen_kernel(const int32_t* a, int64_t* b) { *b = *a; }
+
+static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+}  // namespace ngs
+
+using namespace ngs;
+
+extern "C" {
+
+int ngs_conformal_children(const int32_t* parents, int64_t np, int32_t stride, ngs_grid finer_parents, int32_t has_finer,
+                           int32_t* children, int32_t* keep, ngs_stream_t s) {
+    if (np <= 0) return NGS_OK;
+    conformal_children_kernel<<<grid1d(np * 8, 256), 256, 0, S(s)>>>(parents, np, stride, finer_parents, has_finer,
+                                                                      children, keep);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_unrefined_flags(const int32_t* coords, int64_t n, ngs_grid finer_parents, int32_t* keep, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    unrefined_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(coords, n, finer_parents, keep);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_compact_rows(const int32_t* rows, int32_t width, const int32_t* keep, int64_t n, int32_t* out, int32_t* offsets,
+                     void* workspace, ngs_stream_t s) {
+    NGS_REQUIRE(width >= 1 && workspace && offsets, "ngs_compact_rows: bad arguments");
+    if (n <= 0) return NGS_OK;
+    int r = scan_i32_internal(keep, n, offsets, workspace, S(s));
+    if (r != NGS_OK) return r;
+    compact_rows_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(rows, width, keep, offsets, n, out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_expand_primal(const int32_t* leaves, int64_t n, int32_t n_upsample, int32_t stride, int32_t* out, ngs_stream_t s) {
+    NGS_REQUIRE(n_upsample >= 1, "ngs_expand_primal: n_upsample must be >= 1");
+    int64_t tot = n * n_upsample * n_upsample * n_upsample;
+    if (tot <= 0) return NGS_OK;
+    expand_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(leaves, n, n_upsample, stride, out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_mark_dual_centres(ngs_grid g, const int32_t* primal, int64_t n, int32_t stride, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    mark_dual_kernel<<<grid1d(n * 8, 256), 256, 0, S(s)>>>(g, primal, n, stride);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_dual_accumulate(ngs_grid centres, const int32_t* primal, int64_t n, int32_t stride, int32_t level, int32_t id_base,
+                        int32_t* acc, int32_t* count, ngs_stream_t s) {
+    NGS_REQUIRE(stride >= 1, "ngs_dual_accumulate: bad stride");
+    if (n <= 0) return NGS_OK;
+    int64_t tot = n * (6LL * stride * stride + 2);
+    dual_accumulate_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, primal, n, stride, level, id_base, acc, count);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_dual_compact(const int32_t* acc, const int32_t* count, int64_t dc, int32_t* cubes, int32_t* flags, int32_t* offsets,
+                     void* workspace, ngs_stream_t s) {
+    NGS_REQUIRE(workspace && flags && offsets, "ngs_dual_compact: bad arguments");
+    if (dc <= 0) return NGS_OK;
+    complete_flags_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(count, dc, flags);
+    NGS_CHECK_LAUNCH();
+    int r = scan_i32_internal(flags, dc, offsets, workspace, S(s));
+    if (r != NGS_OK) return r;
+    compact_cubes_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(acc, flags, offsets, dc, cubes);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_dual_corner_positions(const int32_t* primal, int64_t n, int32_t stride, float scale, float* pos, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    corner_pos_kernel<<<grid1d(3 * n, 256), 256, 0, S(s)>>>(primal, n, stride, scale, pos);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_mc_count(const int32_t* cubes, int64_t k, const float* values, int32_t* vcount, ngs_stream_t s) {
+    if (k <= 0) return NGS_OK;
+    mc_count_kernel<<<grid1d(k, 256), 256, 0, S(s)>>>(cubes, k, values, vcount);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_mc_emit(const int32_t* cubes, int64_t k, const float* values, const float* pos, const int32_t* voffset, float* tris,
+                uint64_t* vkeys, ngs_stream_t s) {
+    if (k <= 0) return NGS_OK;
+    mc_emit_kernel<<<grid1d(k, 128), 128, 0, S(s)>>>(cubes, k, values, pos, voffset, tris, vkeys);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+static int dedup_layout(int64_t n, size_t* tmp, size_t off[6], size_t* total) {
+    size_t t = 0;
+    int r = sort_pairs_u64(nullptr, nullptr, nullptr, nullptr, n, 64, nullptr, &t, nullptr);
+    if (r != NGS_OK) return r;
+    *tmp = t;
+    off[0] = al256(t);                          // sorted keys
+    off[1] = off[0] + al256(8 * (size_t)n);     // iota
+    off[2] = off[1] + al256(4 * (size_t)n);     // sorted idx
+    off[3] = off[2] + al256(4 * (size_t)n);     // heads
+    off[4] = off[3] + al256(4 * (size_t)n);     // offsets (n + 1)
+    off[5] = off[4] + al256(4 * (size_t)(n + 1));
+    *total = off[5] + al256((size_t)ngs_scan_workspace_bytes(n));
+    return NGS_OK;
+}
+
+int64_t ngs_dedup_workspace_bytes(int64_t nkeys) {
+    size_t tmp, off[6], total;
+    if (dedup_layout(nkeys, &tmp, off, &total) != NGS_OK) return -1;
+    return (int64_t)total;
+}
+
+int ngs_dedup_vertices(const uint64_t* vkeys, const float* tris, int64_t nkeys, int64_t* faces, float* verts, int64_t* nverts,
+                       void* workspace, int64_t workspace_bytes, ngs_stream_t s) {
+    NGS_REQUIRE(workspace, "ngs_dedup_vertices: null workspace");
+    if (nkeys <= 0) return NGS_OK;
+    size_t tmp, off[6], total;
+    int r = dedup_layout(nkeys, &tmp, off, &total);
+    if (r != NGS_OK) return r;
+    NGS_REQUIRE((size_t)workspace_bytes >= total, "ngs_dedup_vertices: workspace too small");
+    char* w = reinterpret_cast<char*>(workspace);
+    hipStream_t st = S(s);
+    uint64_t* kout = reinterpret_cast<uint64_t*>(w + off[0]);
+    int32_t* vin = reinterpret_cast<int32_t*>(w + off[1]);
+    int32_t* vout = reinterpret_cast<int32_t*>(w + off[2]);
+    int32_t* head = reinterpret_cast<int32_t*>(w + off[3]);
+    int32_t* offs = reinterpret_cast<int32_t*>(w + off[4]);
+    iota_kernel<<<grid1d(nkeys, 256), 256, 0, st>>>(vin, nkeys);
+    NGS_CHECK_LAUNCH();
+    size_t tb = tmp;
+    r = sort_pairs_u64(vkeys, kout, vin, vout, nkeys, 64, w, &tb, st);
+    if (r != NGS_OK) return r;
+    run_heads_kernel<<<grid1d(nkeys, 256), 256, 0, st>>>(kout, nkeys, head);
+    NGS_CHECK_LAUNCH();
+    r = scan_i32_internal(head, nkeys, offs, w + off[5], st);
+    if (r != NGS_OK) return r;
+    dedup_scatter_kernel<<<grid1d(nkeys, 256), 256, 0, st>>>(head, offs, vout, tris, nkeys, faces, verts);
+    NGS_CHECK_LAUNCH();
+    if (nverts) {
+        widen_kernel<<<1, 1, 0, st>>>(&offs[nkeys], nverts);
+        NGS_CHECK_LAUNCH();
+    }
+    return NGS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,448 @@
+// Jacobi-preconditioned conjugate gradients on the assembled CSR system.
+//
+// One iteration = three kernels, no host round trip:
+//   K1 spmv_dot:   q = A p and p.q; the last block to finish computes alpha = rz / pq.
+//   K2 update:     x += alpha p, r -= alpha q, rz' = r.(D^-1 r), rr = r.r; the last block decides
+//                  convergence (rr <= tol^2 bb), counts the iteration and forms beta = rz'/rz.
+//   K3 direction:  p = D^-1 r + beta p.
+// Reductions are FP64, per block then over the block partials in a fixed order by the last block
+// (deterministic). Rows are partitioned in contiguous chunks, and block -> chunk is mapped so that
+// all blocks of one XCD (blockIdx % 8) own one contiguous eighth of the rows: each XCD's L2 then
+// serves the vector gathers of its own slab. Chunks of iterations are captured once into a
+// hipGraph and replayed; kernels of a converged system return immediately.
+#include "common.h"
+
+#include <vector>
+
+namespace ngs {
+
+constexpr int PCG_THREADS = 256;
+constexpr int NUM_XCD = 8;
+
+struct PcgState {
+    double rz, pq, rr, bb, tol2, alpha, beta, rz_new;
+    int32_t iter, done, max_iter, pad;
+    uint32_t counter[4];
+};
+
+struct PcgArgs {
+    const int32_t* rowptr;
+    const int32_t* cols;
+    const float* vals;
+    const float* dinv;
+    const float* b;
+    float* x;
+    float* r;
+    float* p;
+    float* q;
+    double* partials;  // 2 * nblocks
+    PcgState* st;
+    int64_t M;
+    int64_t chunk;     // rows per block
+    int nblocks;
+};
+
+__device__ __forceinline__ int logical_block() {
+    // Blocks are dispatched round-robin over the XCDs; give each XCD a contiguous range.
+    const int nb = gridDim.x, b = blockIdx.x;
+    if (nb % NUM_XCD) return b;
+    return (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Block reduce of two doubles (256 threads), result valid in thread 0.
+__device__ __forceinline__ void block_sum2(double& a, double& b) {
+    __shared__ double sa[PCG_THREADS / 64], sb[PCG_THREADS / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { sa[w] = a; sb[w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0, tb = 0;
+        for (int i = 0; i < PCG_THREADS / 64; ++i) { ta += sa[i]; tb += sb[i]; }
+        a = ta;
+        b = tb;
+    }
+    __syncthreads();
+}
+
+// Publish this block's partials; returns true in every thread of the last block to arrive, after
+// which sums (fixed order) of both partial arrays are in (a, b) of thread 0.
+__device__ bool last_block_reduce(double& a, double& b, double* partials, uint32_t* counter, int nblocks) {
+    __shared__ bool last;
+    block_sum2(a, b);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[nblocks + blockIdx.x], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        uint32_t ticket = atomicAdd(counter, 1u);
+        last = (ticket == (uint32_t)nblocks - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double ta = 0, tb = 0;
+    for (int i = threadIdx.x; i < nblocks; i += PCG_THREADS) {
+        ta += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tb += __hip_atomic_load(&partials[nblocks + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum2(ta, tb);
+    a = ta;
+    b = tb;
+    if (threadIdx.x == 0) *counter = 0u;
+    return true;
+}
+
+__global__ void __launch_bounds__(PCG_THREADS) pcg_init_kernel(PcgArgs A, double tol) {
+    const int lb = logical_block();
+    const int64_t r0 = (int64_t)lb * A.chunk;
+    const int64_t r1 = min(A.M, r0 + A.chunk);
+    double bb = 0, rz = 0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) {
+        float bi = A.b[i];
+        float z = A.dinv[i] * bi;
+        A.x[i] = 0.f;
+        A.r[i] = bi;
+        A.p[i] = z;
+        bb += (double)bi * bi;
+        rz += (double)bi * z;
+    }
+    if (last_block_reduce(bb, rz, A.partials, &A.st->counter[0], A.nblocks) && threadIdx.x == 0) {
+        PcgState* s = A.st;
+        s->bb = bb;
+        s->rz = rz;
+        s->tol2 = tol * tol;
+        s->iter = 0;
+        s->done = (bb == 0.0) ? 1 : 0;
+    }
+}
+
+template <int G>
+__global__ void __launch_bounds__(PCG_THREADS) pcg_spmv_dot_kernel(PcgArgs A) {
+    if (A.st->done) return;
+    const int lb = logical_block();
+    const int64_t r0 = (int64_t)lb * A.chunk;
+    const int64_t r1 = min(A.M, r0 + A.chunk);
+    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    constexpr int GROUPS = PCG_THREADS / G;
+    double pq = 0;
+    for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+        const int32_t k0 = A.rowptr[row], k1 = A.rowptr[row + 1];
+        float sum = 0.f;
+        for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
+            const float4 v = *reinterpret_cast<const float4*>(A.vals + k);
+            const int4 c = *reinterpret_cast<const int4*>(A.cols + k);
+            sum += v.x * A.p[c.x] + v.y * A.p[c.y] + v.z * A.p[c.z] + v.w * A.p[c.w];
+        }
+#pragma unroll
+        for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        if (lane == 0) {
+            A.q[row] = sum;
+            pq += (double)A.p[row] * sum;
+        }
+    }
+    double dummy = 0;
+    if (last_block_reduce(pq, dummy, A.partials, &A.st->counter[1], A.nblocks) && threadIdx.x == 0) {
+        PcgState* s = A.st;
+        s->pq = pq;
+        s->alpha = s->rz / pq;
+    }
+}
+
+__global__ void __launch_bounds__(PCG_THREADS) pcg_update_kernel(PcgArgs A) {
+    if (A.st->done) return;
+    const int lb = logical_block();
+    const int64_t r0 = (int64_t)lb * A.chunk;
+    const int64_t r1 = min(A.M, r0 + A.chunk);
+    const float alpha = (float)A.st->alpha;
+    double rz = 0, rr = 0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) {
+        float pi = A.p[i];
+        A.x[i] += alpha * pi;
+        float ri = A.r[i] - alpha * A.q[i];
+        A.r[i] = ri;
+        float z = A.dinv[i] * ri;
+        rz += (double)ri * z;
+        rr += (double)ri * ri;
+    }
+    if (last_block_reduce(rz, rr, A.partials, &A.st->counter[2], A.nblocks) && threadIdx.x == 0) {
+        PcgState* s = A.st;
+        s->iter += 1;
+        s->rr = rr;
+        if (rr <= s->tol2 * s->bb) {
+            s->done = 1;
+        } else if (s->iter >= s->max_iter) {
+            s->done = 2;
+        } else {
+            s->beta = rz / s->rz;
+            s->rz = rz;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(PCG_THREADS) pcg_direction_kernel(PcgArgs A) {
+    if (A.st->done) return;
+    const int lb = logical_block();
+    const int64_t r0 = (int64_t)lb * A.chunk;
+    const int64_t r1 = min(A.M, r0 + A.chunk);
+    const float beta = (float)A.st->beta;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) A.p[i] = A.dinv[i] * A.r[i] + beta * A.p[i];
+}
+
+template <int G>
+__global__ void __launch_bounds__(PCG_THREADS) spmv_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                                                           const float* __restrict__ vals, const float* __restrict__ x,
+                                                           float* __restrict__ y, int64_t M, int64_t chunk) {
+    const int lb = logical_block();
+    const int64_t r0 = (int64_t)lb * chunk;
+    const int64_t r1 = min(M, r0 + chunk);
+    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    constexpr int GROUPS = PCG_THREADS / G;
+    for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+        const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+        float sum = 0.f;
+        for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
+            const float4 v = *reinterpret_cast<const float4*>(vals + k);
+            const int4 c = *reinterpret_cast<const int4*>(cols + k);
+            sum += v.x * x[c.x] + v.y * x[c.y] + v.z * x[c.z] + v.w * x[c.w];
+        }
+#pragma unroll
+        for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        if (lane == 0) y[row] = sum;
+    }
+}
+
+__global__ void __launch_bounds__(PCG_THREADS) residual_f64_kernel(const int32_t* __restrict__ rowptr,
+                                                                   const int32_t* __restrict__ cols,
+                                                                   const float* __restrict__ vals, const double* __restrict__ b,
+                                                                   const double* __restrict__ x, double* __restrict__ r,
+                                                                   int64_t M, int64_t chunk, double* partials,
+                                                                   uint32_t* counter, double* norms) {
+    constexpr int G = 16;
+    const int lb = logical_block();
+    const int64_t r0 = (int64_t)lb * chunk;
+    const int64_t r1 = min(M, r0 + chunk);
+    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    constexpr int GROUPS = PCG_THREADS / G;
+    double rr = 0, bb = 0;
+    for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+        const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+        double sum = 0.0;
+        for (int32_t k = k0 + lane; k < k1; k += G) sum += (double)vals[k] * x[cols[k]];
+#pragma unroll
+        for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        if (lane == 0) {
+            double ri = b[row] - sum;
+            r[row] = ri;
+            rr += ri * ri;
+            bb += b[row] * b[row];
+        }
+    }
+    if (last_block_reduce(rr, bb, partials, counter, gridDim.x) && threadIdx.x == 0) {
+        norms[0] = rr;
+        norms[1] = bb;
+    }
+}
+
+__global__ void zero_state(PcgState* s, int max_iter) {
+    if (threadIdx.x == 0) {
+        s->counter[0] = s->counter[1] = s->counter[2] = s->counter[3] = 0u;
+        s->max_iter = max_iter;
+        s->done = 0;
+        s->iter = 0;
+    }
+}
+
+static int pcg_blocks(int64_t M, int64_t* chunk) {
+    int64_t nb = (M + 1023) / 1024;  // ~1024 rows per block
+    if (nb > 2048) nb = 2048;
+    if (nb >= NUM_XCD) nb = (nb / NUM_XCD) * NUM_XCD;
+    if (nb < 1) nb = 1;
+    *chunk = (M + nb - 1) / nb;
+    return (int)nb;
+}
+
+static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+static int pcg_layout(int64_t M, PcgArgs* A, char* ws) {
+    int64_t chunk;
+    int nb = pcg_blocks(M, &chunk);
+    size_t off = 0;
+    A->st = reinterpret_cast<PcgState*>(ws + off);
+    off += al256(sizeof(PcgState));
+    A->partials = reinterpret_cast<double*>(ws + off);
+    off += al256(sizeof(double) * 2 * nb);
+    A->r = reinterpret_cast<float*>(ws + off);
+    off += al256(sizeof(float) * M);
+    A->p = reinterpret_cast<float*>(ws + off);
+    off += al256(sizeof(float) * M);
+    A->q = reinterpret_cast<float*>(ws + off);
+    off += al256(sizeof(float) * M);
+    A->nblocks = nb;
+    A->chunk = chunk;
+    A->M = M;
+    return (int)0;
+}
+
+static int64_t pcg_ws_bytes(int64_t M) {
+    int64_t chunk;
+    int nb = pcg_blocks(M, &chunk);
+    return (int64_t)(al256(sizeof(PcgState)) + al256(sizeof(double) * 2 * nb) + 3 * al256(sizeof(float) * M));
+}
+
+static int choose_group(const int32_t* /*rowptr*/, int64_t /*M*/) { return 16; }
+
+static void launch_iteration(const PcgArgs& A, int G, hipStream_t st) {
+    dim3 grid(A.nblocks), block(PCG_THREADS);
+    if (G == 8)
+        pcg_spmv_dot_kernel<8><<<grid, block, 0, st>>>(A);
+    else if (G == 32)
+        pcg_spmv_dot_kernel<32><<<grid, block, 0, st>>>(A);
+    else
+        pcg_spmv_dot_kernel<16><<<grid, block, 0, st>>>(A);
+    pcg_update_kernel<<<grid, block, 0, st>>>(A);
+    pcg_direction_kernel<<<grid, block, 0, st>>>(A);
+}
+
+struct HostState {
+    int32_t* pinned = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+static HostState& host_state() {
+    static thread_local HostState hs;
+    return hs;
+}
+
+}  // namespace ngs
+
+using namespace ngs;
+
+extern "C" {
+
+int64_t ngs_pcg_workspace_bytes(int64_t M) { return pcg_ws_bytes(M); }
+
+// Environment-free knobs for measurement: graph use and lanes per row (0 = automatic).
+static int g_pcg_use_graph = 1;
+static int g_pcg_group = 0;
+int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row) {
+    NGS_REQUIRE(lanes_per_row == 0 || lanes_per_row == 8 || lanes_per_row == 16 || lanes_per_row == 32,
+                "ngs_pcg_config: lanes_per_row must be 0, 8, 16 or 32");
+    g_pcg_use_graph = use_graph ? 1 : 0;
+    g_pcg_group = lanes_per_row;
+    return NGS_OK;
+}
+
+int ngs_pcg_solve(const int32_t* rowptr, const int32_t* cols, const float* vals, const float* inv_diag, const float* b,
+                  float* x, int64_t M, double tol, int32_t max_iter, int32_t chunk, void* workspace, ngs_stream_t s,
+                  int32_t* iters, double* rel_resid) {
+    NGS_REQUIRE(M >= 0 && workspace, "ngs_pcg_solve: bad arguments");
+    NGS_REQUIRE(tol >= 0, "ngs_pcg_solve: negative tolerance");
+    if (M == 0) {
+        if (iters) *iters = 0;
+        if (rel_resid) *rel_resid = 0;
+        return NGS_OK;
+    }
+    if (chunk <= 0) chunk = 32;
+    if (max_iter < 0) max_iter = 10000000;
+    hipStream_t st = S(s);
+    HostState& hs = host_state();
+    if (!hs.pinned) NGS_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&hs.pinned), 64, hipHostMallocDefault));
+    if (!hs.stream) NGS_CHECK_HIP(hipStreamCreateWithFlags(&hs.stream, hipStreamNonBlocking));
+    hipStream_t work = hs.stream;
+    // Order the private capture stream after the caller's stream.
+    hipEvent_t ev;
+    NGS_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    NGS_CHECK_HIP(hipEventRecord(ev, st));
+    NGS_CHECK_HIP(hipStreamWaitEvent(work, ev, 0));
+
+    PcgArgs A{};
+    pcg_layout(M, &A, reinterpret_cast<char*>(workspace));
+    A.rowptr = rowptr;
+    A.cols = cols;
+    A.vals = vals;
+    A.dinv = inv_diag;
+    A.b = b;
+    A.x = x;
+    const int G = g_pcg_group ? g_pcg_group : choose_group(rowptr, M);
+    zero_state<<<1, 64, 0, work>>>(A.st, max_iter);
+    pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, work>>>(A, tol);
+    NGS_CHECK_LAUNCH();
+
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (g_pcg_use_graph) {
+        NGS_CHECK_HIP(hipStreamBeginCapture(work, hipStreamCaptureModeThreadLocal));
+        for (int c = 0; c < chunk; ++c) launch_iteration(A, G, work);
+        NGS_CHECK_HIP(hipStreamEndCapture(work, &graph));
+        NGS_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    int status = NGS_OK;
+    int32_t it = 0, done = 0;
+    while (true) {
+        if (exec) {
+            NGS_CHECK_HIP(hipGraphLaunch(exec, work));
+        } else {
+            for (int c = 0; c < chunk; ++c) launch_iteration(A, G, work);
+            NGS_CHECK_LAUNCH();
+        }
+        NGS_CHECK_HIP(hipMemcpyAsync(hs.pinned, &A.st->iter, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, work));
+        NGS_CHECK_HIP(hipStreamSynchronize(work));
+        it = hs.pinned[0];
+        done = hs.pinned[1];
+        if (done) break;
+    }
+    double sc[2];
+    NGS_CHECK_HIP(hipMemcpy(sc, &A.st->rr, sizeof(double) * 2, hipMemcpyDeviceToHost));  // rr, bb
+    if (exec) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    }
+    (void)hipEventDestroy(ev);
+    if (iters) *iters = it;
+    if (rel_resid) *rel_resid = sc[1] > 0 ? std::sqrt(sc[0] / sc[1]) : 0.0;
+    if (done == 2) {
+        set_error("ngs_pcg_solve: iteration cap reached before the tolerance");
+        status = NGS_ERR_NOT_CONVERGED;
+    }
+    return status;
+}
+
+int ngs_spmv(const int32_t* rowptr, const int32_t* cols, const float* vals, const float* x, float* y, int64_t M,
+             ngs_stream_t s) {
+    if (M <= 0) return NGS_OK;
+    int64_t chunk;
+    int nb = pcg_blocks(M, &chunk);
+    const int G = g_pcg_group ? g_pcg_group : 16;
+    if (G == 8)
+        spmv_kernel<8><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
+    else if (G == 32)
+        spmv_kernel<32><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
+    else
+        spmv_kernel<16><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_residual_f64(const int32_t* rowptr, const int32_t* cols, const float* vals, const double* b, const double* x,
+                     double* r, int64_t M, double* norms, void* workspace, ngs_stream_t s) {
+    NGS_REQUIRE(workspace, "ngs_residual_f64: null workspace");
+    if (M <= 0) return NGS_OK;
+    int64_t chunk;
+    int nb = pcg_blocks(M, &chunk);
+    char* w = reinterpret_cast<char*>(workspace);
+    PcgState* stt = reinterpret_cast<PcgState*>(w);
+    double* partials = reinterpret_cast<double*>(w + al256(sizeof(PcgState)));
+    zero_state<<<1, 64, 0, S(s)>>>(stt, 0);
+    residual_f64_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, b, x, r, M, chunk, partials, &stt->counter[3], norms);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+"""ctypes binding of libngspsr.so (the C ABI declared in include/ngspsr.h).
+
+The library is built in-tree (python -m ngspsr._build, or __graft_entry__.build()). Every
+product code path goes through it; there is no CPU fallback: loading fails loudly when the
+library is absent.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libngspsr.so")
+
+c_int32, c_int64, c_float, c_double, c_void_p, c_uint64 = (ctypes.c_int32, ctypes.c_int64, ctypes.c_float,
+                                                            ctypes.c_double, ctypes.c_void_p, ctypes.c_uint64)
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("origin", c_int32 * 3), ("dims", c_int32 * 3), ("log2_stride", c_int32), ("pad_", c_int32),
+                ("nwords", c_int64), ("bits", c_void_p), ("rank", c_void_p)]
+
+
+class Level(ctypes.Structure):
+    _fields_ = [("grid", Grid), ("coords", c_void_p), ("M", c_int64), ("stride", c_int32), ("pad_", c_int32)]
+
+
+class PointBins(ctypes.Structure):
+    _fields_ = [("cells", Grid), ("cell_start", c_void_p), ("perm", c_void_p), ("b", c_void_p), ("alpha", c_void_p),
+                ("alpha_scalar", c_float), ("pad_", c_float), ("n", c_int64)]
+
+
+class HashTable(ctypes.Structure):
+    _fields_ = [("table_size", c_int64), ("num_buckets", c_int32), ("evict_bound", c_int32), ("keys", c_void_p),
+                ("vals", c_void_p), ("mult", c_uint64 * 3), ("inserted", c_int64), ("attempts", c_int32),
+                ("pad_", c_int32)]
+
+
+P = c_void_p
+_SIGNATURES = {
+    "ngs_last_error": ([], ctypes.c_char_p),
+    "ngs_abi_version": ([], c_int32),
+    "ngs_hash_coords": ([P, c_int64, c_int32, P, P], c_int32),
+    "ngs_kernel_hash": ([P, c_int64, c_int32, P, c_int32, P, P], c_int32),
+    "ngs_hash_sizing": ([c_int64, c_int32, ctypes.POINTER(c_int64), ctypes.POINTER(c_int32),
+                         ctypes.POINTER(c_int32)], c_int32),
+    "ngs_hash_workspace_bytes": ([c_int64], c_int64),
+    "ngs_hash_build": ([ctypes.POINTER(HashTable), P, P, c_int64, c_uint64, P, P], c_int32),
+    "ngs_hash_query": ([ctypes.POINTER(HashTable), P, c_int64, P, P], c_int32),
+    "ngs_grid_words": ([ctypes.POINTER(c_int32)], c_int64),
+    "ngs_scan_workspace_bytes": ([c_int64], c_int64),
+    "ngs_grid_clear": ([Grid, P], c_int32),
+    "ngs_grid_mark_coords": ([Grid, P, c_int64, P], c_int32),
+    "ngs_grid_mark_corners": ([Grid, P, c_int64, c_float, P, c_int32, P], c_int32),
+    "ngs_grid_finalize": ([Grid, P, P, P], c_int32),
+    "ngs_grid_emit": ([Grid, P, P], c_int32),
+    "ngs_grid_query": ([Grid, P, c_int64, P, c_int32, P, P], c_int32),
+    "ngs_grid_rank_rows": ([Grid, P, c_int64, P, P], c_int32),
+    "ngs_point_bounds": ([P, c_int64, c_float, P, P], c_int32),
+    "ngs_scan_i32": ([P, c_int64, P, P, P], c_int32),
+    "ngs_splat": ([Grid, P, c_int64, c_float, P, c_int32, P, P, c_int32, c_double, P, P], c_int32),
+    "ngs_eval_field": ([Grid, P, P, c_int64, c_float, c_float, c_int32, P, P, P], c_int32),
+    "ngs_point_levels": ([P, c_int64, c_float, c_float, c_int32, P, P], c_int32),
+    "ngs_point_basis": ([P, c_int64, c_float, c_int32, P, P, P], c_int32),
+    "ngs_bucket_workspace_bytes": ([c_int64, c_int64], c_int64),
+    "ngs_bucket_points": ([P, c_int64, c_int64, P, P, P, c_int64, P], c_int32),
+    "ngs_set_tables": ([P, P, c_int32], c_int32),
+    "ngs_lhs_row_counts": ([Level, P, P], c_int32),
+    "ngs_lhs_assemble": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P, P], c_int32),
+    "ngs_rhs_cross_stiffness": ([Level, Level, P, P, P], c_int32),
+    "ngs_rhs_normals": ([Level, Level, P, P, P], c_int32),
+    "ngs_pcg_workspace_bytes": ([c_int64], c_int64),
+    "ngs_pcg_config": ([c_int32, c_int32], c_int32),
+    "ngs_pcg_solve": ([P, P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
+                       ctypes.POINTER(c_double)], c_int32),
+    "ngs_spmv": ([P, P, P, P, P, c_int64, P], c_int32),
+    "ngs_residual_f64": ([P, P, P, P, P, P, c_int64, P, P, P], c_int32),
+    "ngs_conformal_children": ([P, c_int64, c_int32, Grid, c_int32, P, P, P], c_int32),
+    "ngs_unrefined_flags": ([P, c_int64, Grid, P, P], c_int32),
+    "ngs_compact_rows": ([P, c_int32, P, c_int64, P, P, P, P], c_int32),
+    "ngs_expand_primal": ([P, c_int64, c_int32, c_int32, P, P], c_int32),
+    "ngs_mark_dual_centres": ([Grid, P, c_int64, c_int32, P], c_int32),
+    "ngs_dual_accumulate": ([Grid, P, c_int64, c_int32, c_int32, c_int32, P, P, P], c_int32),
+    "ngs_dual_compact": ([P, P, c_int64, P, P, P, P, P], c_int32),
+    "ngs_dual_corner_positions": ([P, c_int64, c_int32, c_float, P, P], c_int32),
+    "ngs_mc_count": ([P, c_int64, P, P, P], c_int32),
+    "ngs_mc_emit": ([P, c_int64, P, P, P, P, P, P], c_int32),
+    "ngs_dedup_workspace_bytes": ([c_int64], c_int64),
+    "ngs_dedup_vertices": ([P, P, c_int64, P, P, P, P, c_int64, P], c_int32),
+    "ngs_f64_to_f32": ([P, P, c_int64, P], c_int32),
+    "ngs_weighted_sum": ([P, P, c_int64, P, P], c_int32),
+}
+
+STATUS = {0: "ok", 1: "bad argument", 2: "HIP error", 3: "hash bucket overflow", 4: "rehash exhausted",
+          5: "not converged", 6: "capacity exceeded"}
+
+
+class NgsError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+_LIB = None
+
+
+def lib():
+    """The loaded library; raises when it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libngspsr.so not found at {LIB_PATH}; build it with `python -m ngspsr._build` "
+                               "(or __graft_entry__.build()). There is no CPU fallback.")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def call(name, *args):
+    st = getattr(lib(), name)(*args)
+    if st != 0:
+        raise NgsError(st, lib().ngs_last_error().decode())
+    return st
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Scratch:
+    """Per-device growable scratch buffer for stream-ordered temporary workspaces."""
+    _bufs = {}
+
+    @classmethod
+    def get(cls, nbytes, device, slot=0):
+        key = (str(device), slot)
+        buf = cls._bufs.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(int(nbytes), 256) + 256, dtype=torch.uint8, device=device)
+            cls._bufs[key] = buf
+        return buf

@@ -1,0 +1,298 @@
+"""Screened Poisson solve and mesh extraction on the GPU (jspsr/core/reconstructor.py)."""
+import numpy as np
+import torch
+
+from . import solver as _solver
+from ._lib import PointBins, Scratch, call, lib, ptr, stream
+from .bases import BezierTensorBasis, ensure_device_tables
+from .grid import SparseGrid
+from .hashtree import HashTree
+
+
+class PointBinning:
+    """Points of the screening term binned by their containing cell at one level, with the 3x3
+    per-axis basis values of the 27 voxels around each point (reconstructor.py:111-120)."""
+
+    def __init__(self, tree: HashTree, xyz, d: int, alpha):
+        n = xyz.shape[0]
+        s = tree.strides[d]
+        dev = tree.device
+        self.n = n
+        self.cell = torch.empty((n, 3), dtype=torch.int32, device=dev)
+        self.b = torch.empty((n, 9), dtype=torch.float32, device=dev)
+        call("ngs_point_basis", ptr(xyz), n, tree.inv_vs, s, ptr(self.cell), ptr(self.b), stream())
+        self.cells = SparseGrid.for_points(tree._qmin, tree._qmax, s, dev)
+        self.cells.mark_coords(self.cell)
+        self.ncells = self.cells.finalize(emit=False)
+        rank = torch.empty(n, dtype=torch.int32, device=dev)
+        call("ngs_grid_rank_rows", self.cells.struct(), ptr(self.cell), n, ptr(rank), stream())
+        self.perm = torch.empty(n, dtype=torch.int32, device=dev)
+        self.cell_start = torch.empty(self.ncells + 1, dtype=torch.int32, device=dev)
+        wsb = lib().ngs_bucket_workspace_bytes(n, self.ncells)
+        ws = Scratch.get(wsb, dev, slot=3)
+        call("ngs_bucket_points", ptr(rank), n, self.ncells, ptr(self.perm), ptr(self.cell_start), ptr(ws), wsb, stream())
+        self.alpha = alpha if isinstance(alpha, torch.Tensor) else None
+        self.alpha_scalar = 0.0 if self.alpha is not None else float(alpha)
+
+    @classmethod
+    def empty(cls, tree):
+        obj = cls.__new__(cls)
+        obj.n = 0
+        obj.cells = SparseGrid.empty(tree.device)
+        obj.cell_start = torch.zeros(1, dtype=torch.int32, device=tree.device)
+        obj.perm = obj.b = obj.alpha = None
+        obj.alpha_scalar = 0.0
+        return obj
+
+    def struct(self) -> PointBins:
+        pb = PointBins()
+        pb.cells = self.cells.struct()
+        pb.cell_start = self.cell_start.data_ptr()
+        pb.perm = None if self.perm is None else self.perm.data_ptr()
+        pb.b = None if self.b is None else self.b.data_ptr()
+        pb.alpha = None if self.alpha is None else self.alpha.data_ptr()
+        pb.alpha_scalar = self.alpha_scalar
+        pb.n = self.n
+        return pb
+
+
+class Reconstructor:
+    def __init__(self, hash_tree: HashTree, basis=None, feat=None):
+        if basis is not None and not isinstance(basis, BezierTensorBasis):
+            raise NotImplementedError("only the degree-2 Bezier tensor basis is supported on this path")
+        self.hash_tree = hash_tree
+        self.basis = basis or BezierTensorBasis()
+        self.solutions = {}
+        self.systems = {}
+        self.solve_info = {}
+        self.sample_weight = None
+        self.fixed_level_set = None
+        self.branch = HashTree.DECODER
+        ensure_device_tables(hash_tree.device)
+
+    def set_fixed_level_set(self, value):
+        self.fixed_level_set = value
+
+    def assemble_level(self, d, bins, coarse, sf, delta, rhs):
+        tree = self.hash_tree
+        lv = tree.level(d)
+        M = lv.M
+        row_nnz = torch.empty(M, dtype=torch.int32, device=tree.device)
+        call("ngs_lhs_row_counts", lv, ptr(row_nnz), stream())
+        rowptr = torch.empty(M + 1, dtype=torch.int32, device=tree.device)
+        ws = Scratch.get(lib().ngs_scan_workspace_bytes(M), tree.device)
+        call("ngs_scan_i32", ptr(row_nnz), M, ptr(rowptr), ptr(ws), stream())
+        nnz = int(rowptr[M].item())
+        cols = torch.empty(nnz, dtype=torch.int32, device=tree.device)
+        vals = torch.empty(nnz, dtype=torch.float32, device=tree.device)
+        inv_diag = torch.empty(M, dtype=torch.float32, device=tree.device)
+        call("ngs_lhs_assemble", lv, bins.struct(), ptr(coarse), float(sf), float(delta), ptr(rowptr), ptr(cols),
+             ptr(vals), ptr(inv_diag), ptr(rhs), stream())
+        return _solver.CSRSystem(rowptr, cols, vals, inv_diag)
+
+    def solve_multigrid(self, start_depth, end_depth, normal_data: dict, screen_alpha=0.0, screen_xyz=None,
+                        screen_delta=0.1, solver: str = "pcg", verbose: bool = False):
+        """reconstructor.py:79-195: coarse-to-fine per-level solves; each level's RHS is corrected by
+        the coarser solutions (cascadic)."""
+        tree = self.hash_tree
+        if isinstance(screen_alpha, torch.Tensor):
+            assert screen_xyz is not None and screen_alpha.shape[0] == screen_xyz.shape[0]
+            should_screen = True
+            alpha = screen_alpha.float().contiguous()
+        else:
+            should_screen = screen_alpha > 0.0
+            alpha = float(screen_alpha)
+        if isinstance(screen_delta, torch.Tensor):
+            raise NotImplementedError("per-point screen_delta is not supported on this path")
+        self.solutions = {}
+        xyz = screen_xyz.float().contiguous() if should_screen else None
+        bins = {}
+        for d in range(end_depth, start_depth + 1):
+            bins[d] = PointBinning(tree, xyz, d, alpha) if should_screen else PointBinning.empty(tree)
+        for d in range(start_depth, end_depth - 1, -1):
+            M = tree.get_coords_size(None, d)
+            if M == 0:
+                self.solutions[d] = torch.zeros(0, dtype=torch.float32, device=tree.device)
+                continue
+            sf = 0.25 ** d
+            rhs = torch.zeros(M, dtype=torch.float64, device=tree.device)
+            for dd, nd in normal_data.items():
+                call("ngs_rhs_normals", tree.level(d), tree.level(dd), ptr(nd.float().contiguous()), ptr(rhs), stream())
+            coarser = [dd for dd in range(start_depth, d, -1) if tree.get_coords_size(None, dd) > 0]
+            coarse = None
+            if should_screen and coarser:
+                coarse = torch.zeros(xyz.shape[0], dtype=torch.float32, device=tree.device)
+                for dd in coarser:
+                    call("ngs_eval_field", tree.grids[dd].struct(), ptr(self.solutions[dd]), ptr(xyz), xyz.shape[0],
+                         tree.inv_vs, 1.0, 1, ptr(coarse), None, stream())
+            for dd in coarser:
+                call("ngs_rhs_cross_stiffness", tree.level(d), tree.level(dd), ptr(self.solutions[dd]), ptr(rhs),
+                     stream())
+            system = self.assemble_level(d, bins[d], coarse, sf, float(screen_delta), rhs)
+            x, info = _solver.solve(system, rhs, solver)
+            self.solutions[d] = x.contiguous()
+            self.systems[d] = (system, rhs)
+            self.solve_info[d] = info
+            if verbose:
+                r = system.matvec(x).double() - rhs
+                print(f"Solving complete at level {d}, residual = {torch.linalg.norm(r).item()}.")
+        return self.solutions
+
+    # -------------------------------------------------------------------------- field
+    def evaluate_raw_chi(self, xyz, compute_grad=False, depths=None):
+        tree = self.hash_tree
+        q = xyz.float().contiguous()
+        n = q.shape[0]
+        out = torch.zeros(n, dtype=torch.float32, device=tree.device)
+        grad = torch.zeros((n, 3), dtype=torch.float32, device=tree.device) if compute_grad else None
+        for d, sol in self.solutions.items():
+            if depths is not None and d not in depths:
+                continue
+            if sol.numel() == 0:
+                continue
+            call("ngs_eval_field", tree.grids[d].struct(), ptr(sol), ptr(q), n, tree.inv_vs, 1.0, 1, ptr(out), ptr(grad),
+                 stream())
+        return (out, grad) if compute_grad else out
+
+    def get_mean_chi(self):
+        """reconstructor.py:228-236: level set = sample-weighted mean of chi at the input points."""
+        if self.fixed_level_set is not None:
+            return float(self.fixed_level_set)
+        chi = self.evaluate_raw_chi(self.hash_tree.xyz)
+        sums = torch.empty(2, dtype=torch.float64, device=chi.device)
+        w = None if self.sample_weight is None else self.sample_weight.float().contiguous()
+        call("ngs_weighted_sum", ptr(chi), ptr(w), chi.shape[0], ptr(sums), stream())
+        s = sums.cpu()
+        return float(s[0] / s[1])
+
+    def evaluate_chi(self, xyz, max_points: int = -1):
+        mean = self.get_mean_chi()
+        return self.evaluate_raw_chi(xyz) - mean
+
+    # -------------------------------------------------------------------------- mesh
+    def extract_multiscale_mesh(self, n_upsample: int = 1, max_depth: int = 100, expand: int = 0, trim: bool = False,
+                                build_o3d_mesh: bool = False, max_points: int = -1, return_info=False):
+        """reconstructor.py:296-397 (expand=0, trim=False): dual marching cubes over the conformal
+        leaves, each split into n_upsample^3 sub-cells. Returns (vertices (V,3) float32,
+        triangles (T,3) int64)."""
+        if expand != 0 or trim:
+            raise NotImplementedError("only expand=0, trim=False is supported on this path")
+        tree = self.hash_tree
+        dev = tree.device
+        L = min(max_depth, tree.depth)
+        # parents (stride 2s) of every level but the last, sorted
+        parents = {}
+        for d in range(L - 1):
+            s = tree.strides[d]
+            g = tree.grids[d]
+            pg = SparseGrid([(o * s) // (2 * s) - 1 for o in g.origin],
+                            [((o + dm) * s) // (2 * s) + 1 for o, dm in zip(g.origin, g.dims)], 2 * s, dev)
+            pg.mark_coords(g.coords)
+            pg.finalize()
+            parents[d] = pg
+        leaves = {}
+        max_d = L
+        for d in range(L):
+            s = tree.strides[d]
+            if d < L - 1:
+                npar = parents[d].count
+                conf = torch.empty((npar * 8, 3), dtype=torch.int32, device=dev)
+                keep = torch.empty(npar * 8, dtype=torch.int32, device=dev)
+                fin = parents[d - 1] if d > 0 else SparseGrid.empty(dev)
+                call("ngs_conformal_children", ptr(parents[d].coords), npar, s, fin.struct(), int(d > 0), ptr(conf),
+                     ptr(keep), stream())
+            else:
+                conf = tree.grids[d].coords
+                keep = torch.ones(conf.shape[0], dtype=torch.int32, device=dev)
+                if d > 0:
+                    call("ngs_unrefined_flags", ptr(conf), conf.shape[0], parents[d - 1].struct(), ptr(keep), stream())
+            n = conf.shape[0]
+            out = torch.empty((n, 3), dtype=torch.int32, device=dev)
+            offs = torch.empty(n + 1, dtype=torch.int32, device=dev)
+            ws = Scratch.get(lib().ngs_scan_workspace_bytes(n), dev)
+            if n:
+                call("ngs_compact_rows", ptr(conf), 3, ptr(keep), n, ptr(out), ptr(offs), ptr(ws), stream())
+                k = int(offs[n].item())
+            else:
+                k = 0
+            if d > 0 and k == 0:
+                max_d = d
+                break
+            leaves[d] = out[:k]
+        L = max_d
+        nu = n_upsample
+        primal, bases_, nprim = {}, {}, 0
+        for d in range(L):
+            s = tree.strides[d]
+            lf = leaves[d]
+            pr = torch.empty((lf.shape[0] * nu ** 3, 3), dtype=torch.int32, device=dev)
+            if lf.shape[0]:
+                call("ngs_expand_primal", ptr(lf), lf.shape[0], nu, s, ptr(pr), stream())
+            primal[d] = pr
+            bases_[d] = nprim
+            nprim += pr.shape[0]
+        # dual centres: sorted unique corners of all primal sub-cells (upsampled units)
+        lo = [min(((o - 2) * tree.strides[d]) * nu for d in range(L) for o in [tree.grids[d].origin[a]]) for a in range(3)]
+        hi = [max(((o + dm + 2) * tree.strides[d]) * nu for d in range(L)
+                  for o, dm in [(tree.grids[d].origin[a], tree.grids[d].dims[a])]) for a in range(3)]
+        cg = SparseGrid(lo, hi, 1, dev)
+        for d in range(L):
+            if primal[d].shape[0]:
+                call("ngs_mark_dual_centres", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d], stream())
+        DC = cg.finalize()
+        acc = torch.zeros((DC, 8, 2), dtype=torch.int32, device=dev)
+        cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
+        for d in range(L):
+            if primal[d].shape[0]:
+                call("ngs_dual_accumulate", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d], d,
+                     bases_[d], ptr(acc), ptr(cnt), stream())
+        cubes = torch.empty((DC, 8), dtype=torch.int32, device=dev)
+        flags = torch.empty(DC, dtype=torch.int32, device=dev)
+        offs = torch.empty(DC + 1, dtype=torch.int32, device=dev)
+        ws = Scratch.get(lib().ngs_scan_workspace_bytes(DC), dev)
+        call("ngs_dual_compact", ptr(acc), ptr(cnt), DC, ptr(cubes), ptr(flags), ptr(offs), ptr(ws), stream())
+        K = int(offs[DC].item()) if DC else 0
+        cubes = cubes[:K]
+        # dual corners = (primal + stride) * voxel_size / n_upsample, and chi there
+        pos = torch.empty((nprim, 3), dtype=torch.float32, device=dev)
+        scale = float(np.float32(tree.voxel_size / nu))
+        for d in range(L):
+            if primal[d].shape[0]:
+                call("ngs_dual_corner_positions", ptr(primal[d]), primal[d].shape[0], tree.strides[d], scale,
+                     ptr(pos[bases_[d]:]), stream())
+        values = self.evaluate_chi(pos, max_points=max_points)
+        v, f = self.dual_marching_cubes(cubes, pos, values)
+        if return_info:
+            return v, f, dict(leaves=leaves, primal=primal, centres=cg.coords, acc=acc, count=cnt, cubes=cubes, pos=pos,
+                              values=values)
+        return v, f
+
+    @staticmethod
+    def dual_marching_cubes(cubes, pos, values):
+        """dual_mc.cu + reconstructor.py:379-383: triangles of the complete dual cubes, vertices
+        de-duplicated by their corner-pair key (sorted)."""
+        dev = pos.device
+        K = cubes.shape[0]
+        if K == 0:
+            return torch.zeros((0, 3), dtype=torch.float32, device=dev), torch.zeros((0, 3), dtype=torch.int64, device=dev)
+        cubes = cubes.contiguous()
+        values = values.float().contiguous()
+        vcount = torch.empty(K, dtype=torch.int32, device=dev)
+        call("ngs_mc_count", ptr(cubes), K, ptr(values), ptr(vcount), stream())
+        voff = torch.empty(K + 1, dtype=torch.int32, device=dev)
+        ws = Scratch.get(lib().ngs_scan_workspace_bytes(K), dev)
+        call("ngs_scan_i32", ptr(vcount), K, ptr(voff), ptr(ws), stream())
+        nv = int(voff[K].item())
+        if nv == 0:
+            return torch.zeros((0, 3), dtype=torch.float32, device=dev), torch.zeros((0, 3), dtype=torch.int64, device=dev)
+        tris = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+        keys = torch.empty(nv, dtype=torch.int64, device=dev)
+        call("ngs_mc_emit", ptr(cubes), K, ptr(values), ptr(pos), ptr(voff), ptr(tris), ptr(keys), stream())
+        faces = torch.empty(nv, dtype=torch.int64, device=dev)
+        verts = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+        nvert = torch.empty(1, dtype=torch.int64, device=dev)
+        wsb = lib().ngs_dedup_workspace_bytes(nv)
+        ws2 = Scratch.get(wsb, dev, slot=4)
+        call("ngs_dedup_vertices", ptr(keys), ptr(tris), nv, ptr(faces), ptr(verts), ptr(nvert), ptr(ws2), wsb, stream())
+        V = int(nvert.item())
+        return verts[:V], faces.view(-1, 3)

@@ -1,0 +1,140 @@
+"""Sparse symmetric solves of the jspsr path (jspsr/core/solver.py), on the HIP PCG.
+
+Solver strings keep the reference's mini-language, "pcg@tol=1e-6@maxiter=1000@res_fix=False"
+(solver.py:180-186), parsed without eval. Methods:
+  pcg    Jacobi-PCG in FP32 with FP64 reductions; stop when ||r|| <= tol ||b|| (recurrence residual)
+         or at maxiter (solve_pcg.cu:15-136 semantics; maxiter < 0 means uncapped in the reference,
+         capped here at 10^7 with an error).
+  mixed  FP64 iterative refinement around FP32 inner PCG solves: r = b - A x in FP64, inner
+         solve A e = r to inner_tol, x += e in FP64, until ||b - A x|| / ||b|| <= tol.
+"""
+import ast
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import NgsError, call, lib, ptr, stream
+
+DEFAULTS = {"pcg": {"tol": 1e-5, "maxiter": -1, "res_fix": False, "verbose": False, "chunk": 32},
+            "mixed": {"tol": 1e-8, "maxiter": 100000, "inner_tol": 1e-4, "max_outer": 20, "verbose": False,
+                      "chunk": 32}}
+
+
+def parse_solver(spec: str):
+    parts = spec.split("@")
+    name = parts[0]
+    if name not in DEFAULTS:
+        raise NotImplementedError(f"solver {name!r} (supported: {sorted(DEFAULTS)})")
+    conf = dict(DEFAULTS[name])
+    for p in parts[1:]:
+        k, v = p.split("=", 1)
+        conf[k] = ast.literal_eval(v)
+    return name, conf
+
+
+class CSRSystem:
+    """Assembled level system: CSR (rows padded to multiples of 4 entries), inverse diagonal."""
+
+    def __init__(self, rowptr, cols, vals, inv_diag):
+        self.rowptr, self.cols, self.vals, self.inv_diag = rowptr, cols, vals, inv_diag
+        self.M = inv_diag.shape[0]
+        self.device = vals.device
+
+    @property
+    def nnz_stored(self):
+        return self.cols.shape[0]
+
+    def nnz(self):
+        """Non-padding entries (excludes the explicit zero padding)."""
+        return int((self.vals != 0).sum().item())
+
+    def matvec(self, x):
+        y = torch.empty(self.M, dtype=torch.float32, device=self.device)
+        if self.M:
+            call("ngs_spmv", ptr(self.rowptr), ptr(self.cols), ptr(self.vals), ptr(x.float().contiguous()), ptr(y),
+                 self.M, stream())
+        return y
+
+    def residual_f64(self, b64, x64):
+        """(r = b - A x in FP64, ||r||^2, ||b||^2)."""
+        r = torch.empty(self.M, dtype=torch.float64, device=self.device)
+        norms = torch.empty(2, dtype=torch.float64, device=self.device)
+        ws = _lib.Scratch.get(lib().ngs_pcg_workspace_bytes(self.M), self.device, slot=1)
+        call("ngs_residual_f64", ptr(self.rowptr), ptr(self.cols), ptr(self.vals), ptr(b64.contiguous()),
+             ptr(x64.contiguous()), ptr(r), self.M, ptr(norms), ptr(ws), stream())
+        nv = norms.cpu()
+        return r, float(nv[0]), float(nv[1])
+
+    def to_scipy(self):
+        import numpy as np
+        import scipy.sparse as sp
+        rp = self.rowptr.cpu().numpy()
+        a = sp.csr_matrix((self.vals.cpu().numpy().astype(np.float64), self.cols.cpu().numpy(), rp),
+                          shape=(self.M, self.M))
+        a.eliminate_zeros()
+        return a
+
+
+class SolveInfo:
+    def __init__(self, iters=0, rel_resid=0.0, outer=0, true_rel_resid=None):
+        self.iters, self.rel_resid, self.outer, self.true_rel_resid = iters, rel_resid, outer, true_rel_resid
+
+    def __repr__(self):
+        return (f"SolveInfo(iters={self.iters}, rel_resid={self.rel_resid:.3e}, outer={self.outer}, "
+                f"true_rel_resid={self.true_rel_resid})")
+
+
+def pcg(system: CSRSystem, b, tol=1e-5, maxiter=-1, chunk=32, x_out=None, raise_on_cap=True):
+    """FP32 Jacobi-PCG from x = 0. Returns (x, SolveInfo)."""
+    M = system.M
+    x = x_out if x_out is not None else torch.empty(M, dtype=torch.float32, device=system.device)
+    if M == 0:
+        return x, SolveInfo()
+    ws = _lib.Scratch.get(lib().ngs_pcg_workspace_bytes(M), system.device, slot=2)
+    it = ctypes.c_int32()
+    rr = ctypes.c_double()
+    b = b.float().contiguous()
+    st = lib().ngs_pcg_solve(ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(system.inv_diag), ptr(b),
+                             ptr(x), M, float(tol), int(maxiter), int(chunk), ptr(ws), stream(), ctypes.byref(it),
+                             ctypes.byref(rr))
+    if st != 0 and not (st == 5 and not raise_on_cap):
+        raise NgsError(st, lib().ngs_last_error().decode())
+    return x, SolveInfo(it.value, rr.value)
+
+
+def mixed(system: CSRSystem, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, chunk=32):
+    """FP64 iterative refinement with FP32 PCG inner solves. Returns (x64, SolveInfo)."""
+    M = system.M
+    b64 = b.double().contiguous()
+    x64 = torch.zeros(M, dtype=torch.float64, device=system.device)
+    info = SolveInfo()
+    if M == 0:
+        return x64, info
+    r64, rr, bb = system.residual_f64(b64, x64)
+    nb = bb ** 0.5
+    rel = (rr ** 0.5) / nb if nb > 0 else 0.0
+    while rel > tol and info.outer < max_outer:
+        e, inner = pcg(system, r64.float(), tol=inner_tol, maxiter=maxiter, chunk=chunk, raise_on_cap=False)
+        x64 += e.double()
+        info.iters += inner.iters
+        info.outer += 1
+        r64, rr, _ = system.residual_f64(b64, x64)
+        rel = (rr ** 0.5) / nb
+    info.rel_resid = rel
+    info.true_rel_resid = rel
+    return x64, info
+
+
+def solve(system: CSRSystem, b, spec: str = "pcg"):
+    """Dispatch on a solver string. Returns (x float32, SolveInfo)."""
+    name, conf = parse_solver(spec)
+    if name == "pcg":
+        x, info = pcg(system, b, tol=conf["tol"], maxiter=conf["maxiter"], chunk=conf["chunk"])
+    else:
+        x64, info = mixed(system, b, tol=conf["tol"], inner_tol=conf["inner_tol"], max_outer=conf["max_outer"],
+                          maxiter=conf["maxiter"], chunk=conf["chunk"])
+        x = x64.float()
+    if conf.get("verbose"):
+        print("PCG Iteration ended in", info.iters)
+    return x, info
