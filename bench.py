@@ -1,0 +1,261 @@
+"""Benchmark of the screened-Poisson reconstruction path on one MI355X.
+
+Default workload (one GPU): 2M points on the union of 16 seeded random ellipsoids (semi-axes in
+[0.05, 0.25], sigma = 0.002 position noise, von Mises-Fisher kappa = 200 normal noise), voxel size
+1/512 (512^3 finest), 4 levels, screening 4.0, FP32 Jacobi-PCG with FP64 reductions inside FP64
+iterative refinement to relative residual 1e-8, dual marching cubes with 2x upsampling. One step
+is one full reconstruction (hierarchy, splat, assembly, solve, mesh) with the points already in
+HBM. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c5] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "neural-galerkin_amd"))
+
+CONFIGS = {
+    "c2": dict(workload="ellipsoids16-2M-vs512-L4-mixed1e-8", gen="ellipsoids", n=2_000_000, vs=1 / 512, depth=4,
+               min_density=1.0, solver="mixed@tol=1e-8@inner_tol=1e-4", sigma=0.002, kappa=200.0),
+    "c1": dict(workload="torus-100k-vs128-L4-pcg1e-6", gen="torus", n=100_000, vs=1 / 128, depth=4, min_density=8.0,
+               solver="pcg@tol=1e-6@maxiter=10000"),
+    "c5": dict(workload="room40boxes-10M-vs1024-L4-mixed1e-7", gen="room", n=10_000_000, vs=1 / 1024, depth=4,
+               min_density=1.0, solver="mixed@tol=1e-7@inner_tol=1e-4", sigma=0.003),
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def make_scene(cfg):
+    from ngspsr import synthetic
+    if cfg["gen"] == "ellipsoids":
+        return synthetic.ellipsoids(cfg["n"], k=16, sigma=cfg["sigma"], kappa=cfg["kappa"], seed=0)
+    if cfg["gen"] == "torus":
+        return synthetic.torus(cfg["n"], seed=0)
+    return synthetic.room(cfg["n"], sigma=cfg["sigma"], seed=0)
+
+
+def spmv_roofline(system, reps=50):
+    """Average duration of the solver's SpMV kernel on the finest-level system, with HIP events on
+    the stream it is launched on, and its algorithmic bytes (8 nnz + 4 (M+1) + 8 M)."""
+    import torch
+    x = torch.randn(system.M, device=system.device)
+    y = torch.empty_like(x)
+    from ngspsr._lib import call, ptr, stream
+    for _ in range(3):
+        call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x), ptr(y), system.M, stream())
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x), ptr(y), system.M, stream())
+    e.record()
+    e.synchronize()
+    t = s.elapsed_time(e) / 1e3 / reps
+    nnz = system.nnz()
+    algo = 8 * nnz + 4 * (system.M + 1) + 8 * system.M
+    return t, algo, nnz
+
+
+def pcg_iteration_rate(system, iters=200):
+    """Seconds per PCG iteration (3 kernels) on the finest system, tolerance 0 so every iteration runs."""
+    import torch
+    from ngspsr import solver as S
+    b = torch.randn(system.M, device=system.device)
+    S.pcg(system, b, tol=0.0, maxiter=20, chunk=20, raise_on_cap=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, info = S.pcg(system, b, tol=0.0, maxiter=iters, chunk=iters, raise_on_cap=False)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / max(info.iters, 1)
+
+
+def cpu_baseline(system, b, budget_s=15.0):
+    """The oracle's C/OpenMP Jacobi-PCG (FP32, FP64 dots) on the finest-level system, timed over a
+    bounded number of iterations (matrix conversion excluded). Returns seconds per iteration."""
+    sys.path.insert(0, ROOT)
+    from oracle import pcg as opcg
+    prep = opcg.Prepared(arrays=(system.rowptr.cpu().numpy(), system.cols.cpu().numpy(), system.vals.cpu().numpy(),
+                                 system.inv_diag.cpu().numpy()))
+    bb = b.cpu().numpy()
+    t0 = time.perf_counter()
+    prep.run(bb, 1)
+    t_one = time.perf_counter() - t0
+    iters = int(max(2, min(200, budget_s / max(t_one, 1e-3))))
+    t0 = time.perf_counter()
+    _, it = prep.run(bb, iters)
+    per_iter = (time.perf_counter() - t0) / max(it, 1)
+    return per_iter, it
+
+
+def serving_scenes(n_req, seed0=1000):
+    """Requests of the serving stream: random-ellipsoid scenes, points log-uniform in [1e4, 1e6],
+    depth uniform in {7, 8, 9} (voxel size 2^-depth), sigma uniform in [0, 0.005]."""
+    import numpy as np
+    from ngspsr import synthetic
+    out = []
+    for i in range(n_req):
+        rng = np.random.default_rng(seed0 + i)
+        n = synthetic.log_uniform_int(rng, 1e4, 1e6)
+        depth = int(rng.integers(7, 10))
+        sigma = float(rng.uniform(0, 0.005))
+        xyz, nrm = synthetic.ellipsoids(n, k=16, sigma=sigma, kappa=200.0, seed=seed0 + i)
+        out.append((xyz, nrm, 2.0 ** -depth))
+    return out
+
+
+def run_serving(n_req=64, workers=8, rate=None):
+    """Plain sequential FP32 path vs the continuous-batching server (mixed precision) on the same
+    request stream; tolerance 1e-6 for both."""
+    import numpy as np
+    import torch
+    import ngspsr
+    from ngspsr import serving
+    dev = torch.device("cuda:0")
+    scenes = serving_scenes(n_req)
+    data = [(torch.from_numpy(x).to(dev), torch.from_numpy(n).to(dev), vs) for x, n, vs in scenes]
+    pts = int(sum(x.shape[0] for x, _, _ in data))
+    # plain: one request at a time, FP32 PCG to 1e-6
+    for x, n, vs in data[:2]:
+        ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_lat = []
+    for x, n, vs in data:
+        t1 = time.perf_counter()
+        ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
+        torch.cuda.synchronize()
+        plain_lat.append(time.perf_counter() - t1)
+    plain_t = time.perf_counter() - t0
+    plain_rate = n_req / plain_t
+    # batched: saturating arrivals (all at t=0) for throughput
+    srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, mixed=True)
+    warm = [serving.Request(-1 - i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data[:workers])]
+    srv.serve(warm)
+    reqs = [serving.Request(i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data)]
+    t0 = time.perf_counter()
+    srv.serve(reqs, t0=t0)
+    torch.cuda.synchronize()
+    batched_t = max(r.finish for r in reqs) - t0
+    errs = [r for r in reqs if r.error is not None]
+    if errs:
+        raise errs[0].error
+    batched_rate = n_req / batched_t
+    res_max = max(r.max_rel_residual for r in reqs)
+    # latency at a Poisson arrival rate of 0.8x the measured batched capacity
+    lam = rate or 0.8 * batched_rate
+    rng = np.random.default_rng(7)
+    arr = np.cumsum(rng.exponential(1.0 / lam, n_req))
+    reqs2 = [serving.Request(i, x, n, vs, arrival=float(a)) for i, ((x, n, vs), a) in enumerate(zip(data, arr))]
+    t0 = time.perf_counter()
+    srv.serve(reqs2, t0=t0)
+    lat = serving.latency_stats(reqs2)
+    pool = srv.pool
+    srv.close()
+    return {
+        "requests": n_req, "points_total": pts, "workers": workers,
+        "plain_fp32_solves_per_s": round(plain_rate, 3),
+        "plain_latency_ms": {"p50": round(1e3 * float(np.percentile(plain_lat, 50)), 2),
+                             "p99": round(1e3 * float(np.percentile(plain_lat, 99)), 2)},
+        "batched_mixed_solves_per_s": round(batched_rate, 3),
+        "batched_vs_plain": round(batched_rate / plain_rate, 3),
+        "batched_max_rel_residual": res_max,
+        "poisson_rate_per_s": round(lam, 3),
+        "poisson_latency_ms": {"p50": round(lat["p50_ms"], 2), "p99": round(lat["p99_ms"], 2)},
+        "pool_launches": pool.launches,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving", type=int, default=64, help="requests of the serving comparison (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding of the path is described "
+                                   "in DESIGN.md and is not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import ngspsr
+
+    cfg = CONFIGS[args.config]
+    dev = torch.device("cuda:0")
+    xyz_np, nrm_np = make_scene(cfg)
+    xyz = torch.from_numpy(xyz_np).to(dev)
+    nrm = torch.from_numpy(nrm_np).to(dev)
+
+    def step():
+        return ngspsr.reconstruct(xyz, nrm, cfg["vs"], depth=cfg["depth"], min_density=cfg["min_density"],
+                                  screen_alpha=4.0, solver=cfg["solver"], return_result=True)
+
+    for _ in range(args.warmup):
+        res = step()
+    torch.cuda.synchronize()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        res = step()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    total = time.perf_counter() - t_start
+    lat = np.array(lat)
+
+    info = res.solve_info
+    total_iters = int(sum(i.iters for i in info.values()))
+    rel = max(float(i.rel_resid) for i in info.values())
+    system, rhs = res.reconstructor.systems[0]
+    t_spmv, algo, nnz = spmv_roofline(system)
+    t_iter = pcg_iteration_rate(system)
+    iter_bytes = algo + 44 * system.M
+    out = {
+        "metric": "solves/s",
+        "value": round(args.steps / total, 4),
+        "unit": "reconstructions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "precision": "FP32 CSR values and PCG vectors; FP64 dot products, residual and refinement",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "points": cfg["n"], "voxel_size": cfg["vs"], "levels": cfg["depth"],
+                   "solver": cfg["solver"], "finest_voxels": int(system.M), "finest_nnz": int(nnz)},
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+        "max_level_rel_residual": rel,
+        "pcg_iterations_total": total_iters,
+        "pcg_iteration": {"ms": round(1e3 * t_iter, 4), "algorithmic_bytes": int(iter_bytes),
+                          "GB/s": round(iter_bytes / t_iter / 1e9, 1)},
+        "roofline": {"kernel": "spmv_kernel<16> (finest-level CSR SpMV of the PCG)", "bound": "hbm",
+                     "achieved": round(algo / t_spmv / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "launch_us": round(1e6 * t_spmv, 2), "algorithmic_bytes": int(algo)},
+        "mesh": {"vertices": int(res.vertices.shape[0]), "triangles": int(res.triangles.shape[0])},
+    }
+    if not args.no_cpu:
+        per_iter, it = cpu_baseline(system, rhs)
+        cpu_rate = 1.0 / (per_iter * total_iters)
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+        out["cpu_baseline"] = {
+            "value": round(cpu_rate, 5), "unit": "reconstructions/s", "cores": cores, "kind": "port",
+            "sample": f"{it} iterations of the oracle's C/OpenMP FP32 Jacobi-PCG (FP64 dots) on the finest-level "
+                      f"system (M={system.M}, nnz={nnz}), {per_iter * 1e3:.1f} ms/iteration, scaled by the GPU run's "
+                      f"{total_iters} PCG iterations over all levels; PCG only (upper bound on CPU solves/s)"}
+    if args.serving > 0:
+        out["serving"] = run_serving(args.serving)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -211,6 +211,18 @@ int ngs_rhs_cross_stiffness(ngs_level fine, ngs_level coarse, const float *x_coa
  * level is gathered with the reference's |rel| <= 1.5 filter (_hash_backend.py:254-262). */
 int ngs_rhs_normals(ngs_level target, ngs_level data, const float *normals, double *rhs, ngs_stream_t s);
 
+/* ----------------------------------------------------------------------- adjoint (backward) */
+
+/* Transpose of ngs_eval_field (the adjoint of evaluate_interpolated, hashtree.py:157-192):
+ * out[i] += scale * sum_p w[p] B_i(p) over the 27 voxels of g around every point. */
+int ngs_basis_splat(ngs_grid g, const float *w, const float *xyz, int64_t n, float inv_voxel_size, double scale,
+                    double *out, ngs_stream_t s);
+/* Transpose of the cross-level stiffness of ngs_rhs_cross_stiffness:
+ * out_coarse[j] += scale * sum_i K_{d,dd}(i, j) g_fine[i]. The backward of the cascadic RHS
+ * correction (reconstructor.py:153-168) through Solver.backward (solver.py:91-98). */
+int ngs_cross_stiffness_transpose(ngs_level fine, ngs_level coarse, const float *g_fine, double scale,
+                                  double *out_coarse, ngs_stream_t s);
+
 /* ----------------------------------------------------------------------------------- solver */
 
 /* Replaces _sparse_solve.solve_pcg_diag (csrc/sparse_solve/solve_pcg.cu:15-136): Jacobi-PCG from
@@ -227,6 +239,25 @@ int ngs_pcg_solve(const int32_t *rowptr, const int32_t *cols, const float *vals,
 /* Solver knobs: use_graph (capture iteration chunks in a hipGraph; default 1) and lanes per CSR
  * row in the SpMV (8, 16, 32; 0 = automatic). */
 int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row);
+
+/* Batched solves (continuous batching of independent level systems; no reference counterpart —
+ * the reference solves one system per call, solve_pcg.cu:15-136). A device table of
+ * ngs_pcg_batch_entry_bytes()-sized entries lists the active systems; blk2sys maps every block of
+ * the launch to its system (-1: idle). Each system keeps its own Krylov state in its workspace
+ * (ngs_pcg_workspace_bytes(M)) and occupies ngs_pcg_batch_blocks(M) consecutive blocks starting
+ * at a multiple of 8. */
+int64_t ngs_pcg_batch_entry_bytes(void);
+int32_t ngs_pcg_batch_blocks(int64_t M);
+/* Initialise a system (x = 0, r = b, p = D^-1 b) on stream s and write its host table entry. */
+int ngs_pcg_batch_prepare(const int32_t *rowptr, const int32_t *cols, const float *vals, const float *inv_diag,
+                          const float *b, float *x, int64_t M, double tol, int32_t max_iter, void *workspace,
+                          int32_t blk0, int32_t nblk, void *entry, ngs_stream_t s);
+/* `iters` iterations of every active system in one launch per kernel (graph-replayed when use_graph). */
+int ngs_pcg_batch_iterate(const void *table, const int32_t *blk2sys, int32_t total_blocks, int32_t iters,
+                          int32_t use_graph, ngs_stream_t s);
+/* (iter, done) of each system of the host copy of the table into states (2 nsys int32, host);
+ * synchronises s. done = 1 converged, 2 iteration cap. */
+int ngs_pcg_batch_status(const void *host_table, int32_t nsys, int32_t *states, ngs_stream_t s);
 
 /* y = A x (FP32), the SpMV of the solver, exposed for measurement. */
 int ngs_spmv(const int32_t *rowptr, const int32_t *cols, const float *vals, const float *x, float *y,

@@ -247,6 +247,65 @@ __global__ void rhs_normals_gather_kernel(ngs_level tgt, ngs_level dat, const fl
     rhs[i] += (double)acc;
 }
 
+// Transpose of the field evaluation: out[i] += scale * sum_p w_p B_i(p) over the 27 voxels of g
+// around each point (the adjoint of hashtree.py:157-192).
+__global__ void basis_splat_kernel(ngs_grid g, const float* __restrict__ w, const float* __restrict__ xyz, int64_t n,
+                                   float inv_vs, double scale, double* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float wi = w[i];
+    if (wi == 0.f) return;
+    const int s = 1 << g.log2_stride;
+    const float inv_s = 1.0f / (float)s;
+    int cell[3];
+    float b[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) axis_basis(xyz[3 * i + a] * inv_vs, s, inv_s, &cell[a], b[a]);
+    for (int kx = 0; kx < 3; ++kx)
+        for (int ky = 0; ky < 3; ++ky)
+            for (int kz = 0; kz < 3; ++kz) {
+                int id = grid_find(g, cell[0] + (kx - 1) * s, cell[1] + (ky - 1) * s, cell[2] + (kz - 1) * s);
+                if (id < 0) continue;
+                atomicAdd(&out[id], scale * (double)(wi * (b[0][kx] * b[1][ky] * b[2][kz])));
+            }
+}
+
+// Transpose of the cross-level stiffness: out_c[j] += scale * sum_i K_{d,dd}(i, j) g_f[i].
+__global__ void cross_stiffness_t_kernel(ngs_level fine, ngs_level coarse, const float* __restrict__ gf, Tables T,
+                                         double scale, double* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fine.M) return;
+    const float gi = gf[i];
+    if (gi == 0.f) return;
+    const int sf_ = fine.stride, sc = coarse.stride, m = T.m;
+    int c[3], Q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c[a] = fine.coords[3 * i + a];
+        int num = 2 * c[a] + sf_, den = 2 * sc;
+        int q = num >= 0 ? num / den : -((-num + den - 1) / den);
+        Q[a] = q * sc;
+    }
+    const int nrow = 3 * m + 2;
+    const float fs = (float)sf_, inv_sc = 1.0f / (float)sc;
+    for (int kx = -2; kx <= 2; ++kx) {
+        int ix = kx * m + (Q[0] - c[0]) / sf_ + 2 * m;
+        for (int ky = -2; ky <= 2; ++ky) {
+            int iy = ky * m + (Q[1] - c[1]) / sf_ + 2 * m;
+            for (int kz = -2; kz <= 2; ++kz) {
+                int iz = kz * m + (Q[2] - c[2]) / sf_ + 2 * m;
+                if (ix < 0 || ix >= nrow || iy < 0 || iy >= nrow || iz < 0 || iz >= nrow) continue;
+                int j = grid_find(coarse.grid, Q[0] + kx * sc, Q[1] + ky * sc, Q[2] + kz * sc);
+                if (j < 0) continue;
+                const float c0 = T.F[ix] * fs, c1 = T.F[iy] * fs, c2 = T.F[iz] * fs;
+                const float g0 = T.G[ix] * inv_sc, g1 = T.G[iy] * inv_sc, g2 = T.G[iz] * inv_sc;
+                const float K = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
+                atomicAdd(&out[j], scale * (double)(K * gi));
+            }
+        }
+    }
+}
+
 }  // namespace ngs
 
 using namespace ngs;
@@ -334,6 +393,25 @@ int ngs_rhs_normals(ngs_level target, ngs_level data, const float* normals, doub
         NGS_REQUIRE(tables_for(data.stride / target.stride, &T), "ngs_rhs_normals: table multiple unavailable");
         rhs_normals_gather_kernel<<<grid1d(target.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);
     }
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_basis_splat(ngs_grid g, const float* w, const float* xyz, int64_t n, float inv_voxel_size, double scale, double* out,
+                    ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    basis_splat_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, w, xyz, n, inv_voxel_size, scale, out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_cross_stiffness_transpose(ngs_level fine, ngs_level coarse, const float* g_fine, double scale, double* out_coarse,
+                                  ngs_stream_t s) {
+    if (fine.M <= 0 || coarse.M <= 0) return NGS_OK;
+    NGS_REQUIRE(coarse.stride > fine.stride, "ngs_cross_stiffness_transpose: coarse stride must exceed fine stride");
+    Tables T;
+    NGS_REQUIRE(tables_for(coarse.stride / fine.stride, &T), "ngs_cross_stiffness_transpose: table multiple unavailable");
+    cross_stiffness_t_kernel<<<grid1d(fine.M, 128), 128, 0, S(s)>>>(fine, coarse, g_fine, T, scale, out_coarse);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -74,12 +74,12 @@ __device__ __forceinline__ void block_sum2(double& a, double& b) {
 
 // Publish this block's partials; returns true in every thread of the last block to arrive, after
 // which sums (fixed order) of both partial arrays are in (a, b) of thread 0.
-__device__ bool last_block_reduce(double& a, double& b, double* partials, uint32_t* counter, int nblocks) {
+__device__ bool last_block_reduce_n(double& a, double& b, double* partials, uint32_t* counter, int nblocks, int bid) {
     __shared__ bool last;
     block_sum2(a, b);
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[nblocks + blockIdx.x], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[bid], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[nblocks + bid], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         uint32_t ticket = atomicAdd(counter, 1u);
         last = (ticket == (uint32_t)nblocks - 1);
@@ -97,6 +97,10 @@ __device__ bool last_block_reduce(double& a, double& b, double* partials, uint32
     b = tb;
     if (threadIdx.x == 0) *counter = 0u;
     return true;
+}
+
+__device__ __forceinline__ bool last_block_reduce(double& a, double& b, double* partials, uint32_t* counter, int nblocks) {
+    return last_block_reduce_n(a, b, partials, counter, nblocks, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(PCG_THREADS) pcg_init_kernel(PcgArgs A, double tol) {
@@ -320,6 +324,115 @@ static HostState& host_state() {
     return hs;
 }
 
+// ------------------------------------------------------------------------------ batched solves
+// Many independent systems advance together: a device table lists the active systems and the
+// contiguous range of blocks each one owns; every kernel looks up its system from its block index,
+// so one launch (and one captured graph, whose arguments are the fixed table pointers) serves any
+// mix of systems. Systems join and leave between launches by rewriting the table.
+struct BatchSys {
+    PcgArgs a;
+    int32_t blk0;
+    int32_t nblk;
+    int32_t pad[2];
+};
+
+__device__ __forceinline__ const BatchSys* batch_lookup(const BatchSys* tab, const int32_t* blk2sys, int* local) {
+    const int sys = blk2sys[blockIdx.x];
+    if (sys < 0) return nullptr;
+    const BatchSys* B = tab + sys;
+    const int lb = blockIdx.x - B->blk0;
+    const int nb = B->nblk;
+    *local = (nb % NUM_XCD) ? lb : (lb % NUM_XCD) * (nb / NUM_XCD) + lb / NUM_XCD;
+    return B;
+}
+
+template <int G>
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_spmv_dot_kernel(const BatchSys* tab, const int32_t* blk2sys) {
+    int lb;
+    const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
+    if (!B) return;
+    const PcgArgs& A = B->a;
+    if (A.st->done) return;
+    const int64_t r0 = (int64_t)lb * A.chunk;
+    const int64_t r1 = min(A.M, r0 + A.chunk);
+    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    constexpr int GROUPS = PCG_THREADS / G;
+    double pq = 0;
+    for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+        const int32_t k0 = A.rowptr[row], k1 = A.rowptr[row + 1];
+        float sum = 0.f;
+        for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
+            const float4 v = *reinterpret_cast<const float4*>(A.vals + k);
+            const int4 c = *reinterpret_cast<const int4*>(A.cols + k);
+            sum += v.x * A.p[c.x] + v.y * A.p[c.y] + v.z * A.p[c.z] + v.w * A.p[c.w];
+        }
+#pragma unroll
+        for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        if (lane == 0) {
+            A.q[row] = sum;
+            pq += (double)A.p[row] * sum;
+        }
+    }
+    double dummy = 0;
+    if (last_block_reduce_n(pq, dummy, A.partials, &A.st->counter[1], B->nblk, lb) && threadIdx.x == 0) {
+        A.st->pq = pq;
+        A.st->alpha = A.st->rz / pq;
+    }
+}
+
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_update_kernel(const BatchSys* tab, const int32_t* blk2sys) {
+    int lb;
+    const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
+    if (!B) return;
+    const PcgArgs& A = B->a;
+    if (A.st->done) return;
+    const int64_t r0 = (int64_t)lb * A.chunk;
+    const int64_t r1 = min(A.M, r0 + A.chunk);
+    const float alpha = (float)A.st->alpha;
+    double rz = 0, rr = 0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) {
+        float pi = A.p[i];
+        A.x[i] += alpha * pi;
+        float ri = A.r[i] - alpha * A.q[i];
+        A.r[i] = ri;
+        float z = A.dinv[i] * ri;
+        rz += (double)ri * z;
+        rr += (double)ri * ri;
+    }
+    if (last_block_reduce_n(rz, rr, A.partials, &A.st->counter[2], B->nblk, lb) && threadIdx.x == 0) {
+        PcgState* s = A.st;
+        s->iter += 1;
+        s->rr = rr;
+        if (rr <= s->tol2 * s->bb) {
+            s->done = 1;
+        } else if (s->iter >= s->max_iter) {
+            s->done = 2;
+        } else {
+            s->beta = rz / s->rz;
+            s->rz = rz;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_direction_kernel(const BatchSys* tab, const int32_t* blk2sys) {
+    int lb;
+    const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
+    if (!B) return;
+    const PcgArgs& A = B->a;
+    if (A.st->done) return;
+    const int64_t r0 = (int64_t)lb * A.chunk;
+    const int64_t r1 = min(A.M, r0 + A.chunk);
+    const float beta = (float)A.st->beta;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) A.p[i] = A.dinv[i] * A.r[i] + beta * A.p[i];
+}
+
+static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, int total_blocks, hipStream_t st) {
+    dim3 grid(total_blocks), block(PCG_THREADS);
+    bpcg_spmv_dot_kernel<16><<<grid, block, 0, st>>>(tab, blk2sys);
+    bpcg_update_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
+    bpcg_direction_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
+}
+
 }  // namespace ngs
 
 using namespace ngs;
@@ -442,6 +555,88 @@ int ngs_residual_f64(const int32_t* rowptr, const int32_t* cols, const float* va
     zero_state<<<1, 64, 0, S(s)>>>(stt, 0);
     residual_f64_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, b, x, r, M, chunk, partials, &stt->counter[3], norms);
     NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+// ----------------------------------------------------------------------------- batched ABI
+
+int32_t ngs_pcg_batch_blocks(int64_t M);
+
+// Size in bytes of one entry of the batch table (BatchSys).
+int64_t ngs_pcg_batch_entry_bytes(void) { return (int64_t)sizeof(BatchSys); }
+
+// Prepare one system for batched iterations: lay out its workspace (as ngs_pcg_solve does),
+// run the initialisation (x = 0, r = b, p = D^-1 b, norms) on `s`, and write its table entry
+// (host memory, `entry`, ngs_pcg_batch_entry_bytes bytes) with the given block range.
+int ngs_pcg_batch_prepare(const int32_t* rowptr, const int32_t* cols, const float* vals, const float* inv_diag,
+                          const float* b, float* x, int64_t M, double tol, int32_t max_iter, void* workspace,
+                          int32_t blk0, int32_t nblk, void* entry, ngs_stream_t s) {
+    NGS_REQUIRE(M > 0 && workspace && entry, "ngs_pcg_batch_prepare: bad arguments");
+    NGS_REQUIRE(nblk == ngs_pcg_batch_blocks(M), "ngs_pcg_batch_prepare: nblk must be ngs_pcg_batch_blocks(M)");
+    BatchSys* E = reinterpret_cast<BatchSys*>(entry);
+    PcgArgs A{};
+    pcg_layout(M, &A, reinterpret_cast<char*>(workspace));
+    A.rowptr = rowptr;
+    A.cols = cols;
+    A.vals = vals;
+    A.dinv = inv_diag;
+    A.b = b;
+    A.x = x;
+    if (max_iter < 0) max_iter = 10000000;
+    zero_state<<<1, 64, 0, S(s)>>>(A.st, max_iter);
+    pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, S(s)>>>(A, tol);
+    NGS_CHECK_LAUNCH();
+    E->a = A;
+    E->blk0 = blk0;
+    E->nblk = nblk;
+    return NGS_OK;
+}
+
+// Blocks a system of M rows occupies in a batch (its single-system grid).
+int32_t ngs_pcg_batch_blocks(int64_t M) {
+    int64_t chunk;
+    return pcg_blocks(M, &chunk);
+}
+
+// Run `iters` batched iterations over the device table (nsys entries) and block map
+// (total_blocks entries, -1 = idle block). With use_graph, the iterations are captured once per
+// (table, map, total_blocks, iters) and replayed; the table contents may change between calls.
+int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t total_blocks, int32_t iters,
+                          int32_t use_graph, ngs_stream_t s) {
+    NGS_REQUIRE(table && blk2sys && total_blocks > 0 && iters > 0, "ngs_pcg_batch_iterate: bad arguments");
+    const BatchSys* tab = reinterpret_cast<const BatchSys*>(table);
+    hipStream_t st = S(s);
+    if (!use_graph) {
+        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, st);
+        NGS_CHECK_LAUNCH();
+        return NGS_OK;
+    }
+    struct Key { const void* t; const int32_t* m; int nb, it; hipStream_t s; hipGraphExec_t e; };
+    static std::vector<Key> cache;
+    for (auto& k : cache)
+        if (k.t == table && k.m == blk2sys && k.nb == total_blocks && k.it == iters && k.s == st) {
+            NGS_CHECK_HIP(hipGraphLaunch(k.e, st));
+            return NGS_OK;
+        }
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    NGS_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, st);
+    NGS_CHECK_HIP(hipStreamEndCapture(st, &graph));
+    NGS_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    cache.push_back({table, blk2sys, total_blocks, iters, st, exec});
+    NGS_CHECK_HIP(hipGraphLaunch(exec, st));
+    return NGS_OK;
+}
+
+// Copy (iter, done) of each system in the table to host memory states[2*i], states[2*i+1]
+// through the device scratch `dev_states` (2*nsys int32); synchronises `s`.
+int ngs_pcg_batch_status(const void* host_table, int32_t nsys, int32_t* states, ngs_stream_t s) {
+    const BatchSys* tab = reinterpret_cast<const BatchSys*>(host_table);
+    for (int i = 0; i < nsys; ++i)
+        NGS_CHECK_HIP(hipMemcpyAsync(states + 2 * i, &tab[i].a.st->iter, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, S(s)));
+    NGS_CHECK_HIP(hipStreamSynchronize(S(s)));
     return NGS_OK;
 }
 

@@ -6,6 +6,7 @@ library is absent.
 """
 import ctypes
 import os
+import threading
 
 import torch
 
@@ -69,10 +70,17 @@ _SIGNATURES = {
     "ngs_lhs_assemble": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P, P], c_int32),
     "ngs_rhs_cross_stiffness": ([Level, Level, P, P, P], c_int32),
     "ngs_rhs_normals": ([Level, Level, P, P, P], c_int32),
+    "ngs_basis_splat": ([Grid, P, P, c_int64, c_float, c_double, P, P], c_int32),
+    "ngs_cross_stiffness_transpose": ([Level, Level, P, c_double, P, P], c_int32),
     "ngs_pcg_workspace_bytes": ([c_int64], c_int64),
     "ngs_pcg_config": ([c_int32, c_int32], c_int32),
     "ngs_pcg_solve": ([P, P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
                        ctypes.POINTER(c_double)], c_int32),
+    "ngs_pcg_batch_entry_bytes": ([], c_int64),
+    "ngs_pcg_batch_blocks": ([c_int64], c_int32),
+    "ngs_pcg_batch_prepare": ([P, P, P, P, P, P, c_int64, c_double, c_int32, P, c_int32, c_int32, P, P], c_int32),
+    "ngs_pcg_batch_iterate": ([P, P, c_int32, c_int32, c_int32, P], c_int32),
+    "ngs_pcg_batch_status": ([P, c_int32, P, P], c_int32),
     "ngs_spmv": ([P, P, P, P, P, c_int64, P], c_int32),
     "ngs_residual_f64": ([P, P, P, P, P, P, c_int64, P, P, P], c_int32),
     "ngs_conformal_children": ([P, c_int64, c_int32, Grid, c_int32, P, P, P], c_int32),
@@ -143,14 +151,20 @@ def stream():
 
 
 class Scratch:
-    """Per-device growable scratch buffer for stream-ordered temporary workspaces."""
-    _bufs = {}
+    """Per-thread, per-device growable scratch buffers for stream-ordered temporary workspaces.
+
+    Buffers are private to the calling thread, so concurrent requests on different streams never
+    share one."""
+    _tls = threading.local()
 
     @classmethod
     def get(cls, nbytes, device, slot=0):
+        bufs = getattr(cls._tls, "bufs", None)
+        if bufs is None:
+            bufs = cls._tls.bufs = {}
         key = (str(device), slot)
-        buf = cls._bufs.get(key)
+        buf = bufs.get(key)
         if buf is None or buf.numel() < nbytes:
             buf = torch.empty(max(int(nbytes), 256) + 256, dtype=torch.uint8, device=device)
-            cls._bufs[key] = buf
+            bufs[key] = buf
         return buf

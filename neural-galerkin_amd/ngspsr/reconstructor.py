@@ -91,9 +91,10 @@ class Reconstructor:
         return _solver.CSRSystem(rowptr, cols, vals, inv_diag)
 
     def solve_multigrid(self, start_depth, end_depth, normal_data: dict, screen_alpha=0.0, screen_xyz=None,
-                        screen_delta=0.1, solver: str = "pcg", verbose: bool = False):
+                        screen_delta=0.1, solver: str = "pcg", verbose: bool = False, solve_fn=None):
         """reconstructor.py:79-195: coarse-to-fine per-level solves; each level's RHS is corrected by
-        the coarser solutions (cascadic)."""
+        the coarser solutions (cascadic). solve_fn(system, rhs) -> (x, info) replaces the solver
+        (the serving engine routes solves through its batched pool)."""
         tree = self.hash_tree
         if isinstance(screen_alpha, torch.Tensor):
             assert screen_xyz is not None and screen_alpha.shape[0] == screen_xyz.shape[0]
@@ -129,7 +130,7 @@ class Reconstructor:
                 call("ngs_rhs_cross_stiffness", tree.level(d), tree.level(dd), ptr(self.solutions[dd]), ptr(rhs),
                      stream())
             system = self.assemble_level(d, bins[d], coarse, sf, float(screen_delta), rhs)
-            x, info = _solver.solve(system, rhs, solver)
+            x, info = solve_fn(system, rhs) if solve_fn is not None else _solver.solve(system, rhs, solver)
             self.solutions[d] = x.contiguous()
             self.systems[d] = (system, rhs)
             self.solve_info[d] = info

@@ -1,0 +1,304 @@
+"""Continuous-batching reconstruction server on one GPU.
+
+Requests of uneven size (point count, depth) are admitted while their estimated HBM footprint fits
+the budget and a worker slot is free. Each admitted request runs its pipeline (hierarchy, splat,
+assembly, mesh) on its own stream in a worker thread and keeps its tree, matrices and Krylov state
+on the device. Every linear solve is handed to one shared pool: a pool thread advances all active
+level systems of all requests together, one launch per kernel per iteration (batched kernels over
+a device work table, replayed as a captured hipGraph), admitting new systems and retiring converged
+ones between chunks of iterations. Mixed precision (FP64 refinement around FP32 inner solves)
+runs per request around the pool's solves.
+"""
+import math
+import queue
+import threading
+import time
+
+import numpy as np
+import torch
+
+from . import solver as _solver
+from ._lib import NgsError, call, lib, ptr, stream
+from .bases import ensure_device_tables
+from .hashtree import HashTree
+from .reconstructor import Reconstructor
+
+
+class _Job:
+    __slots__ = ("system", "b", "x", "ws", "tol", "max_iter", "done", "iters", "status", "ready", "nblk", "t0")
+
+    def __init__(self, system, b, tol, max_iter):
+        self.system = system
+        self.b = b.float().contiguous()
+        self.x = torch.empty(system.M, dtype=torch.float32, device=system.device)
+        self.ws = torch.empty(lib().ngs_pcg_workspace_bytes(system.M), dtype=torch.uint8, device=system.device)
+        self.tol = tol
+        self.max_iter = max_iter
+        self.done = threading.Event()
+        self.iters = 0
+        self.status = 0
+        self.nblk = int(lib().ngs_pcg_batch_blocks(system.M))
+        self.ready = torch.cuda.Event()
+        self.ready.record(torch.cuda.current_stream())
+
+
+class BatchedPCG:
+    """Pool of concurrently iterating PCG systems (one batched launch per kernel per iteration)."""
+
+    def __init__(self, device, capacity_blocks=4096, max_systems=256, chunk=8, use_graph=True):
+        self.device = torch.device(device)
+        self.cap = int(capacity_blocks)
+        self.max_sys = int(max_systems)
+        self.chunk = int(chunk)
+        self.use_graph = bool(use_graph)
+        self.eb = int(lib().ngs_pcg_batch_entry_bytes())
+        self.host_table = torch.zeros(self.max_sys * self.eb, dtype=torch.uint8).pin_memory()
+        self.dev_table = torch.zeros(self.max_sys * self.eb, dtype=torch.uint8, device=self.device)
+        self.host_map = torch.full((self.cap,), -1, dtype=torch.int32).pin_memory()
+        self.dev_map = torch.full((self.cap,), -1, dtype=torch.int32, device=self.device)
+        self.states = torch.zeros(2 * self.max_sys, dtype=torch.int32).pin_memory()
+        self.stream = torch.cuda.Stream(self.device)
+        self.pending = queue.Queue()
+        self.active = []
+        self.launches = 0
+        self.iterations = 0
+        self.busy_s = 0.0
+        self._stop = False
+        self._err = None
+        self._wake = threading.Event()
+        self.thread = threading.Thread(target=self._run, name="ngspsr-pcg-pool", daemon=True)
+        self.thread.start()
+
+    def solve(self, system, b, tol=1e-6, max_iter=100000):
+        """Blocking solve through the pool (callable from any worker thread)."""
+        if self._err is not None:
+            raise self._err
+        job = _Job(system, b, tol, max_iter)
+        self.pending.put(job)
+        self._wake.set()
+        job.done.wait()
+        if job.status == 2 and self._err is not None:
+            raise self._err
+        return job.x, _solver.SolveInfo(job.iters, 0.0)
+
+    def close(self):
+        self._stop = True
+        self._wake.set()
+        self.thread.join()
+
+    def _layout(self):
+        blk = 0
+        hm = self.host_map.numpy()
+        hm[:] = -1
+        for i, job in enumerate(self.active):
+            blk0 = blk
+            hm[blk0:blk0 + job.nblk] = i
+            job_entry = self.host_table[i * self.eb:(i + 1) * self.eb]
+            # patch the block range of the prepared entry (two int32 after the PcgArgs)
+            e = job_entry.numpy().view(np.int32)
+            e[-4] = blk0
+            e[-3] = job.nblk
+            blk = blk0 + ((job.nblk + 7) // 8) * 8
+        return blk
+
+    def _admit(self):
+        used = sum(((j.nblk + 7) // 8) * 8 for j in self.active)
+        admitted = []
+        while len(self.active) < self.max_sys:
+            try:
+                job = self.pending.get_nowait()
+            except queue.Empty:
+                break
+            need = ((job.nblk + 7) // 8) * 8
+            if used + need > self.cap and self.active:
+                self.pending.put(job)  # retry after something retires
+                break
+            used += need
+            self.stream.wait_event(job.ready)
+            slot = len(self.active)
+            entry = self.host_table[slot * self.eb:(slot + 1) * self.eb]
+            s = job.system
+            call("ngs_pcg_batch_prepare", ptr(s.rowptr), ptr(s.cols), ptr(s.vals), ptr(s.inv_diag), ptr(job.b),
+                 ptr(job.x), s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
+                 self.stream.cuda_stream)
+            self.active.append(job)
+            admitted.append(job)
+        return admitted
+
+    def _run(self):
+        torch.cuda.set_device(self.device)
+        try:
+            with torch.cuda.stream(self.stream):
+                while not self._stop:
+                    changed = bool(self._admit())
+                    if not self.active:
+                        self._wake.wait(0.01)
+                        self._wake.clear()
+                        continue
+                    t0 = time.perf_counter()
+                    if changed or self._dirty:
+                        self._layout()
+                        n = len(self.active)
+                        self.dev_table[:n * self.eb].copy_(self.host_table[:n * self.eb], non_blocking=True)
+                        self.dev_map.copy_(self.host_map, non_blocking=True)
+                        self._dirty = False
+                    call("ngs_pcg_batch_iterate", ptr(self.dev_table), ptr(self.dev_map), self.cap, self.chunk,
+                         int(self.use_graph), self.stream.cuda_stream)
+                    self.launches += 1
+                    self.iterations += self.chunk
+                    n = len(self.active)
+                    call("ngs_pcg_batch_status", self.host_table.data_ptr(), n, self.states.data_ptr(),
+                         self.stream.cuda_stream)
+                    self.busy_s += time.perf_counter() - t0
+                    st = self.states.numpy()[:2 * n].reshape(n, 2)
+                    keep = []
+                    for i, job in enumerate(self.active):
+                        if st[i, 1]:
+                            job.iters, job.status = int(st[i, 0]), int(st[i, 1])
+                            job.done.set()
+                        else:
+                            keep.append(job)
+                    if len(keep) != len(self.active):
+                        # compact the table: entries move with their jobs
+                        old = self.host_table.clone()
+                        for j, job in enumerate(keep):
+                            i = self.active.index(job)
+                            self.host_table[j * self.eb:(j + 1) * self.eb] = old[i * self.eb:(i + 1) * self.eb]
+                        self.active = keep
+                        self._dirty = True
+        except Exception as e:  # surface pool failures to waiting requests
+            self._err = e
+            for job in self.active:
+                job.status = 2
+                job.done.set()
+            while not self.pending.empty():
+                j = self.pending.get_nowait()
+                j.status = 2
+                j.done.set()
+
+    _dirty = False
+
+
+def estimate_footprint(n_points: int, voxel_size: float, area: float = 1.5) -> int:
+    """HBM bytes of one request (finest CSR dominates: ~1 KB per finest voxel, voxels bounded by the
+    8 trilinear corners of every point and by a 3-voxel band over the surface area)."""
+    m0 = min(8.0 * n_points, 3.5 * area / (voxel_size * voxel_size))
+    return int(1.3 * (1100.0 * m0 + 400.0 * n_points) + 64e6)
+
+
+class Request:
+    def __init__(self, rid, xyz, normal, voxel_size, depth=4, min_density=1.0, screen_alpha=4.0, arrival=0.0):
+        self.rid, self.xyz, self.normal = rid, xyz, normal
+        self.voxel_size, self.depth, self.min_density, self.screen_alpha = voxel_size, depth, min_density, screen_alpha
+        self.arrival = arrival
+        self.footprint = estimate_footprint(xyz.shape[0], voxel_size)
+        self.start = self.finish = None
+        self.result = None
+        self.error = None
+        self.max_rel_residual = None
+
+
+class ReconstructionServer:
+    """Continuous batching of reconstructions: admission by HBM footprint, concurrent pipelines,
+    one shared batched PCG pool."""
+
+    def __init__(self, device="cuda:0", workers=8, hbm_budget=None, tol=1e-6, mixed=True, inner_tol=1e-4,
+                 chunk=8, capacity_blocks=4096, use_graph=True):
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        ensure_device_tables(self.device)
+        free, total = torch.cuda.mem_get_info(self.device)
+        self.budget = int(hbm_budget if hbm_budget is not None else 0.6 * free)
+        self.tol, self.mixed, self.inner_tol = tol, mixed, inner_tol
+        self.pool = BatchedPCG(self.device, capacity_blocks=capacity_blocks, chunk=chunk, use_graph=use_graph)
+        self.workers = workers
+        self.in_use = 0
+        self.lock = threading.Condition()
+        self.completed = []
+
+    # ---------------------------------------------------------------------- solves via the pool
+    def _solve(self, system, rhs):
+        if not self.mixed:
+            x, info = self.pool.solve(system, rhs, tol=self.tol)
+            return x, info
+        b64 = rhs.double()
+        x64 = torch.zeros(system.M, dtype=torch.float64, device=system.device)
+        r64, rr, bb = system.residual_f64(b64, x64)
+        nb = math.sqrt(bb)
+        rel = math.sqrt(rr) / nb if nb > 0 else 0.0
+        info = _solver.SolveInfo()
+        while rel > self.tol and info.outer < 20:
+            e, inner = self.pool.solve(system, r64.float(), tol=self.inner_tol)
+            x64 += e.double()
+            info.iters += inner.iters
+            info.outer += 1
+            r64, rr, _ = system.residual_f64(b64, x64)
+            rel = math.sqrt(rr) / nb
+        info.rel_resid = info.true_rel_resid = rel
+        return x64.float(), info
+
+    def run_request(self, req: Request):
+        s = torch.cuda.Stream(self.device)
+        try:
+            with torch.cuda.stream(s):
+                req.start = time.perf_counter()
+                tree = HashTree(req.xyz, req.voxel_size, req.depth)
+                tree.build_encoder_hierarchy_adaptive(min_density=req.min_density)
+                w = 1.0 / tree.xyz_density
+                counts = tree.level_point_counts()
+                nd = {}
+                for d in range(req.depth):
+                    if counts[d]:
+                        nd[d] = tree.splat_data(req.xyz, None, d, req.normal, scale=w, levels=tree.xyz_depth,
+                                                level=d) / float(tree.strides[d] ** 3)
+                rec = Reconstructor(tree)
+                rec.sample_weight = w
+                rec.solve_multigrid(req.depth - 1, 0, nd, screen_alpha=req.screen_alpha, screen_xyz=req.xyz,
+                                    solve_fn=self._solve)
+                v, f = rec.extract_multiscale_mesh(n_upsample=2)
+                s.synchronize()
+                req.result = (v, f)
+                infos = rec.solve_info.values()
+                req.max_rel_residual = max((i.rel_resid for i in infos), default=0.0)
+        except Exception as e:  # reported per request
+            req.error = e
+        finally:
+            req.finish = time.perf_counter()
+            with self.lock:
+                self.in_use -= req.footprint
+                self.workers_busy -= 1
+                self.completed.append(req)
+                self.lock.notify_all()
+
+    workers_busy = 0
+
+    def serve(self, requests, t0=None):
+        """Runs the requests honouring their arrival offsets (seconds after t0); returns them."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        t0 = time.perf_counter() if t0 is None else t0
+        threads = []
+        for req in reqs:
+            delay = t0 + req.arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            req.arrival_abs = t0 + req.arrival
+            with self.lock:
+                while self.workers_busy >= self.workers or (self.in_use + req.footprint > self.budget and self.in_use > 0):
+                    self.lock.wait()
+                self.workers_busy += 1
+                self.in_use += req.footprint
+            th = threading.Thread(target=self.run_request, args=(req,), daemon=True)
+            th.start()
+            threads.append(th)
+        for th in threads:
+            th.join()
+        return reqs
+
+    def close(self):
+        self.pool.close()
+
+
+def latency_stats(reqs):
+    lat = np.array([r.finish - r.arrival_abs for r in reqs])
+    return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3)}

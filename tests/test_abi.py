@@ -1,0 +1,51 @@
+"""The C ABI library loads and exports every entry point declared in include/ngspsr.h (no GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ngspsr.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ngs_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_declared_symbols():
+    from ngspsr import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libngspsr.so is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # every declared entry point has a ctypes signature on the host side
+    assert sorted(set(names) - set(_lib.exported_symbols())) == []
+
+
+def test_host_only_entry_points():
+    from ngspsr import _lib
+    lib = _lib.lib()
+    assert lib.ngs_abi_version() == 1
+    ts, nb, eb = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+    assert lib.ngs_hash_sizing(86415, 0, ctypes.byref(ts), ctypes.byref(nb), ctypes.byref(eb)) == 0
+    assert (ts.value, nb.value, eb.value) == (262144, 512, 136)
+    assert lib.ngs_hash_sizing(0, 0, ctypes.byref(ts), ctypes.byref(nb), ctypes.byref(eb)) == 1
+    assert b"positive" in lib.ngs_last_error()
+    dims = (ctypes.c_int32 * 3)(10, 10, 33)
+    assert lib.ngs_grid_words(dims) == (3300 + 31) // 32
+    huge = (ctypes.c_int32 * 3)(100000, 100000, 100000)
+    assert lib.ngs_grid_words(huge) == 0
+
+
+def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
+    from ngspsr import _lib
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_lib, "_LIB", None)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _lib.lib()
