@@ -236,9 +236,9 @@ int ngs_pcg_solve(const int32_t *rowptr, const int32_t *cols, const float *vals,
                   const float *b, float *x, int64_t M, double tol, int32_t max_iter, int32_t chunk,
                   void *workspace, ngs_stream_t s, int32_t *iters, double *rel_resid);
 
-/* Solver knobs: use_graph (capture iteration chunks in a hipGraph; default 1) and lanes per CSR
- * row in the SpMV (8, 16, 32; 0 = automatic). */
-int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row);
+/* Solver knobs: use_graph (capture iteration chunks in a hipGraph; default 1), lanes per CSR row
+ * in the SpMV (8, 16, 32; 0 = 16) and non-temporal loads of the matrix stream (default 1). */
+int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal);
 
 /* Batched solves (continuous batching of independent level systems; no reference counterpart —
  * the reference solves one system per call, solve_pcg.cu:15-136). A device table of

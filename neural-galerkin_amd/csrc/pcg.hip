@@ -1,6 +1,7 @@
 // Jacobi-preconditioned conjugate gradients on the assembled CSR system.
 //
-// One iteration = three kernels, no host round trip:
+// One iteration = three kernels, no host round trip (bpcg_* below; a solve of one system is a batch
+// of one):
 //   K1 spmv_dot:   q = A p and p.q; the last block to finish computes alpha = rz / pq.
 //   K2 update:     x += alpha p, r -= alpha q, rz' = r.(D^-1 r), rr = r.r; the last block decides
 //                  convergence (rr <= tol^2 bb), counts the iteration and forms beta = rz'/rz.
@@ -12,6 +13,7 @@
 // hipGraph and replayed; kernels of a converged system return immediately.
 #include "common.h"
 
+#include <type_traits>
 #include <vector>
 
 namespace ngs {
@@ -103,6 +105,33 @@ __device__ __forceinline__ bool last_block_reduce(double& a, double& b, double* 
     return last_block_reduce_n(a, b, partials, counter, nblocks, blockIdx.x);
 }
 
+// Dot product of CSR row `row` with x by G lanes (16-byte loads of 4 values and 4 columns per lane).
+// NT: non-temporal loads for the matrix stream, so the gathered vector stays resident in L2.
+template <int G, bool NT>
+__device__ __forceinline__ float row_dot(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                                         const float* __restrict__ vals, const float* __restrict__ x, int64_t row,
+                                         int lane) {
+    const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    float sum = 0.f;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
+        f4 v;
+        i4 c;
+        if (NT) {
+            v = __builtin_nontemporal_load(reinterpret_cast<const f4*>(vals + k));
+            c = __builtin_nontemporal_load(reinterpret_cast<const i4*>(cols + k));
+        } else {
+            v = *reinterpret_cast<const f4*>(vals + k);
+            c = *reinterpret_cast<const i4*>(cols + k);
+        }
+        sum += v.x * x[c.x] + v.y * x[c.y] + v.z * x[c.z] + v.w * x[c.w];
+    }
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    return sum;
+}
+
 __global__ void __launch_bounds__(PCG_THREADS) pcg_init_kernel(PcgArgs A, double tol) {
     const int lb = logical_block();
     const int64_t r0 = (int64_t)lb * A.chunk;
@@ -127,79 +156,7 @@ __global__ void __launch_bounds__(PCG_THREADS) pcg_init_kernel(PcgArgs A, double
     }
 }
 
-template <int G>
-__global__ void __launch_bounds__(PCG_THREADS) pcg_spmv_dot_kernel(PcgArgs A) {
-    if (A.st->done) return;
-    const int lb = logical_block();
-    const int64_t r0 = (int64_t)lb * A.chunk;
-    const int64_t r1 = min(A.M, r0 + A.chunk);
-    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
-    constexpr int GROUPS = PCG_THREADS / G;
-    double pq = 0;
-    for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
-        const int32_t k0 = A.rowptr[row], k1 = A.rowptr[row + 1];
-        float sum = 0.f;
-        for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
-            const float4 v = *reinterpret_cast<const float4*>(A.vals + k);
-            const int4 c = *reinterpret_cast<const int4*>(A.cols + k);
-            sum += v.x * A.p[c.x] + v.y * A.p[c.y] + v.z * A.p[c.z] + v.w * A.p[c.w];
-        }
-#pragma unroll
-        for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
-        if (lane == 0) {
-            A.q[row] = sum;
-            pq += (double)A.p[row] * sum;
-        }
-    }
-    double dummy = 0;
-    if (last_block_reduce(pq, dummy, A.partials, &A.st->counter[1], A.nblocks) && threadIdx.x == 0) {
-        PcgState* s = A.st;
-        s->pq = pq;
-        s->alpha = s->rz / pq;
-    }
-}
-
-__global__ void __launch_bounds__(PCG_THREADS) pcg_update_kernel(PcgArgs A) {
-    if (A.st->done) return;
-    const int lb = logical_block();
-    const int64_t r0 = (int64_t)lb * A.chunk;
-    const int64_t r1 = min(A.M, r0 + A.chunk);
-    const float alpha = (float)A.st->alpha;
-    double rz = 0, rr = 0;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) {
-        float pi = A.p[i];
-        A.x[i] += alpha * pi;
-        float ri = A.r[i] - alpha * A.q[i];
-        A.r[i] = ri;
-        float z = A.dinv[i] * ri;
-        rz += (double)ri * z;
-        rr += (double)ri * ri;
-    }
-    if (last_block_reduce(rz, rr, A.partials, &A.st->counter[2], A.nblocks) && threadIdx.x == 0) {
-        PcgState* s = A.st;
-        s->iter += 1;
-        s->rr = rr;
-        if (rr <= s->tol2 * s->bb) {
-            s->done = 1;
-        } else if (s->iter >= s->max_iter) {
-            s->done = 2;
-        } else {
-            s->beta = rz / s->rz;
-            s->rz = rz;
-        }
-    }
-}
-
-__global__ void __launch_bounds__(PCG_THREADS) pcg_direction_kernel(PcgArgs A) {
-    if (A.st->done) return;
-    const int lb = logical_block();
-    const int64_t r0 = (int64_t)lb * A.chunk;
-    const int64_t r1 = min(A.M, r0 + A.chunk);
-    const float beta = (float)A.st->beta;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) A.p[i] = A.dinv[i] * A.r[i] + beta * A.p[i];
-}
-
-template <int G>
+template <int G, bool NT>
 __global__ void __launch_bounds__(PCG_THREADS) spmv_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
                                                            const float* __restrict__ vals, const float* __restrict__ x,
                                                            float* __restrict__ y, int64_t M, int64_t chunk) {
@@ -209,15 +166,7 @@ __global__ void __launch_bounds__(PCG_THREADS) spmv_kernel(const int32_t* __rest
     const int lane = threadIdx.x % G, grp = threadIdx.x / G;
     constexpr int GROUPS = PCG_THREADS / G;
     for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
-        const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
-        float sum = 0.f;
-        for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
-            const float4 v = *reinterpret_cast<const float4*>(vals + k);
-            const int4 c = *reinterpret_cast<const int4*>(cols + k);
-            sum += v.x * x[c.x] + v.y * x[c.y] + v.z * x[c.z] + v.w * x[c.w];
-        }
-#pragma unroll
-        for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        float sum = row_dot<G, NT>(rowptr, cols, vals, x, row, lane);
         if (lane == 0) y[row] = sum;
     }
 }
@@ -274,7 +223,7 @@ static int pcg_blocks(int64_t M, int64_t* chunk) {
 
 static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
 
-static int pcg_layout(int64_t M, PcgArgs* A, char* ws) {
+static int pcg_layout(int64_t M, PcgArgs* A, char* ws, void** tab = nullptr, int32_t** map = nullptr) {
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
     size_t off = 0;
@@ -288,6 +237,9 @@ static int pcg_layout(int64_t M, PcgArgs* A, char* ws) {
     off += al256(sizeof(float) * M);
     A->q = reinterpret_cast<float*>(ws + off);
     off += al256(sizeof(float) * M);
+    if (tab) *tab = ws + off;
+    off += 512;  // one batch table entry
+    if (map) *map = reinterpret_cast<int32_t*>(ws + off);
     A->nblocks = nb;
     A->chunk = chunk;
     A->M = M;
@@ -297,21 +249,8 @@ static int pcg_layout(int64_t M, PcgArgs* A, char* ws) {
 static int64_t pcg_ws_bytes(int64_t M) {
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
-    return (int64_t)(al256(sizeof(PcgState)) + al256(sizeof(double) * 2 * nb) + 3 * al256(sizeof(float) * M));
-}
-
-static int choose_group(const int32_t* /*rowptr*/, int64_t /*M*/) { return 16; }
-
-static void launch_iteration(const PcgArgs& A, int G, hipStream_t st) {
-    dim3 grid(A.nblocks), block(PCG_THREADS);
-    if (G == 8)
-        pcg_spmv_dot_kernel<8><<<grid, block, 0, st>>>(A);
-    else if (G == 32)
-        pcg_spmv_dot_kernel<32><<<grid, block, 0, st>>>(A);
-    else
-        pcg_spmv_dot_kernel<16><<<grid, block, 0, st>>>(A);
-    pcg_update_kernel<<<grid, block, 0, st>>>(A);
-    pcg_direction_kernel<<<grid, block, 0, st>>>(A);
+    return (int64_t)(al256(sizeof(PcgState)) + al256(sizeof(double) * 2 * nb) + 3 * al256(sizeof(float) * M) + 512 +
+                     al256(sizeof(int32_t) * nb));
 }
 
 struct HostState {
@@ -335,6 +274,7 @@ struct BatchSys {
     int32_t nblk;
     int32_t pad[2];
 };
+static_assert(sizeof(BatchSys) <= 512, "batch table entry must fit its workspace slot");
 
 __device__ __forceinline__ const BatchSys* batch_lookup(const BatchSys* tab, const int32_t* blk2sys, int* local) {
     const int sys = blk2sys[blockIdx.x];
@@ -346,7 +286,7 @@ __device__ __forceinline__ const BatchSys* batch_lookup(const BatchSys* tab, con
     return B;
 }
 
-template <int G>
+template <int G, bool NT>
 __global__ void __launch_bounds__(PCG_THREADS) bpcg_spmv_dot_kernel(const BatchSys* tab, const int32_t* blk2sys) {
     int lb;
     const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
@@ -359,15 +299,7 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_spmv_dot_kernel(const BatchS
     constexpr int GROUPS = PCG_THREADS / G;
     double pq = 0;
     for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
-        const int32_t k0 = A.rowptr[row], k1 = A.rowptr[row + 1];
-        float sum = 0.f;
-        for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
-            const float4 v = *reinterpret_cast<const float4*>(A.vals + k);
-            const int4 c = *reinterpret_cast<const int4*>(A.cols + k);
-            sum += v.x * A.p[c.x] + v.y * A.p[c.y] + v.z * A.p[c.z] + v.w * A.p[c.w];
-        }
-#pragma unroll
-        for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        float sum = row_dot<G, NT>(A.rowptr, A.cols, A.vals, A.p, row, lane);
         if (lane == 0) {
             A.q[row] = sum;
             pq += (double)A.p[row] * sum;
@@ -426,9 +358,35 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_direction_kernel(const Batch
     for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) A.p[i] = A.dinv[i] * A.r[i] + beta * A.p[i];
 }
 
+__global__ void write_entry_kernel(BatchSys* dst, BatchSys src) { *dst = src; }
+
+__global__ void fill_map_kernel(int32_t* map, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) map[i] = 0;
+}
+
+// SpMV variant: lanes per row (8, 16, 32) and non-temporal matrix loads; set by ngs_pcg_config.
+static int g_spmv_lanes = 16;
+static int g_spmv_nt = 1;
+
+template <class F>
+static void dispatch_spmv(F&& f) {
+    if (g_spmv_nt) {
+        if (g_spmv_lanes == 8) f(std::integral_constant<int, 8>(), std::true_type());
+        else if (g_spmv_lanes == 32) f(std::integral_constant<int, 32>(), std::true_type());
+        else f(std::integral_constant<int, 16>(), std::true_type());
+    } else {
+        if (g_spmv_lanes == 8) f(std::integral_constant<int, 8>(), std::false_type());
+        else if (g_spmv_lanes == 32) f(std::integral_constant<int, 32>(), std::false_type());
+        else f(std::integral_constant<int, 16>(), std::false_type());
+    }
+}
+
 static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, int total_blocks, hipStream_t st) {
     dim3 grid(total_blocks), block(PCG_THREADS);
-    bpcg_spmv_dot_kernel<16><<<grid, block, 0, st>>>(tab, blk2sys);
+    dispatch_spmv([&](auto g, auto nt) {
+        bpcg_spmv_dot_kernel<decltype(g)::value, decltype(nt)::value><<<grid, block, 0, st>>>(tab, blk2sys);
+    });
     bpcg_update_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
     bpcg_direction_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
 }
@@ -440,15 +398,19 @@ using namespace ngs;
 extern "C" {
 
 int64_t ngs_pcg_workspace_bytes(int64_t M) { return pcg_ws_bytes(M); }
+int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t total_blocks, int32_t iters,
+                          int32_t use_graph, ngs_stream_t s);
 
 // Environment-free knobs for measurement: graph use and lanes per row (0 = automatic).
 static int g_pcg_use_graph = 1;
 static int g_pcg_group = 0;
-int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row) {
+int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal) {
     NGS_REQUIRE(lanes_per_row == 0 || lanes_per_row == 8 || lanes_per_row == 16 || lanes_per_row == 32,
                 "ngs_pcg_config: lanes_per_row must be 0, 8, 16 or 32");
     g_pcg_use_graph = use_graph ? 1 : 0;
     g_pcg_group = lanes_per_row;
+    g_spmv_lanes = lanes_per_row ? lanes_per_row : 16;
+    g_spmv_nt = nontemporal ? 1 : 0;
     return NGS_OK;
 }
 
@@ -475,36 +437,33 @@ int ngs_pcg_solve(const int32_t* rowptr, const int32_t* cols, const float* vals,
     NGS_CHECK_HIP(hipEventRecord(ev, st));
     NGS_CHECK_HIP(hipStreamWaitEvent(work, ev, 0));
 
+    // The solve runs through the table-driven kernels: the single-entry table and its block map live
+    // in the workspace, so the captured chunk graph depends only on (workspace, blocks, chunk) and is
+    // replayed by every later solve that reuses the workspace.
     PcgArgs A{};
-    pcg_layout(M, &A, reinterpret_cast<char*>(workspace));
+    void* tab = nullptr;
+    int32_t* map = nullptr;
+    pcg_layout(M, &A, reinterpret_cast<char*>(workspace), &tab, &map);
     A.rowptr = rowptr;
     A.cols = cols;
     A.vals = vals;
     A.dinv = inv_diag;
     A.b = b;
     A.x = x;
-    const int G = g_pcg_group ? g_pcg_group : choose_group(rowptr, M);
     zero_state<<<1, 64, 0, work>>>(A.st, max_iter);
     pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, work>>>(A, tol);
+    BatchSys E{};
+    E.a = A;
+    E.blk0 = 0;
+    E.nblk = A.nblocks;
+    write_entry_kernel<<<1, 1, 0, work>>>(reinterpret_cast<BatchSys*>(tab), E);
+    fill_map_kernel<<<grid1d(A.nblocks, 256), 256, 0, work>>>(map, A.nblocks);
     NGS_CHECK_LAUNCH();
-
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    if (g_pcg_use_graph) {
-        NGS_CHECK_HIP(hipStreamBeginCapture(work, hipStreamCaptureModeThreadLocal));
-        for (int c = 0; c < chunk; ++c) launch_iteration(A, G, work);
-        NGS_CHECK_HIP(hipStreamEndCapture(work, &graph));
-        NGS_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
     int status = NGS_OK;
     int32_t it = 0, done = 0;
     while (true) {
-        if (exec) {
-            NGS_CHECK_HIP(hipGraphLaunch(exec, work));
-        } else {
-            for (int c = 0; c < chunk; ++c) launch_iteration(A, G, work);
-            NGS_CHECK_LAUNCH();
-        }
+        int r = ngs_pcg_batch_iterate(tab, map, A.nblocks, chunk, g_pcg_use_graph, reinterpret_cast<ngs_stream_t>(work));
+        if (r != NGS_OK) return r;
         NGS_CHECK_HIP(hipMemcpyAsync(hs.pinned, &A.st->iter, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, work));
         NGS_CHECK_HIP(hipStreamSynchronize(work));
         it = hs.pinned[0];
@@ -513,10 +472,6 @@ int ngs_pcg_solve(const int32_t* rowptr, const int32_t* cols, const float* vals,
     }
     double sc[2];
     NGS_CHECK_HIP(hipMemcpy(sc, &A.st->rr, sizeof(double) * 2, hipMemcpyDeviceToHost));  // rr, bb
-    if (exec) {
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-    }
     (void)hipEventDestroy(ev);
     if (iters) *iters = it;
     if (rel_resid) *rel_resid = sc[1] > 0 ? std::sqrt(sc[0] / sc[1]) : 0.0;
@@ -532,13 +487,9 @@ int ngs_spmv(const int32_t* rowptr, const int32_t* cols, const float* vals, cons
     if (M <= 0) return NGS_OK;
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
-    const int G = g_pcg_group ? g_pcg_group : 16;
-    if (G == 8)
-        spmv_kernel<8><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
-    else if (G == 32)
-        spmv_kernel<32><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
-    else
-        spmv_kernel<16><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
+    dispatch_spmv([&](auto g, auto nt) {
+        spmv_kernel<decltype(g)::value, decltype(nt)::value><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
+    });
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
@@ -611,21 +562,26 @@ int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t tot
         NGS_CHECK_LAUNCH();
         return NGS_OK;
     }
+    // Graphs depend only on (table, map, blocks, iterations, stream). The cache is per thread (each
+    // thread launches on its own streams), bounded, and evicts its oldest entry.
     struct Key { const void* t; const int32_t* m; int nb, it; hipStream_t s; hipGraphExec_t e; };
-    static std::vector<Key> cache;
+    static thread_local std::vector<Key> cache;
+    hipGraphExec_t exec = nullptr;
     for (auto& k : cache)
-        if (k.t == table && k.m == blk2sys && k.nb == total_blocks && k.it == iters && k.s == st) {
-            NGS_CHECK_HIP(hipGraphLaunch(k.e, st));
-            return NGS_OK;
+        if (k.t == table && k.m == blk2sys && k.nb == total_blocks && k.it == iters && k.s == st) exec = k.e;
+    if (!exec) {
+        hipGraph_t graph;
+        NGS_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, st);
+        NGS_CHECK_HIP(hipStreamEndCapture(st, &graph));
+        NGS_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        if (cache.size() >= 256) {
+            (void)hipGraphExecDestroy(cache.front().e);
+            cache.erase(cache.begin());
         }
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    NGS_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, st);
-    NGS_CHECK_HIP(hipStreamEndCapture(st, &graph));
-    NGS_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(graph);
-    cache.push_back({table, blk2sys, total_blocks, iters, st, exec});
+        cache.push_back({table, blk2sys, total_blocks, iters, st, exec});
+    }
     NGS_CHECK_HIP(hipGraphLaunch(exec, st));
     return NGS_OK;
 }

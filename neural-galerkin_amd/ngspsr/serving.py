@@ -76,7 +76,9 @@ class BatchedPCG:
         job = _Job(system, b, tol, max_iter)
         self.pending.put(job)
         self._wake.set()
-        job.done.wait()
+        while not job.done.wait(0.5):
+            if self._err is not None or not self.thread.is_alive():
+                raise self._err or RuntimeError("PCG pool thread stopped")
         if job.status == 2 and self._err is not None:
             raise self._err
         return job.x, _solver.SolveInfo(job.iters, 0.0)

@@ -1,0 +1,81 @@
+"""Full-size configurations checked through size-independent properties (MI355X)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_tree(tree, xyz_t):
+    for d in range(tree.depth):
+        c = tree.get_coords(None, d)
+        if c.shape[0] == 0:
+            continue
+        s = tree.strides[d]
+        assert bool((c % s == 0).all())
+        k = (c[:, 0].long() << 42) + ((c[:, 1].long() + (1 << 20)) << 21) + (c[:, 2].long() + (1 << 20))
+        assert bool((k[1:] > k[:-1]).all()), "coords must be strictly increasing (torch.unique order)"
+        # every point of level <= d has its 8 trilinear corners in the set
+        sel = tree.xyz_depth <= d
+        q = xyz_t[sel] * tree.inv_vs
+        base = torch.floor(q / s) * s
+        sign = torch.sign(q - base - s / 2.0)
+        for o in range(8):
+            off = torch.tensor([(o >> 2) & 1, (o >> 1) & 1, o & 1], device=q.device)
+            corner = (base + off * sign * s).int()
+            assert bool((tree.grids[d].query(corner) >= 0).all()), (d, o)
+        # hash round trip on a sample
+        idx = torch.randint(0, c.shape[0], (min(200000, c.shape[0]),), device=c.device)
+        assert torch.equal(tree.hash_table(d).query(c[idx]), idx)
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2"])
+def test_full_size_properties(cuda, cfg):
+    import ngspsr
+    from ngspsr import synthetic
+    print(f"[case] full size {cfg}", flush=True)
+    if cfg == "c1":
+        xyz, nrm = synthetic.torus(100000, seed=0)
+        vs, mind, solver, tol = 1 / 128, 8.0, "pcg@tol=1e-6@maxiter=10000", 1e-6
+    else:
+        xyz, nrm = synthetic.ellipsoids(2000000, k=16, sigma=0.002, kappa=200.0, seed=0)
+        vs, mind, solver, tol = 1 / 512, 1.0, "mixed@tol=1e-8@inner_tol=1e-4", 1e-8
+    xyz_t, nrm_t = torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda)
+    res = ngspsr.reconstruct(xyz_t, nrm_t, vs, depth=4, min_density=mind, solver=solver, return_result=True)
+    tree = res.tree
+    _check_tree(tree, xyz_t)
+    for d, (system, rhs) in res.reconstructor.systems.items():
+        x = res.reconstructor.solutions[d].double()
+        _, rr, bb = system.residual_f64(rhs.double(), x)
+        rel = (rr / bb) ** 0.5
+        # FP32 storage of x adds ~1e-7 relative on top of the solver's residual
+        assert rel <= max(tol, 3e-7) * 1.01, (d, rel)
+    v, f = res.vertices, res.triangles
+    assert f.shape[0] > 1000
+    assert int(f.max()) < v.shape[0] and int(f.min()) >= 0
+    e = torch.sort(torch.cat([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), dim=1).values
+    _, cnt = torch.unique(e, dim=0, return_counts=True)
+    assert float((cnt == 2).float().mean()) > 0.95
+
+
+def test_c1_solution_error_vs_fp64_oracle(cuda):
+    """Config 1 (100k-point torus, 128^3): per-level solution error of the FP32 and the
+    mixed-precision solves against an FP64 solve of the same system."""
+    import ngspsr
+    from ngspsr import synthetic
+    from oracle import pcg
+    print("[case] c1 error vs fp64", flush=True)
+    xyz, nrm = synthetic.torus(100000, seed=0)
+    xyz_t, nrm_t = torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda)
+    out = {}
+    for name, solver in (("fp32", "pcg@tol=1e-6@maxiter=10000"), ("mixed", "mixed@tol=1e-8@inner_tol=1e-4")):
+        res = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 128, depth=4, min_density=8.0, solver=solver, return_result=True)
+        system, rhs = res.reconstructor.systems[0]
+        a = system.to_scipy()
+        b = rhs.cpu().numpy()
+        x64, _ = pcg.solve_pcg(a, b, tol=1e-13, max_iter=20000, dtype=np.float64)
+        x = res.reconstructor.solutions[0].double().cpu().numpy()
+        out[name] = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    print(f"[result] c1 level-0 solution error vs FP64: fp32 {out['fp32']:.3e} mixed {out['mixed']:.3e}", flush=True)
+    assert out["mixed"] < out["fp32"]
+    assert out["mixed"] < 1e-6
