@@ -38,25 +38,56 @@ def make_scene(cfg):
     return synthetic.room(cfg["n"], sigma=cfg["sigma"], seed=0)
 
 
-def spmv_roofline(system, reps=50):
-    """Average duration of the solver's SpMV kernel on the finest-level system, with HIP events on
-    the stream it is launched on, and its algorithmic bytes (8 nnz + 4 (M+1) + 8 M)."""
+def _event_time(fn, reps):
+    """Average seconds of fn() over reps launches, HIP events on the current stream (the stream the
+    library launches on)."""
     import torch
-    x = torch.randn(system.M, device=system.device)
-    y = torch.empty_like(x)
-    from ngspsr._lib import call, ptr, stream
     for _ in range(3):
-        call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x), ptr(y), system.M, stream())
+        fn()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s.record()
     for _ in range(reps):
-        call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x), ptr(y), system.M, stream())
+        fn()
     e.record()
     e.synchronize()
-    t = s.elapsed_time(e) / 1e3 / reps
+    return s.elapsed_time(e) / 1e3 / reps
+
+
+def spmv_roofline(system, reps=50):
+    """Average duration of the CSR SpMV kernel (spmv_kernel) on a level system and its algorithmic
+    bytes (8 nnz + 4 (M+1) + 8 M)."""
+    import torch
+    from ngspsr._lib import call, ptr, stream
+    x = torch.randn(system.M, device=system.device)
+    y = torch.empty_like(x)
+    t = _event_time(lambda: call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x), ptr(y),
+                                 system.M, stream()), reps)
     nnz = system.nnz()
     algo = 8 * nnz + 4 * (system.M + 1) + 8 * system.M
     return t, algo, nnz
+
+
+def stencil_roofline(system, reps=50):
+    """Average durations of the two passes of the stencil operator (stencil_points_kernel,
+    stencil_rows_kernel) and their algorithmic bytes, each array counted once:
+      points: cell coords 12 n + basis 36 n (+ alpha 4 n) + x 4 M + level bitmap 8 W + u 4 n
+      rows:   coords 12 M + x 4 M + y 4 M + level bitmap 8 W + cell bitmap 8 Wc + cell_start 4 (C+1)
+              + basis 36 n + u 4 n
+    (W, Wc: 32-bit words of the level and cell bitmaps, bits + rank)."""
+    import torch
+    from ngspsr._lib import call, ptr, stream
+    x = torch.randn(system.M, device=system.device)
+    y = torch.empty_like(x)
+    st = system.struct()
+    t_pts = _event_time(lambda: call("ngs_stencil_pass", st, ptr(x), ptr(y), 1, stream()), reps)
+    t_rows = _event_time(lambda: call("ngs_stencil_pass", st, ptr(x), ptr(y), 2, stream()), reps)
+    M, n = system.M, system.n
+    W = int(st.level.grid.nwords)
+    Wc = int(st.cells.nwords)
+    C = system.bins.ncells if n else 0
+    b_pts = n * (12 + 36 + 4 + (4 if system.alpha is not None else 0)) + 4 * M + 8 * W
+    b_rows = 12 * M + 8 * M + 8 * W + 8 * Wc + 4 * (C + 1) + 40 * n
+    return t_pts, b_pts, t_rows, b_rows
 
 
 def pcg_iteration_rate(system, iters=200):
@@ -211,10 +242,31 @@ def main():
     info = res.solve_info
     total_iters = int(sum(i.iters for i in info.values()))
     rel = max(float(i.rel_resid) for i in info.values())
+    from ngspsr.solver import StencilSystem
     system, rhs = res.reconstructor.systems[0]
-    t_spmv, algo, nnz = spmv_roofline(system)
+    forms = {int(d): ("stencil" if isinstance(s_, StencilSystem) else "csr")
+             for d, (s_, _) in sorted(res.reconstructor.systems.items())}
+    csr = system.to_csr() if isinstance(system, StencilSystem) else system
+    t_spmv, algo, nnz = spmv_roofline(csr)
     t_iter = pcg_iteration_rate(system)
-    iter_bytes = algo + 44 * system.M
+    spmv_obj = {"kernel": "spmv_kernel<16,1> (CSR SpMV, the solve_pcg_diag operator)", "bound": "hbm",
+                "achieved": round(algo / t_spmv / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                "launch_us": round(1e6 * t_spmv, 2), "algorithmic_bytes": int(algo)}
+    if isinstance(system, StencilSystem):
+        t_pts, b_pts, t_rows, b_rows = stencil_roofline(system)
+        iter_bytes = b_pts + b_rows + 44 * system.M
+        roof = {"kernel": "stencil_rows_kernel (finest-level row pass of the stencil operator)", "bound": "hbm",
+                "achieved": round(b_rows / t_rows / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(b_rows / t_rows / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                "launch_us": round(1e6 * t_rows, 2), "algorithmic_bytes": int(b_rows),
+                "point_pass": {"kernel": "stencil_points_kernel", "launch_us": round(1e6 * t_pts, 2),
+                               "algorithmic_bytes": int(b_pts), "achieved": round(b_pts / t_pts / 1e9, 1)},
+                "operator_us": round(1e6 * (t_pts + t_rows), 2),
+                "csr_spmv_us_same_system": round(1e6 * t_spmv, 2)}
+    else:
+        iter_bytes = algo + 44 * system.M
+        roof = dict(spmv_obj)
     out = {
         "metric": "solves/s",
         "value": round(args.steps / total, 4),
@@ -227,24 +279,24 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "precision": "FP32 CSR values and PCG vectors; FP64 dot products, residual and refinement",
+        "precision": "FP32 level operators (stencil form where cheaper per iteration, CSR elsewhere) and PCG "
+                     "vectors; FP64 dot products, residual and refinement",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "points": cfg["n"], "voxel_size": cfg["vs"], "levels": cfg["depth"],
-                   "solver": cfg["solver"], "finest_voxels": int(system.M), "finest_nnz": int(nnz)},
+                   "solver": cfg["solver"], "finest_voxels": int(system.M), "finest_nnz": int(nnz),
+                   "operator_forms": forms},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "max_level_rel_residual": rel,
         "pcg_iterations_total": total_iters,
         "pcg_iteration": {"ms": round(1e3 * t_iter, 4), "algorithmic_bytes": int(iter_bytes),
                           "GB/s": round(iter_bytes / t_iter / 1e9, 1)},
-        "roofline": {"kernel": "spmv_kernel<16> (finest-level CSR SpMV of the PCG)", "bound": "hbm",
-                     "achieved": round(algo / t_spmv / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "launch_us": round(1e6 * t_spmv, 2), "algorithmic_bytes": int(algo)},
+        "roofline": roof,
+        "csr_spmv": spmv_obj,
         "mesh": {"vertices": int(res.vertices.shape[0]), "triangles": int(res.triangles.shape[0])},
     }
     if not args.no_cpu:
-        per_iter, it = cpu_baseline(system, rhs)
+        per_iter, it = cpu_baseline(csr, rhs)
         cpu_rate = 1.0 / (per_iter * total_iters)
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
         out["cpu_baseline"] = {

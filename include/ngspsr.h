@@ -185,6 +185,27 @@ typedef struct {
     int64_t n;
 } ngs_point_bins;
 
+/* Stencil form of a level system (no reference counterpart: the reference always assembles the
+ * matrix, reconstructor.py:170-183 + solver.py:40-48). The operator of ngs_lhs_assemble, applied
+ * from the level bitmap, the 125 Laplacian coefficients and the binned screening points:
+ *   (A x)_i = sum_o lap[o] x_{i+o} + sf * sum_{p near i} B_i(p) u_p,
+ *   u_p     = alpha_p * sum_{j near p} B_j(p) x_j.
+ * Point arrays are in binned order (ngs_stencil_bin_points). */
+typedef struct {
+    ngs_level level;           /* rows: the voxels of the level */
+    ngs_grid cells;            /* cells holding >= 1 screening point (same stride) */
+    const int32_t *cell_start; /* ncells + 1 */
+    const int32_t *point_cell; /* n x 3 cell coordinates of the binned points */
+    const float *basis;        /* n x 9 per-axis basis values of the binned points */
+    const float *alpha;        /* n screening weights of the binned points, or NULL */
+    const int32_t *perm;       /* n original point id of each binned point */
+    const float *lap;          /* 125 Laplacian coefficients (ngs_stencil_laplacian) */
+    float *u;                  /* n FP32 scratch of the point pass (PCG and ngs_stencil_apply) */
+    float alpha_scalar;
+    float sf;                  /* screening factor of the level (4^-d) */
+    int64_t n;                 /* binned points (0: no screening) */
+} ngs_stencil;
+
 /* Entries per row of the level's 125-stencil (existing neighbours), rounded up to a multiple of
  * 4 (padding entries are explicit zeros on the diagonal column). */
 int ngs_lhs_row_counts(ngs_level lv, int32_t *row_nnz, ngs_stream_t s);
@@ -207,9 +228,34 @@ int ngs_rhs_cross_stiffness(ngs_level fine, ngs_level coarse, const float *x_coa
 
 /* Normal-field RHS (reconstructor.py:125-141, integrate_const_deriv_product
  * bezier_tensor.py:98-121): rhs_i += int grad B_i . (-N) over the data voxels of a data level.
- * Data at a finer or equal level is scattered (27 targets per data voxel); data at a coarser
+ * Data at the same level is gathered (27 data voxels per target, no atomics); data at a finer
+ * level is scattered (27 targets per data voxel, FP64 atomics); data at a coarser
  * level is gathered with the reference's |rel| <= 1.5 filter (_hash_backend.py:254-262). */
 int ngs_rhs_normals(ngs_level target, ngs_level data, const float *normals, double *rhs, ngs_stream_t s);
+
+/* The 125 same-level stiffness coefficients K(o), o in [-2,2]^3 lexicographic, of stride `stride`
+ * (the K of ngs_lhs_assemble, computed the same way from the m = 1 tables). */
+int ngs_stencil_laplacian(int32_t stride, float *lap, ngs_stream_t s);
+/* Binned copies of the point data of `pts`: point_cell[k] = cell[perm[k]] (n x 3),
+ * basis[k] = pts.b[perm[k]] (n x 9), alpha_out[k] = pts.alpha[perm[k]] when pts.alpha is set. */
+int ngs_stencil_bin_points(ngs_point_bins pts, const int32_t *cell, int32_t *point_cell, float *basis,
+                           float *alpha_out, ngs_stream_t s);
+/* inv_diag and the screening RHS terms of ngs_lhs_assemble for the stencil form (same values,
+ * same summation order): inv_diag_i = 1 / A_ii, rhs_i += sf delta sum_p alpha_p B_i(p)
+ * - sf sum_p alpha_p B_i(p) coarse_p (coarse in original point order, may be NULL). */
+int ngs_stencil_diag_rhs(ngs_stencil op, const float *coarse, float delta, float *inv_diag, double *rhs,
+                         ngs_stream_t s);
+/* Existing neighbours in the 125-stencil of each row (the entries of the equivalent CSR, unpadded). */
+int ngs_stencil_row_counts(ngs_level lv, int32_t *row_nnz, ngs_stream_t s);
+/* y = A x (FP32): point pass into op.u, then the row pass. */
+int ngs_stencil_apply(ngs_stencil op, const float *x, float *y, ngs_stream_t s);
+/* The passes of ngs_stencil_apply separately, for measurement: passes bit 0 = point pass
+ * (op.u from x), bit 1 = row pass (y from x and the current op.u). */
+int ngs_stencil_pass(ngs_stencil op, const float *x, float *y, int32_t passes, ngs_stream_t s);
+/* r = b - A x in FP64 (the operator evaluated in FP64 from the same FP32 data), ||r||^2 and
+ * ||b||^2 into norms[0..1]; u64: n doubles of scratch; workspace: ngs_pcg_workspace_bytes(M). */
+int ngs_stencil_residual_f64(ngs_stencil op, const double *b, const double *x, double *r, double *u64,
+                             double *norms, void *workspace, ngs_stream_t s);
 
 /* ----------------------------------------------------------------------- adjoint (backward) */
 
@@ -236,6 +282,12 @@ int ngs_pcg_solve(const int32_t *rowptr, const int32_t *cols, const float *vals,
                   const float *b, float *x, int64_t M, double tol, int32_t max_iter, int32_t chunk,
                   void *workspace, ngs_stream_t s, int32_t *iters, double *rel_resid);
 
+/* ngs_pcg_solve on the stencil form of the system (same iteration, same stop rule; the SpMV is the
+ * point pass + row pass of ngs_stencil_apply). */
+int ngs_pcg_solve_stencil(ngs_stencil op, const float *inv_diag, const float *b, float *x, double tol,
+                          int32_t max_iter, int32_t chunk, void *workspace, ngs_stream_t s, int32_t *iters,
+                          double *rel_resid);
+
 /* Solver knobs: use_graph (capture iteration chunks in a hipGraph; default 1), lanes per CSR row
  * in the SpMV (8, 16, 32; 0 = 16) and non-temporal loads of the matrix stream (default 1). */
 int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal);
@@ -252,9 +304,16 @@ int32_t ngs_pcg_batch_blocks(int64_t M);
 int ngs_pcg_batch_prepare(const int32_t *rowptr, const int32_t *cols, const float *vals, const float *inv_diag,
                           const float *b, float *x, int64_t M, double tol, int32_t max_iter, void *workspace,
                           int32_t blk0, int32_t nblk, void *entry, ngs_stream_t s);
-/* `iters` iterations of every active system in one launch per kernel (graph-replayed when use_graph). */
+/* Set the first block of a prepared host table entry (a multiple of 8). */
+int ngs_pcg_batch_place(void *entry, int32_t blk0);
+/* The same as ngs_pcg_batch_prepare for a system in stencil form. */
+int ngs_pcg_batch_prepare_stencil(ngs_stencil op, const float *inv_diag, const float *b, float *x, double tol,
+                                  int32_t max_iter, void *workspace, int32_t blk0, int32_t nblk, void *entry,
+                                  ngs_stream_t s);
+/* `iters` iterations of every active system in one launch per kernel (graph-replayed when use_graph).
+ * flags bit 0: the table may hold stencil systems (adds their point pass to each iteration). */
 int ngs_pcg_batch_iterate(const void *table, const int32_t *blk2sys, int32_t total_blocks, int32_t iters,
-                          int32_t use_graph, ngs_stream_t s);
+                          int32_t use_graph, int32_t flags, ngs_stream_t s);
 /* (iter, done) of each system of the host copy of the table into states (2 nsys int32, host);
  * synchronises s. done = 1 converged, 2 iteration cap. */
 int ngs_pcg_batch_status(const void *host_table, int32_t nsys, int32_t *states, ngs_stream_t s);
@@ -295,6 +354,14 @@ int ngs_dual_accumulate(ngs_grid centres, const int32_t *primal, int64_t n, int3
  * offsets (dc + 1, offsets[dc] = K) are written on the way. */
 int ngs_dual_compact(const int32_t *acc, const int32_t *count, int64_t dc, int32_t *cubes, int32_t *flags,
                      int32_t *offsets, void *workspace, ngs_stream_t s);
+/* Leaf forms of the two calls above (same sets, fewer atomics): the dual centres of all sub-cells
+ * of each leaf (lattice leaf*n + o*stride, o in [0, n]^3), and the dual-cube assembly of stride-1
+ * leaves (one count update per leaf and lattice point; identical to the per-sub-cell form whenever no
+ * dual centre is over-registered). */
+int ngs_mark_dual_leaves(ngs_grid g, const int32_t *leaves, int64_t n, int32_t stride, int32_t n_upsample,
+                         ngs_stream_t s);
+int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t *leaves, int64_t n, int32_t n_upsample, int32_t level,
+                               int32_t id_base, int32_t *acc, int32_t *count, ngs_stream_t s);
 /* Dual-corner positions (primal + stride) * voxel_size / n_upsample (reconstructor.py:368-373). */
 int ngs_dual_corner_positions(const int32_t *primal, int64_t n, int32_t stride, float scale, float *pos,
                               ngs_stream_t s);

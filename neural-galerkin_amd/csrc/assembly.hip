@@ -21,6 +21,7 @@ struct Tables {
     int m;
 };
 
+template <bool PAD>
 __global__ void lhs_count_kernel(ngs_level lv, int32_t* __restrict__ row_nnz) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= lv.M) return;
@@ -30,7 +31,7 @@ __global__ void lhs_count_kernel(ngs_level lv, int32_t* __restrict__ row_nnz) {
     for (int dx = -2; dx <= 2; ++dx)
         for (int dy = -2; dy <= 2; ++dy)
             for (int dz = -2; dz <= 2; ++dz) cnt += grid_test(lv.grid, cx + dx * s, cy + dy * s, cz + dz * s) ? 1 : 0;
-    row_nnz[i] = (cnt + 3) & ~3;
+    row_nnz[i] = PAD ? (cnt + 3) & ~3 : cnt;
 }
 
 template <bool HAS_ALPHA, bool HAS_COARSE>
@@ -116,6 +117,77 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
     rhs[i] += (double)(sf_delta * rs) - (double)(sf * rc);
 }
 
+// 125 same-level stiffness coefficients of stride s, computed exactly as in lhs_assemble_kernel.
+__global__ void stencil_laplacian_kernel(const float* __restrict__ F1, const float* __restrict__ G1, int s,
+                                         float* __restrict__ lap) {
+    const int t = threadIdx.x;
+    if (t >= 125) return;
+    const int dx = t / 25 - 2, dy = (t / 5) % 5 - 2, dz = t % 5 - 2;
+    const float fs = (float)s, inv_s = 1.0f / (float)s;
+    {
+#pragma clang fp contract(off)
+        const float c0 = F1[dx + 2] * fs, c1 = F1[dy + 2] * fs, c2 = F1[dz + 2] * fs;
+        const float g0 = G1[dx + 2] * inv_s, g1 = G1[dy + 2] * inv_s, g2 = G1[dz + 2] * inv_s;
+        lap[t] = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
+    }
+}
+
+__global__ void stencil_bin_kernel(ngs_point_bins pts, const int32_t* __restrict__ cell, int32_t* __restrict__ pcell,
+                                   float* __restrict__ basis, float* __restrict__ alpha) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= pts.n) return;
+    const int64_t p = pts.perm[k];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pcell[3 * k + a] = cell[3 * p + a];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) basis[9 * k + j] = pts.b[9 * p + j];
+    if (alpha) alpha[k] = pts.alpha[p];
+}
+
+// Diagonal and screening RHS of row i for the stencil form: the same products, in the same
+// order, as the diagonal entry and rs/rc of lhs_assemble_kernel (cells in (ox, oy, oz) order,
+// points in bin order).
+__global__ void __launch_bounds__(256) stencil_diag_rhs_kernel(ngs_stencil op, const float* __restrict__ coarse,
+                                                               float sf_delta,
+                                                               float* __restrict__ inv_diag, double* __restrict__ rhs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= op.level.M) return;
+    const int s = op.level.stride;
+    const int cx = op.level.coords[3 * i], cy = op.level.coords[3 * i + 1], cz = op.level.coords[3 * i + 2];
+    float dd = 0.f, rs = 0.f, rc = 0.f;
+    if (op.n > 0) {
+#pragma unroll
+        for (int ox = -1; ox <= 1; ++ox) {
+#pragma unroll
+            for (int oy = -1; oy <= 1; ++oy) {
+#pragma unroll
+                for (int oz = -1; oz <= 1; ++oz) {
+                    const int cid = grid_find(op.cells, cx + ox * s, cy + oy * s, cz + oz * s);
+                    if (cid < 0) continue;
+                    const int k0 = op.cell_start[cid], k1 = op.cell_start[cid + 1];
+                    for (int k = k0; k < k1; ++k) {
+                        const float* bp = op.basis + 9 * (int64_t)k;
+                        const float bx = bp[1 - ox], by = bp[4 - oy], bz = bp[7 - oz];
+                        const float a = op.alpha ? op.alpha[k] : op.alpha_scalar;
+                        const float w = a * (bx * by * bz);
+                        rs += w;
+                        if (coarse) rc += w * coarse[op.perm[k]];
+                        const float wxy = w * (bx * by);
+                        dd += wxy * bz;
+                    }
+                }
+            }
+        }
+    }
+    float v;
+    {
+#pragma clang fp contract(off)
+        v = op.lap[62] + op.sf * dd;  // K(0) + sf * S_ii
+    }
+    inv_diag[i] = 1.0f / v;
+    if (op.n > 0) rhs[i] += (double)(sf_delta * rs) - (double)(op.sf * rc);
+}
+
 // rel (in units of the fine stride) and table row for the cross-level pair (fine voxel at c,
 // coarse voxel at Q + k s_c): rel = k m + (Q - c)/s_f + (m - 1)/2, row = rel + 1.5 m + 0.5.
 __global__ void cross_stiffness_kernel(ngs_level fine, ngs_level coarse, const float* __restrict__ xc, Tables T,
@@ -198,6 +270,41 @@ __global__ void rhs_normals_scatter_kernel(ngs_level tgt, ngs_level dat, const f
             }
         }
     }
+}
+
+// Data at the same level: the transpose of the scatter above for m = 1, gathered per target voxel t
+// from the data voxels t - k s, k in [-1,1]^3, with the same FP32 terms summed in FP64 in a fixed
+// order (no atomics).
+__global__ void rhs_normals_same_kernel(ngs_level tgt, ngs_level dat, const float* __restrict__ nrm, Tables T,
+                                        double* __restrict__ rhs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tgt.M) return;
+    const int s = tgt.stride;
+    const int cx = tgt.coords[3 * i], cy = tgt.coords[3 * i + 1], cz = tgt.coords[3 * i + 2];
+    const float fsd = (float)s, imult = 1.0f;
+    double acc = 0.0;
+#pragma unroll
+    for (int kx = -1; kx <= 1; ++kx) {
+#pragma unroll
+        for (int ky = -1; ky <= 1; ++ky) {
+#pragma unroll
+            for (int kz = -1; kz <= 1; ++kz) {
+                const int v = grid_find(dat.grid, cx - kx * s, cy - ky * s, cz - kz * s);
+                if (v < 0) continue;
+                const float n0 = -nrm[3 * v], n1 = -nrm[3 * v + 1], n2 = -nrm[3 * v + 2];
+                const int ix = kx + 1, iy = ky + 1, iz = kz + 1;
+                float r;
+                {
+#pragma clang fp contract(off)
+                    const float c0 = T.P[ix] * fsd, c1 = T.P[iy] * fsd, c2 = T.P[iz] * fsd;
+                    const float g0 = T.Q[ix] * imult, g1 = T.Q[iy] * imult, g2 = T.Q[iz] * imult;
+                    r = n0 * g0 * c1 * c2 + n1 * c0 * g1 * c2 + n2 * c0 * c1 * g2;
+                }
+                acc += (double)r;
+            }
+        }
+    }
+    rhs[i] += acc;
 }
 
 // Data at a coarser level: gather, per target voxel, the data voxels around the data cell
@@ -345,7 +452,14 @@ int ngs_set_tables(const float* tables, const int64_t* offsets, int32_t levels) 
 
 int ngs_lhs_row_counts(ngs_level lv, int32_t* row_nnz, ngs_stream_t s) {
     if (lv.M <= 0) return NGS_OK;
-    lhs_count_kernel<<<grid1d(lv.M, 256), 256, 0, S(s)>>>(lv, row_nnz);
+    lhs_count_kernel<true><<<grid1d(lv.M, 256), 256, 0, S(s)>>>(lv, row_nnz);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_stencil_row_counts(ngs_level lv, int32_t* row_nnz, ngs_stream_t s) {
+    if (lv.M <= 0) return NGS_OK;
+    lhs_count_kernel<false><<<grid1d(lv.M, 256), 256, 0, S(s)>>>(lv, row_nnz);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
@@ -373,6 +487,36 @@ int ngs_lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, floa
     return NGS_OK;
 }
 
+int ngs_stencil_laplacian(int32_t stride, float* lap, ngs_stream_t s) {
+    NGS_REQUIRE(stride >= 1 && (stride & (stride - 1)) == 0 && lap, "ngs_stencil_laplacian: bad arguments");
+    Tables T;
+    NGS_REQUIRE(tables_for(1, &T), "ngs_stencil_laplacian: integral tables not registered");
+    stencil_laplacian_kernel<<<1, 128, 0, S(s)>>>(T.F, T.G, stride, lap);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_stencil_bin_points(ngs_point_bins pts, const int32_t* cell, int32_t* point_cell, float* basis, float* alpha_out,
+                           ngs_stream_t s) {
+    if (pts.n <= 0) return NGS_OK;
+    NGS_REQUIRE(pts.perm && pts.b && cell && point_cell && basis && (!pts.alpha || alpha_out),
+                "ngs_stencil_bin_points: bad arguments");
+    stencil_bin_kernel<<<grid1d(pts.n, 256), 256, 0, S(s)>>>(pts, cell, point_cell, basis,
+                                                             pts.alpha ? alpha_out : nullptr);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_stencil_diag_rhs(ngs_stencil op, const float* coarse, float delta, float* inv_diag, double* rhs, ngs_stream_t s) {
+    if (op.level.M <= 0) return NGS_OK;
+    NGS_REQUIRE(inv_diag && rhs && op.level.coords && op.lap && (op.n == 0 || (op.basis && op.perm && op.cell_start)),
+                "ngs_stencil_diag_rhs: bad arguments");
+    const float sfd = op.sf * delta;
+    stencil_diag_rhs_kernel<<<grid1d(op.level.M, 256), 256, 0, S(s)>>>(op, coarse, sfd, inv_diag, rhs);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
 int ngs_rhs_cross_stiffness(ngs_level fine, ngs_level coarse, const float* x_coarse, double* rhs, ngs_stream_t s) {
     if (fine.M <= 0 || coarse.M <= 0) return NGS_OK;
     NGS_REQUIRE(coarse.stride > fine.stride, "ngs_rhs_cross_stiffness: coarse stride must exceed fine stride");
@@ -386,7 +530,10 @@ int ngs_rhs_cross_stiffness(ngs_level fine, ngs_level coarse, const float* x_coa
 int ngs_rhs_normals(ngs_level target, ngs_level data, const float* normals, double* rhs, ngs_stream_t s) {
     if (target.M <= 0 || data.M <= 0) return NGS_OK;
     Tables T;
-    if (target.stride >= data.stride) {
+    if (target.stride == data.stride) {
+        NGS_REQUIRE(tables_for(1, &T), "ngs_rhs_normals: table multiple unavailable");
+        rhs_normals_same_kernel<<<grid1d(target.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);
+    } else if (target.stride > data.stride) {
         NGS_REQUIRE(tables_for(target.stride / data.stride, &T), "ngs_rhs_normals: table multiple unavailable");
         rhs_normals_scatter_kernel<<<grid1d(data.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);
     } else {

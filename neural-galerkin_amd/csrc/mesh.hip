@@ -305,6 +305,71 @@ __global__ void dedup_scatter_kernel(const int32_t* __restrict__ head, const int
     }
 }
 
+// All sub-cell corners of a leaf form the lattice leaf*nu + o*s, o in [0, nu]^3. Along z the lattice
+// points of one (x, y) column are bits of the same or the next word, so each column is marked with
+// one or two multi-bit atomicOr instead of one atomic per sub-cell corner.
+__global__ void mark_dual_leaves_kernel(ngs_grid g, const int32_t* __restrict__ leaves, int64_t n, int s, int nu) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int cols = (nu + 1) * (nu + 1);
+    if (t >= n * cols) return;
+    int64_t l = t / cols;
+    int c = (int)(t % cols);
+    int ox = c / (nu + 1), oy = c % (nu + 1);
+    int x = leaves[3 * l] * nu + ox * s, y = leaves[3 * l + 1] * nu + oy * s, z0 = leaves[3 * l + 2] * nu;
+    int64_t cur_word = -1;
+    uint32_t mask = 0;
+    for (int oz = 0; oz <= nu; ++oz) {
+        int64_t lin;
+        if (!grid_cell(g, x, y, z0 + oz * s, &lin)) continue;
+        int64_t w = lin >> 5;
+        if (w != cur_word) {
+            if (mask) atomicOr(&g.bits[cur_word], mask);
+            cur_word = w;
+            mask = 0;
+        }
+        mask |= 1u << (lin & 31);
+    }
+    if (mask) atomicOr(&g.bits[cur_word], mask);
+}
+
+// Dual-cube assembly for leaves of stride 1 (the finest level): one thread per (leaf, lattice point
+// o in [0, nu]^3) registers every sub-cell of the leaf that has the point on its boundary, with one
+// aggregated count update. Per axis, o = 0 is covered by sub-cell 0 from the positive side, o = nu
+// by sub-cell nu-1 from the negative side, and 0 < o < nu by sub-cells o-1 (negative) and o (positive).
+__global__ void dual_accumulate_leaves_kernel(ngs_grid cen, const int32_t* __restrict__ leaves, int64_t n, int nu,
+                                              int level, int id_base, int32_t* __restrict__ acc,
+                                              int32_t* __restrict__ cnt) {
+    const int per = (nu + 1) * (nu + 1) * (nu + 1);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * per) return;
+    int64_t l = t / per;
+    int r = (int)(t % per);
+    int o[3] = {r / ((nu + 1) * (nu + 1)), (r / (nu + 1)) % (nu + 1), r % (nu + 1)};
+    int dc = grid_find(cen, leaves[3 * l] * nu + o[0], leaves[3 * l + 1] * nu + o[1], leaves[3 * l + 2] * nu + o[2]);
+    if (dc < 0) return;
+    // per axis: up to two (sub-cell index, side) pairs; side 1 = cell on the positive side
+    int ce[3][2], sd[3][2], nc[3];
+    int nsl = 1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        nc[a] = 0;
+        if (o[a] > 0) { ce[a][nc[a]] = o[a] - 1; sd[a][nc[a]] = 0; ++nc[a]; }
+        if (o[a] < nu) { ce[a][nc[a]] = o[a]; sd[a][nc[a]] = 1; ++nc[a]; }
+        nsl *= nc[a];
+    }
+    int old = atomicAdd(&cnt[dc], nsl);
+    if (old >= 8) return;
+    const int64_t base = (int64_t)id_base + l * (int64_t)(nu * nu * nu);
+    for (int i = 0; i < nc[0]; ++i)
+        for (int j = 0; j < nc[1]; ++j)
+            for (int k = 0; k < nc[2]; ++k) {
+                int sl = octant_slot(sd[0][i], sd[1][j], sd[2][k]);
+                int e = (ce[0][i] * nu + ce[1][j]) * nu + ce[2][k];
+                acc[((int64_t)dc * 8 + sl) * 2] = (int32_t)(base + e);
+                acc[((int64_t)dc * 8 + sl) * 2 + 1] = level;
+            }
+}
+
 __global__ void widen_kernel(const int32_t* a, int64_t* b) { *b = *a; }
 
 static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
@@ -454,6 +519,26 @@ int ngs_dedup_vertices(const uint64_t* vkeys, const float* tris, int64_t nkeys, 
         widen_kernel<<<1, 1, 0, st>>>(&offs[nkeys], nverts);
         NGS_CHECK_LAUNCH();
     }
+    return NGS_OK;
+}
+
+int ngs_mark_dual_leaves(ngs_grid g, const int32_t* leaves, int64_t n, int32_t stride, int32_t n_upsample, ngs_stream_t s) {
+    NGS_REQUIRE(n_upsample >= 1 && stride >= 1, "ngs_mark_dual_leaves: bad arguments");
+    int64_t tot = n * (n_upsample + 1) * (n_upsample + 1);
+    if (tot <= 0) return NGS_OK;
+    mark_dual_leaves_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(g, leaves, n, stride, n_upsample);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t* leaves, int64_t n, int32_t n_upsample, int32_t level,
+                               int32_t id_base, int32_t* acc, int32_t* count, ngs_stream_t s) {
+    NGS_REQUIRE(n_upsample >= 1, "ngs_dual_accumulate_leaves: bad n_upsample");
+    int64_t tot = n * (n_upsample + 1) * (n_upsample + 1) * (n_upsample + 1);
+    if (tot <= 0) return NGS_OK;
+    dual_accumulate_leaves_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, leaves, n, n_upsample, level, id_base, acc,
+                                                                      count);
+    NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
 

@@ -12,6 +12,7 @@
 // serves the vector gathers of its own slab. Chunks of iterations are captured once into a
 // hipGraph and replayed; kernels of a converged system return immediately.
 #include "common.h"
+#include "stencil.h"
 
 #include <type_traits>
 #include <vector>
@@ -203,6 +204,47 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_kernel(const int32_t
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(PCG_THREADS) stencil_points_kernel(ngs_stencil op, const T* __restrict__ x,
+                                                                     T* __restrict__ u) {
+    const int64_t k = (int64_t)blockIdx.x * PCG_THREADS + threadIdx.x;
+    if (k < op.n) u[k] = stencil_point<T>(op, x, k);
+}
+
+__global__ void __launch_bounds__(PCG_THREADS) stencil_rows_kernel(ngs_stencil op, const float* __restrict__ x,
+                                                                   float* __restrict__ y, int64_t chunk) {
+    __shared__ float lap[125];
+    for (int t = threadIdx.x; t < 125; t += PCG_THREADS) lap[t] = op.lap[t];
+    __syncthreads();
+    const int64_t r0 = (int64_t)logical_block() * chunk;
+    const int64_t r1 = min(op.level.M, r0 + chunk);
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) y[row] = stencil_row<float>(op, lap, x, op.u, row);
+}
+
+__global__ void __launch_bounds__(PCG_THREADS) stencil_residual_kernel(ngs_stencil op, const double* __restrict__ b,
+                                                                       const double* __restrict__ x,
+                                                                       const double* __restrict__ u,
+                                                                       double* __restrict__ r, int64_t chunk,
+                                                                       double* partials, uint32_t* counter,
+                                                                       double* norms) {
+    __shared__ float lap[125];
+    for (int t = threadIdx.x; t < 125; t += PCG_THREADS) lap[t] = op.lap[t];
+    __syncthreads();
+    const int64_t r0 = (int64_t)logical_block() * chunk;
+    const int64_t r1 = min(op.level.M, r0 + chunk);
+    double rr = 0, bb = 0;
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
+        const double ri = b[row] - stencil_row<double>(op, lap, x, u, row);
+        r[row] = ri;
+        rr += ri * ri;
+        bb += b[row] * b[row];
+    }
+    if (last_block_reduce(rr, bb, partials, counter, gridDim.x) && threadIdx.x == 0) {
+        norms[0] = rr;
+        norms[1] = bb;
+    }
+}
+
 __global__ void zero_state(PcgState* s, int max_iter) {
     if (threadIdx.x == 0) {
         s->counter[0] = s->counter[1] = s->counter[2] = s->counter[3] = 0u;
@@ -268,11 +310,15 @@ static HostState& host_state() {
 // contiguous range of blocks each one owns; every kernel looks up its system from its block index,
 // so one launch (and one captured graph, whose arguments are the fixed table pointers) serves any
 // mix of systems. Systems join and leave between launches by rewriting the table.
+enum : int32_t { SYS_CSR = 0, SYS_STENCIL = 1 };
+
 struct BatchSys {
     PcgArgs a;
+    ngs_stencil op;  // kind == SYS_STENCIL
+    int32_t kind;
     int32_t blk0;
     int32_t nblk;
-    int32_t pad[2];
+    int32_t pad;
 };
 static_assert(sizeof(BatchSys) <= 512, "batch table entry must fit its workspace slot");
 
@@ -298,11 +344,22 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_spmv_dot_kernel(const BatchS
     const int lane = threadIdx.x % G, grp = threadIdx.x / G;
     constexpr int GROUPS = PCG_THREADS / G;
     double pq = 0;
-    for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
-        float sum = row_dot<G, NT>(A.rowptr, A.cols, A.vals, A.p, row, lane);
-        if (lane == 0) {
+    if (B->kind == SYS_STENCIL) {
+        __shared__ float lap[125];
+        for (int t = threadIdx.x; t < 125; t += PCG_THREADS) lap[t] = B->op.lap[t];
+        __syncthreads();
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
+            const float sum = stencil_row<float>(B->op, lap, A.p, B->op.u, row);
             A.q[row] = sum;
             pq += (double)A.p[row] * sum;
+        }
+    } else {
+        for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+            float sum = row_dot<G, NT>(A.rowptr, A.cols, A.vals, A.p, row, lane);
+            if (lane == 0) {
+                A.q[row] = sum;
+                pq += (double)A.p[row] * sum;
+            }
         }
     }
     double dummy = 0;
@@ -310,6 +367,21 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_spmv_dot_kernel(const BatchS
         A.st->pq = pq;
         A.st->alpha = A.st->rz / pq;
     }
+}
+
+// Point pass of the stencil systems: u = point form of p, the points split evenly over the
+// system's blocks (same XCD-contiguous mapping as its rows).
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_points_kernel(const BatchSys* tab, const int32_t* blk2sys) {
+    int lb;
+    const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
+    if (!B || B->kind != SYS_STENCIL) return;
+    const PcgArgs& A = B->a;
+    if (A.st->done) return;
+    const ngs_stencil& op = B->op;
+    const int64_t per = (op.n + B->nblk - 1) / B->nblk;
+    const int64_t k0 = (int64_t)lb * per;
+    const int64_t k1 = min(op.n, k0 + per);
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += PCG_THREADS) op.u[k] = stencil_point<float>(op, A.p, k);
 }
 
 __global__ void __launch_bounds__(PCG_THREADS) bpcg_update_kernel(const BatchSys* tab, const int32_t* blk2sys) {
@@ -382,8 +454,10 @@ static void dispatch_spmv(F&& f) {
     }
 }
 
-static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, int total_blocks, hipStream_t st) {
+static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, int total_blocks, int flags,
+                                   hipStream_t st) {
     dim3 grid(total_blocks), block(PCG_THREADS);
+    if (flags & 1) bpcg_points_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
     dispatch_spmv([&](auto g, auto nt) {
         bpcg_spmv_dot_kernel<decltype(g)::value, decltype(nt)::value><<<grid, block, 0, st>>>(tab, blk2sys);
     });
@@ -399,7 +473,7 @@ extern "C" {
 
 int64_t ngs_pcg_workspace_bytes(int64_t M) { return pcg_ws_bytes(M); }
 int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t total_blocks, int32_t iters,
-                          int32_t use_graph, ngs_stream_t s);
+                          int32_t use_graph, int32_t flags, ngs_stream_t s);
 
 // Environment-free knobs for measurement: graph use and lanes per row (0 = automatic).
 static int g_pcg_use_graph = 1;
@@ -414,16 +488,11 @@ int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal
     return NGS_OK;
 }
 
-int ngs_pcg_solve(const int32_t* rowptr, const int32_t* cols, const float* vals, const float* inv_diag, const float* b,
-                  float* x, int64_t M, double tol, int32_t max_iter, int32_t chunk, void* workspace, ngs_stream_t s,
-                  int32_t* iters, double* rel_resid) {
-    NGS_REQUIRE(M >= 0 && workspace, "ngs_pcg_solve: bad arguments");
-    NGS_REQUIRE(tol >= 0, "ngs_pcg_solve: negative tolerance");
-    if (M == 0) {
-        if (iters) *iters = 0;
-        if (rel_resid) *rel_resid = 0;
-        return NGS_OK;
-    }
+// One system through the table-driven kernels: the single-entry table and its block map live in
+// the workspace, so the captured chunk graph depends only on (workspace, blocks, chunk) and is
+// replayed by every later solve that reuses the workspace.
+static int pcg_solve_entry(BatchSys E, int64_t M, double tol, int32_t max_iter, int32_t chunk, void* workspace,
+                           ngs_stream_t s, int32_t* iters, double* rel_resid) {
     if (chunk <= 0) chunk = 32;
     if (max_iter < 0) max_iter = 10000000;
     hipStream_t st = S(s);
@@ -436,33 +505,24 @@ int ngs_pcg_solve(const int32_t* rowptr, const int32_t* cols, const float* vals,
     NGS_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     NGS_CHECK_HIP(hipEventRecord(ev, st));
     NGS_CHECK_HIP(hipStreamWaitEvent(work, ev, 0));
-
-    // The solve runs through the table-driven kernels: the single-entry table and its block map live
-    // in the workspace, so the captured chunk graph depends only on (workspace, blocks, chunk) and is
-    // replayed by every later solve that reuses the workspace.
-    PcgArgs A{};
     void* tab = nullptr;
     int32_t* map = nullptr;
+    PcgArgs A = E.a;
     pcg_layout(M, &A, reinterpret_cast<char*>(workspace), &tab, &map);
-    A.rowptr = rowptr;
-    A.cols = cols;
-    A.vals = vals;
-    A.dinv = inv_diag;
-    A.b = b;
-    A.x = x;
     zero_state<<<1, 64, 0, work>>>(A.st, max_iter);
     pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, work>>>(A, tol);
-    BatchSys E{};
     E.a = A;
     E.blk0 = 0;
     E.nblk = A.nblocks;
     write_entry_kernel<<<1, 1, 0, work>>>(reinterpret_cast<BatchSys*>(tab), E);
     fill_map_kernel<<<grid1d(A.nblocks, 256), 256, 0, work>>>(map, A.nblocks);
     NGS_CHECK_LAUNCH();
+    const int flags = E.kind == SYS_STENCIL ? 1 : 0;
     int status = NGS_OK;
     int32_t it = 0, done = 0;
     while (true) {
-        int r = ngs_pcg_batch_iterate(tab, map, A.nblocks, chunk, g_pcg_use_graph, reinterpret_cast<ngs_stream_t>(work));
+        int r = ngs_pcg_batch_iterate(tab, map, A.nblocks, chunk, g_pcg_use_graph, flags,
+                                      reinterpret_cast<ngs_stream_t>(work));
         if (r != NGS_OK) return r;
         NGS_CHECK_HIP(hipMemcpyAsync(hs.pinned, &A.st->iter, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, work));
         NGS_CHECK_HIP(hipStreamSynchronize(work));
@@ -480,6 +540,52 @@ int ngs_pcg_solve(const int32_t* rowptr, const int32_t* cols, const float* vals,
         status = NGS_ERR_NOT_CONVERGED;
     }
     return status;
+}
+
+static bool stencil_ok(const ngs_stencil& op) {
+    return op.level.coords && op.level.grid.bits && op.level.grid.rank && op.lap &&
+           (op.n == 0 || (op.u && op.basis && op.point_cell && op.cell_start && op.cells.bits && op.cells.rank));
+}
+
+int ngs_pcg_solve(const int32_t* rowptr, const int32_t* cols, const float* vals, const float* inv_diag, const float* b,
+                  float* x, int64_t M, double tol, int32_t max_iter, int32_t chunk, void* workspace, ngs_stream_t s,
+                  int32_t* iters, double* rel_resid) {
+    NGS_REQUIRE(M >= 0 && workspace, "ngs_pcg_solve: bad arguments");
+    NGS_REQUIRE(tol >= 0, "ngs_pcg_solve: negative tolerance");
+    if (M == 0) {
+        if (iters) *iters = 0;
+        if (rel_resid) *rel_resid = 0;
+        return NGS_OK;
+    }
+    BatchSys E{};
+    E.kind = SYS_CSR;
+    E.a.rowptr = rowptr;
+    E.a.cols = cols;
+    E.a.vals = vals;
+    E.a.dinv = inv_diag;
+    E.a.b = b;
+    E.a.x = x;
+    return pcg_solve_entry(E, M, tol, max_iter, chunk, workspace, s, iters, rel_resid);
+}
+
+int ngs_pcg_solve_stencil(ngs_stencil op, const float* inv_diag, const float* b, float* x, double tol, int32_t max_iter,
+                          int32_t chunk, void* workspace, ngs_stream_t s, int32_t* iters, double* rel_resid) {
+    const int64_t M = op.level.M;
+    NGS_REQUIRE(M >= 0 && workspace && inv_diag && b && x, "ngs_pcg_solve_stencil: bad arguments");
+    NGS_REQUIRE(tol >= 0, "ngs_pcg_solve_stencil: negative tolerance");
+    if (M == 0) {
+        if (iters) *iters = 0;
+        if (rel_resid) *rel_resid = 0;
+        return NGS_OK;
+    }
+    NGS_REQUIRE(stencil_ok(op), "ngs_pcg_solve_stencil: incomplete operator");
+    BatchSys E{};
+    E.kind = SYS_STENCIL;
+    E.op = op;
+    E.a.dinv = inv_diag;
+    E.a.b = b;
+    E.a.x = x;
+    return pcg_solve_entry(E, M, tol, max_iter, chunk, workspace, s, iters, rel_resid);
 }
 
 int ngs_spmv(const int32_t* rowptr, const int32_t* cols, const float* vals, const float* x, float* y, int64_t M,
@@ -509,6 +615,41 @@ int ngs_residual_f64(const int32_t* rowptr, const int32_t* cols, const float* va
     return NGS_OK;
 }
 
+int ngs_stencil_pass(ngs_stencil op, const float* x, float* y, int32_t passes, ngs_stream_t s) {
+    const int64_t M = op.level.M;
+    if (M <= 0) return NGS_OK;
+    NGS_REQUIRE(x && (y || !(passes & 2)) && (passes & 3) && stencil_ok(op), "ngs_stencil_pass: bad arguments");
+    if ((passes & 1) && op.n > 0)
+        stencil_points_kernel<float><<<grid1d(op.n, PCG_THREADS), PCG_THREADS, 0, S(s)>>>(op, x, op.u);
+    if (passes & 2) {
+        int64_t chunk;
+        int nb = pcg_blocks(M, &chunk);
+        stencil_rows_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(op, x, y, chunk);
+    }
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_stencil_apply(ngs_stencil op, const float* x, float* y, ngs_stream_t s) { return ngs_stencil_pass(op, x, y, 3, s); }
+
+int ngs_stencil_residual_f64(ngs_stencil op, const double* b, const double* x, double* r, double* u64, double* norms,
+                             void* workspace, ngs_stream_t s) {
+    const int64_t M = op.level.M;
+    if (M <= 0) return NGS_OK;
+    NGS_REQUIRE(b && x && r && norms && workspace && (op.n == 0 || u64) && stencil_ok(op),
+                "ngs_stencil_residual_f64: bad arguments");
+    if (op.n > 0) stencil_points_kernel<double><<<grid1d(op.n, PCG_THREADS), PCG_THREADS, 0, S(s)>>>(op, x, u64);
+    int64_t chunk;
+    int nb = pcg_blocks(M, &chunk);
+    char* w = reinterpret_cast<char*>(workspace);
+    PcgState* stt = reinterpret_cast<PcgState*>(w);
+    double* partials = reinterpret_cast<double*>(w + al256(sizeof(PcgState)));
+    zero_state<<<1, 64, 0, S(s)>>>(stt, 0);
+    stencil_residual_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(op, b, x, u64, r, chunk, partials, &stt->counter[3], norms);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
 // ----------------------------------------------------------------------------- batched ABI
 
 int32_t ngs_pcg_batch_blocks(int64_t M);
@@ -519,20 +660,10 @@ int64_t ngs_pcg_batch_entry_bytes(void) { return (int64_t)sizeof(BatchSys); }
 // Prepare one system for batched iterations: lay out its workspace (as ngs_pcg_solve does),
 // run the initialisation (x = 0, r = b, p = D^-1 b, norms) on `s`, and write its table entry
 // (host memory, `entry`, ngs_pcg_batch_entry_bytes bytes) with the given block range.
-int ngs_pcg_batch_prepare(const int32_t* rowptr, const int32_t* cols, const float* vals, const float* inv_diag,
-                          const float* b, float* x, int64_t M, double tol, int32_t max_iter, void* workspace,
-                          int32_t blk0, int32_t nblk, void* entry, ngs_stream_t s) {
-    NGS_REQUIRE(M > 0 && workspace && entry, "ngs_pcg_batch_prepare: bad arguments");
-    NGS_REQUIRE(nblk == ngs_pcg_batch_blocks(M), "ngs_pcg_batch_prepare: nblk must be ngs_pcg_batch_blocks(M)");
-    BatchSys* E = reinterpret_cast<BatchSys*>(entry);
-    PcgArgs A{};
+static int batch_prepare(BatchSys* E, int64_t M, double tol, int32_t max_iter, void* workspace, int32_t blk0,
+                         int32_t nblk, ngs_stream_t s) {
+    PcgArgs A = E->a;
     pcg_layout(M, &A, reinterpret_cast<char*>(workspace));
-    A.rowptr = rowptr;
-    A.cols = cols;
-    A.vals = vals;
-    A.dinv = inv_diag;
-    A.b = b;
-    A.x = x;
     if (max_iter < 0) max_iter = 10000000;
     zero_state<<<1, 64, 0, S(s)>>>(A.st, max_iter);
     pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, S(s)>>>(A, tol);
@@ -540,6 +671,49 @@ int ngs_pcg_batch_prepare(const int32_t* rowptr, const int32_t* cols, const floa
     E->a = A;
     E->blk0 = blk0;
     E->nblk = nblk;
+    return NGS_OK;
+}
+
+int ngs_pcg_batch_prepare(const int32_t* rowptr, const int32_t* cols, const float* vals, const float* inv_diag,
+                          const float* b, float* x, int64_t M, double tol, int32_t max_iter, void* workspace,
+                          int32_t blk0, int32_t nblk, void* entry, ngs_stream_t s) {
+    NGS_REQUIRE(M > 0 && workspace && entry, "ngs_pcg_batch_prepare: bad arguments");
+    NGS_REQUIRE(nblk == ngs_pcg_batch_blocks(M), "ngs_pcg_batch_prepare: nblk must be ngs_pcg_batch_blocks(M)");
+    BatchSys E{};
+    E.kind = SYS_CSR;
+    E.a.rowptr = rowptr;
+    E.a.cols = cols;
+    E.a.vals = vals;
+    E.a.dinv = inv_diag;
+    E.a.b = b;
+    E.a.x = x;
+    int r = batch_prepare(&E, M, tol, max_iter, workspace, blk0, nblk, s);
+    if (r == NGS_OK) *reinterpret_cast<BatchSys*>(entry) = E;
+    return r;
+}
+
+int ngs_pcg_batch_prepare_stencil(ngs_stencil op, const float* inv_diag, const float* b, float* x, double tol,
+                                  int32_t max_iter, void* workspace, int32_t blk0, int32_t nblk, void* entry,
+                                  ngs_stream_t s) {
+    const int64_t M = op.level.M;
+    NGS_REQUIRE(M > 0 && workspace && entry && inv_diag && b && x, "ngs_pcg_batch_prepare_stencil: bad arguments");
+    NGS_REQUIRE(stencil_ok(op), "ngs_pcg_batch_prepare_stencil: incomplete operator");
+    NGS_REQUIRE(nblk == ngs_pcg_batch_blocks(M), "ngs_pcg_batch_prepare_stencil: nblk must be ngs_pcg_batch_blocks(M)");
+    BatchSys E{};
+    E.kind = SYS_STENCIL;
+    E.op = op;
+    E.a.dinv = inv_diag;
+    E.a.b = b;
+    E.a.x = x;
+    int r = batch_prepare(&E, M, tol, max_iter, workspace, blk0, nblk, s);
+    if (r == NGS_OK) *reinterpret_cast<BatchSys*>(entry) = E;
+    return r;
+}
+
+// Moves a prepared host table entry to the block range starting at blk0.
+int ngs_pcg_batch_place(void* entry, int32_t blk0) {
+    NGS_REQUIRE(entry && blk0 >= 0 && blk0 % NUM_XCD == 0, "ngs_pcg_batch_place: bad arguments");
+    reinterpret_cast<BatchSys*>(entry)->blk0 = blk0;
     return NGS_OK;
 }
 
@@ -553,26 +727,27 @@ int32_t ngs_pcg_batch_blocks(int64_t M) {
 // (total_blocks entries, -1 = idle block). With use_graph, the iterations are captured once per
 // (table, map, total_blocks, iters) and replayed; the table contents may change between calls.
 int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t total_blocks, int32_t iters,
-                          int32_t use_graph, ngs_stream_t s) {
+                          int32_t use_graph, int32_t flags, ngs_stream_t s) {
     NGS_REQUIRE(table && blk2sys && total_blocks > 0 && iters > 0, "ngs_pcg_batch_iterate: bad arguments");
     const BatchSys* tab = reinterpret_cast<const BatchSys*>(table);
     hipStream_t st = S(s);
     if (!use_graph) {
-        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, st);
+        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, flags, st);
         NGS_CHECK_LAUNCH();
         return NGS_OK;
     }
     // Graphs depend only on (table, map, blocks, iterations, stream). The cache is per thread (each
     // thread launches on its own streams), bounded, and evicts its oldest entry.
-    struct Key { const void* t; const int32_t* m; int nb, it; hipStream_t s; hipGraphExec_t e; };
+    struct Key { const void* t; const int32_t* m; int nb, it, fl; hipStream_t s; hipGraphExec_t e; };
     static thread_local std::vector<Key> cache;
     hipGraphExec_t exec = nullptr;
     for (auto& k : cache)
-        if (k.t == table && k.m == blk2sys && k.nb == total_blocks && k.it == iters && k.s == st) exec = k.e;
+        if (k.t == table && k.m == blk2sys && k.nb == total_blocks && k.it == iters && k.fl == flags && k.s == st)
+            exec = k.e;
     if (!exec) {
         hipGraph_t graph;
         NGS_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, st);
+        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, flags, st);
         NGS_CHECK_HIP(hipStreamEndCapture(st, &graph));
         NGS_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
@@ -580,7 +755,7 @@ int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t tot
             (void)hipGraphExecDestroy(cache.front().e);
             cache.erase(cache.begin());
         }
-        cache.push_back({table, blk2sys, total_blocks, iters, st, exec});
+        cache.push_back({table, blk2sys, total_blocks, iters, flags, st, exec});
     }
     NGS_CHECK_HIP(hipGraphLaunch(exec, st));
     return NGS_OK;

@@ -1,0 +1,123 @@
+// Stencil form of a level system: the operator that ngs_lhs_assemble writes as CSR, applied
+// without stored entries.
+//
+//   (A x)_i = sum_{o in [-2,2]^3} lap[o] x_{i+o}  +  sf * sum_{p: cell(p) in i + [-1,1]^3} B_i(p) u_p
+//   u_p     = alpha_p * sum_{j in cell(p) + [-1,1]^3} B_j(p) x_j
+//
+// Neighbour ranks come from the level's ordered bitmap: the five z-neighbours of a voxel at one
+// (dx, dy) are consecutive lattice bits, and the ranks of the present ones are consecutive, so
+// one bitmap window and one rank read give a whole column. Per-row HBM traffic is a few bytes of
+// bitmap plus the gathered vector, instead of 8 bytes per stored entry.
+#pragma once
+
+#include "common.h"
+
+namespace ngs {
+
+// Presence mask of the n (<= 8) consecutive lattice points (lx, ly, lz0 + k), k < n, of g (lattice
+// coordinates relative to the origin), and in *r0 the rank of the first present one. Present
+// points have consecutive ranks: point k has rank r0 + popc(mask & ((1 << k) - 1)).
+__device__ __forceinline__ uint32_t grid_column(const ngs_grid& g, int lx, int ly, int lz0, int n, int32_t* r0) {
+    if ((unsigned)lx >= (unsigned)g.dims[0] || (unsigned)ly >= (unsigned)g.dims[1]) return 0u;
+    const int zlo = max(lz0, 0), zhi = min(lz0 + n, g.dims[2]);
+    if (zlo >= zhi) return 0u;
+    const int64_t lin = grid_linear(g, lx, ly, zlo);
+    const int64_t w = lin >> 5;
+    const int b = (int)(lin & 31);
+    const int len = zhi - zlo;
+    const uint32_t w0 = g.bits[w];
+    uint64_t v = w0;
+    // the window's last bit is a lattice point of the box, so word w + 1 exists when it is needed
+    if (b + len > 32) v |= (uint64_t)g.bits[w + 1] << 32;
+    const uint32_t m = (uint32_t)(v >> b) & ((1u << len) - 1u);
+    *r0 = g.rank[w] + __popc(w0 & ((1u << b) - 1u));
+    return m << (zlo - lz0);
+}
+
+// Row i of A x. lap: the 125 Laplacian coefficients (lexicographic offsets), typically in LDS.
+// u: the point pass of the same x.
+template <typename T>
+__device__ __forceinline__ T stencil_row(const ngs_stencil& op, const float* lap, const T* __restrict__ x,
+                                         const T* __restrict__ u, int64_t i) {
+    const ngs_grid& g = op.level.grid;
+    const int cx = op.level.coords[3 * i], cy = op.level.coords[3 * i + 1], cz = op.level.coords[3 * i + 2];
+    const int l2 = g.log2_stride;
+    const int lx = (cx >> l2) - g.origin[0], ly = (cy >> l2) - g.origin[1], lz = (cz >> l2) - g.origin[2];
+    T acc = 0;
+#pragma unroll 1
+    for (int dx = 0; dx < 5; ++dx) {
+#pragma unroll
+        for (int dy = 0; dy < 5; ++dy) {
+            int32_t r;
+            const uint32_t m = grid_column(g, lx + dx - 2, ly + dy - 2, lz - 2, 5, &r);
+            const float* lp = lap + (dx * 5 + dy) * 5;
+#pragma unroll
+            for (int dz = 0; dz < 5; ++dz) {
+                if ((m >> dz) & 1u) {
+                    acc += (T)lp[dz] * x[r];
+                    ++r;
+                }
+            }
+        }
+    }
+    if (op.n == 0) return acc;
+    const ngs_grid& cg = op.cells;
+    const int c2 = cg.log2_stride;
+    const int mx = (cx >> c2) - cg.origin[0], my = (cy >> c2) - cg.origin[1], mz = (cz >> c2) - cg.origin[2];
+    T sc = 0;
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox) {
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy) {
+            int32_t c;
+            const uint32_t m = grid_column(cg, mx + ox - 1, my + oy - 1, mz - 1, 3, &c);
+            if (!m) continue;
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                if (!((m >> oz) & 1u)) continue;
+                const int32_t k0 = op.cell_start[c], k1 = op.cell_start[c + 1];
+                ++c;
+                // the cell sits at offset (ox-1, oy-1, oz-1) from voxel i: B_i(p) = bx[2-ox] by[2-oy] bz[2-oz]
+                for (int32_t k = k0; k < k1; ++k) {
+                    const float* bp = op.basis + 9 * (int64_t)k;
+                    const T w = (T)bp[2 - ox] * (T)bp[5 - oy] * (T)bp[8 - oz];
+                    sc += w * u[k];
+                }
+            }
+        }
+    }
+    return acc + (T)op.sf * sc;
+}
+
+// u_k of binned point k for the vector x: alpha_k sum_j B_j(p_k) x_j over the present voxels j of
+// the 27 around the point's cell.
+template <typename T>
+__device__ __forceinline__ T stencil_point(const ngs_stencil& op, const T* __restrict__ x, int64_t k) {
+    const ngs_grid& g = op.level.grid;
+    const int l2 = g.log2_stride;
+    const int32_t* pc = op.point_cell + 3 * k;
+    const int lx = (pc[0] >> l2) - g.origin[0], ly = (pc[1] >> l2) - g.origin[1], lz = (pc[2] >> l2) - g.origin[2];
+    const float* bp = op.basis + 9 * k;
+    T sum = 0;
+#pragma unroll
+    for (int vx = 0; vx < 3; ++vx) {
+#pragma unroll
+        for (int vy = 0; vy < 3; ++vy) {
+            int32_t r;
+            const uint32_t m = grid_column(g, lx + vx - 1, ly + vy - 1, lz - 1, 3, &r);
+            if (!m) continue;
+            const T wxy = (T)bp[vx] * (T)bp[3 + vy];
+#pragma unroll
+            for (int vz = 0; vz < 3; ++vz) {
+                if ((m >> vz) & 1u) {
+                    sum += wxy * (T)bp[6 + vz] * x[r];
+                    ++r;
+                }
+            }
+        }
+    }
+    const T a = op.alpha ? (T)op.alpha[k] : (T)op.alpha_scalar;
+    return a * sum;
+}
+
+}  // namespace ngs

@@ -39,7 +39,7 @@ class ReconstructionResult:
 
 def reconstruct(xyz, normal, voxel_size: float, depth: int = 4, min_density: float = 32.0, screen_alpha: float = 4.0,
                 *, degree: int = 2, solver: str = "pcg@maxiter=100000", n_upsample: int = 2, xyz_depth=None,
-                return_result: bool = False, device=None):
+                operator: str = "auto", return_result: bool = False, device=None):
     """Screened Poisson reconstruction of oriented points (jspsr/__init__.py:8-61).
 
     :param xyz: (N, 3) points (numpy array or torch tensor)
@@ -52,6 +52,8 @@ def reconstruct(xyz, normal, voxel_size: float, depth: int = 4, min_density: flo
     :param solver: solver string ("pcg@tol=1e-6@maxiter=10000", "mixed@tol=1e-8", ...); the default
         keeps the reference's PCG tolerance (1e-5 of ||b||) with an iteration cap
     :param xyz_depth: optional imposed per-point levels (skips the density-derived assignment)
+    :param operator: level operator form, "csr" (assembled matrix, as the reference), "stencil"
+        (applied from the voxel bitmap and the binned points) or "auto" (cheaper per iteration)
     :return: (vertices (V, 3), triangles (T, 3)) as numpy arrays for numpy input, device tensors
         otherwise; with return_result, a ReconstructionResult holding the solution field too.
     """
@@ -77,6 +79,7 @@ def reconstruct(xyz, normal, voxel_size: float, depth: int = 4, min_density: flo
             nd = tree.splat_data(xyz_t, None, d, nrm_t, scale=sample_weight, levels=tree.xyz_depth, level=d)
             normal_data[d] = nd / float(tree.strides[d] ** 3)
         recon = Reconstructor(tree)
+        recon.operator = operator
         recon.sample_weight = sample_weight
         recon.solve_multigrid(depth - 1, 0, normal_data, screen_alpha=screen_alpha, screen_xyz=xyz_t, solver=solver)
         v, f = recon.extract_multiscale_mesh(n_upsample=n_upsample)

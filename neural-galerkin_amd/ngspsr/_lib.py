@@ -31,6 +31,12 @@ class PointBins(ctypes.Structure):
                 ("alpha_scalar", c_float), ("pad_", c_float), ("n", c_int64)]
 
 
+class Stencil(ctypes.Structure):
+    _fields_ = [("level", Level), ("cells", Grid), ("cell_start", c_void_p), ("point_cell", c_void_p),
+                ("basis", c_void_p), ("alpha", c_void_p), ("perm", c_void_p), ("lap", c_void_p), ("u", c_void_p),
+                ("alpha_scalar", c_float), ("sf", c_float), ("n", c_int64)]
+
+
 class HashTable(ctypes.Structure):
     _fields_ = [("table_size", c_int64), ("num_buckets", c_int32), ("evict_bound", c_int32), ("keys", c_void_p),
                 ("vals", c_void_p), ("mult", c_uint64 * 3), ("inserted", c_int64), ("attempts", c_int32),
@@ -79,7 +85,18 @@ _SIGNATURES = {
     "ngs_pcg_batch_entry_bytes": ([], c_int64),
     "ngs_pcg_batch_blocks": ([c_int64], c_int32),
     "ngs_pcg_batch_prepare": ([P, P, P, P, P, P, c_int64, c_double, c_int32, P, c_int32, c_int32, P, P], c_int32),
-    "ngs_pcg_batch_iterate": ([P, P, c_int32, c_int32, c_int32, P], c_int32),
+    "ngs_pcg_batch_prepare_stencil": ([Stencil, P, P, P, c_double, c_int32, P, c_int32, c_int32, P, P], c_int32),
+    "ngs_pcg_batch_place": ([P, c_int32], c_int32),
+    "ngs_pcg_batch_iterate": ([P, P, c_int32, c_int32, c_int32, c_int32, P], c_int32),
+    "ngs_pcg_solve_stencil": ([Stencil, P, P, P, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
+                               ctypes.POINTER(c_double)], c_int32),
+    "ngs_stencil_laplacian": ([c_int32, P, P], c_int32),
+    "ngs_stencil_bin_points": ([PointBins, P, P, P, P, P], c_int32),
+    "ngs_stencil_diag_rhs": ([Stencil, P, c_float, P, P, P], c_int32),
+    "ngs_stencil_apply": ([Stencil, P, P, P], c_int32),
+    "ngs_stencil_pass": ([Stencil, P, P, c_int32, P], c_int32),
+    "ngs_stencil_row_counts": ([Level, P, P], c_int32),
+    "ngs_stencil_residual_f64": ([Stencil, P, P, P, P, P, P, P], c_int32),
     "ngs_pcg_batch_status": ([P, c_int32, P, P], c_int32),
     "ngs_spmv": ([P, P, P, P, P, c_int64, P], c_int32),
     "ngs_residual_f64": ([P, P, P, P, P, P, c_int64, P, P, P], c_int32),
@@ -89,6 +106,8 @@ _SIGNATURES = {
     "ngs_expand_primal": ([P, c_int64, c_int32, c_int32, P, P], c_int32),
     "ngs_mark_dual_centres": ([Grid, P, c_int64, c_int32, P], c_int32),
     "ngs_dual_accumulate": ([Grid, P, c_int64, c_int32, c_int32, c_int32, P, P, P], c_int32),
+    "ngs_mark_dual_leaves": ([Grid, P, c_int64, c_int32, c_int32, P], c_int32),
+    "ngs_dual_accumulate_leaves": ([Grid, P, c_int64, c_int32, c_int32, c_int32, P, P, P], c_int32),
     "ngs_dual_compact": ([P, P, c_int64, P, P, P, P, P], c_int32),
     "ngs_dual_corner_positions": ([P, c_int64, c_int32, c_float, P, P], c_int32),
     "ngs_mc_count": ([P, c_int64, P, P, P], c_int32),

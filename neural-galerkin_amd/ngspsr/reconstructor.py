@@ -68,15 +68,37 @@ class Reconstructor:
         self.sample_weight = None
         self.fixed_level_set = None
         self.branch = HashTree.DECODER
+        self.operator = "auto"  # level operator form: "auto", "csr" or "stencil"
         ensure_device_tables(hash_tree.device)
 
     def set_fixed_level_set(self, value):
         self.fixed_level_set = value
 
-    def assemble_level(self, d, bins, coarse, sf, delta, rhs):
+    # Per-iteration cost model of the two operator forms, in bytes of HBM-equivalent traffic: CSR
+    # streams ~8 bytes for each of ~90 entries per row; the stencil form reads ~60 bytes per binned
+    # point and spends the equivalent of ~150 bytes per row on bitmap and gather work.
+    STENCIL_BYTES_PER_POINT = 60.0
+    STENCIL_BYTES_PER_ROW = 150.0
+    CSR_BYTES_PER_ROW = 720.0
+
+    def operator_form(self, M, n_points):
+        if self.operator in ("csr", "stencil"):
+            return self.operator
+        stencil = self.STENCIL_BYTES_PER_POINT * n_points + self.STENCIL_BYTES_PER_ROW * M
+        return "stencil" if stencil < self.CSR_BYTES_PER_ROW * M else "csr"
+
+    def assemble_level(self, d, bins, coarse, sf, delta, rhs, form=None):
+        """Level-d system (reconstructor.py:143-183); the screening RHS terms are added to rhs.
+        form: "csr" (assembled, ngs_lhs_assemble), "stencil" (ngs_stencil_*), None = operator_form."""
         tree = self.hash_tree
         lv = tree.level(d)
         M = lv.M
+        if form is None:
+            form = self.operator_form(M, bins.n)
+        if form == "stencil":
+            system = _solver.StencilSystem(lv, bins, sf)
+            system.diag_rhs(coarse, delta, rhs)
+            return system
         row_nnz = torch.empty(M, dtype=torch.int32, device=tree.device)
         call("ngs_lhs_row_counts", lv, ptr(row_nnz), stream())
         rowptr = torch.empty(M + 1, dtype=torch.int32, device=tree.device)
@@ -238,13 +260,19 @@ class Reconstructor:
                   for o, dm in [(tree.grids[d].origin[a], tree.grids[d].dims[a])]) for a in range(3)]
         cg = SparseGrid(lo, hi, 1, dev)
         for d in range(L):
-            if primal[d].shape[0]:
-                call("ngs_mark_dual_centres", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d], stream())
+            if leaves[d].shape[0]:
+                call("ngs_mark_dual_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], tree.strides[d], nu,
+                     stream())
         DC = cg.finalize()
         acc = torch.zeros((DC, 8, 2), dtype=torch.int32, device=dev)
         cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
         for d in range(L):
-            if primal[d].shape[0]:
+            if not primal[d].shape[0]:
+                continue
+            if tree.strides[d] == 1:
+                call("ngs_dual_accumulate_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], nu, d, bases_[d],
+                     ptr(acc), ptr(cnt), stream())
+            else:
                 call("ngs_dual_accumulate", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d], d,
                      bases_[d], ptr(acc), ptr(cnt), stream())
         cubes = torch.empty((DC, 8), dtype=torch.int32, device=dev)

@@ -95,11 +95,7 @@ class BatchedPCG:
         for i, job in enumerate(self.active):
             blk0 = blk
             hm[blk0:blk0 + job.nblk] = i
-            job_entry = self.host_table[i * self.eb:(i + 1) * self.eb]
-            # patch the block range of the prepared entry (two int32 after the PcgArgs)
-            e = job_entry.numpy().view(np.int32)
-            e[-4] = blk0
-            e[-3] = job.nblk
+            call("ngs_pcg_batch_place", self.host_table[i * self.eb:].data_ptr(), blk0)
             blk = blk0 + ((job.nblk + 7) // 8) * 8
         return blk
 
@@ -120,9 +116,15 @@ class BatchedPCG:
             slot = len(self.active)
             entry = self.host_table[slot * self.eb:(slot + 1) * self.eb]
             s = job.system
-            call("ngs_pcg_batch_prepare", ptr(s.rowptr), ptr(s.cols), ptr(s.vals), ptr(s.inv_diag), ptr(job.b),
-                 ptr(job.x), s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
-                 self.stream.cuda_stream)
+            if isinstance(s, _solver.StencilSystem):
+                # one system's point scratch (s.u) serves one solve at a time
+                call("ngs_pcg_batch_prepare_stencil", s.struct(), ptr(s.inv_diag), ptr(job.b), ptr(job.x),
+                     float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
+                     self.stream.cuda_stream)
+            else:
+                call("ngs_pcg_batch_prepare", ptr(s.rowptr), ptr(s.cols), ptr(s.vals), ptr(s.inv_diag), ptr(job.b),
+                     ptr(job.x), s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
+                     self.stream.cuda_stream)
             self.active.append(job)
             admitted.append(job)
         return admitted
@@ -144,8 +146,9 @@ class BatchedPCG:
                         self.dev_table[:n * self.eb].copy_(self.host_table[:n * self.eb], non_blocking=True)
                         self.dev_map.copy_(self.host_map, non_blocking=True)
                         self._dirty = False
+                    flags = int(any(isinstance(j.system, _solver.StencilSystem) for j in self.active))
                     call("ngs_pcg_batch_iterate", ptr(self.dev_table), ptr(self.dev_map), self.cap, self.chunk,
-                         int(self.use_graph), self.stream.cuda_stream)
+                         int(self.use_graph), flags, self.stream.cuda_stream)
                     self.launches += 1
                     self.iterations += self.chunk
                     n = len(self.active)

@@ -76,6 +76,116 @@ class CSRSystem:
         return a
 
 
+class StencilSystem:
+    """Level system in stencil form: the operator of ngs_lhs_assemble applied from the level bitmap,
+    the 125 Laplacian coefficients of the level and the screening points binned by cell, without
+    stored entries (per-iteration traffic a few bytes per row instead of 8 per entry). Same
+    interface as CSRSystem; to_csr() assembles the identical operator as CSR on demand."""
+
+    def __init__(self, level, bins, sf: float):
+        from ._lib import PointBins, Stencil
+        self.device = bins.cells.device if bins.n else level.coords.device
+        self.level = level            # ctypes Level (its tensors are owned by the tree)
+        self.bins = bins
+        self.sf = float(sf)
+        self.M = int(level.M)
+        dev = self.device
+        self.lap = torch.empty(125, dtype=torch.float32, device=dev)
+        call("ngs_stencil_laplacian", int(level.stride), ptr(self.lap), stream())
+        n = bins.n
+        self.n = n
+        self.perm = bins.perm
+        if n:
+            self.point_cell = torch.empty((n, 3), dtype=torch.int32, device=dev)
+            self.basis = torch.empty((n, 9), dtype=torch.float32, device=dev)
+            self.alpha = torch.empty(n, dtype=torch.float32, device=dev) if bins.alpha is not None else None
+            call("ngs_stencil_bin_points", bins.struct(), ptr(bins.cell), ptr(self.point_cell), ptr(self.basis),
+                 ptr(self.alpha), stream())
+            self.u = torch.empty(n, dtype=torch.float32, device=dev)
+        else:
+            self.point_cell = self.basis = self.alpha = self.u = None
+        self.inv_diag = torch.empty(self.M, dtype=torch.float32, device=dev)
+        st = Stencil()
+        st.level = level
+        st.cells = bins.cells.struct()
+        st.cell_start = bins.cell_start.data_ptr()
+        st.point_cell = None if self.point_cell is None else self.point_cell.data_ptr()
+        st.basis = None if self.basis is None else self.basis.data_ptr()
+        st.alpha = None if self.alpha is None else self.alpha.data_ptr()
+        st.perm = None if self.perm is None else self.perm.data_ptr()
+        st.lap = self.lap.data_ptr()
+        st.u = None if self.u is None else self.u.data_ptr()
+        st.alpha_scalar = float(bins.alpha_scalar)
+        st.sf = self.sf
+        st.n = n
+        self._struct = st
+
+    def struct(self):
+        return self._struct
+
+    def diag_rhs(self, coarse, delta, rhs):
+        """inv_diag and the screening RHS terms (ngs_stencil_diag_rhs)."""
+        if self.M:
+            call("ngs_stencil_diag_rhs", self._struct, ptr(coarse), float(delta), ptr(self.inv_diag), ptr(rhs),
+                 stream())
+
+    def nnz(self):
+        """Entries of the equivalent CSR (existing neighbours in the 125-stencil of each row)."""
+        if not self.M:
+            return 0
+        cnt = torch.empty(self.M, dtype=torch.int32, device=self.device)
+        call("ngs_stencil_row_counts", self.level, ptr(cnt), stream())
+        return int(cnt.sum(dtype=torch.int64).item())
+
+    def matvec(self, x):
+        y = torch.empty(self.M, dtype=torch.float32, device=self.device)
+        if self.M:
+            call("ngs_stencil_apply", self._struct, ptr(x.float().contiguous()), ptr(y), stream())
+        return y
+
+    def residual_f64(self, b64, x64):
+        """(r = b - A x in FP64, ||r||^2, ||b||^2), the operator evaluated in FP64."""
+        r = torch.empty(self.M, dtype=torch.float64, device=self.device)
+        norms = torch.empty(2, dtype=torch.float64, device=self.device)
+        ws = _lib.Scratch.get(lib().ngs_pcg_workspace_bytes(self.M), self.device, slot=1)
+        u64 = _lib.Scratch.get(8 * max(self.n, 1), self.device, slot=4)
+        call("ngs_stencil_residual_f64", self._struct, ptr(b64.contiguous()), ptr(x64.contiguous()), ptr(r), ptr(u64),
+             ptr(norms), ptr(ws), stream())
+        nv = norms.cpu()
+        return r, float(nv[0]), float(nv[1])
+
+    def to_csr(self):
+        """The same operator assembled as CSR (ngs_lhs_assemble over the binned points)."""
+        from ._lib import PointBins
+        M, dev = self.M, self.device
+        row_nnz = torch.empty(M, dtype=torch.int32, device=dev)
+        call("ngs_lhs_row_counts", self.level, ptr(row_nnz), stream())
+        rowptr = torch.empty(M + 1, dtype=torch.int32, device=dev)
+        ws = _lib.Scratch.get(lib().ngs_scan_workspace_bytes(M), dev)
+        call("ngs_scan_i32", ptr(row_nnz), M, ptr(rowptr), ptr(ws), stream())
+        nnz = int(rowptr[M].item())
+        cols = torch.empty(nnz, dtype=torch.int32, device=dev)
+        vals = torch.empty(nnz, dtype=torch.float32, device=dev)
+        inv_diag = torch.empty(M, dtype=torch.float32, device=dev)
+        rhs = torch.zeros(M, dtype=torch.float64, device=dev)
+        pb = PointBins()
+        pb.cells = self._struct.cells
+        pb.cell_start = self._struct.cell_start
+        ident = torch.arange(self.n, dtype=torch.int32, device=dev) if self.n else None
+        pb.perm = None if ident is None else ident.data_ptr()
+        pb.b = self._struct.basis
+        pb.alpha = self._struct.alpha
+        pb.alpha_scalar = self._struct.alpha_scalar
+        pb.n = self.n
+        call("ngs_lhs_assemble", self.level, pb, None, self.sf, 0.0, ptr(rowptr), ptr(cols), ptr(vals), ptr(inv_diag),
+             ptr(rhs), stream())
+        torch.cuda.current_stream().synchronize()
+        return CSRSystem(rowptr, cols, vals, inv_diag)
+
+    def to_scipy(self):
+        return self.to_csr().to_scipy()
+
+
 class SolveInfo:
     def __init__(self, iters=0, rel_resid=0.0, outer=0, true_rel_resid=None):
         self.iters, self.rel_resid, self.outer, self.true_rel_resid = iters, rel_resid, outer, true_rel_resid
@@ -85,7 +195,7 @@ class SolveInfo:
                 f"true_rel_resid={self.true_rel_resid})")
 
 
-def pcg(system: CSRSystem, b, tol=1e-5, maxiter=-1, chunk=32, x_out=None, raise_on_cap=True):
+def pcg(system, b, tol=1e-5, maxiter=-1, chunk=32, x_out=None, raise_on_cap=True):
     """FP32 Jacobi-PCG from x = 0. Returns (x, SolveInfo)."""
     M = system.M
     x = x_out if x_out is not None else torch.empty(M, dtype=torch.float32, device=system.device)
@@ -95,15 +205,20 @@ def pcg(system: CSRSystem, b, tol=1e-5, maxiter=-1, chunk=32, x_out=None, raise_
     it = ctypes.c_int32()
     rr = ctypes.c_double()
     b = b.float().contiguous()
-    st = lib().ngs_pcg_solve(ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(system.inv_diag), ptr(b),
-                             ptr(x), M, float(tol), int(maxiter), int(chunk), ptr(ws), stream(), ctypes.byref(it),
-                             ctypes.byref(rr))
+    if isinstance(system, StencilSystem):
+        st = lib().ngs_pcg_solve_stencil(system.struct(), ptr(system.inv_diag), ptr(b), ptr(x), float(tol),
+                                         int(maxiter), int(chunk), ptr(ws), stream(), ctypes.byref(it),
+                                         ctypes.byref(rr))
+    else:
+        st = lib().ngs_pcg_solve(ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(system.inv_diag),
+                                 ptr(b), ptr(x), M, float(tol), int(maxiter), int(chunk), ptr(ws), stream(),
+                                 ctypes.byref(it), ctypes.byref(rr))
     if st != 0 and not (st == 5 and not raise_on_cap):
         raise NgsError(st, lib().ngs_last_error().decode())
     return x, SolveInfo(it.value, rr.value)
 
 
-def mixed(system: CSRSystem, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, chunk=32):
+def mixed(system, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, chunk=32):
     """FP64 iterative refinement with FP32 PCG inner solves. Returns (x64, SolveInfo)."""
     M = system.M
     b64 = b.double().contiguous()
@@ -126,7 +241,7 @@ def mixed(system: CSRSystem, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=
     return x64, info
 
 
-def solve(system: CSRSystem, b, spec: str = "pcg"):
+def solve(system, b, spec: str = "pcg"):
     """Dispatch on a solver string. Returns (x float32, SolveInfo)."""
     name, conf = parse_solver(spec)
     if name == "pcg":

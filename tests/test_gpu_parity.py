@@ -15,21 +15,23 @@ def _scene(kind, n, seed):
     return synthetic.sphere(n, seed=seed)
 
 
-CASES = [("torus", 4000, 1 / 32, 3, 8.0), ("ellipsoids", 6000, 1 / 64, 4, 8.0), ("sphere", 3000, 1 / 32, 3, 32.0)]
+# (scene, points, voxel size, depth, min_density, level operator form)
+CASES = [("torus", 4000, 1 / 32, 3, 8.0, "csr"), ("torus", 4000, 1 / 32, 3, 8.0, "stencil"),
+         ("ellipsoids", 6000, 1 / 64, 4, 8.0, "stencil"), ("sphere", 3000, 1 / 32, 3, 32.0, "auto")]
 
 
-@pytest.fixture(scope="module", params=CASES, ids=lambda c: f"{c[0]}-{c[1]}-d{c[3]}")
+@pytest.fixture(scope="module", params=CASES, ids=lambda c: f"{c[0]}-{c[1]}-d{c[3]}-{c[5]}")
 def pair(request, cuda):
     """GPU reconstruction and oracle on the same input; the oracle's multigrid is fed the GPU
     solutions of the coarser levels so that per-level systems are compared on equal inputs."""
     import ngspsr
     from oracle import assembly, hierarchy, mesh
-    kind, n, vs, depth, mind = request.param
-    print(f"[case] {kind} n={n} vs={vs} depth={depth} min_density={mind}", flush=True)
+    kind, n, vs, depth, mind, form = request.param
+    print(f"[case] {kind} n={n} vs={vs} depth={depth} min_density={mind} operator={form}", flush=True)
     xyz, nrm = _scene(kind, n, seed=7)
     res = ngspsr.reconstruct(torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda), vs, depth=depth,
                              min_density=mind, screen_alpha=4.0, solver="pcg@tol=1e-7@maxiter=20000",
-                             return_result=True)
+                             operator=form, return_result=True)
     torch.cuda.synchronize()
     gpu_depth = res.tree.xyz_depth.cpu().numpy()
     h = hierarchy.Hierarchy(depth, vs)
@@ -168,3 +170,36 @@ def test_mesh_close_to_surface(pair):
     from scipy.spatial import cKDTree
     d, _ = cKDTree(xyz).query(v)
     assert np.median(d) < pair["vs"], np.median(d)
+
+
+@pytest.mark.parametrize("nu", [1, 2, 3])
+def test_dual_leaf_forms_equal_per_cell(pair, nu):
+    """The leaf forms of dual-centre marking and stride-1 dual-cube assembly give the same bitmap,
+    counts and complete cubes as the per-sub-cell forms."""
+    from ngspsr._lib import call, ptr, stream
+    from ngspsr.grid import SparseGrid
+    res = pair["res"]
+    _, _, info = res.reconstructor.extract_multiscale_mesh(n_upsample=nu, return_info=True)
+    tree = res.tree
+    cg_leaf = info["centres"]
+    leaves, primal = info["leaves"], info["primal"]
+    dev = cg_leaf.device
+    lo = [int(cg_leaf[:, a].min()) - 1 for a in range(3)]
+    hi = [int(cg_leaf[:, a].max()) + 2 for a in range(3)]
+    cg = SparseGrid(lo, hi, 1, dev)
+    for d, pr in primal.items():
+        if pr.shape[0]:
+            call("ngs_mark_dual_centres", cg.struct(), ptr(pr), pr.shape[0], tree.strides[d], stream())
+    DC = cg.finalize()
+    assert torch.equal(cg.coords, cg_leaf)
+    acc = torch.zeros((DC, 8, 2), dtype=torch.int32, device=dev)
+    cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
+    base = 0
+    for d, pr in primal.items():
+        if pr.shape[0]:
+            call("ngs_dual_accumulate", cg.struct(), ptr(pr), pr.shape[0], tree.strides[d], d, base, ptr(acc),
+                 ptr(cnt), stream())
+        base += pr.shape[0]
+    assert torch.equal(cnt, info["count"])
+    full = cnt == 8
+    assert torch.equal(acc[full], info["acc"][full])

@@ -1,0 +1,142 @@
+"""Stencil form of the level systems against the assembled CSR form (MI355X).
+
+The stencil operator is the CSR operator re-associated: the diagonal, the screening RHS terms and
+the materialised matrix are bit-identical; products agree to FP32 reordering; solves agree to the
+solver tolerance."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(cuda, per_point_alpha, seed=3):
+    import ngspsr
+    from ngspsr import synthetic
+    from ngspsr.reconstructor import PointBinning
+    xyz, nrm = synthetic.ellipsoids(20000, k=4, sigma=0.002, seed=seed)
+    xyz_t, nrm_t = torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda)
+    res = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 64, depth=4, min_density=2.0, solver="pcg@tol=1e-7@maxiter=20000",
+                             operator="csr", return_result=True)
+    tree, recon = res.tree, res.reconstructor
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    alpha = (torch.rand(xyz.shape[0], generator=g) * 8.0).to(cuda) if per_point_alpha else 4.0
+    bins = {d: PointBinning(tree, xyz_t, d, alpha) for d in range(tree.depth)}
+    return tree, recon, bins, xyz_t
+
+
+@pytest.mark.parametrize("per_point_alpha", [False, True])
+def test_stencil_equals_csr_operator(cuda, per_point_alpha):
+    print(f"[case] stencil vs csr operator alpha={'point' if per_point_alpha else 'scalar'}", flush=True)
+    tree, recon, bins, xyz_t = _setup(cuda, per_point_alpha)
+    g = torch.Generator(device="cpu").manual_seed(11)
+    for d in range(tree.depth):
+        M = tree.get_coords_size(None, d)
+        if M == 0:
+            continue
+        sf = 0.25 ** d
+        coarse = (torch.randn(xyz_t.shape[0], generator=g) * 0.1).to(cuda) if d < tree.depth - 1 else None
+        base = torch.randn(M, generator=g, dtype=torch.float64).to(cuda)
+        rhs_c, rhs_s = base.clone(), base.clone()
+        A = recon.assemble_level(d, bins[d], coarse, sf, 0.1, rhs_c, form="csr")
+        S = recon.assemble_level(d, bins[d], coarse, sf, 0.1, rhs_s, form="stencil")
+        torch.cuda.synchronize()
+        assert torch.equal(A.inv_diag, S.inv_diag), d
+        assert torch.equal(rhs_c, rhs_s), d
+        B = S.to_csr()
+        assert torch.equal(A.rowptr, B.rowptr) and torch.equal(A.cols, B.cols) and torch.equal(A.vals, B.vals), d
+        assert S.nnz() == A.nnz(), d
+        x = torch.randn(M, generator=g).to(cuda)
+        yc, ys = A.matvec(x), S.matvec(x)
+        # FP32 reassociation: bounded by a few ulps of sum_j |A_ij x_j|
+        absA = type(A)(A.rowptr, A.cols, A.vals.abs(), A.inv_diag)
+        bound = absA.matvec(x.abs())
+        assert bool(((yc - ys).abs() <= 1e-5 * bound + 1e-30).all()), (d, float(((yc - ys).abs() / bound).max()))
+        x64 = x.double()
+        b64 = torch.randn(M, generator=g, dtype=torch.float64).to(cuda)
+        rc, rrc, bbc = A.residual_f64(b64, x64)
+        rs, rrs, bbs = S.residual_f64(b64, x64)
+        assert bbc == bbs
+        # the FP64 stencil operator differs from the FP32-rounded CSR entries by < 2^-23 |A| |x|
+        bound64 = absA.matvec(x.abs()).double()
+        assert bool(((rc - rs).abs() <= 2e-7 * bound64 + 1e-300).all()), d
+
+
+def test_stencil_solve_matches_csr_solve(cuda):
+    from ngspsr import solver
+    print("[case] stencil vs csr solve", flush=True)
+    tree, recon, bins, xyz_t = _setup(cuda, False, seed=5)
+    for d in range(tree.depth):
+        M = tree.get_coords_size(None, d)
+        if M == 0:
+            continue
+        rhs_c = torch.zeros(M, dtype=torch.float64, device=cuda)
+        rhs_s = torch.zeros(M, dtype=torch.float64, device=cuda)
+        A = recon.assemble_level(d, bins[d], None, 0.25 ** d, 0.1, rhs_c, form="csr")
+        S = recon.assemble_level(d, bins[d], None, 0.25 ** d, 0.1, rhs_s, form="stencil")
+        b = (rhs_c + torch.randn(M, dtype=torch.float64, device=cuda)).float()
+        xc, ic = solver.pcg(A, b, tol=1e-7, maxiter=20000)
+        xs, is_ = solver.pcg(S, b, tol=1e-7, maxiter=20000)
+        assert abs(ic.iters - is_.iters) <= max(2, ic.iters // 20), (d, ic.iters, is_.iters)
+        assert is_.rel_resid <= 1e-7 * 1.0001
+        # both satisfy the FP64 operator
+        _, rr, bb = S.residual_f64(b.double(), xs.double())
+        assert (rr / bb) ** 0.5 < 2e-6, d
+        err = float((xs - xc).norm() / xc.norm())
+        assert err < 1e-4, (d, err)
+        x64, info = solver.mixed(S, b, tol=1e-10, inner_tol=1e-4)
+        assert info.true_rel_resid <= 1e-10, (d, info)
+
+
+def test_pool_mixes_stencil_and_csr(cuda):
+    """Batched pool with CSR and stencil systems together: each result equals its single solve."""
+    import threading
+    from ngspsr import serving, solver
+    print("[case] pool stencil + csr", flush=True)
+    tree, recon, bins, xyz_t = _setup(cuda, False, seed=9)
+    systems = []
+    for d in range(2):
+        M = tree.get_coords_size(None, d)
+        for form in ("csr", "stencil"):
+            rhs = torch.zeros(M, dtype=torch.float64, device=cuda)
+            systems.append((recon.assemble_level(d, bins[d], None, 0.25 ** d, 0.1, rhs, form=form), rhs.float()))
+    torch.cuda.synchronize()
+    pool = serving.BatchedPCG(cuda, chunk=4)
+    outs = [None] * len(systems)
+    try:
+        def work(i):
+            s, b = systems[i]
+            outs[i] = pool.solve(s, b, tol=1e-6)
+        th = [threading.Thread(target=work, args=(i,)) for i in range(len(systems))]
+        [t.start() for t in th]
+        [t.join() for t in th]
+    finally:
+        pool.close()
+    for (s, b), (xb, ib) in zip(systems, outs):
+        xs, info = solver.pcg(s, b, tol=1e-6)
+        torch.cuda.synchronize()
+        assert ib.iters == info.iters
+        assert torch.equal(xb, xs), float((xb - xs).abs().max())
+
+
+@pytest.mark.parametrize("operator", ["csr", "stencil"])
+def test_reconstruct_operator_forms_agree(cuda, operator):
+    import ngspsr
+    from ngspsr import synthetic
+    print(f"[case] reconstruct operator={operator}", flush=True)
+    xyz, nrm = synthetic.torus(30000, seed=2)
+    xyz_t, nrm_t = torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda)
+    ref = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 64, depth=4, min_density=4.0, solver="mixed@tol=1e-9",
+                             operator="csr", return_result=True)
+    got = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 64, depth=4, min_density=4.0, solver="mixed@tol=1e-9",
+                             operator=operator, return_result=True)
+    for d, x in ref.reconstructor.solutions.items():
+        y = got.reconstructor.solutions[d]
+        err = float((x - y).norm() / max(float(x.norm()), 1e-30))
+        assert err < 1e-6, (d, err)
+    v0, f0 = ref.vertices.cpu().numpy(), ref.triangles.cpu().numpy()
+    v1, f1 = got.vertices.cpu().numpy(), got.triangles.cpu().numpy()
+    assert abs(v0.shape[0] - v1.shape[0]) <= max(4, v0.shape[0] // 1000)
+    from scipy.spatial import cKDTree
+    dist, _ = cKDTree(v0).query(v1)
+    assert float(np.quantile(dist, 0.99)) < 1e-4
