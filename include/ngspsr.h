@@ -289,7 +289,7 @@ int ngs_pcg_solve_stencil(ngs_stencil op, const float *inv_diag, const float *b,
                           double *rel_resid);
 
 /* Solver knobs: use_graph (capture iteration chunks in a hipGraph; default 1), lanes per CSR row
- * in the SpMV (8, 16, 32; 0 = 16) and non-temporal loads of the matrix stream (default 1). */
+ * in the SpMV (8, 16, 32; 0 = 8) and non-temporal loads of the matrix stream (default 0). */
 int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal);
 
 /* Batched solves (continuous batching of independent level systems; no reference counterpart —

@@ -65,17 +65,29 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                     float by[3] = {bp[3], bp[4], bp[5]};
                     float bz[3] = {bp[6], bp[7], bp[8]};
                     const float a = HAS_ALPHA ? pts.alpha[p] : pts.alpha_scalar;
-                    const float w = a * (bx[1 - ox] * by[1 - oy] * bz[1 - oz]);
+                    // products rounded on their own, sums by explicit FMA: the same arithmetic as
+                    // stencil_diag_rhs_kernel, whatever the contraction choices around it
+                    float w;
+                    {
+#pragma clang fp contract(off)
+                        w = a * (bx[1 - ox] * by[1 - oy] * bz[1 - oz]);
+                    }
                     rs += w;
-                    if (HAS_COARSE) rc += w * coarse[p];
+                    if (HAS_COARSE) rc = fmaf(w, coarse[p], rc);
 #pragma unroll
                     for (int vx = -1; vx <= 1; ++vx) {
 #pragma unroll
                         for (int vy = -1; vy <= 1; ++vy) {
-                            const float wxy = w * (bx[vx + 1] * by[vy + 1]);
+                            float wxy;
+                            {
+#pragma clang fp contract(off)
+                                wxy = w * (bx[vx + 1] * by[vy + 1]);
+                            }
 #pragma unroll
-                            for (int vz = -1; vz <= 1; ++vz)
-                                acc[(ox + vx + 2) * 25 + (oy + vy + 2) * 5 + (oz + vz + 2)] += wxy * bz[vz + 1];
+                            for (int vz = -1; vz <= 1; ++vz) {
+                                float& e = acc[(ox + vx + 2) * 25 + (oy + vy + 2) * 5 + (oz + vz + 2)];
+                                e = fmaf(wxy, bz[vz + 1], e);
+                            }
                         }
                     }
                 }
@@ -169,11 +181,15 @@ __global__ void __launch_bounds__(256) stencil_diag_rhs_kernel(ngs_stencil op, c
                         const float* bp = op.basis + 9 * (int64_t)k;
                         const float bx = bp[1 - ox], by = bp[4 - oy], bz = bp[7 - oz];
                         const float a = op.alpha ? op.alpha[k] : op.alpha_scalar;
-                        const float w = a * (bx * by * bz);
+                        float w, wxy;
+                        {
+#pragma clang fp contract(off)
+                            w = a * (bx * by * bz);
+                            wxy = w * (bx * by);
+                        }
                         rs += w;
-                        if (coarse) rc += w * coarse[op.perm[k]];
-                        const float wxy = w * (bx * by);
-                        dd += wxy * bz;
+                        if (coarse) rc = fmaf(w, coarse[op.perm[k]], rc);
+                        dd = fmaf(wxy, bz, dd);
                     }
                 }
             }

@@ -96,6 +96,26 @@ __device__ __forceinline__ void grid_mark(const ngs_grid& g, int cx, int cy, int
     if (grid_cell(g, cx, cy, cz, &lin)) atomicOr(&g.bits[lin >> 5], 1u << (lin & 31));
 }
 
+// Presence mask of the n (<= 8) consecutive lattice points (lx, ly, lz0 + k), k < n, of g (lattice
+// coordinates relative to the origin), and in *r0 the rank of the first present one. Present
+// points have consecutive ranks: point k has rank r0 + popc(mask & ((1 << k) - 1)).
+__device__ __forceinline__ uint32_t grid_column(const ngs_grid& g, int lx, int ly, int lz0, int n, int32_t* r0) {
+    if ((unsigned)lx >= (unsigned)g.dims[0] || (unsigned)ly >= (unsigned)g.dims[1]) return 0u;
+    const int zlo = max(lz0, 0), zhi = min(lz0 + n, g.dims[2]);
+    if (zlo >= zhi) return 0u;
+    const int64_t lin = grid_linear(g, lx, ly, zlo);
+    const int64_t w = lin >> 5;
+    const int b = (int)(lin & 31);
+    const int len = zhi - zlo;
+    const uint32_t w0 = g.bits[w];
+    uint64_t v = w0;
+    // the window's last bit is a lattice point of the box, so word w + 1 exists when it is needed
+    if (b + len > 32) v |= (uint64_t)g.bits[w + 1] << 32;
+    const uint32_t m = (uint32_t)(v >> b) & ((1u << len) - 1u);
+    *r0 = g.rank[w] + __popc(w0 & ((1u << b) - 1u));
+    return m << (zlo - lz0);
+}
+
 // ----------------------------------------------------------------------------------- FNV keys
 // fold60(FNV-1a-64 over the uint32 words of a coordinate row); csrc/common/hashmap_cuda.cu:248-261.
 __device__ __forceinline__ uint64_t fnv_fold60(const int32_t* c, int dim) {

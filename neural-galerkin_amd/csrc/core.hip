@@ -442,12 +442,17 @@ __global__ void eval_field_kernel(ngs_grid g, const float* __restrict__ field, c
 #pragma unroll
     for (int a = 0; a < 3; ++a) axis_basis_grad(xyz[3 * i + a] * inv_vs, s, inv_s, &cell[a], b[a], db[a]);
     float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    const int l2 = g.log2_stride;
+    const int lx = (cell[0] >> l2) - g.origin[0], ly = (cell[1] >> l2) - g.origin[1], lz = (cell[2] >> l2) - g.origin[2];
+    // the 27 voxels around the cell in (kx, ky, kz) order, one bitmap column per (kx, ky)
     for (int kx = 0; kx < 3; ++kx)
-        for (int ky = 0; ky < 3; ++ky)
+        for (int ky = 0; ky < 3; ++ky) {
+            int32_t id;
+            const uint32_t m = grid_column(g, lx + kx - 1, ly + ky - 1, lz - 1, 3, &id);
+            if (!m) continue;
             for (int kz = 0; kz < 3; ++kz) {
-                int id = grid_find(g, cell[0] + (kx - 1) * s, cell[1] + (ky - 1) * s, cell[2] + (kz - 1) * s);
-                if (id < 0) continue;
-                float f = field[id];
+                if (!((m >> kz) & 1u)) continue;
+                float f = field[id++];
                 v += (b[0][kx] * b[1][ky] * b[2][kz]) * f;
                 if (grad) {
                     gx += (db[0][kx] * b[1][ky] * b[2][kz]) * f;
@@ -455,6 +460,7 @@ __global__ void eval_field_kernel(ngs_grid g, const float* __restrict__ field, c
                     gz += (b[0][kx] * b[1][ky] * db[2][kz]) * f;
                 }
             }
+        }
     out[i] = (accumulate ? out[i] : 0.f) + v * scale;
     if (grad) {
         float* gp = grad + 3 * i;

@@ -438,8 +438,8 @@ __global__ void fill_map_kernel(int32_t* map, int n) {
 }
 
 // SpMV variant: lanes per row (8, 16, 32) and non-temporal matrix loads; set by ngs_pcg_config.
-static int g_spmv_lanes = 16;
-static int g_spmv_nt = 1;
+static int g_spmv_lanes = 8;
+static int g_spmv_nt = 0;
 
 template <class F>
 static void dispatch_spmv(F&& f) {
@@ -483,7 +483,7 @@ int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal
                 "ngs_pcg_config: lanes_per_row must be 0, 8, 16 or 32");
     g_pcg_use_graph = use_graph ? 1 : 0;
     g_pcg_group = lanes_per_row;
-    g_spmv_lanes = lanes_per_row ? lanes_per_row : 16;
+    g_spmv_lanes = lanes_per_row ? lanes_per_row : 8;
     g_spmv_nt = nontemporal ? 1 : 0;
     return NGS_OK;
 }
@@ -738,11 +738,14 @@ int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t tot
     }
     // Graphs depend only on (table, map, blocks, iterations, stream). The cache is per thread (each
     // thread launches on its own streams), bounded, and evicts its oldest entry.
-    struct Key { const void* t; const int32_t* m; int nb, it, fl; hipStream_t s; hipGraphExec_t e; };
+    // the SpMV variant is part of the key: a graph records the kernels of the variant it was captured with
+    struct Key { const void* t; const int32_t* m; int nb, it, fl, var; hipStream_t s; hipGraphExec_t e; };
+    const int var = g_spmv_lanes * 2 + g_spmv_nt;
     static thread_local std::vector<Key> cache;
     hipGraphExec_t exec = nullptr;
     for (auto& k : cache)
-        if (k.t == table && k.m == blk2sys && k.nb == total_blocks && k.it == iters && k.fl == flags && k.s == st)
+        if (k.t == table && k.m == blk2sys && k.nb == total_blocks && k.it == iters && k.fl == flags && k.var == var &&
+            k.s == st)
             exec = k.e;
     if (!exec) {
         hipGraph_t graph;
@@ -755,7 +758,7 @@ int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t tot
             (void)hipGraphExecDestroy(cache.front().e);
             cache.erase(cache.begin());
         }
-        cache.push_back({table, blk2sys, total_blocks, iters, flags, st, exec});
+        cache.push_back({table, blk2sys, total_blocks, iters, flags, var, st, exec});
     }
     NGS_CHECK_HIP(hipGraphLaunch(exec, st));
     return NGS_OK;

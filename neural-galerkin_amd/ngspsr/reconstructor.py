@@ -96,7 +96,7 @@ class Reconstructor:
         if form is None:
             form = self.operator_form(M, bins.n)
         if form == "stencil":
-            system = _solver.StencilSystem(lv, bins, sf)
+            system = _solver.StencilSystem(tree.grids[d], tree.strides[d], bins, sf)
             system.diag_rhs(coarse, delta, rhs)
             return system
         row_nnz = torch.empty(M, dtype=torch.int32, device=tree.device)

@@ -82,10 +82,18 @@ class StencilSystem:
     stored entries (per-iteration traffic a few bytes per row instead of 8 per entry). Same
     interface as CSRSystem; to_csr() assembles the identical operator as CSR on demand."""
 
-    def __init__(self, level, bins, sf: float):
-        from ._lib import PointBins, Stencil
-        self.device = bins.cells.device if bins.n else level.coords.device
-        self.level = level            # ctypes Level (its tensors are owned by the tree)
+    def __init__(self, grid, stride: int, bins, sf: float):
+        """grid: the level's SparseGrid (voxel bitmap, rank and coordinates), held by the system so
+        that the raw pointers of the kernel arguments stay valid for the system's lifetime."""
+        from ._lib import Level, Stencil
+        self.grid = grid
+        level = Level()
+        level.grid = grid.struct()
+        level.coords = grid.coords.data_ptr() if grid.count else None
+        level.M = grid.count
+        level.stride = int(stride)
+        self.device = grid.device
+        self.level = level
         self.bins = bins
         self.sf = float(sf)
         self.M = int(level.M)

@@ -57,9 +57,10 @@ def test_stencil_equals_csr_operator(cuda, per_point_alpha):
         rc, rrc, bbc = A.residual_f64(b64, x64)
         rs, rrs, bbs = S.residual_f64(b64, x64)
         assert bbc == bbs
-        # the FP64 stencil operator differs from the FP32-rounded CSR entries by < 2^-23 |A| |x|
+        # the CSR entries are FP32 sums over up to ~100 point terms (error up to ~100 ulp of the
+        # entry); the FP64 stencil operator evaluates the same terms exactly
         bound64 = absA.matvec(x.abs()).double()
-        assert bool(((rc - rs).abs() <= 2e-7 * bound64 + 1e-300).all()), d
+        assert bool(((rc - rs).abs() <= 1e-5 * bound64 + 1e-300).all()), d
 
 
 def test_stencil_solve_matches_csr_solve(cuda):
@@ -130,10 +131,13 @@ def test_reconstruct_operator_forms_agree(cuda, operator):
                              operator="csr", return_result=True)
     got = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 64, depth=4, min_density=4.0, solver="mixed@tol=1e-9",
                              operator=operator, return_result=True)
+    # the mixed solver converges each form to its own FP64 operator: the CSR one uses the assembled
+    # FP32 entries (FP32 sums over the points), the stencil one the exact terms; the two operators
+    # differ by ~1e-6 relative, so the solutions agree to ~cond x 1e-6, not to the 1e-9 tolerance
     for d, x in ref.reconstructor.solutions.items():
         y = got.reconstructor.solutions[d]
         err = float((x - y).norm() / max(float(x.norm()), 1e-30))
-        assert err < 1e-6, (d, err)
+        assert err < 2e-5, (d, err)
     v0, f0 = ref.vertices.cpu().numpy(), ref.triangles.cpu().numpy()
     v1, f1 = got.vertices.cpu().numpy(), got.triangles.cpu().numpy()
     assert abs(v0.shape[0] - v1.shape[0]) <= max(4, v0.shape[0] // 1000)
