@@ -289,6 +289,7 @@ def main():
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "max_level_rel_residual": rel,
         "pcg_iterations_total": total_iters,
+        "pcg_iterations_by_level": {int(d): int(i.iters) for d, i in sorted(info.items())},
         "pcg_iteration": {"ms": round(1e3 * t_iter, 4), "algorithmic_bytes": int(iter_bytes),
                           "GB/s": round(iter_bytes / t_iter / 1e9, 1)},
         "roofline": roof,

@@ -252,8 +252,9 @@ int ngs_stencil_apply(ngs_stencil op, const float *x, float *y, ngs_stream_t s);
 /* The passes of ngs_stencil_apply separately, for measurement: passes bit 0 = point pass
  * (op.u from x), bit 1 = row pass (y from x and the current op.u). */
 int ngs_stencil_pass(ngs_stencil op, const float *x, float *y, int32_t passes, ngs_stream_t s);
-/* r = b - A x in FP64 (the operator evaluated in FP64 from the same FP32 data), ||r||^2 and
- * ||b||^2 into norms[0..1]; u64: n doubles of scratch; workspace: ngs_pcg_workspace_bytes(M). */
+/* r = b - A x in FP64 (the operator evaluated in FP64 from the same FP32 data; x NULL = 0),
+ * ||r||^2 and ||b||^2 into norms[0..1]; u64: n doubles of scratch; workspace:
+ * ngs_pcg_workspace_bytes(M). */
 int ngs_stencil_residual_f64(ngs_stencil op, const double *b, const double *x, double *r, double *u64,
                              double *norms, void *workspace, ngs_stream_t s);
 
@@ -322,7 +323,8 @@ int ngs_pcg_batch_status(const void *host_table, int32_t nsys, int32_t *states, 
 int ngs_spmv(const int32_t *rowptr, const int32_t *cols, const float *vals, const float *x, float *y,
              int64_t M, ngs_stream_t s);
 
-/* r = b - A x in FP64 (x FP64), and ||r||^2, ||b||^2 into norms[0..1] (device double). */
+/* r = b - A x in FP64 (x FP64; NULL = 0, no matrix pass), and ||r||^2, ||b||^2 into norms[0..1]
+ * (device double). */
 int ngs_residual_f64(const int32_t *rowptr, const int32_t *cols, const float *vals, const double *b,
                      const double *x, double *r, int64_t M, double *norms, void *workspace, ngs_stream_t s);
 

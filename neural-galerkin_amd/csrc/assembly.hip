@@ -323,6 +323,54 @@ __global__ void rhs_normals_same_kernel(ngs_level tgt, ngs_level dat, const floa
     rhs[i] += acc;
 }
 
+// Data at a finer level (m = s_t / s_d >= 2), gathered per target voxel t: the data voxels whose
+// target cell is t - k s_t (k in [-1,1]^3) are those of the 3m-wide window starting at t - s_t on
+// each axis, and the data voxel at window position j has table row 3m - 1 - j (the row the
+// scatter kernel computes for the same pair). Window columns are read from the data bitmap; the
+// same FP32 terms are summed in FP64 in a fixed order (no atomics).
+__global__ void rhs_normals_fine_gather_kernel(ngs_level tgt, ngs_level dat, const float* __restrict__ nrm, Tables T,
+                                               double* __restrict__ rhs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tgt.M) return;
+    const int sd = dat.stride, st = tgt.stride, m = T.m, w3 = 3 * m;
+    const ngs_grid& g = dat.grid;
+    const int l2 = g.log2_stride;
+    const int lx0 = ((tgt.coords[3 * i] - st) >> l2) - g.origin[0];
+    const int ly0 = ((tgt.coords[3 * i + 1] - st) >> l2) - g.origin[1];
+    const int lz0 = ((tgt.coords[3 * i + 2] - st) >> l2) - g.origin[2];
+    const float fsd = (float)sd, imult = (float)sd / (float)st;
+    double acc = 0.0;
+    for (int jx = 0; jx < w3; ++jx) {
+        const int ix = w3 - 1 - jx;
+        const float px = T.P[ix] * fsd, qx = T.Q[ix] * imult;
+        for (int jy = 0; jy < w3; ++jy) {
+            const int iy = w3 - 1 - jy;
+            const float py = T.P[iy] * fsd, qy = T.Q[iy] * imult;
+            for (int z0 = 0; z0 < w3; z0 += 24) {
+                const int len = min(24, w3 - z0);
+                int32_t r;
+                uint32_t msk = grid_column(g, lx0 + jx, ly0 + jy, lz0 + z0, len, &r);
+                while (msk) {
+                    const int jz = z0 + __ffs(msk) - 1;
+                    msk &= msk - 1u;
+                    const int iz = w3 - 1 - jz;
+                    const float n0 = -nrm[3 * r], n1 = -nrm[3 * r + 1], n2 = -nrm[3 * r + 2];
+                    ++r;
+                    float v;
+                    {
+#pragma clang fp contract(off)
+                        const float c0 = px, c1 = py, c2 = T.P[iz] * fsd;
+                        const float g0 = qx, g1 = qy, g2 = T.Q[iz] * imult;
+                        v = n0 * g0 * c1 * c2 + n1 * c0 * g1 * c2 + n2 * c0 * c1 * g2;
+                    }
+                    acc += (double)v;
+                }
+            }
+        }
+    }
+    rhs[i] += acc;
+}
+
 // Data at a coarser level: gather, per target voxel, the data voxels around the data cell
 // containing its centre, kept when |rel| <= 1.5 target strides (_hash_backend.py:254-262).
 __global__ void rhs_normals_gather_kernel(ngs_level tgt, ngs_level dat, const float* __restrict__ nrm, Tables T,
@@ -550,8 +598,14 @@ int ngs_rhs_normals(ngs_level target, ngs_level data, const float* normals, doub
         NGS_REQUIRE(tables_for(1, &T), "ngs_rhs_normals: table multiple unavailable");
         rhs_normals_same_kernel<<<grid1d(target.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);
     } else if (target.stride > data.stride) {
-        NGS_REQUIRE(tables_for(target.stride / data.stride, &T), "ngs_rhs_normals: table multiple unavailable");
-        rhs_normals_scatter_kernel<<<grid1d(data.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);
+        const int m = target.stride / data.stride;
+        NGS_REQUIRE(tables_for(m, &T), "ngs_rhs_normals: table multiple unavailable");
+        // gather (window scan per target, no atomics) unless the data is much sparser than the
+        // targets' windows, where the 27-target scatter touches less
+        if ((double)target.M * (3.0 * m) * (3.0 * m) < 4.0 * 27.0 * (double)data.M)
+            rhs_normals_fine_gather_kernel<<<grid1d(target.M, 128), 128, 0, S(s)>>>(target, data, normals, T, rhs);
+        else
+            rhs_normals_scatter_kernel<<<grid1d(data.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);
     } else {
         NGS_REQUIRE(tables_for(data.stride / target.stride, &T), "ngs_rhs_normals: table multiple unavailable");
         rhs_normals_gather_kernel<<<grid1d(target.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);

@@ -96,7 +96,7 @@ __device__ __forceinline__ void grid_mark(const ngs_grid& g, int cx, int cy, int
     if (grid_cell(g, cx, cy, cz, &lin)) atomicOr(&g.bits[lin >> 5], 1u << (lin & 31));
 }
 
-// Presence mask of the n (<= 8) consecutive lattice points (lx, ly, lz0 + k), k < n, of g (lattice
+// Presence mask of the n (<= 31) consecutive lattice points (lx, ly, lz0 + k), k < n, of g (lattice
 // coordinates relative to the origin), and in *r0 the rank of the first present one. Present
 // points have consecutive ranks: point k has rank r0 + popc(mask & ((1 << k) - 1)).
 __device__ __forceinline__ uint32_t grid_column(const ngs_grid& g, int lx, int ly, int lz0, int n, int32_t* r0) {

@@ -172,13 +172,17 @@ __global__ void __launch_bounds__(PCG_THREADS) spmv_kernel(const int32_t* __rest
     }
 }
 
+// r = b - A x in FP64 (FP32 entries, FP64 x and products), 8 lanes per row with 16-byte loads of
+// entries and columns. x == nullptr means x = 0 (r = b, no matrix pass).
 __global__ void __launch_bounds__(PCG_THREADS) residual_f64_kernel(const int32_t* __restrict__ rowptr,
                                                                    const int32_t* __restrict__ cols,
                                                                    const float* __restrict__ vals, const double* __restrict__ b,
                                                                    const double* __restrict__ x, double* __restrict__ r,
                                                                    int64_t M, int64_t chunk, double* partials,
                                                                    uint32_t* counter, double* norms) {
-    constexpr int G = 16;
+    constexpr int G = 8;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef int i4 __attribute__((ext_vector_type(4)));
     const int lb = logical_block();
     const int64_t r0 = (int64_t)lb * chunk;
     const int64_t r1 = min(M, r0 + chunk);
@@ -186,16 +190,23 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_kernel(const int32_t
     constexpr int GROUPS = PCG_THREADS / G;
     double rr = 0, bb = 0;
     for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
-        const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
         double sum = 0.0;
-        for (int32_t k = k0 + lane; k < k1; k += G) sum += (double)vals[k] * x[cols[k]];
+        if (x) {
+            const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+            for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
+                const f4 v = *reinterpret_cast<const f4*>(vals + k);
+                const i4 c = *reinterpret_cast<const i4*>(cols + k);
+                sum += (double)v.x * x[c.x] + (double)v.y * x[c.y] + (double)v.z * x[c.z] + (double)v.w * x[c.w];
+            }
 #pragma unroll
-        for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+            for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        }
         if (lane == 0) {
-            double ri = b[row] - sum;
+            const double bi = b[row];
+            const double ri = bi - sum;
             r[row] = ri;
             rr += ri * ri;
-            bb += b[row] * b[row];
+            bb += bi * bi;
         }
     }
     if (last_block_reduce(rr, bb, partials, counter, gridDim.x) && threadIdx.x == 0) {
@@ -234,7 +245,7 @@ __global__ void __launch_bounds__(PCG_THREADS) stencil_residual_kernel(ngs_stenc
     const int64_t r1 = min(op.level.M, r0 + chunk);
     double rr = 0, bb = 0;
     for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
-        const double ri = b[row] - stencil_row<double>(op, lap, x, u, row);
+        const double ri = x ? b[row] - stencil_row<double>(op, lap, x, u, row) : b[row];
         r[row] = ri;
         rr += ri * ri;
         bb += b[row] * b[row];
@@ -602,7 +613,7 @@ int ngs_spmv(const int32_t* rowptr, const int32_t* cols, const float* vals, cons
 
 int ngs_residual_f64(const int32_t* rowptr, const int32_t* cols, const float* vals, const double* b, const double* x,
                      double* r, int64_t M, double* norms, void* workspace, ngs_stream_t s) {
-    NGS_REQUIRE(workspace, "ngs_residual_f64: null workspace");
+    NGS_REQUIRE(workspace && b && r && norms && (!x || (rowptr && cols && vals)), "ngs_residual_f64: bad arguments");
     if (M <= 0) return NGS_OK;
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
@@ -636,9 +647,9 @@ int ngs_stencil_residual_f64(ngs_stencil op, const double* b, const double* x, d
                              void* workspace, ngs_stream_t s) {
     const int64_t M = op.level.M;
     if (M <= 0) return NGS_OK;
-    NGS_REQUIRE(b && x && r && norms && workspace && (op.n == 0 || u64) && stencil_ok(op),
+    NGS_REQUIRE(b && r && norms && workspace && (op.n == 0 || u64) && stencil_ok(op),
                 "ngs_stencil_residual_f64: bad arguments");
-    if (op.n > 0) stencil_points_kernel<double><<<grid1d(op.n, PCG_THREADS), PCG_THREADS, 0, S(s)>>>(op, x, u64);
+    if (x && op.n > 0) stencil_points_kernel<double><<<grid1d(op.n, PCG_THREADS), PCG_THREADS, 0, S(s)>>>(op, x, u64);
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
     char* w = reinterpret_cast<char*>(workspace);

@@ -41,6 +41,7 @@ class PointBinning:
         obj.cells = SparseGrid.empty(tree.device)
         obj.cell_start = torch.zeros(1, dtype=torch.int32, device=tree.device)
         obj.perm = obj.b = obj.alpha = None
+        obj.ncells = 0
         obj.alpha_scalar = 0.0
         return obj
 
@@ -74,18 +75,21 @@ class Reconstructor:
     def set_fixed_level_set(self, value):
         self.fixed_level_set = value
 
-    # Per-iteration cost model of the two operator forms, in bytes of HBM-equivalent traffic: CSR
-    # streams ~8 bytes for each of ~90 entries per row; the stencil form reads ~60 bytes per binned
-    # point and spends the equivalent of ~150 bytes per row on bitmap and gather work.
-    STENCIL_BYTES_PER_POINT = 60.0
-    STENCIL_BYTES_PER_ROW = 150.0
-    CSR_BYTES_PER_ROW = 720.0
+    # Per-application time model of the two operator forms (picoseconds, fitted to MI355X kernel
+    # timings on 0.05M-5.6M-row levels): CSR ~2.5 ps per stored entry (~90 entries per row); the
+    # stencil ~122 ps per row for the Laplacian part, ~3.5 ps per (row, point) visit of the
+    # screening part (27 cells around each row) and ~50 ps per point for the point pass.
+    CSR_PS_PER_ROW = 2.5 * 90
+    STENCIL_PS_PER_ROW = 122.0
+    STENCIL_PS_PER_VISIT = 3.5
+    STENCIL_PS_PER_POINT = 50.0
 
-    def operator_form(self, M, n_points):
+    def operator_form(self, M, n_points, n_cells=0):
         if self.operator in ("csr", "stencil"):
             return self.operator
-        stencil = self.STENCIL_BYTES_PER_POINT * n_points + self.STENCIL_BYTES_PER_ROW * M
-        return "stencil" if stencil < self.CSR_BYTES_PER_ROW * M else "csr"
+        visits = 27.0 * n_points / max(n_cells, 1) if n_points else 0.0
+        stencil = (self.STENCIL_PS_PER_ROW + self.STENCIL_PS_PER_VISIT * visits) * M + self.STENCIL_PS_PER_POINT * n_points
+        return "stencil" if stencil < self.CSR_PS_PER_ROW * M else "csr"
 
     def assemble_level(self, d, bins, coarse, sf, delta, rhs, form=None):
         """Level-d system (reconstructor.py:143-183); the screening RHS terms are added to rhs.
@@ -94,7 +98,7 @@ class Reconstructor:
         lv = tree.level(d)
         M = lv.M
         if form is None:
-            form = self.operator_form(M, bins.n)
+            form = self.operator_form(M, bins.n, getattr(bins, "ncells", 0))
         if form == "stencil":
             system = _solver.StencilSystem(tree.grids[d], tree.strides[d], bins, sf)
             system.diag_rhs(coarse, delta, rhs)

@@ -228,7 +228,7 @@ class ReconstructionServer:
             return x, info
         b64 = rhs.double()
         x64 = torch.zeros(system.M, dtype=torch.float64, device=system.device)
-        r64, rr, bb = system.residual_f64(b64, x64)
+        r64, rr, bb = system.residual_f64(b64, None)  # x = 0: r = b without a matrix pass
         nb = math.sqrt(bb)
         rel = math.sqrt(rr) / nb if nb > 0 else 0.0
         info = _solver.SolveInfo()

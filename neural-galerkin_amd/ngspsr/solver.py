@@ -57,12 +57,12 @@ class CSRSystem:
         return y
 
     def residual_f64(self, b64, x64):
-        """(r = b - A x in FP64, ||r||^2, ||b||^2)."""
+        """(r = b - A x in FP64, ||r||^2, ||b||^2); x64 None means x = 0."""
         r = torch.empty(self.M, dtype=torch.float64, device=self.device)
         norms = torch.empty(2, dtype=torch.float64, device=self.device)
         ws = _lib.Scratch.get(lib().ngs_pcg_workspace_bytes(self.M), self.device, slot=1)
         call("ngs_residual_f64", ptr(self.rowptr), ptr(self.cols), ptr(self.vals), ptr(b64.contiguous()),
-             ptr(x64.contiguous()), ptr(r), self.M, ptr(norms), ptr(ws), stream())
+             None if x64 is None else ptr(x64.contiguous()), ptr(r), self.M, ptr(norms), ptr(ws), stream())
         nv = norms.cpu()
         return r, float(nv[0]), float(nv[1])
 
@@ -152,13 +152,13 @@ class StencilSystem:
         return y
 
     def residual_f64(self, b64, x64):
-        """(r = b - A x in FP64, ||r||^2, ||b||^2), the operator evaluated in FP64."""
+        """(r = b - A x in FP64, ||r||^2, ||b||^2), the operator evaluated in FP64; x64 None = 0."""
         r = torch.empty(self.M, dtype=torch.float64, device=self.device)
         norms = torch.empty(2, dtype=torch.float64, device=self.device)
         ws = _lib.Scratch.get(lib().ngs_pcg_workspace_bytes(self.M), self.device, slot=1)
         u64 = _lib.Scratch.get(8 * max(self.n, 1), self.device, slot=4)
-        call("ngs_stencil_residual_f64", self._struct, ptr(b64.contiguous()), ptr(x64.contiguous()), ptr(r), ptr(u64),
-             ptr(norms), ptr(ws), stream())
+        call("ngs_stencil_residual_f64", self._struct, ptr(b64.contiguous()),
+             None if x64 is None else ptr(x64.contiguous()), ptr(r), ptr(u64), ptr(norms), ptr(ws), stream())
         nv = norms.cpu()
         return r, float(nv[0]), float(nv[1])
 
@@ -234,7 +234,7 @@ def mixed(system, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, chu
     info = SolveInfo()
     if M == 0:
         return x64, info
-    r64, rr, bb = system.residual_f64(b64, x64)
+    r64, rr, bb = system.residual_f64(b64, None)  # x = 0: r = b without a matrix pass
     nb = bb ** 0.5
     rel = (rr ** 0.5) / nb if nb > 0 else 0.0
     while rel > tol and info.outer < max_outer:
