@@ -290,8 +290,9 @@ int ngs_pcg_solve_stencil(ngs_stencil op, const float *inv_diag, const float *b,
                           double *rel_resid);
 
 /* Solver knobs: use_graph (capture iteration chunks in a hipGraph; default 1), lanes per CSR row
- * in the SpMV (8, 16, 32; 0 = 8) and non-temporal loads of the matrix stream (default 0). */
-int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal);
+ * in the SpMV (8, 16, 32; 0 = 8), non-temporal loads of the matrix stream (default 0) and
+ * interleaved row tiles (blocks of one XCD sweep the XCD's row slab together; default 1). */
+int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal, int32_t interleave);
 
 /* Batched solves (continuous batching of independent level systems; no reference counterpart —
  * the reference solves one system per call, solve_pcg.cu:15-136). A device table of

@@ -107,26 +107,53 @@ __device__ __forceinline__ bool last_block_reduce(double& a, double& b, double* 
 }
 
 // Dot product of CSR row `row` with x by G lanes (16-byte loads of 4 values and 4 columns per lane).
-// NT: non-temporal loads for the matrix stream, so the gathered vector stays resident in L2.
+// Rows hold multiples of 4 entries. The first 128 entries (every assembled row: at most 125
+// neighbours, padded) are read without branches: each lane loads its groups from an index clamped
+// to the row's last group (always in bounds; the surplus is masked out of the sum), so the compiler
+// issues all matrix loads, then all gathers, with no wait between them. Longer rows of a caller's
+// CSR continue in a plain loop. NT: non-temporal loads for the matrix stream.
 template <int G, bool NT>
 __device__ __forceinline__ float row_dot(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
                                          const float* __restrict__ vals, const float* __restrict__ x, int64_t row,
                                          int lane) {
-    const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
-    float sum = 0.f;
     typedef float f4 __attribute__((ext_vector_type(4)));
     typedef int i4 __attribute__((ext_vector_type(4)));
-    for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
-        f4 v;
-        i4 c;
-        if (NT) {
-            v = __builtin_nontemporal_load(reinterpret_cast<const f4*>(vals + k));
-            c = __builtin_nontemporal_load(reinterpret_cast<const i4*>(cols + k));
-        } else {
-            v = *reinterpret_cast<const f4*>(vals + k);
-            c = *reinterpret_cast<const i4*>(cols + k);
+    constexpr int U = 128 / (4 * G);
+    const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    float sum = 0.f;
+    if (k1 > k0) {
+        const int32_t klast = k1 - 4;
+        f4 v[U];
+        i4 c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t k = min(k0 + 4 * lane + 4 * G * u, klast);
+            if (NT) {
+                v[u] = __builtin_nontemporal_load(reinterpret_cast<const f4*>(vals + k));
+                c[u] = __builtin_nontemporal_load(reinterpret_cast<const i4*>(cols + k));
+            } else {
+                v[u] = *reinterpret_cast<const f4*>(vals + k);
+                c[u] = *reinterpret_cast<const i4*>(cols + k);
+            }
         }
-        sum += v.x * x[c.x] + v.y * x[c.y] + v.z * x[c.z] + v.w * x[c.w];
+        float xs[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xs[u][0] = x[c[u].x];
+            xs[u][1] = x[c[u].y];
+            xs[u][2] = x[c[u].z];
+            xs[u][3] = x[c[u].w];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float part = v[u].x * xs[u][0] + v[u].y * xs[u][1] + v[u].z * xs[u][2] + v[u].w * xs[u][3];
+            sum += (k0 + 4 * lane + 4 * G * u < k1) ? part : 0.f;
+        }
+        for (int32_t k = k0 + 128 + 4 * lane; k < k1; k += 4 * G) {
+            const f4 vv = *reinterpret_cast<const f4*>(vals + k);
+            const i4 cc = *reinterpret_cast<const i4*>(cols + k);
+            sum += vv.x * x[cc.x] + vv.y * x[cc.y] + vv.z * x[cc.z] + vv.w * x[cc.w];
+        }
     }
 #pragma unroll
     for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
@@ -157,16 +184,48 @@ __global__ void __launch_bounds__(PCG_THREADS) pcg_init_kernel(PcgArgs A, double
     }
 }
 
-template <int G, bool NT>
-__global__ void __launch_bounds__(PCG_THREADS) spmv_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+// Row tiles of one block in the SpMV kernels. Contiguous: block chunk [lb*chunk, (lb+1)*chunk).
+// Interleaved (nb a multiple of 8): the rows are cut into 8 slabs, one per XCD (block index mod 8),
+// and each XCD's blocks take tiles of `tile` consecutive rows round-robin, so all blocks of an XCD
+// sweep their slab together and the gathered vector window stays small enough for the XCD's L2.
+struct RowTiles {
+    int64_t first, end, step;  // tile starts: first, first + step, ... < end
+    int64_t hi;                // rows of a tile are clipped to hi
+};
+
+__device__ __forceinline__ RowTiles tiles_interleaved(int64_t M, int nb, int raw, int tile) {
+    RowTiles t;
+    const int nbx = nb / NUM_XCD, xcd = raw % NUM_XCD, j = raw / NUM_XCD;
+    const int64_t slab = (M + NUM_XCD - 1) / NUM_XCD;
+    const int64_t lo = min(M, (int64_t)xcd * slab);
+    t.hi = min(M, lo + slab);
+    t.first = lo + (int64_t)j * tile;
+    t.end = t.hi;
+    t.step = (int64_t)nbx * tile;
+    return t;
+}
+
+__device__ __forceinline__ RowTiles tiles_contiguous(int64_t M, int64_t chunk, int local, int tile) {
+    RowTiles t;
+    t.first = (int64_t)local * chunk;
+    t.hi = min(M, t.first + chunk);
+    t.end = t.hi;
+    t.step = tile;
+    return t;
+}
+
+// IL: interleaved tiles (the host launches it only with a grid that is a multiple of 8).
+template <int G, bool NT, bool IL>
+__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(8))) spmv_kernel(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
                                                            const float* __restrict__ vals, const float* __restrict__ x,
                                                            float* __restrict__ y, int64_t M, int64_t chunk) {
-    const int lb = logical_block();
-    const int64_t r0 = (int64_t)lb * chunk;
-    const int64_t r1 = min(M, r0 + chunk);
     const int lane = threadIdx.x % G, grp = threadIdx.x / G;
     constexpr int GROUPS = PCG_THREADS / G;
-    for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+    const RowTiles t = IL ? tiles_interleaved(M, gridDim.x, blockIdx.x, GROUPS)
+                          : tiles_contiguous(M, chunk, logical_block(), GROUPS);
+    for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
+        const int64_t row = t0 + grp;
+        if (row >= t.hi) continue;
         float sum = row_dot<G, NT>(rowptr, cols, vals, x, row, lane);
         if (lane == 0) y[row] = sum;
     }
@@ -192,11 +251,38 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_kernel(const int32_t
     for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
         double sum = 0.0;
         if (x) {
+            // same branch-free first 128 entries as row_dot, then a loop for longer rows
+            constexpr int U = 128 / (4 * G);
             const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
-            for (int32_t k = k0 + 4 * lane; k < k1; k += 4 * G) {
-                const f4 v = *reinterpret_cast<const f4*>(vals + k);
-                const i4 c = *reinterpret_cast<const i4*>(cols + k);
-                sum += (double)v.x * x[c.x] + (double)v.y * x[c.y] + (double)v.z * x[c.z] + (double)v.w * x[c.w];
+            if (k1 > k0) {
+                const int32_t klast = k1 - 4;
+                f4 v[U];
+                i4 c[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int32_t k = min(k0 + 4 * lane + 4 * G * u, klast);
+                    v[u] = *reinterpret_cast<const f4*>(vals + k);
+                    c[u] = *reinterpret_cast<const i4*>(cols + k);
+                }
+                double xs[U][4];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    xs[u][0] = x[c[u].x];
+                    xs[u][1] = x[c[u].y];
+                    xs[u][2] = x[c[u].z];
+                    xs[u][3] = x[c[u].w];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const double part = (double)v[u].x * xs[u][0] + (double)v[u].y * xs[u][1] +
+                                        (double)v[u].z * xs[u][2] + (double)v[u].w * xs[u][3];
+                    sum += (k0 + 4 * lane + 4 * G * u < k1) ? part : 0.0;
+                }
+                for (int32_t k = k0 + 128 + 4 * lane; k < k1; k += 4 * G) {
+                    const f4 vv = *reinterpret_cast<const f4*>(vals + k);
+                    const i4 cc = *reinterpret_cast<const i4*>(cols + k);
+                    sum += (double)vv.x * x[cc.x] + (double)vv.y * x[cc.y] + (double)vv.z * x[cc.z] + (double)vv.w * x[cc.w];
+                }
             }
 #pragma unroll
             for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
@@ -343,8 +429,8 @@ __device__ __forceinline__ const BatchSys* batch_lookup(const BatchSys* tab, con
     return B;
 }
 
-template <int G, bool NT>
-__global__ void __launch_bounds__(PCG_THREADS) bpcg_spmv_dot_kernel(const BatchSys* tab, const int32_t* blk2sys) {
+template <int G, bool NT, bool IL>
+__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(8))) bpcg_spmv_dot_kernel(const BatchSys* tab, const int32_t* blk2sys) {
     int lb;
     const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
     if (!B) return;
@@ -365,7 +451,12 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_spmv_dot_kernel(const BatchS
             pq += (double)A.p[row] * sum;
         }
     } else {
-        for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+        const RowTiles t = (IL && B->nblk % NUM_XCD == 0)
+                               ? tiles_interleaved(A.M, B->nblk, blockIdx.x - B->blk0, GROUPS)
+                               : tiles_contiguous(A.M, A.chunk, lb, GROUPS);
+        for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
+            const int64_t row = t0 + grp;
+            if (row >= t.hi) continue;
             float sum = row_dot<G, NT>(A.rowptr, A.cols, A.vals, A.p, row, lane);
             if (lane == 0) {
                 A.q[row] = sum;
@@ -448,29 +539,38 @@ __global__ void fill_map_kernel(int32_t* map, int n) {
     if (i < n) map[i] = 0;
 }
 
-// SpMV variant: lanes per row (8, 16, 32) and non-temporal matrix loads; set by ngs_pcg_config.
+// SpMV variant: lanes per row (8, 16, 32), non-temporal matrix loads and interleaved row tiles;
+// set by ngs_pcg_config.
 static int g_spmv_lanes = 8;
 static int g_spmv_nt = 0;
+static int g_spmv_il = 1;
+
+template <class F, class NT>
+static void dispatch_spmv_g(F&& f, NT nt) {
+    if (g_spmv_il) {
+        if (g_spmv_lanes == 8) f(std::integral_constant<int, 8>(), nt, std::true_type());
+        else if (g_spmv_lanes == 32) f(std::integral_constant<int, 32>(), nt, std::true_type());
+        else f(std::integral_constant<int, 16>(), nt, std::true_type());
+    } else {
+        if (g_spmv_lanes == 8) f(std::integral_constant<int, 8>(), nt, std::false_type());
+        else if (g_spmv_lanes == 32) f(std::integral_constant<int, 32>(), nt, std::false_type());
+        else f(std::integral_constant<int, 16>(), nt, std::false_type());
+    }
+}
 
 template <class F>
 static void dispatch_spmv(F&& f) {
-    if (g_spmv_nt) {
-        if (g_spmv_lanes == 8) f(std::integral_constant<int, 8>(), std::true_type());
-        else if (g_spmv_lanes == 32) f(std::integral_constant<int, 32>(), std::true_type());
-        else f(std::integral_constant<int, 16>(), std::true_type());
-    } else {
-        if (g_spmv_lanes == 8) f(std::integral_constant<int, 8>(), std::false_type());
-        else if (g_spmv_lanes == 32) f(std::integral_constant<int, 32>(), std::false_type());
-        else f(std::integral_constant<int, 16>(), std::false_type());
-    }
+    if (g_spmv_nt) dispatch_spmv_g(f, std::true_type());
+    else dispatch_spmv_g(f, std::false_type());
 }
 
 static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, int total_blocks, int flags,
                                    hipStream_t st) {
     dim3 grid(total_blocks), block(PCG_THREADS);
     if (flags & 1) bpcg_points_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
-    dispatch_spmv([&](auto g, auto nt) {
-        bpcg_spmv_dot_kernel<decltype(g)::value, decltype(nt)::value><<<grid, block, 0, st>>>(tab, blk2sys);
+    dispatch_spmv([&](auto g, auto nt, auto il) {
+        bpcg_spmv_dot_kernel<decltype(g)::value, decltype(nt)::value, decltype(il)::value>
+            <<<grid, block, 0, st>>>(tab, blk2sys);
     });
     bpcg_update_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
     bpcg_direction_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
@@ -489,13 +589,14 @@ int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t tot
 // Environment-free knobs for measurement: graph use and lanes per row (0 = automatic).
 static int g_pcg_use_graph = 1;
 static int g_pcg_group = 0;
-int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal) {
+int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal, int32_t interleave) {
     NGS_REQUIRE(lanes_per_row == 0 || lanes_per_row == 8 || lanes_per_row == 16 || lanes_per_row == 32,
                 "ngs_pcg_config: lanes_per_row must be 0, 8, 16 or 32");
     g_pcg_use_graph = use_graph ? 1 : 0;
     g_pcg_group = lanes_per_row;
     g_spmv_lanes = lanes_per_row ? lanes_per_row : 8;
     g_spmv_nt = nontemporal ? 1 : 0;
+    g_spmv_il = interleave ? 1 : 0;
     return NGS_OK;
 }
 
@@ -604,9 +705,13 @@ int ngs_spmv(const int32_t* rowptr, const int32_t* cols, const float* vals, cons
     if (M <= 0) return NGS_OK;
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
-    dispatch_spmv([&](auto g, auto nt) {
-        spmv_kernel<decltype(g)::value, decltype(nt)::value><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
+    const int il_saved = g_spmv_il;
+    if (nb % NUM_XCD) g_spmv_il = 0;  // interleaved tiles need a grid that is a multiple of 8
+    dispatch_spmv([&](auto g, auto nt, auto il) {
+        spmv_kernel<decltype(g)::value, decltype(nt)::value, decltype(il)::value>
+            <<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
     });
+    g_spmv_il = il_saved;
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
@@ -751,7 +856,7 @@ int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t tot
     // thread launches on its own streams), bounded, and evicts its oldest entry.
     // the SpMV variant is part of the key: a graph records the kernels of the variant it was captured with
     struct Key { const void* t; const int32_t* m; int nb, it, fl, var; hipStream_t s; hipGraphExec_t e; };
-    const int var = g_spmv_lanes * 2 + g_spmv_nt;
+    const int var = g_spmv_lanes * 4 + g_spmv_nt * 2 + g_spmv_il;
     static thread_local std::vector<Key> cache;
     hipGraphExec_t exec = nullptr;
     for (auto& k : cache)

@@ -79,7 +79,7 @@ _SIGNATURES = {
     "ngs_basis_splat": ([Grid, P, P, c_int64, c_float, c_double, P, P], c_int32),
     "ngs_cross_stiffness_transpose": ([Level, Level, P, c_double, P, P], c_int32),
     "ngs_pcg_workspace_bytes": ([c_int64], c_int64),
-    "ngs_pcg_config": ([c_int32, c_int32, c_int32], c_int32),
+    "ngs_pcg_config": ([c_int32, c_int32, c_int32, c_int32], c_int32),
     "ngs_pcg_solve": ([P, P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
                        ctypes.POINTER(c_double)], c_int32),
     "ngs_pcg_batch_entry_bytes": ([], c_int64),
