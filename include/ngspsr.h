@@ -173,13 +173,14 @@ typedef struct {
     int32_t pad_;
 } ngs_level;
 
-/* Points binned by their level-d cell, with per-point basis values at level d. */
+/* Points binned by their level-d cell, with per-point basis values at level d. Per-point arrays are
+ * in bin order (bin k holds point perm[k]; ngs_bin_points writes them). */
 typedef struct {
     ngs_grid cells;          /* cells containing >= 1 point */
     const int32_t *cell_start; /* ncells + 1 */
-    const int32_t *perm;     /* n: point ids ordered by cell */
-    const float *b;          /* n x 9 basis values at level d (original point order) */
-    const float *alpha;      /* n screening weights, or NULL with alpha_scalar */
+    const int32_t *perm;     /* n: original point id of each bin slot */
+    const float *b;          /* n x 9 basis values at level d, bin order */
+    const float *alpha;      /* n screening weights in bin order, or NULL with alpha_scalar */
     float alpha_scalar;
     float pad_;
     int64_t n;
@@ -190,7 +191,7 @@ typedef struct {
  * from the level bitmap, the 125 Laplacian coefficients and the binned screening points:
  *   (A x)_i = sum_o lap[o] x_{i+o} + sf * sum_{p near i} B_i(p) u_p,
  *   u_p     = alpha_p * sum_{j near p} B_j(p) x_j.
- * Point arrays are in binned order (ngs_stencil_bin_points). */
+ * Point arrays are in bin order (ngs_bin_points). */
 typedef struct {
     ngs_level level;           /* rows: the voxels of the level */
     ngs_grid cells;            /* cells holding >= 1 screening point (same stride) */
@@ -236,10 +237,10 @@ int ngs_rhs_normals(ngs_level target, ngs_level data, const float *normals, doub
 /* The 125 same-level stiffness coefficients K(o), o in [-2,2]^3 lexicographic, of stride `stride`
  * (the K of ngs_lhs_assemble, computed the same way from the m = 1 tables). */
 int ngs_stencil_laplacian(int32_t stride, float *lap, ngs_stream_t s);
-/* Binned copies of the point data of `pts`: point_cell[k] = cell[perm[k]] (n x 3),
- * basis[k] = pts.b[perm[k]] (n x 9), alpha_out[k] = pts.alpha[perm[k]] when pts.alpha is set. */
-int ngs_stencil_bin_points(ngs_point_bins pts, const int32_t *cell, int32_t *point_cell, float *basis,
-                           float *alpha_out, ngs_stream_t s);
+/* Bin-order copies of per-point data: point_cell[k] = cell[perm[k]] (n x 3), basis[k] = b[perm[k]]
+ * (n x 9), alpha_out[k] = alpha[perm[k]] when alpha is not NULL. */
+int ngs_bin_points(const int32_t *perm, int64_t n, const int32_t *cell, const float *b, const float *alpha,
+                   int32_t *point_cell, float *basis, float *alpha_out, ngs_stream_t s);
 /* inv_diag and the screening RHS terms of ngs_lhs_assemble for the stencil form (same values,
  * same summation order): inv_diag_i = 1 / A_ii, rhs_i += sf delta sum_p alpha_p B_i(p)
  * - sf sum_p alpha_p B_i(p) coarse_p (coarse in original point order, may be NULL). */

@@ -25,15 +25,23 @@ template <bool PAD>
 __global__ void lhs_count_kernel(ngs_level lv, int32_t* __restrict__ row_nnz) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= lv.M) return;
-    const int s = lv.stride;
-    const int cx = lv.coords[3 * i], cy = lv.coords[3 * i + 1], cz = lv.coords[3 * i + 2];
+    const ngs_grid& g = lv.grid;
+    const int l2 = g.log2_stride;
+    const int lx = (lv.coords[3 * i] >> l2) - g.origin[0], ly = (lv.coords[3 * i + 1] >> l2) - g.origin[1];
+    const int lz = (lv.coords[3 * i + 2] >> l2) - g.origin[2];
     int cnt = 0;
     for (int dx = -2; dx <= 2; ++dx)
-        for (int dy = -2; dy <= 2; ++dy)
-            for (int dz = -2; dz <= 2; ++dz) cnt += grid_test(lv.grid, cx + dx * s, cy + dy * s, cz + dz * s) ? 1 : 0;
+        for (int dy = -2; dy <= 2; ++dy) {
+            int32_t r;
+            cnt += __popc(grid_column(g, lx + dx, ly + dy, lz - 2, 5, &r));
+        }
     row_nnz[i] = PAD ? (cnt + 3) & ~3 : cnt;
 }
 
+// One thread per row: the 125 entries accumulate in registers over the points of the 27 cells
+// around the voxel (points in bin order, basis values and weights stored in bin order), then the
+// row is written in increasing column order. Cells and neighbours are found one bitmap column
+// (3 or 5 consecutive lattice points along z) at a time.
 template <bool HAS_ALPHA, bool HAS_COARSE>
 __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_point_bins pts,
                                                           const float* __restrict__ coarse, float sf, float sf_delta,
@@ -45,26 +53,33 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
     if (i >= lv.M) return;
     const int s = lv.stride;
     const int cx = lv.coords[3 * i], cy = lv.coords[3 * i + 1], cz = lv.coords[3 * i + 2];
+    const float* __restrict__ pb_all = pts.b;
+    const float* __restrict__ pa_all = pts.alpha;
+    const int32_t* __restrict__ cstart = pts.cell_start;
     float acc[125];
 #pragma unroll
     for (int k = 0; k < 125; ++k) acc[k] = 0.f;
     float rs = 0.f, rc = 0.f;
+    const ngs_grid& cg = pts.cells;
+    const int c2 = cg.log2_stride;
+    const int mx = (cx >> c2) - cg.origin[0], my = (cy >> c2) - cg.origin[1], mz = (cz >> c2) - cg.origin[2];
 #pragma unroll
     for (int ox = -1; ox <= 1; ++ox) {
 #pragma unroll
         for (int oy = -1; oy <= 1; ++oy) {
+            int32_t cid;
+            const uint32_t cm = grid_column(cg, mx + ox, my + oy, mz - 1, 3, &cid);
 #pragma unroll
             for (int oz = -1; oz <= 1; ++oz) {
-                int cid = grid_find(pts.cells, cx + ox * s, cy + oy * s, cz + oz * s);
-                if (cid < 0) continue;
-                const int k0 = pts.cell_start[cid], k1 = pts.cell_start[cid + 1];
+                if (!((cm >> (oz + 1)) & 1u)) continue;
+                const int k0 = cstart[cid], k1 = cstart[cid + 1];
+                ++cid;
                 for (int k = k0; k < k1; ++k) {
-                    const int p = pts.perm[k];
-                    const float* bp = pts.b + 9 * (int64_t)p;
+                    const float* bp = pb_all + 9 * (int64_t)k;
                     float bx[3] = {bp[0], bp[1], bp[2]};
                     float by[3] = {bp[3], bp[4], bp[5]};
                     float bz[3] = {bp[6], bp[7], bp[8]};
-                    const float a = HAS_ALPHA ? pts.alpha[p] : pts.alpha_scalar;
+                    const float a = HAS_ALPHA ? pa_all[k] : pts.alpha_scalar;
                     // products rounded on their own, sums by explicit FMA: the same arithmetic as
                     // stencil_diag_rhs_kernel, whatever the contraction choices around it
                     float w;
@@ -73,7 +88,7 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                         w = a * (bx[1 - ox] * by[1 - oy] * bz[1 - oz]);
                     }
                     rs += w;
-                    if (HAS_COARSE) rc = fmaf(w, coarse[p], rc);
+                    if (HAS_COARSE) rc = fmaf(w, coarse[pts.perm[k]], rc);
 #pragma unroll
                     for (int vx = -1; vx <= 1; ++vx) {
 #pragma unroll
@@ -98,14 +113,18 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
     int32_t base = rowptr[i];
     int cnt = 0;
     const float fs = (float)s, inv_s = 1.0f / (float)s;
+    const ngs_grid& g = lv.grid;
+    const int l2 = g.log2_stride;
+    const int lx = (cx >> l2) - g.origin[0], ly = (cy >> l2) - g.origin[1], lz = (cz >> l2) - g.origin[2];
 #pragma unroll
     for (int dx = -2; dx <= 2; ++dx) {
 #pragma unroll
         for (int dy = -2; dy <= 2; ++dy) {
+            int32_t j;
+            const uint32_t m = grid_column(g, lx + dx, ly + dy, lz - 2, 5, &j);
 #pragma unroll
             for (int dz = -2; dz <= 2; ++dz) {
-                int j = grid_find(lv.grid, cx + dx * s, cy + dy * s, cz + dz * s);
-                if (j < 0) continue;
+                if (!((m >> (dz + 2)) & 1u)) continue;
                 float v;
                 {
 #pragma clang fp contract(off)
@@ -114,7 +133,7 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                     const float K = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
                     v = K + sf * acc[(dx + 2) * 25 + (dy + 2) * 5 + (dz + 2)];
                 }
-                cols[base + cnt] = j;
+                cols[base + cnt] = j++;
                 vals[base + cnt] = v;
                 ++cnt;
                 if (dx == 0 && dy == 0 && dz == 0) inv_diag[i] = 1.0f / v;
@@ -144,16 +163,17 @@ __global__ void stencil_laplacian_kernel(const float* __restrict__ F1, const flo
     }
 }
 
-__global__ void stencil_bin_kernel(ngs_point_bins pts, const int32_t* __restrict__ cell, int32_t* __restrict__ pcell,
-                                   float* __restrict__ basis, float* __restrict__ alpha) {
+__global__ void bin_points_kernel(const int32_t* __restrict__ perm, int64_t n, const int32_t* __restrict__ cell,
+                                  const float* __restrict__ b, const float* __restrict__ alpha,
+                                  int32_t* __restrict__ pcell, float* __restrict__ basis, float* __restrict__ alpha_out) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= pts.n) return;
-    const int64_t p = pts.perm[k];
+    if (k >= n) return;
+    const int64_t p = perm[k];
 #pragma unroll
     for (int a = 0; a < 3; ++a) pcell[3 * k + a] = cell[3 * p + a];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) basis[9 * k + j] = pts.b[9 * p + j];
-    if (alpha) alpha[k] = pts.alpha[p];
+    for (int j = 0; j < 9; ++j) basis[9 * k + j] = b[9 * p + j];
+    if (alpha) alpha_out[k] = alpha[p];
 }
 
 // Diagonal and screening RHS of row i for the stencil form: the same products, in the same
@@ -223,15 +243,20 @@ __global__ void cross_stiffness_kernel(ngs_level fine, ngs_level coarse, const f
     }
     const int nrow = 3 * m + 2;
     const float fs = (float)sf_, inv_sc = 1.0f / (float)sc;
+    const ngs_grid& g = coarse.grid;
+    const int l2 = g.log2_stride;
+    const int qx = (Q[0] >> l2) - g.origin[0], qy = (Q[1] >> l2) - g.origin[1], qz = (Q[2] >> l2) - g.origin[2];
     float acc = 0.f;
     for (int kx = -2; kx <= 2; ++kx) {
         int ix = kx * m + (Q[0] - c[0]) / sf_ + 2 * m;
         for (int ky = -2; ky <= 2; ++ky) {
             int iy = ky * m + (Q[1] - c[1]) / sf_ + 2 * m;
+            int32_t j;
+            const uint32_t msk = grid_column(g, qx + kx, qy + ky, qz - 2, 5, &j);
             for (int kz = -2; kz <= 2; ++kz) {
+                if (!((msk >> (kz + 2)) & 1u)) continue;
+                const int32_t jj = j++;
                 int iz = kz * m + (Q[2] - c[2]) / sf_ + 2 * m;
-                int j = grid_find(coarse.grid, Q[0] + kx * sc, Q[1] + ky * sc, Q[2] + kz * sc);
-                if (j < 0) continue;
                 if (ix < 0 || ix >= nrow || iy < 0 || iy >= nrow || iz < 0 || iz >= nrow) continue;
                 float K;
                 {
@@ -240,7 +265,7 @@ __global__ void cross_stiffness_kernel(ngs_level fine, ngs_level coarse, const f
                     const float g0 = T.G[ix] * inv_sc, g1 = T.G[iy] * inv_sc, g2 = T.G[iz] * inv_sc;
                     K = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
                 }
-                acc += K * xc[j];
+                acc += K * xc[jj];
             }
         }
     }
@@ -298,15 +323,23 @@ __global__ void rhs_normals_same_kernel(ngs_level tgt, ngs_level dat, const floa
     const int s = tgt.stride;
     const int cx = tgt.coords[3 * i], cy = tgt.coords[3 * i + 1], cz = tgt.coords[3 * i + 2];
     const float fsd = (float)s, imult = 1.0f;
+    const ngs_grid& g = dat.grid;
+    const int l2 = g.log2_stride;
+    const int lx = (cx >> l2) - g.origin[0], ly = (cy >> l2) - g.origin[1], lz = (cz >> l2) - g.origin[2];
     double acc = 0.0;
 #pragma unroll
     for (int kx = -1; kx <= 1; ++kx) {
 #pragma unroll
         for (int ky = -1; ky <= 1; ++ky) {
+            // data voxels t - k s for kz = 1, 0, -1: the column (lz - 1 .. lz + 1), read in kz order
+            int32_t r0;
+            const uint32_t msk = grid_column(g, lx - kx, ly - ky, lz - 1, 3, &r0);
+            if (!msk) continue;
 #pragma unroll
             for (int kz = -1; kz <= 1; ++kz) {
-                const int v = grid_find(dat.grid, cx - kx * s, cy - ky * s, cz - kz * s);
-                if (v < 0) continue;
+                const int bit = 1 - kz;  // lattice offset of t - kz s in the column
+                if (!((msk >> bit) & 1u)) continue;
+                const int v = r0 + __popc(msk & ((1u << bit) - 1u));
                 const float n0 = -nrm[3 * v], n1 = -nrm[3 * v + 1], n2 = -nrm[3 * v + 2];
                 const int ix = kx + 1, iy = ky + 1, iz = kz + 1;
                 float r;
@@ -560,13 +593,11 @@ int ngs_stencil_laplacian(int32_t stride, float* lap, ngs_stream_t s) {
     return NGS_OK;
 }
 
-int ngs_stencil_bin_points(ngs_point_bins pts, const int32_t* cell, int32_t* point_cell, float* basis, float* alpha_out,
-                           ngs_stream_t s) {
-    if (pts.n <= 0) return NGS_OK;
-    NGS_REQUIRE(pts.perm && pts.b && cell && point_cell && basis && (!pts.alpha || alpha_out),
-                "ngs_stencil_bin_points: bad arguments");
-    stencil_bin_kernel<<<grid1d(pts.n, 256), 256, 0, S(s)>>>(pts, cell, point_cell, basis,
-                                                             pts.alpha ? alpha_out : nullptr);
+int ngs_bin_points(const int32_t* perm, int64_t n, const int32_t* cell, const float* b, const float* alpha,
+                   int32_t* point_cell, float* basis, float* alpha_out, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    NGS_REQUIRE(perm && cell && b && point_cell && basis && (!alpha || alpha_out), "ngs_bin_points: bad arguments");
+    bin_points_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(perm, n, cell, b, alpha, point_cell, basis, alpha_out);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -445,22 +445,30 @@ __global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_e
         __shared__ float lap[125];
         for (int t = threadIdx.x; t < 125; t += PCG_THREADS) lap[t] = B->op.lap[t];
         __syncthreads();
+        const ngs_stencil op = B->op;
+        const float* __restrict__ p = A.p;
+        float* __restrict__ q = A.q;
         for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
-            const float sum = stencil_row<float>(B->op, lap, A.p, B->op.u, row);
-            A.q[row] = sum;
-            pq += (double)A.p[row] * sum;
+            const float sum = stencil_row<float>(op, lap, p, op.u, row);
+            q[row] = sum;
+            pq += (double)p[row] * sum;
         }
     } else {
         const RowTiles t = (IL && B->nblk % NUM_XCD == 0)
                                ? tiles_interleaved(A.M, B->nblk, blockIdx.x - B->blk0, GROUPS)
                                : tiles_contiguous(A.M, A.chunk, lb, GROUPS);
+        const int32_t* __restrict__ rowptr = A.rowptr;
+        const int32_t* __restrict__ cols = A.cols;
+        const float* __restrict__ vals = A.vals;
+        const float* __restrict__ p = A.p;
+        float* __restrict__ q = A.q;
         for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
             const int64_t row = t0 + grp;
             if (row >= t.hi) continue;
-            float sum = row_dot<G, NT>(A.rowptr, A.cols, A.vals, A.p, row, lane);
+            float sum = row_dot<G, NT>(rowptr, cols, vals, p, row, lane);
             if (lane == 0) {
-                A.q[row] = sum;
-                pq += (double)A.p[row] * sum;
+                q[row] = sum;
+                pq += (double)p[row] * sum;
             }
         }
     }
@@ -479,11 +487,13 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_points_kernel(const BatchSys
     if (!B || B->kind != SYS_STENCIL) return;
     const PcgArgs& A = B->a;
     if (A.st->done) return;
-    const ngs_stencil& op = B->op;
+    const ngs_stencil op = B->op;
     const int64_t per = (op.n + B->nblk - 1) / B->nblk;
     const int64_t k0 = (int64_t)lb * per;
     const int64_t k1 = min(op.n, k0 + per);
-    for (int64_t k = k0 + threadIdx.x; k < k1; k += PCG_THREADS) op.u[k] = stencil_point<float>(op, A.p, k);
+    const float* __restrict__ p = A.p;
+    float* __restrict__ u = op.u;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += PCG_THREADS) u[k] = stencil_point<float>(op, p, k);
 }
 
 __global__ void __launch_bounds__(PCG_THREADS) bpcg_update_kernel(const BatchSys* tab, const int32_t* blk2sys) {
@@ -495,13 +505,20 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_update_kernel(const BatchSys
     const int64_t r0 = (int64_t)lb * A.chunk;
     const int64_t r1 = min(A.M, r0 + A.chunk);
     const float alpha = (float)A.st->alpha;
+    // local restrict copies: the stores below cannot alias the table or the other vectors
+    float* __restrict__ x = A.x;
+    float* __restrict__ r = A.r;
+    const float* __restrict__ p = A.p;
+    const float* __restrict__ q = A.q;
+    const float* __restrict__ dinv = A.dinv;
     double rz = 0, rr = 0;
+#pragma unroll 4
     for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) {
-        float pi = A.p[i];
-        A.x[i] += alpha * pi;
-        float ri = A.r[i] - alpha * A.q[i];
-        A.r[i] = ri;
-        float z = A.dinv[i] * ri;
+        const float pi = p[i];
+        x[i] += alpha * pi;
+        const float ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const float z = dinv[i] * ri;
         rz += (double)ri * z;
         rr += (double)ri * ri;
     }
@@ -529,7 +546,11 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_direction_kernel(const Batch
     const int64_t r0 = (int64_t)lb * A.chunk;
     const int64_t r1 = min(A.M, r0 + A.chunk);
     const float beta = (float)A.st->beta;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) A.p[i] = A.dinv[i] * A.r[i] + beta * A.p[i];
+    float* __restrict__ p = A.p;
+    const float* __restrict__ r = A.r;
+    const float* __restrict__ dinv = A.dinv;
+#pragma unroll 4
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
 __global__ void write_entry_kernel(BatchSys* dst, BatchSys src) { *dst = src; }

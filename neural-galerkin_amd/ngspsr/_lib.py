@@ -91,7 +91,7 @@ _SIGNATURES = {
     "ngs_pcg_solve_stencil": ([Stencil, P, P, P, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
                                ctypes.POINTER(c_double)], c_int32),
     "ngs_stencil_laplacian": ([c_int32, P, P], c_int32),
-    "ngs_stencil_bin_points": ([PointBins, P, P, P, P, P], c_int32),
+    "ngs_bin_points": ([P, c_int64, P, P, P, P, P, P, P], c_int32),
     "ngs_stencil_diag_rhs": ([Stencil, P, c_float, P, P, P], c_int32),
     "ngs_stencil_apply": ([Stencil, P, P, P], c_int32),
     "ngs_stencil_pass": ([Stencil, P, P, c_int32, P], c_int32),

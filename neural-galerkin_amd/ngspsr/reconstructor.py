@@ -11,28 +11,34 @@ from .hashtree import HashTree
 
 class PointBinning:
     """Points of the screening term binned by their containing cell at one level, with the 3x3
-    per-axis basis values of the 27 voxels around each point (reconstructor.py:111-120)."""
+    per-axis basis values of the 27 voxels around each point (reconstructor.py:111-120). Per-point
+    arrays are stored in bin order: slot k holds point perm[k]."""
 
     def __init__(self, tree: HashTree, xyz, d: int, alpha):
         n = xyz.shape[0]
         s = tree.strides[d]
         dev = tree.device
         self.n = n
-        self.cell = torch.empty((n, 3), dtype=torch.int32, device=dev)
-        self.b = torch.empty((n, 9), dtype=torch.float32, device=dev)
-        call("ngs_point_basis", ptr(xyz), n, tree.inv_vs, s, ptr(self.cell), ptr(self.b), stream())
+        cell = torch.empty((n, 3), dtype=torch.int32, device=dev)
+        b = torch.empty((n, 9), dtype=torch.float32, device=dev)
+        call("ngs_point_basis", ptr(xyz), n, tree.inv_vs, s, ptr(cell), ptr(b), stream())
         self.cells = SparseGrid.for_points(tree._qmin, tree._qmax, s, dev)
-        self.cells.mark_coords(self.cell)
+        self.cells.mark_coords(cell)
         self.ncells = self.cells.finalize(emit=False)
         rank = torch.empty(n, dtype=torch.int32, device=dev)
-        call("ngs_grid_rank_rows", self.cells.struct(), ptr(self.cell), n, ptr(rank), stream())
+        call("ngs_grid_rank_rows", self.cells.struct(), ptr(cell), n, ptr(rank), stream())
         self.perm = torch.empty(n, dtype=torch.int32, device=dev)
         self.cell_start = torch.empty(self.ncells + 1, dtype=torch.int32, device=dev)
         wsb = lib().ngs_bucket_workspace_bytes(n, self.ncells)
         ws = Scratch.get(wsb, dev, slot=3)
         call("ngs_bucket_points", ptr(rank), n, self.ncells, ptr(self.perm), ptr(self.cell_start), ptr(ws), wsb, stream())
-        self.alpha = alpha if isinstance(alpha, torch.Tensor) else None
-        self.alpha_scalar = 0.0 if self.alpha is not None else float(alpha)
+        alpha_t = alpha.float().contiguous() if isinstance(alpha, torch.Tensor) else None
+        self.point_cell = torch.empty((n, 3), dtype=torch.int32, device=dev)
+        self.b = torch.empty((n, 9), dtype=torch.float32, device=dev)
+        self.alpha = torch.empty(n, dtype=torch.float32, device=dev) if alpha_t is not None else None
+        call("ngs_bin_points", ptr(self.perm), n, ptr(cell), ptr(b), ptr(alpha_t), ptr(self.point_cell), ptr(self.b),
+             ptr(self.alpha), stream())
+        self.alpha_scalar = 0.0 if alpha_t is not None else float(alpha)
 
     @classmethod
     def empty(cls, tree):
@@ -40,7 +46,7 @@ class PointBinning:
         obj.n = 0
         obj.cells = SparseGrid.empty(tree.device)
         obj.cell_start = torch.zeros(1, dtype=torch.int32, device=tree.device)
-        obj.perm = obj.b = obj.alpha = None
+        obj.perm = obj.b = obj.alpha = obj.point_cell = None
         obj.ncells = 0
         obj.alpha_scalar = 0.0
         return obj

@@ -104,11 +104,8 @@ class StencilSystem:
         self.n = n
         self.perm = bins.perm
         if n:
-            self.point_cell = torch.empty((n, 3), dtype=torch.int32, device=dev)
-            self.basis = torch.empty((n, 9), dtype=torch.float32, device=dev)
-            self.alpha = torch.empty(n, dtype=torch.float32, device=dev) if bins.alpha is not None else None
-            call("ngs_stencil_bin_points", bins.struct(), ptr(bins.cell), ptr(self.point_cell), ptr(self.basis),
-                 ptr(self.alpha), stream())
+            # the bin-order point arrays of the binning
+            self.point_cell, self.basis, self.alpha = bins.point_cell, bins.b, bins.alpha
             self.u = torch.empty(n, dtype=torch.float32, device=dev)
         else:
             self.point_cell = self.basis = self.alpha = self.u = None
@@ -163,8 +160,7 @@ class StencilSystem:
         return r, float(nv[0]), float(nv[1])
 
     def to_csr(self):
-        """The same operator assembled as CSR (ngs_lhs_assemble over the binned points)."""
-        from ._lib import PointBins
+        """The same operator assembled as CSR (ngs_lhs_assemble over the same binned points)."""
         M, dev = self.M, self.device
         row_nnz = torch.empty(M, dtype=torch.int32, device=dev)
         call("ngs_lhs_row_counts", self.level, ptr(row_nnz), stream())
@@ -176,18 +172,8 @@ class StencilSystem:
         vals = torch.empty(nnz, dtype=torch.float32, device=dev)
         inv_diag = torch.empty(M, dtype=torch.float32, device=dev)
         rhs = torch.zeros(M, dtype=torch.float64, device=dev)
-        pb = PointBins()
-        pb.cells = self._struct.cells
-        pb.cell_start = self._struct.cell_start
-        ident = torch.arange(self.n, dtype=torch.int32, device=dev) if self.n else None
-        pb.perm = None if ident is None else ident.data_ptr()
-        pb.b = self._struct.basis
-        pb.alpha = self._struct.alpha
-        pb.alpha_scalar = self._struct.alpha_scalar
-        pb.n = self.n
-        call("ngs_lhs_assemble", self.level, pb, None, self.sf, 0.0, ptr(rowptr), ptr(cols), ptr(vals), ptr(inv_diag),
-             ptr(rhs), stream())
-        torch.cuda.current_stream().synchronize()
+        call("ngs_lhs_assemble", self.level, self.bins.struct(), None, self.sf, 0.0, ptr(rowptr), ptr(cols),
+             ptr(vals), ptr(inv_diag), ptr(rhs), stream())
         return CSRSystem(rowptr, cols, vals, inv_diag)
 
     def to_scipy(self):
