@@ -11,6 +11,15 @@
 
 namespace ngs {
 
+// Device buffers reached through struct fields (tables, grids) are global memory; casting them to
+// the global address space lets the compiler use global loads with per-counter waits instead of
+// flat accesses. Never apply it to LDS or host pointers.
+#define NGS_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ NGS_GLOBAL T* as_global(T* p) {
+    return (NGS_GLOBAL T*)p;
+}
+
 // ----------------------------------------------------------------------------------- errors
 void set_error(const std::string& msg);
 
@@ -77,10 +86,10 @@ __device__ __forceinline__ int32_t grid_find(const ngs_grid& g, int cx, int cy, 
     if (((cx | cy | cz) & mask) != 0) return -1;
     int64_t lin;
     if (!grid_cell(g, cx, cy, cz, &lin)) return -1;
-    uint32_t w = g.bits[lin >> 5];
+    uint32_t w = as_global(g.bits)[lin >> 5];
     uint32_t b = (uint32_t)(lin & 31);
     if (!((w >> b) & 1u)) return -1;
-    return g.rank[lin >> 5] + __popc(w & ((1u << b) - 1u));
+    return as_global(g.rank)[lin >> 5] + __popc(w & ((1u << b) - 1u));
 }
 
 __device__ __forceinline__ bool grid_test(const ngs_grid& g, int cx, int cy, int cz) {
@@ -88,7 +97,7 @@ __device__ __forceinline__ bool grid_test(const ngs_grid& g, int cx, int cy, int
     if (((cx | cy | cz) & mask) != 0) return false;
     int64_t lin;
     if (!grid_cell(g, cx, cy, cz, &lin)) return false;
-    return (g.bits[lin >> 5] >> (lin & 31)) & 1u;
+    return (as_global(g.bits)[lin >> 5] >> (lin & 31)) & 1u;
 }
 
 __device__ __forceinline__ void grid_mark(const ngs_grid& g, int cx, int cy, int cz) {
@@ -107,12 +116,13 @@ __device__ __forceinline__ uint32_t grid_column(const ngs_grid& g, int lx, int l
     const int64_t w = lin >> 5;
     const int b = (int)(lin & 31);
     const int len = zhi - zlo;
-    const uint32_t w0 = g.bits[w];
+    const NGS_GLOBAL uint32_t* bits = as_global(g.bits);
+    const uint32_t w0 = bits[w];
     uint64_t v = w0;
     // the window's last bit is a lattice point of the box, so word w + 1 exists when it is needed
-    if (b + len > 32) v |= (uint64_t)g.bits[w + 1] << 32;
+    if (b + len > 32) v |= (uint64_t)bits[w + 1] << 32;
     const uint32_t m = (uint32_t)(v >> b) & ((1u << len) - 1u);
-    *r0 = g.rank[w] + __popc(w0 & ((1u << b) - 1u));
+    *r0 = as_global(g.rank)[w] + __popc(w0 & ((1u << b) - 1u));
     return m << (zlo - lz0);
 }
 

@@ -113,9 +113,10 @@ __device__ __forceinline__ bool last_block_reduce(double& a, double& b, double* 
 // issues all matrix loads, then all gathers, with no wait between them. Longer rows of a caller's
 // CSR continue in a plain loop. NT: non-temporal loads for the matrix stream.
 template <int G, bool NT>
-__device__ __forceinline__ float row_dot(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
-                                         const float* __restrict__ vals, const float* __restrict__ x, int64_t row,
-                                         int lane) {
+__device__ __forceinline__ float row_dot(const NGS_GLOBAL int32_t* __restrict__ rowptr,
+                                         const NGS_GLOBAL int32_t* __restrict__ cols,
+                                         const NGS_GLOBAL float* __restrict__ vals,
+                                         const NGS_GLOBAL float* __restrict__ x, int64_t row, int lane) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     typedef int i4 __attribute__((ext_vector_type(4)));
     constexpr int U = 128 / (4 * G);
@@ -129,11 +130,11 @@ __device__ __forceinline__ float row_dot(const int32_t* __restrict__ rowptr, con
         for (int u = 0; u < U; ++u) {
             const int32_t k = min(k0 + 4 * lane + 4 * G * u, klast);
             if (NT) {
-                v[u] = __builtin_nontemporal_load(reinterpret_cast<const f4*>(vals + k));
-                c[u] = __builtin_nontemporal_load(reinterpret_cast<const i4*>(cols + k));
+                v[u] = __builtin_nontemporal_load(reinterpret_cast<const NGS_GLOBAL f4*>(vals + k));
+                c[u] = __builtin_nontemporal_load(reinterpret_cast<const NGS_GLOBAL i4*>(cols + k));
             } else {
-                v[u] = *reinterpret_cast<const f4*>(vals + k);
-                c[u] = *reinterpret_cast<const i4*>(cols + k);
+                v[u] = *reinterpret_cast<const NGS_GLOBAL f4*>(vals + k);
+                c[u] = *reinterpret_cast<const NGS_GLOBAL i4*>(cols + k);
             }
         }
         float xs[U][4];
@@ -150,8 +151,8 @@ __device__ __forceinline__ float row_dot(const int32_t* __restrict__ rowptr, con
             sum += (k0 + 4 * lane + 4 * G * u < k1) ? part : 0.f;
         }
         for (int32_t k = k0 + 128 + 4 * lane; k < k1; k += 4 * G) {
-            const f4 vv = *reinterpret_cast<const f4*>(vals + k);
-            const i4 cc = *reinterpret_cast<const i4*>(cols + k);
+            const f4 vv = *reinterpret_cast<const NGS_GLOBAL f4*>(vals + k);
+            const i4 cc = *reinterpret_cast<const NGS_GLOBAL i4*>(cols + k);
             sum += vv.x * x[cc.x] + vv.y * x[cc.y] + vv.z * x[cc.z] + vv.w * x[cc.w];
         }
     }
@@ -165,12 +166,17 @@ __global__ void __launch_bounds__(PCG_THREADS) pcg_init_kernel(PcgArgs A, double
     const int64_t r0 = (int64_t)lb * A.chunk;
     const int64_t r1 = min(A.M, r0 + A.chunk);
     double bb = 0, rz = 0;
+    const NGS_GLOBAL float* __restrict__ b = as_global(A.b);
+    const NGS_GLOBAL float* __restrict__ dinv = as_global(A.dinv);
+    NGS_GLOBAL float* __restrict__ x = as_global(A.x);
+    NGS_GLOBAL float* __restrict__ r = as_global(A.r);
+    NGS_GLOBAL float* __restrict__ p = as_global(A.p);
     for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) {
-        float bi = A.b[i];
-        float z = A.dinv[i] * bi;
-        A.x[i] = 0.f;
-        A.r[i] = bi;
-        A.p[i] = z;
+        float bi = b[i];
+        float z = dinv[i] * bi;
+        x[i] = 0.f;
+        r[i] = bi;
+        p[i] = z;
         bb += (double)bi * bi;
         rz += (double)bi * z;
     }
@@ -226,7 +232,7 @@ __global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_e
     for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
         const int64_t row = t0 + grp;
         if (row >= t.hi) continue;
-        float sum = row_dot<G, NT>(rowptr, cols, vals, x, row, lane);
+        float sum = row_dot<G, NT>(as_global(rowptr), as_global(cols), as_global(vals), as_global(x), row, lane);
         if (lane == 0) y[row] = sum;
     }
 }
@@ -457,11 +463,11 @@ __global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_e
         const RowTiles t = (IL && B->nblk % NUM_XCD == 0)
                                ? tiles_interleaved(A.M, B->nblk, blockIdx.x - B->blk0, GROUPS)
                                : tiles_contiguous(A.M, A.chunk, lb, GROUPS);
-        const int32_t* __restrict__ rowptr = A.rowptr;
-        const int32_t* __restrict__ cols = A.cols;
-        const float* __restrict__ vals = A.vals;
-        const float* __restrict__ p = A.p;
-        float* __restrict__ q = A.q;
+        const NGS_GLOBAL int32_t* __restrict__ rowptr = as_global(A.rowptr);
+        const NGS_GLOBAL int32_t* __restrict__ cols = as_global(A.cols);
+        const NGS_GLOBAL float* __restrict__ vals = as_global(A.vals);
+        const NGS_GLOBAL float* __restrict__ p = as_global(A.p);
+        NGS_GLOBAL float* __restrict__ q = as_global(A.q);
         for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
             const int64_t row = t0 + grp;
             if (row >= t.hi) continue;
@@ -506,11 +512,11 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_update_kernel(const BatchSys
     const int64_t r1 = min(A.M, r0 + A.chunk);
     const float alpha = (float)A.st->alpha;
     // local restrict copies: the stores below cannot alias the table or the other vectors
-    float* __restrict__ x = A.x;
-    float* __restrict__ r = A.r;
-    const float* __restrict__ p = A.p;
-    const float* __restrict__ q = A.q;
-    const float* __restrict__ dinv = A.dinv;
+    NGS_GLOBAL float* __restrict__ x = as_global(A.x);
+    NGS_GLOBAL float* __restrict__ r = as_global(A.r);
+    const NGS_GLOBAL float* __restrict__ p = as_global(A.p);
+    const NGS_GLOBAL float* __restrict__ q = as_global(A.q);
+    const NGS_GLOBAL float* __restrict__ dinv = as_global(A.dinv);
     double rz = 0, rr = 0;
 #pragma unroll 4
     for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) {
@@ -546,9 +552,9 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_direction_kernel(const Batch
     const int64_t r0 = (int64_t)lb * A.chunk;
     const int64_t r1 = min(A.M, r0 + A.chunk);
     const float beta = (float)A.st->beta;
-    float* __restrict__ p = A.p;
-    const float* __restrict__ r = A.r;
-    const float* __restrict__ dinv = A.dinv;
+    NGS_GLOBAL float* __restrict__ p = as_global(A.p);
+    const NGS_GLOBAL float* __restrict__ r = as_global(A.r);
+    const NGS_GLOBAL float* __restrict__ dinv = as_global(A.dinv);
 #pragma unroll 4
     for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) p[i] = dinv[i] * r[i] + beta * p[i];
 }

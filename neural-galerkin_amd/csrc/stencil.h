@@ -17,10 +17,13 @@ namespace ngs {
 // Row i of A x. lap: the 125 Laplacian coefficients (lexicographic offsets), typically in LDS.
 // u: the point pass of the same x.
 template <typename T>
-__device__ __forceinline__ T stencil_row(const ngs_stencil& op, const float* lap, const T* __restrict__ x,
-                                         const T* __restrict__ u, int64_t i) {
+__device__ __forceinline__ T stencil_row(const ngs_stencil& op, const float* lap, const T* __restrict__ x_,
+                                         const T* __restrict__ u_, int64_t i) {
     const ngs_grid& g = op.level.grid;
-    const int cx = op.level.coords[3 * i], cy = op.level.coords[3 * i + 1], cz = op.level.coords[3 * i + 2];
+    const NGS_GLOBAL T* __restrict__ x = as_global(x_);
+    const NGS_GLOBAL T* __restrict__ u = as_global(u_);
+    const NGS_GLOBAL int32_t* coords = as_global(op.level.coords);
+    const int cx = coords[3 * i], cy = coords[3 * i + 1], cz = coords[3 * i + 2];
     const int l2 = g.log2_stride;
     const int lx = (cx >> l2) - g.origin[0], ly = (cy >> l2) - g.origin[1], lz = (cz >> l2) - g.origin[2];
     T acc = 0;
@@ -55,11 +58,11 @@ __device__ __forceinline__ T stencil_row(const ngs_stencil& op, const float* lap
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
                 if (!((m >> oz) & 1u)) continue;
-                const int32_t k0 = op.cell_start[c], k1 = op.cell_start[c + 1];
+                const int32_t k0 = as_global(op.cell_start)[c], k1 = as_global(op.cell_start)[c + 1];
                 ++c;
                 // the cell sits at offset (ox-1, oy-1, oz-1) from voxel i: B_i(p) = bx[2-ox] by[2-oy] bz[2-oz]
                 for (int32_t k = k0; k < k1; ++k) {
-                    const float* bp = op.basis + 9 * (int64_t)k;
+                    const NGS_GLOBAL float* bp = as_global(op.basis) + 9 * (int64_t)k;
                     const T w = (T)bp[2 - ox] * (T)bp[5 - oy] * (T)bp[8 - oz];
                     sc += w * u[k];
                 }
@@ -72,12 +75,13 @@ __device__ __forceinline__ T stencil_row(const ngs_stencil& op, const float* lap
 // u_k of binned point k for the vector x: alpha_k sum_j B_j(p_k) x_j over the present voxels j of
 // the 27 around the point's cell.
 template <typename T>
-__device__ __forceinline__ T stencil_point(const ngs_stencil& op, const T* __restrict__ x, int64_t k) {
+__device__ __forceinline__ T stencil_point(const ngs_stencil& op, const T* __restrict__ x_, int64_t k) {
     const ngs_grid& g = op.level.grid;
+    const NGS_GLOBAL T* __restrict__ x = as_global(x_);
     const int l2 = g.log2_stride;
-    const int32_t* pc = op.point_cell + 3 * k;
+    const NGS_GLOBAL int32_t* pc = as_global(op.point_cell) + 3 * k;
     const int lx = (pc[0] >> l2) - g.origin[0], ly = (pc[1] >> l2) - g.origin[1], lz = (pc[2] >> l2) - g.origin[2];
-    const float* bp = op.basis + 9 * k;
+    const NGS_GLOBAL float* bp = as_global(op.basis) + 9 * k;
     T sum = 0;
 #pragma unroll
     for (int vx = 0; vx < 3; ++vx) {
@@ -96,7 +100,7 @@ __device__ __forceinline__ T stencil_point(const ngs_stencil& op, const T* __res
             }
         }
     }
-    const T a = op.alpha ? (T)op.alpha[k] : (T)op.alpha_scalar;
+    const T a = op.alpha ? (T)as_global(op.alpha)[k] : (T)op.alpha_scalar;
     return a * sum;
 }
 
