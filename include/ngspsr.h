@@ -314,7 +314,8 @@ int ngs_pcg_batch_prepare_stencil(ngs_stencil op, const float *inv_diag, const f
                                   int32_t max_iter, void *workspace, int32_t blk0, int32_t nblk, void *entry,
                                   ngs_stream_t s);
 /* `iters` iterations of every active system in one launch per kernel (graph-replayed when use_graph).
- * flags bit 0: the table may hold stencil systems (adds their point pass to each iteration). */
+ * flags bit 0: the table may hold stencil systems (adds their point and row passes to each
+ * iteration). */
 int ngs_pcg_batch_iterate(const void *table, const int32_t *blk2sys, int32_t total_blocks, int32_t iters,
                           int32_t use_graph, int32_t flags, ngs_stream_t s);
 /* (iter, done) of each system of the host copy of the table into states (2 nsys int32, host);

@@ -1,16 +1,23 @@
-// Jacobi-preconditioned conjugate gradients on the assembled CSR system.
+// Jacobi-preconditioned conjugate gradients on the assembled CSR system (or its stencil form).
 //
 // One iteration = three kernels, no host round trip (bpcg_* below; a solve of one system is a batch
 // of one):
-//   K1 spmv_dot:   q = A p and p.q; the last block to finish computes alpha = rz / pq.
-//   K2 update:     x += alpha p, r -= alpha q, rz' = r.(D^-1 r), rr = r.r; the last block decides
-//                  convergence (rr <= tol^2 bb), counts the iteration and forms beta = rz'/rz.
-//   K3 direction:  p = D^-1 r + beta p.
-// Reductions are FP64, per block then over the block partials in a fixed order by the last block
-// (deterministic). Rows are partitioned in contiguous chunks, and block -> chunk is mapped so that
-// all blocks of one XCD (blockIdx % 8) own one contiguous eighth of the rows: each XCD's L2 then
-// serves the vector gathers of its own slab. Chunks of iterations are captured once into a
-// hipGraph and replayed; kernels of a converged system return immediately.
+//   K1 spmv_dot:   q = A p; per-block partials of p.q.
+//   K2 update:     alpha = rz / sum(p.q); x += alpha p, r -= alpha q; per-block partials of
+//                  rz' = r.(D^-1 r) and rr = r.r.
+//   K3 direction:  rz', rr summed; convergence (rr <= tol^2 bb) or the iteration cap ends the solve,
+//                  otherwise p = D^-1 r + (rz'/rz) p.
+// No kernel waits for another block of its own launch: a kernel that needs a global sum reads all
+// block partials of the previous kernel and sums them in a fixed order in every block (identical,
+// deterministic FP64 results everywhere), and kernel boundaries make the partials visible across
+// XCDs. This avoids per-block release fences, which on this GPU write back the XCD's L2. The scalar
+// state that K3 advances (rz, iteration count, done flag) is double-buffered by the parity of the
+// step within a launch sequence: step u reads slot u&1 and writes slot (u+1)&1, and sequences
+// always have an even number of steps, so a new sequence starts on slot 0.
+// Row chunks: blocks of one XCD (blockIdx % 8) own one contiguous eighth of the rows (the SpMV
+// deals its row tiles round-robin inside that slab), so each XCD's L2 serves the gathers of its
+// own slab. Sequences of iterations are captured once into a hipGraph and replayed; kernels of a
+// converged system return immediately.
 #include "common.h"
 #include "stencil.h"
 
@@ -22,10 +29,17 @@ namespace ngs {
 constexpr int PCG_THREADS = 256;
 constexpr int NUM_XCD = 8;
 
+struct PcgSlot {
+    int32_t iter;  // iterations completed
+    int32_t done;  // 0 running, 1 converged, 2 iteration cap (set in both slots when the solve ends)
+};
+
 struct PcgState {
-    double rz, pq, rr, bb, tol2, alpha, beta, rz_new;
-    int32_t iter, done, max_iter, pad;
-    uint32_t counter[4];
+    double rz[2];   // r.(D^-1 r) of the current iterate, by step parity
+    double rr, bb;  // ||r||^2 of the last completed iteration (recurrence), ||b||^2
+    double tol2;
+    int32_t max_iter, pad;
+    PcgSlot slot[2];
 };
 
 struct PcgArgs {
@@ -38,7 +52,7 @@ struct PcgArgs {
     float* r;
     float* p;
     float* q;
-    double* partials;  // 2 * nblocks
+    double* partials;  // 3 * nblocks: p.q | r.z | r.r (per block)
     PcgState* st;
     int64_t M;
     int64_t chunk;     // rows per block
@@ -58,52 +72,45 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block reduce of two doubles (256 threads), result valid in thread 0.
+// Block reduce of two doubles (256 threads); the sums are returned in every thread.
 __device__ __forceinline__ void block_sum2(double& a, double& b) {
     __shared__ double sa[PCG_THREADS / 64], sb[PCG_THREADS / 64];
     a = wave_sum(a);
     b = wave_sum(b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();  // the arrays may still be read by a previous call
     if (lane == 0) { sa[w] = a; sb[w] = b; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double ta = 0, tb = 0;
-        for (int i = 0; i < PCG_THREADS / 64; ++i) { ta += sa[i]; tb += sb[i]; }
-        a = ta;
-        b = tb;
-    }
-    __syncthreads();
+    double ta = 0, tb = 0;
+#pragma unroll
+    for (int i = 0; i < PCG_THREADS / 64; ++i) { ta += sa[i]; tb += sb[i]; }
+    a = ta;
+    b = tb;
 }
 
-// Publish this block's partials; returns true in every thread of the last block to arrive, after
-// which sums (fixed order) of both partial arrays are in (a, b) of thread 0.
-__device__ bool last_block_reduce_n(double& a, double& b, double* partials, uint32_t* counter, int nblocks, int bid) {
-    __shared__ bool last;
-    block_sum2(a, b);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[bid], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[nblocks + bid], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        uint32_t ticket = atomicAdd(counter, 1u);
-        last = (ticket == (uint32_t)nblocks - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+// Sums of two partial arrays (n entries each) in a fixed order, returned in every thread of the
+// block: every block that calls it computes bit-identical values.
+__device__ __forceinline__ void sum_partials2(const double* pa, const double* pb, int n, double& a, double& b) {
+    const NGS_GLOBAL double* ga = as_global(pa);
+    const NGS_GLOBAL double* gb = as_global(pb);
     double ta = 0, tb = 0;
-    for (int i = threadIdx.x; i < nblocks; i += PCG_THREADS) {
-        ta += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tb += __hip_atomic_load(&partials[nblocks + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll 8
+    for (int i = threadIdx.x; i < n; i += PCG_THREADS) {
+        ta += ga[i];
+        tb += gb ? gb[i] : 0.0;
     }
     block_sum2(ta, tb);
     a = ta;
     b = tb;
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
 }
 
-__device__ __forceinline__ bool last_block_reduce(double& a, double& b, double* partials, uint32_t* counter, int nblocks) {
-    return last_block_reduce_n(a, b, partials, counter, nblocks, blockIdx.x);
+// Block sums of (a, b) written to partials pa[bid], pb[bid] (pb may be null).
+__device__ __forceinline__ void store_partials2(double a, double b, double* pa, double* pb, int bid) {
+    block_sum2(a, b);
+    if (threadIdx.x == 0) {
+        as_global(pa)[bid] = a;
+        if (pb) as_global(pb)[bid] = b;
+    }
 }
 
 // Dot product of CSR row `row` with x by G lanes (16-byte loads of 4 values and 4 columns per lane).
@@ -161,7 +168,7 @@ __device__ __forceinline__ float row_dot(const NGS_GLOBAL int32_t* __restrict__ 
     return sum;
 }
 
-__global__ void __launch_bounds__(PCG_THREADS) pcg_init_kernel(PcgArgs A, double tol) {
+__global__ void __launch_bounds__(PCG_THREADS) pcg_init_kernel(PcgArgs A) {
     const int lb = logical_block();
     const int64_t r0 = (int64_t)lb * A.chunk;
     const int64_t r1 = min(A.M, r0 + A.chunk);
@@ -180,13 +187,24 @@ __global__ void __launch_bounds__(PCG_THREADS) pcg_init_kernel(PcgArgs A, double
         bb += (double)bi * bi;
         rz += (double)bi * z;
     }
-    if (last_block_reduce(bb, rz, A.partials, &A.st->counter[0], A.nblocks) && threadIdx.x == 0) {
+    store_partials2(rz, bb, A.partials + A.nblocks, A.partials + 2 * A.nblocks, lb);
+}
+
+// One block: initial state from the partials of pcg_init_kernel (r.z in slot 0, ||b||^2).
+__global__ void __launch_bounds__(PCG_THREADS) pcg_init_finish_kernel(PcgArgs A, double tol, int max_iter) {
+    double rz, bb;
+    sum_partials2(A.partials + A.nblocks, A.partials + 2 * A.nblocks, A.nblocks, rz, bb);
+    if (threadIdx.x == 0) {
         PcgState* s = A.st;
+        s->rz[0] = rz;
+        s->rz[1] = rz;
         s->bb = bb;
-        s->rz = rz;
+        s->rr = bb;
         s->tol2 = tol * tol;
-        s->iter = 0;
-        s->done = (bb == 0.0) ? 1 : 0;
+        s->max_iter = max_iter;
+        const int done = (bb == 0.0) ? 1 : 0;
+        s->slot[0].iter = s->slot[1].iter = 0;
+        s->slot[0].done = s->slot[1].done = done;
     }
 }
 
@@ -243,8 +261,7 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_kernel(const int32_t
                                                                    const int32_t* __restrict__ cols,
                                                                    const float* __restrict__ vals, const double* __restrict__ b,
                                                                    const double* __restrict__ x, double* __restrict__ r,
-                                                                   int64_t M, int64_t chunk, double* partials,
-                                                                   uint32_t* counter, double* norms) {
+                                                                   int64_t M, int64_t chunk, double* partials) {
     constexpr int G = 8;
     typedef float f4 __attribute__((ext_vector_type(4)));
     typedef int i4 __attribute__((ext_vector_type(4)));
@@ -301,9 +318,16 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_kernel(const int32_t
             bb += bi * bi;
         }
     }
-    if (last_block_reduce(rr, bb, partials, counter, gridDim.x) && threadIdx.x == 0) {
-        norms[0] = rr;
-        norms[1] = bb;
+    store_partials2(rr, bb, partials, partials + gridDim.x, lb);
+}
+
+// One block: norms[0..1] = sums of the two partial arrays of a residual kernel (nb entries each).
+__global__ void __launch_bounds__(PCG_THREADS) norms_finish_kernel(const double* partials, int nb, double* norms) {
+    double a, b;
+    sum_partials2(partials, partials + nb, nb, a, b);
+    if (threadIdx.x == 0) {
+        norms[0] = a;
+        norms[1] = b;
     }
 }
 
@@ -328,8 +352,7 @@ __global__ void __launch_bounds__(PCG_THREADS) stencil_residual_kernel(ngs_stenc
                                                                        const double* __restrict__ x,
                                                                        const double* __restrict__ u,
                                                                        double* __restrict__ r, int64_t chunk,
-                                                                       double* partials, uint32_t* counter,
-                                                                       double* norms) {
+                                                                       double* partials) {
     __shared__ float lap[125];
     for (int t = threadIdx.x; t < 125; t += PCG_THREADS) lap[t] = op.lap[t];
     __syncthreads();
@@ -342,23 +365,13 @@ __global__ void __launch_bounds__(PCG_THREADS) stencil_residual_kernel(ngs_stenc
         rr += ri * ri;
         bb += b[row] * b[row];
     }
-    if (last_block_reduce(rr, bb, partials, counter, gridDim.x) && threadIdx.x == 0) {
-        norms[0] = rr;
-        norms[1] = bb;
-    }
-}
-
-__global__ void zero_state(PcgState* s, int max_iter) {
-    if (threadIdx.x == 0) {
-        s->counter[0] = s->counter[1] = s->counter[2] = s->counter[3] = 0u;
-        s->max_iter = max_iter;
-        s->done = 0;
-        s->iter = 0;
-    }
+    store_partials2(rr, bb, partials, partials + gridDim.x, logical_block());
 }
 
 static int pcg_blocks(int64_t M, int64_t* chunk) {
-    int64_t nb = (M + 1023) / 1024;  // ~1024 rows per block
+    // ~256 rows per block (8 rows per 8-lane group): small levels get enough blocks that each
+    // group walks only a few rows, so their iterations are not bound by a long latency chain
+    int64_t nb = (M + 255) / 256;
     if (nb > 2048) nb = 2048;
     if (nb >= NUM_XCD) nb = (nb / NUM_XCD) * NUM_XCD;
     if (nb < 1) nb = 1;
@@ -375,7 +388,7 @@ static int pcg_layout(int64_t M, PcgArgs* A, char* ws, void** tab = nullptr, int
     A->st = reinterpret_cast<PcgState*>(ws + off);
     off += al256(sizeof(PcgState));
     A->partials = reinterpret_cast<double*>(ws + off);
-    off += al256(sizeof(double) * 2 * nb);
+    off += al256(sizeof(double) * 3 * nb);
     A->r = reinterpret_cast<float*>(ws + off);
     off += al256(sizeof(float) * M);
     A->p = reinterpret_cast<float*>(ws + off);
@@ -394,7 +407,7 @@ static int pcg_layout(int64_t M, PcgArgs* A, char* ws, void** tab = nullptr, int
 static int64_t pcg_ws_bytes(int64_t M) {
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
-    return (int64_t)(al256(sizeof(PcgState)) + al256(sizeof(double) * 2 * nb) + 3 * al256(sizeof(float) * M) + 512 +
+    return (int64_t)(al256(sizeof(PcgState)) + al256(sizeof(double) * 3 * nb) + 3 * al256(sizeof(float) * M) + 512 +
                      al256(sizeof(int32_t) * nb));
 }
 
@@ -435,82 +448,100 @@ __device__ __forceinline__ const BatchSys* batch_lookup(const BatchSys* tab, con
     return B;
 }
 
+// Kernels of one iteration step take the step parity `ph` (0, 1): the state slot they read.
+
+// q = A p and per-block p.q partials for the CSR systems of the table (stencil systems:
+// bpcg_stencil_dot_kernel).
 template <int G, bool NT, bool IL>
-__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(8))) bpcg_spmv_dot_kernel(const BatchSys* tab, const int32_t* blk2sys) {
+__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(7)))
+bpcg_spmv_dot_kernel(const BatchSys* tab, const int32_t* blk2sys, int ph) {
     int lb;
     const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
-    if (!B) return;
+    if (!B || B->kind != SYS_CSR) return;
     const PcgArgs& A = B->a;
-    if (A.st->done) return;
-    const int64_t r0 = (int64_t)lb * A.chunk;
-    const int64_t r1 = min(A.M, r0 + A.chunk);
+    if (A.st->slot[ph].done) return;
     const int lane = threadIdx.x % G, grp = threadIdx.x / G;
     constexpr int GROUPS = PCG_THREADS / G;
+    const RowTiles t = (IL && B->nblk % NUM_XCD == 0) ? tiles_interleaved(A.M, B->nblk, blockIdx.x - B->blk0, GROUPS)
+                                                      : tiles_contiguous(A.M, A.chunk, lb, GROUPS);
+    const NGS_GLOBAL int32_t* __restrict__ rowptr = as_global(A.rowptr);
+    const NGS_GLOBAL int32_t* __restrict__ cols = as_global(A.cols);
+    const NGS_GLOBAL float* __restrict__ vals = as_global(A.vals);
+    const NGS_GLOBAL float* __restrict__ p = as_global(A.p);
+    NGS_GLOBAL float* __restrict__ q = as_global(A.q);
+    double* partials = A.partials;
     double pq = 0;
-    if (B->kind == SYS_STENCIL) {
-        __shared__ float lap[125];
-        for (int t = threadIdx.x; t < 125; t += PCG_THREADS) lap[t] = B->op.lap[t];
-        __syncthreads();
-        const ngs_stencil op = B->op;
-        const float* __restrict__ p = A.p;
-        float* __restrict__ q = A.q;
-        for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
-            const float sum = stencil_row<float>(op, lap, p, op.u, row);
+    for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
+        const int64_t row = t0 + grp;
+        if (row >= t.hi) continue;
+        const float prow = p[row];  // read with the row's other loads, not after its store
+        const float sum = row_dot<G, NT>(rowptr, cols, vals, p, row, lane);
+        if (lane == 0) {
             q[row] = sum;
-            pq += (double)p[row] * sum;
-        }
-    } else {
-        const RowTiles t = (IL && B->nblk % NUM_XCD == 0)
-                               ? tiles_interleaved(A.M, B->nblk, blockIdx.x - B->blk0, GROUPS)
-                               : tiles_contiguous(A.M, A.chunk, lb, GROUPS);
-        const NGS_GLOBAL int32_t* __restrict__ rowptr = as_global(A.rowptr);
-        const NGS_GLOBAL int32_t* __restrict__ cols = as_global(A.cols);
-        const NGS_GLOBAL float* __restrict__ vals = as_global(A.vals);
-        const NGS_GLOBAL float* __restrict__ p = as_global(A.p);
-        NGS_GLOBAL float* __restrict__ q = as_global(A.q);
-        for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
-            const int64_t row = t0 + grp;
-            if (row >= t.hi) continue;
-            float sum = row_dot<G, NT>(rowptr, cols, vals, p, row, lane);
-            if (lane == 0) {
-                q[row] = sum;
-                pq += (double)p[row] * sum;
-            }
+            pq += (double)prow * sum;
         }
     }
-    double dummy = 0;
-    if (last_block_reduce_n(pq, dummy, A.partials, &A.st->counter[1], B->nblk, lb) && threadIdx.x == 0) {
-        A.st->pq = pq;
-        A.st->alpha = A.st->rz / pq;
-    }
+    store_partials2(pq, 0.0, partials, nullptr, lb);
 }
 
-// Point pass of the stencil systems: u = point form of p, the points split evenly over the
-// system's blocks (same XCD-contiguous mapping as its rows).
-__global__ void __launch_bounds__(PCG_THREADS) bpcg_points_kernel(const BatchSys* tab, const int32_t* blk2sys) {
+// The same for the stencil systems of the table (row pass; the point pass ran before).
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_stencil_dot_kernel(const BatchSys* tab, const int32_t* blk2sys,
+                                                                       int ph) {
     int lb;
     const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
     if (!B || B->kind != SYS_STENCIL) return;
     const PcgArgs& A = B->a;
-    if (A.st->done) return;
+    if (A.st->slot[ph].done) return;
+    const int64_t r0 = (int64_t)lb * A.chunk;
+    const int64_t r1 = min(A.M, r0 + A.chunk);
+    __shared__ float lap[125];
+    for (int t = threadIdx.x; t < 125; t += PCG_THREADS) lap[t] = as_global(B->op.lap)[t];
+    __syncthreads();
+    const ngs_stencil op = B->op;
+    const NGS_GLOBAL float* __restrict__ p = as_global(A.p);
+    NGS_GLOBAL float* __restrict__ q = as_global(A.q);
+    double* partials = A.partials;
+    double pq = 0;
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
+        const float sum = stencil_row<float>(op, lap, A.p, op.u, row);
+        q[row] = sum;
+        pq += (double)p[row] * sum;
+    }
+    store_partials2(pq, 0.0, partials, nullptr, lb);
+}
+
+// Point pass of the stencil systems: u = point form of p, the points split evenly over the
+// system's blocks (same XCD-contiguous mapping as its rows).
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_points_kernel(const BatchSys* tab, const int32_t* blk2sys, int ph) {
+    int lb;
+    const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
+    if (!B || B->kind != SYS_STENCIL) return;
+    const PcgArgs& A = B->a;
+    if (A.st->slot[ph].done) return;
     const ngs_stencil op = B->op;
     const int64_t per = (op.n + B->nblk - 1) / B->nblk;
     const int64_t k0 = (int64_t)lb * per;
     const int64_t k1 = min(op.n, k0 + per);
     const float* __restrict__ p = A.p;
-    float* __restrict__ u = op.u;
+    NGS_GLOBAL float* __restrict__ u = as_global(op.u);
     for (int64_t k = k0 + threadIdx.x; k < k1; k += PCG_THREADS) u[k] = stencil_point<float>(op, p, k);
 }
 
-__global__ void __launch_bounds__(PCG_THREADS) bpcg_update_kernel(const BatchSys* tab, const int32_t* blk2sys) {
+// alpha = rz / sum(p.q) (every block sums the partials of the spmv_dot step), then the updates of
+// x and r and per-block partials of r.z and r.r.
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_update_kernel(const BatchSys* tab, const int32_t* blk2sys, int ph) {
     int lb;
     const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
     if (!B) return;
     const PcgArgs& A = B->a;
-    if (A.st->done) return;
+    if (A.st->slot[ph].done) return;
+    const int nb = B->nblk;
+    double* partials = A.partials;
+    double pq, unused;
+    sum_partials2(partials, nullptr, nb, pq, unused);
+    const float alpha = (float)(A.st->rz[ph] / pq);
     const int64_t r0 = (int64_t)lb * A.chunk;
     const int64_t r1 = min(A.M, r0 + A.chunk);
-    const float alpha = (float)A.st->alpha;
     // local restrict copies: the stores below cannot alias the table or the other vectors
     NGS_GLOBAL float* __restrict__ x = as_global(A.x);
     NGS_GLOBAL float* __restrict__ r = as_global(A.r);
@@ -518,45 +549,98 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_update_kernel(const BatchSys
     const NGS_GLOBAL float* __restrict__ q = as_global(A.q);
     const NGS_GLOBAL float* __restrict__ dinv = as_global(A.dinv);
     double rz = 0, rr = 0;
-#pragma unroll 4
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) {
-        const float pi = p[i];
-        x[i] += alpha * pi;
+    // four rows per thread per step: every load of the step is issued before the first store (a
+    // store counts against the same wait counter as the loads); single rows for the tail
+    constexpr int R = 4;
+    int64_t i = r0 + threadIdx.x;
+    for (; i + (R - 1) * PCG_THREADS < r1; i += R * PCG_THREADS) {
+        float pv[R], xv[R], rv[R], qv[R], dv[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int64_t k = i + u * PCG_THREADS;
+            pv[u] = p[k];
+            xv[u] = x[k];
+            rv[u] = r[k];
+            qv[u] = q[k];
+            dv[u] = dinv[k];
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int64_t k = i + u * PCG_THREADS;
+            const float ri = rv[u] - alpha * qv[u];
+            const float z = dv[u] * ri;
+            rz += (double)ri * z;
+            rr += (double)ri * ri;
+            x[k] = xv[u] + alpha * pv[u];
+            r[k] = ri;
+        }
+    }
+    for (; i < r1; i += PCG_THREADS) {
         const float ri = r[i] - alpha * q[i];
-        r[i] = ri;
         const float z = dinv[i] * ri;
         rz += (double)ri * z;
         rr += (double)ri * ri;
+        x[i] += alpha * p[i];
+        r[i] = ri;
     }
-    if (last_block_reduce_n(rz, rr, A.partials, &A.st->counter[2], B->nblk, lb) && threadIdx.x == 0) {
-        PcgState* s = A.st;
-        s->iter += 1;
-        s->rr = rr;
-        if (rr <= s->tol2 * s->bb) {
-            s->done = 1;
-        } else if (s->iter >= s->max_iter) {
-            s->done = 2;
-        } else {
-            s->beta = rz / s->rz;
-            s->rz = rz;
-        }
-    }
+    store_partials2(rz, rr, partials + nb, partials + 2 * nb, lb);
 }
 
-__global__ void __launch_bounds__(PCG_THREADS) bpcg_direction_kernel(const BatchSys* tab, const int32_t* blk2sys) {
+// Every block sums rz' and rr of the update step; the iteration ends the solve on convergence
+// (rr <= tol^2 bb) or at the cap, otherwise p = D^-1 r + (rz'/rz) p. Block 0 advances the state
+// into slot ph^1 (slot ph is still being read by the other blocks).
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_direction_kernel(const BatchSys* tab, const int32_t* blk2sys,
+                                                                     int ph) {
     int lb;
     const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
     if (!B) return;
     const PcgArgs& A = B->a;
-    if (A.st->done) return;
-    const int64_t r0 = (int64_t)lb * A.chunk;
-    const int64_t r1 = min(A.M, r0 + A.chunk);
-    const float beta = (float)A.st->beta;
-    NGS_GLOBAL float* __restrict__ p = as_global(A.p);
-    const NGS_GLOBAL float* __restrict__ r = as_global(A.r);
-    const NGS_GLOBAL float* __restrict__ dinv = as_global(A.dinv);
-#pragma unroll 4
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) p[i] = dinv[i] * r[i] + beta * p[i];
+    PcgState* st = A.st;
+    if (st->slot[ph].done) return;
+    const int nb = B->nblk;
+    double rz_new, rr;
+    sum_partials2(A.partials + nb, A.partials + 2 * nb, nb, rz_new, rr);
+    const int it = st->slot[ph].iter + 1;
+    const bool conv = rr <= st->tol2 * st->bb;
+    const bool cap = !conv && it >= st->max_iter;
+    if (!conv && !cap) {
+        const float beta = (float)(rz_new / st->rz[ph]);
+        const int64_t r0 = (int64_t)lb * A.chunk;
+        const int64_t r1 = min(A.M, r0 + A.chunk);
+        NGS_GLOBAL float* __restrict__ p = as_global(A.p);
+        const NGS_GLOBAL float* __restrict__ r = as_global(A.r);
+        const NGS_GLOBAL float* __restrict__ dinv = as_global(A.dinv);
+        constexpr int R = 4;
+        int64_t i = r0 + threadIdx.x;
+        for (; i + (R - 1) * PCG_THREADS < r1; i += R * PCG_THREADS) {
+            float pv[R], rv[R], dv[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int64_t k = i + u * PCG_THREADS;
+                pv[u] = p[k];
+                rv[u] = r[k];
+                dv[u] = dinv[k];
+            }
+#pragma unroll
+            for (int u = 0; u < R; ++u) p[i + u * PCG_THREADS] = dv[u] * rv[u] + beta * pv[u];
+        }
+        for (; i < r1; i += PCG_THREADS) p[i] = dinv[i] * r[i] + beta * p[i];
+    }
+    if (lb == 0 && threadIdx.x == 0) {
+        st->rr = rr;
+        const int nph = ph ^ 1;
+        if (conv || cap) {
+            // final in both slots: later steps of either parity see the solve as ended
+            const int d = conv ? 1 : 2;
+            st->slot[nph].iter = it;
+            st->slot[nph].done = d;
+            st->slot[ph].iter = it;
+            st->slot[ph].done = d;
+        } else {
+            st->rz[nph] = rz_new;
+            st->slot[nph].iter = it;
+        }
+    }
 }
 
 __global__ void write_entry_kernel(BatchSys* dst, BatchSys src) { *dst = src; }
@@ -591,16 +675,17 @@ static void dispatch_spmv(F&& f) {
     else dispatch_spmv_g(f, std::false_type());
 }
 
-static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, int total_blocks, int flags,
+static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, int total_blocks, int flags, int ph,
                                    hipStream_t st) {
     dim3 grid(total_blocks), block(PCG_THREADS);
-    if (flags & 1) bpcg_points_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
+    if (flags & 1) bpcg_points_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
     dispatch_spmv([&](auto g, auto nt, auto il) {
         bpcg_spmv_dot_kernel<decltype(g)::value, decltype(nt)::value, decltype(il)::value>
-            <<<grid, block, 0, st>>>(tab, blk2sys);
+            <<<grid, block, 0, st>>>(tab, blk2sys, ph);
     });
-    bpcg_update_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
-    bpcg_direction_kernel<<<grid, block, 0, st>>>(tab, blk2sys);
+    if (flags & 1) bpcg_stencil_dot_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
+    bpcg_update_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
+    bpcg_direction_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
 }
 
 }  // namespace ngs
@@ -633,6 +718,7 @@ int ngs_pcg_config(int32_t use_graph, int32_t lanes_per_row, int32_t nontemporal
 static int pcg_solve_entry(BatchSys E, int64_t M, double tol, int32_t max_iter, int32_t chunk, void* workspace,
                            ngs_stream_t s, int32_t* iters, double* rel_resid) {
     if (chunk <= 0) chunk = 32;
+    chunk += chunk & 1;  // launch sequences have an even number of steps (state slot parity)
     if (max_iter < 0) max_iter = 10000000;
     hipStream_t st = S(s);
     HostState& hs = host_state();
@@ -648,8 +734,8 @@ static int pcg_solve_entry(BatchSys E, int64_t M, double tol, int32_t max_iter, 
     int32_t* map = nullptr;
     PcgArgs A = E.a;
     pcg_layout(M, &A, reinterpret_cast<char*>(workspace), &tab, &map);
-    zero_state<<<1, 64, 0, work>>>(A.st, max_iter);
-    pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, work>>>(A, tol);
+    pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, work>>>(A);
+    pcg_init_finish_kernel<<<1, PCG_THREADS, 0, work>>>(A, tol, max_iter);
     E.a = A;
     E.blk0 = 0;
     E.nblk = A.nblocks;
@@ -663,7 +749,7 @@ static int pcg_solve_entry(BatchSys E, int64_t M, double tol, int32_t max_iter, 
         int r = ngs_pcg_batch_iterate(tab, map, A.nblocks, chunk, g_pcg_use_graph, flags,
                                       reinterpret_cast<ngs_stream_t>(work));
         if (r != NGS_OK) return r;
-        NGS_CHECK_HIP(hipMemcpyAsync(hs.pinned, &A.st->iter, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, work));
+        NGS_CHECK_HIP(hipMemcpyAsync(hs.pinned, &A.st->slot[0], sizeof(PcgSlot), hipMemcpyDeviceToHost, work));
         NGS_CHECK_HIP(hipStreamSynchronize(work));
         it = hs.pinned[0];
         done = hs.pinned[1];
@@ -750,10 +836,9 @@ int ngs_residual_f64(const int32_t* rowptr, const int32_t* cols, const float* va
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
     char* w = reinterpret_cast<char*>(workspace);
-    PcgState* stt = reinterpret_cast<PcgState*>(w);
     double* partials = reinterpret_cast<double*>(w + al256(sizeof(PcgState)));
-    zero_state<<<1, 64, 0, S(s)>>>(stt, 0);
-    residual_f64_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, b, x, r, M, chunk, partials, &stt->counter[3], norms);
+    residual_f64_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, b, x, r, M, chunk, partials);
+    norms_finish_kernel<<<1, PCG_THREADS, 0, S(s)>>>(partials, nb, norms);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
@@ -785,10 +870,9 @@ int ngs_stencil_residual_f64(ngs_stencil op, const double* b, const double* x, d
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
     char* w = reinterpret_cast<char*>(workspace);
-    PcgState* stt = reinterpret_cast<PcgState*>(w);
     double* partials = reinterpret_cast<double*>(w + al256(sizeof(PcgState)));
-    zero_state<<<1, 64, 0, S(s)>>>(stt, 0);
-    stencil_residual_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(op, b, x, u64, r, chunk, partials, &stt->counter[3], norms);
+    stencil_residual_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(op, b, x, u64, r, chunk, partials);
+    norms_finish_kernel<<<1, PCG_THREADS, 0, S(s)>>>(partials, nb, norms);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
@@ -808,8 +892,8 @@ static int batch_prepare(BatchSys* E, int64_t M, double tol, int32_t max_iter, v
     PcgArgs A = E->a;
     pcg_layout(M, &A, reinterpret_cast<char*>(workspace));
     if (max_iter < 0) max_iter = 10000000;
-    zero_state<<<1, 64, 0, S(s)>>>(A.st, max_iter);
-    pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, S(s)>>>(A, tol);
+    pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, S(s)>>>(A);
+    pcg_init_finish_kernel<<<1, PCG_THREADS, 0, S(s)>>>(A, tol, max_iter);
     NGS_CHECK_LAUNCH();
     E->a = A;
     E->blk0 = blk0;
@@ -871,11 +955,12 @@ int32_t ngs_pcg_batch_blocks(int64_t M) {
 // (table, map, total_blocks, iters) and replayed; the table contents may change between calls.
 int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t total_blocks, int32_t iters,
                           int32_t use_graph, int32_t flags, ngs_stream_t s) {
-    NGS_REQUIRE(table && blk2sys && total_blocks > 0 && iters > 0, "ngs_pcg_batch_iterate: bad arguments");
+    NGS_REQUIRE(table && blk2sys && total_blocks > 0 && iters > 0 && iters % 2 == 0,
+                "ngs_pcg_batch_iterate: bad arguments (iters must be even and positive)");
     const BatchSys* tab = reinterpret_cast<const BatchSys*>(table);
     hipStream_t st = S(s);
     if (!use_graph) {
-        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, flags, st);
+        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, flags, i & 1, st);
         NGS_CHECK_LAUNCH();
         return NGS_OK;
     }
@@ -893,7 +978,7 @@ int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t tot
     if (!exec) {
         hipGraph_t graph;
         NGS_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, flags, st);
+        for (int i = 0; i < iters; ++i) launch_batch_iteration(tab, blk2sys, total_blocks, flags, i & 1, st);
         NGS_CHECK_HIP(hipStreamEndCapture(st, &graph));
         NGS_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
@@ -912,7 +997,7 @@ int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t tot
 int ngs_pcg_batch_status(const void* host_table, int32_t nsys, int32_t* states, ngs_stream_t s) {
     const BatchSys* tab = reinterpret_cast<const BatchSys*>(host_table);
     for (int i = 0; i < nsys; ++i)
-        NGS_CHECK_HIP(hipMemcpyAsync(states + 2 * i, &tab[i].a.st->iter, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, S(s)));
+        NGS_CHECK_HIP(hipMemcpyAsync(states + 2 * i, &tab[i].a.st->slot[0], sizeof(PcgSlot), hipMemcpyDeviceToHost, S(s)));
     NGS_CHECK_HIP(hipStreamSynchronize(S(s)));
     return NGS_OK;
 }
