@@ -121,6 +121,39 @@ def cpu_baseline(system, b, budget_s=15.0):
     return per_iter, it
 
 
+COPY_BYTES = 1 << 30
+
+
+def pmc_probe(res):
+    """Kernels for the HBM counter runs: a 1 GiB device copy (its FETCH_SIZE / WRITE_SIZE readings
+    calibrate the counters against 1 GiB read + 1 GiB written) and the finest-level CSR SpMV."""
+    import torch
+    from ngspsr._lib import call, ptr, stream
+    from ngspsr.solver import StencilSystem
+    system = res.reconstructor.systems[0][0]
+    csr = system.to_csr() if isinstance(system, StencilSystem) else system
+    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=csr.device)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    x = torch.randn(csr.M, device=csr.device)
+    y = torch.empty_like(x)
+    for _ in range(5):
+        call("ngs_spmv", ptr(csr.rowptr), ptr(csr.cols), ptr(csr.vals), ptr(x), ptr(y), csr.M, stream())
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_probe": True, "M": csr.M, "nnz": csr.nnz(), "copy_bytes": COPY_BYTES}), flush=True)
+
+
+def measured_traffic(kernel_prefix="spmv_kernel"):
+    """HBM bytes per launch of the SpMV from the committed counter summary (profiles/), if present."""
+    path = os.path.join(ROOT, "profiles", "traffic_spmv_ellipsoids2M_512.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("hbm_bytes_per_launch")
+
+
 def serving_scenes(n_req, seed0=1000):
     """Requests of the serving stream: random-ellipsoid scenes, points log-uniform in [1e4, 1e6],
     depth uniform in {7, 8, 9} (voxel size 2^-depth), sigma uniform in [0, 0.005]."""
@@ -207,6 +240,9 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving", type=int, default=64, help="requests of the serving comparison (0: skip)")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="counter probe: one reconstruction, then 3 device copies of 1 GiB (calibration) and 5 "
+                         "launches of the finest-level SpMV; run under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding of the path is described "
@@ -225,6 +261,10 @@ def main():
     def step():
         return ngspsr.reconstruct(xyz, nrm, cfg["vs"], depth=cfg["depth"], min_density=cfg["min_density"],
                                   screen_alpha=4.0, solver=cfg["solver"], return_result=True)
+
+    if args.pmc_probe:
+        pmc_probe(step())
+        return
 
     for _ in range(args.warmup):
         res = step()
@@ -249,9 +289,9 @@ def main():
     csr = system.to_csr() if isinstance(system, StencilSystem) else system
     t_spmv, algo, nnz = spmv_roofline(csr)
     t_iter = pcg_iteration_rate(system)
-    spmv_obj = {"kernel": "spmv_kernel<16,1> (CSR SpMV, the solve_pcg_diag operator)", "bound": "hbm",
+    spmv_obj = {"kernel": "spmv_kernel<8,0,1> (finest-level CSR SpMV, the solve_pcg_diag operator)", "bound": "hbm",
                 "achieved": round(algo / t_spmv / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                 "launch_us": round(1e6 * t_spmv, 2), "algorithmic_bytes": int(algo)}
     if isinstance(system, StencilSystem):
         t_pts, b_pts, t_rows, b_rows = stencil_roofline(system)
