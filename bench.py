@@ -53,17 +53,22 @@ def _event_time(fn, reps):
     return s.elapsed_time(e) / 1e3 / reps
 
 
-def spmv_roofline(system, reps=50):
-    """Average duration of the CSR SpMV kernel (spmv_kernel) on a level system and its algorithmic
-    bytes (8 nnz + 4 (M+1) + 8 M)."""
+def spmv_roofline(system, reps=50, packed=False):
+    """Average duration of the CSR SpMV kernel on a level system and its algorithmic bytes:
+    int32 columns (spmv_kernel): 8 nnz + 4 (M+1) + 8 M; packed columns (spmv_packed_kernel):
+    6 nnz + 20 M (plane bases) + 4 (M+1) + 8 M."""
     import torch
     from ngspsr._lib import call, ptr, stream
     x = torch.randn(system.M, device=system.device)
     y = torch.empty_like(x)
-    t = _event_time(lambda: call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x), ptr(y),
-                                 system.M, stream()), reps)
+    if packed:
+        t = _event_time(lambda: call("ngs_spmv_packed", ptr(system.rowptr), ptr(system.codes), ptr(system.pbase),
+                                     ptr(system.vals), ptr(x), ptr(y), system.M, stream()), reps)
+    else:
+        t = _event_time(lambda: call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x),
+                                     ptr(y), system.M, stream()), reps)
     nnz = system.nnz()
-    algo = 8 * nnz + 4 * (system.M + 1) + 8 * system.M
+    algo = (6 * nnz + 20 * system.M if packed else 8 * nnz) + 4 * (system.M + 1) + 8 * system.M
     return t, algo, nnz
 
 
@@ -140,18 +145,23 @@ def pmc_probe(res):
     y = torch.empty_like(x)
     for _ in range(5):
         call("ngs_spmv", ptr(csr.rowptr), ptr(csr.cols), ptr(csr.vals), ptr(x), ptr(y), csr.M, stream())
+    if csr.packed:
+        for _ in range(5):
+            call("ngs_spmv_packed", ptr(csr.rowptr), ptr(csr.codes), ptr(csr.pbase), ptr(csr.vals), ptr(x), ptr(y),
+                 csr.M, stream())
     torch.cuda.synchronize()
     print(json.dumps({"pmc_probe": True, "M": csr.M, "nnz": csr.nnz(), "copy_bytes": COPY_BYTES}), flush=True)
 
 
-def measured_traffic(kernel_prefix="spmv_kernel"):
-    """HBM bytes per launch of the SpMV from the committed counter summary (profiles/), if present."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of an SpMV kernel ("spmv_kernel" or "spmv_packed_kernel") at the default
+    workload, from the committed counter summary (profiles/), if present."""
     path = os.path.join(ROOT, "profiles", "traffic_spmv_ellipsoids2M_512.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    return t.get("hbm_bytes_per_launch")
+    return t.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
 
 
 def serving_scenes(n_req, seed0=1000):
@@ -289,10 +299,18 @@ def main():
     csr = system.to_csr() if isinstance(system, StencilSystem) else system
     t_spmv, algo, nnz = spmv_roofline(csr)
     t_iter = pcg_iteration_rate(system)
-    spmv_obj = {"kernel": "spmv_kernel<8,0,1> (finest-level CSR SpMV, the solve_pcg_diag operator)", "bound": "hbm",
-                "achieved": round(algo / t_spmv / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+    spmv_obj = {"kernel": "spmv_kernel<8,0,1> (finest-level CSR SpMV, int32 columns, the solve_pcg_diag operator)",
+                "bound": "hbm", "achieved": round(algo / t_spmv / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": measured_traffic("spmv_kernel"),
                 "launch_us": round(1e6 * t_spmv, 2), "algorithmic_bytes": int(algo)}
+    packed_obj = None
+    if getattr(csr, "packed", False):
+        t_pk, algo_pk, _ = spmv_roofline(csr, packed=True)
+        packed_obj = {"kernel": "spmv_packed_kernel<8,1> (finest-level CSR SpMV, packed 16-bit columns; the PCG's "
+                                "operator)", "bound": "hbm", "achieved": round(algo_pk / t_pk / 1e9, 1),
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(algo_pk / t_pk / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": measured_traffic("spmv_packed_kernel"), "launch_us": round(1e6 * t_pk, 2),
+                      "algorithmic_bytes": int(algo_pk)}
     if isinstance(system, StencilSystem):
         t_pts, b_pts, t_rows, b_rows = stencil_roofline(system)
         iter_bytes = b_pts + b_rows + 44 * system.M
@@ -304,6 +322,9 @@ def main():
                                "algorithmic_bytes": int(b_pts), "achieved": round(b_pts / t_pts / 1e9, 1)},
                 "operator_us": round(1e6 * (t_pts + t_rows), 2),
                 "csr_spmv_us_same_system": round(1e6 * t_spmv, 2)}
+    elif packed_obj is not None:
+        iter_bytes = packed_obj["algorithmic_bytes"] + 44 * system.M
+        roof = dict(packed_obj)
     else:
         iter_bytes = algo + 44 * system.M
         roof = dict(spmv_obj)

@@ -221,6 +221,16 @@ int ngs_lhs_assemble(ngs_level lv, ngs_point_bins pts, const float *coarse, floa
                      const int32_t *rowptr, int32_t *cols, float *vals, float *inv_diag, double *rhs,
                      ngs_stream_t s);
 
+/* ngs_lhs_assemble plus the packed-column form of the same matrix (no reference counterpart; the
+ * reference stores int32 columns, solver.py:40-48): for entry k of row i with neighbour offset
+ * (dx, dy, dz), codes[k] = (dx + 2) << 13 | (column - plane base of dx), where the plane base is the
+ * row's first column with that dx; pbase[8 i + d] = plane base of dx = d - 2 (0 if absent), pbase
+ * entries 5..7 = 0. *overflow (device int32, caller-zeroed) is set when an offset needs more than
+ * 13 bits; the packed arrays are then unusable and the int32 columns remain valid. */
+int ngs_lhs_assemble_packed(ngs_level lv, ngs_point_bins pts, const float *coarse, float sf, float delta,
+                            const int32_t *rowptr, int32_t *cols, uint16_t *codes, int32_t *pbase, int32_t *overflow,
+                            float *vals, float *inv_diag, double *rhs, ngs_stream_t s);
+
 /* Stiffness part of the cross-level correction (reconstructor.py:153-168, COO of
  * get_neighbours(d, dd, 5), integrate_deriv_deriv_product bezier_tensor.py:75-96):
  * rhs_i -= sum_j K_{d,dd}(i, j) x_dd[j] over the 125 dd-voxels around the dd-cell of i. */
@@ -284,6 +294,12 @@ int ngs_pcg_solve(const int32_t *rowptr, const int32_t *cols, const float *vals,
                   const float *b, float *x, int64_t M, double tol, int32_t max_iter, int32_t chunk,
                   void *workspace, ngs_stream_t s, int32_t *iters, double *rel_resid);
 
+/* ngs_pcg_solve on the packed-column form of the matrix (ngs_lhs_assemble_packed): same iteration,
+ * 6 instead of 8 bytes per stored entry in the SpMV. */
+int ngs_pcg_solve_packed(const int32_t *rowptr, const uint16_t *codes, const int32_t *pbase, const float *vals,
+                         const float *inv_diag, const float *b, float *x, int64_t M, double tol, int32_t max_iter,
+                         int32_t chunk, void *workspace, ngs_stream_t s, int32_t *iters, double *rel_resid);
+
 /* ngs_pcg_solve on the stencil form of the system (same iteration, same stop rule; the SpMV is the
  * point pass + row pass of ngs_stencil_apply). */
 int ngs_pcg_solve_stencil(ngs_stencil op, const float *inv_diag, const float *b, float *x, double tol,
@@ -309,18 +325,28 @@ int ngs_pcg_batch_prepare(const int32_t *rowptr, const int32_t *cols, const floa
                           int32_t blk0, int32_t nblk, void *entry, ngs_stream_t s);
 /* Set the first block of a prepared host table entry (a multiple of 8). */
 int ngs_pcg_batch_place(void *entry, int32_t blk0);
+/* The same as ngs_pcg_batch_prepare for a system in packed-column form. */
+int ngs_pcg_batch_prepare_packed(const int32_t *rowptr, const uint16_t *codes, const int32_t *pbase, const float *vals,
+                                 const float *inv_diag, const float *b, float *x, int64_t M, double tol,
+                                 int32_t max_iter, void *workspace, int32_t blk0, int32_t nblk, void *entry,
+                                 ngs_stream_t s);
 /* The same as ngs_pcg_batch_prepare for a system in stencil form. */
 int ngs_pcg_batch_prepare_stencil(ngs_stencil op, const float *inv_diag, const float *b, float *x, double tol,
                                   int32_t max_iter, void *workspace, int32_t blk0, int32_t nblk, void *entry,
                                   ngs_stream_t s);
-/* `iters` iterations of every active system in one launch per kernel (graph-replayed when use_graph).
- * flags bit 0: the table may hold stencil systems (adds their point and row passes to each
- * iteration). */
+/* `iters` (even) iterations of every active system in one launch per kernel (graph-replayed when
+ * use_graph). flags: bit 0 the table may hold stencil systems (adds their point and row passes),
+ * bit 1 it may hold packed-column systems (adds their SpMV), bit 2 it holds no int32-column CSR
+ * systems (skips that SpMV). */
 int ngs_pcg_batch_iterate(const void *table, const int32_t *blk2sys, int32_t total_blocks, int32_t iters,
                           int32_t use_graph, int32_t flags, ngs_stream_t s);
 /* (iter, done) of each system of the host copy of the table into states (2 nsys int32, host);
  * synchronises s. done = 1 converged, 2 iteration cap. */
 int ngs_pcg_batch_status(const void *host_table, int32_t nsys, int32_t *states, ngs_stream_t s);
+
+/* y = A x (FP32) on the packed-column form, exposed for measurement. */
+int ngs_spmv_packed(const int32_t *rowptr, const uint16_t *codes, const int32_t *pbase, const float *vals,
+                    const float *x, float *y, int64_t M, ngs_stream_t s);
 
 /* y = A x (FP32), the SpMV of the solver, exposed for measurement. */
 int ngs_spmv(const int32_t *rowptr, const int32_t *cols, const float *vals, const float *x, float *y,

@@ -45,6 +45,8 @@ struct PcgState {
 struct PcgArgs {
     const int32_t* rowptr;
     const int32_t* cols;
+    const uint16_t* codes;  // packed columns (kind SYS_PACKED): (dx + 2) << 13 | offset
+    const int32_t* pbase;   // packed columns: 8 int32 per row, the five plane bases
     const float* vals;
     const float* dinv;
     const float* b;
@@ -120,14 +122,12 @@ __device__ __forceinline__ void store_partials2(double a, double b, double* pa, 
 // issues all matrix loads, then all gathers, with no wait between them. Longer rows of a caller's
 // CSR continue in a plain loop. NT: non-temporal loads for the matrix stream.
 template <int G, bool NT>
-__device__ __forceinline__ float row_dot(const NGS_GLOBAL int32_t* __restrict__ rowptr,
-                                         const NGS_GLOBAL int32_t* __restrict__ cols,
+__device__ __forceinline__ float row_dot(int32_t k0, int32_t k1, const NGS_GLOBAL int32_t* __restrict__ cols,
                                          const NGS_GLOBAL float* __restrict__ vals,
-                                         const NGS_GLOBAL float* __restrict__ x, int64_t row, int lane) {
+                                         const NGS_GLOBAL float* __restrict__ x, int lane) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     typedef int i4 __attribute__((ext_vector_type(4)));
     constexpr int U = 128 / (4 * G);
-    const int32_t k0 = rowptr[row], k1 = rowptr[row + 1];
     float sum = 0.f;
     if (k1 > k0) {
         const int32_t klast = k1 - 4;
@@ -161,6 +161,59 @@ __device__ __forceinline__ float row_dot(const NGS_GLOBAL int32_t* __restrict__ 
             const f4 vv = *reinterpret_cast<const NGS_GLOBAL f4*>(vals + k);
             const i4 cc = *reinterpret_cast<const NGS_GLOBAL i4*>(cols + k);
             sum += vv.x * x[cc.x] + vv.y * x[cc.y] + vv.z * x[cc.z] + vv.w * x[cc.w];
+        }
+    }
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    return sum;
+}
+
+// Row dot of the packed-column CSR: the same schedule as row_dot, with each 16-bit column code
+// decoded against the row's five plane bases (column = base[code >> 13] + (code & 0x1fff)).
+template <int G>
+__device__ __forceinline__ float row_dot_packed(int32_t k0, int32_t k1, const NGS_GLOBAL uint16_t* __restrict__ codes,
+                                                const NGS_GLOBAL int32_t* __restrict__ pbase,
+                                                const NGS_GLOBAL float* __restrict__ vals,
+                                                const NGS_GLOBAL float* __restrict__ x, int64_t row, int lane) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    typedef unsigned short us4 __attribute__((ext_vector_type(4)));
+    constexpr int U = 128 / (4 * G);
+    const i4 b03 = *reinterpret_cast<const NGS_GLOBAL i4*>(pbase + 8 * row);
+    const int32_t b4 = pbase[8 * row + 4];
+    float sum = 0.f;
+    auto decode = [&](unsigned code) -> int32_t {
+        const unsigned d = code >> 13;
+        const int32_t base = d == 0 ? b03.x : d == 1 ? b03.y : d == 2 ? b03.z : d == 3 ? b03.w : b4;
+        return base + (int32_t)(code & 0x1fffu);
+    };
+    if (k1 > k0) {
+        const int32_t klast = k1 - 4;
+        f4 v[U];
+        us4 c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t k = min(k0 + 4 * lane + 4 * G * u, klast);
+            v[u] = *reinterpret_cast<const NGS_GLOBAL f4*>(vals + k);
+            c[u] = *reinterpret_cast<const NGS_GLOBAL us4*>(codes + k);
+        }
+        float xs[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            xs[u][0] = x[decode(c[u].x)];
+            xs[u][1] = x[decode(c[u].y)];
+            xs[u][2] = x[decode(c[u].z)];
+            xs[u][3] = x[decode(c[u].w)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float part = v[u].x * xs[u][0] + v[u].y * xs[u][1] + v[u].z * xs[u][2] + v[u].w * xs[u][3];
+            sum += (k0 + 4 * lane + 4 * G * u < k1) ? part : 0.f;
+        }
+        for (int32_t k = k0 + 128 + 4 * lane; k < k1; k += 4 * G) {
+            const f4 vv = *reinterpret_cast<const NGS_GLOBAL f4*>(vals + k);
+            const us4 cc = *reinterpret_cast<const NGS_GLOBAL us4*>(codes + k);
+            sum += vv.x * x[decode(cc.x)] + vv.y * x[decode(cc.y)] + vv.z * x[decode(cc.z)] + vv.w * x[decode(cc.w)];
         }
     }
 #pragma unroll
@@ -247,10 +300,30 @@ __global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_e
     constexpr int GROUPS = PCG_THREADS / G;
     const RowTiles t = IL ? tiles_interleaved(M, gridDim.x, blockIdx.x, GROUPS)
                           : tiles_contiguous(M, chunk, logical_block(), GROUPS);
+    const NGS_GLOBAL int32_t* rp = as_global(rowptr);
     for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
         const int64_t row = t0 + grp;
         if (row >= t.hi) continue;
-        float sum = row_dot<G, NT>(as_global(rowptr), as_global(cols), as_global(vals), as_global(x), row, lane);
+        float sum = row_dot<G, NT>(rp[row], rp[row + 1], as_global(cols), as_global(vals), as_global(x), lane);
+        if (lane == 0) y[row] = sum;
+    }
+}
+
+template <int G, bool IL>
+__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(7)))
+spmv_packed_kernel(const int32_t* __restrict__ rowptr, const uint16_t* __restrict__ codes,
+                   const int32_t* __restrict__ pbase, const float* __restrict__ vals, const float* __restrict__ x,
+                   float* __restrict__ y, int64_t M, int64_t chunk) {
+    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    constexpr int GROUPS = PCG_THREADS / G;
+    const RowTiles t = IL ? tiles_interleaved(M, gridDim.x, blockIdx.x, GROUPS)
+                          : tiles_contiguous(M, chunk, logical_block(), GROUPS);
+    const NGS_GLOBAL int32_t* rp = as_global(rowptr);
+    for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
+        const int64_t row = t0 + grp;
+        if (row >= t.hi) continue;
+        float sum = row_dot_packed<G>(rp[row], rp[row + 1], as_global(codes), as_global(pbase), as_global(vals),
+                                      as_global(x), row, lane);
         if (lane == 0) y[row] = sum;
     }
 }
@@ -426,7 +499,10 @@ static HostState& host_state() {
 // contiguous range of blocks each one owns; every kernel looks up its system from its block index,
 // so one launch (and one captured graph, whose arguments are the fixed table pointers) serves any
 // mix of systems. Systems join and leave between launches by rewriting the table.
-enum : int32_t { SYS_CSR = 0, SYS_STENCIL = 1 };
+enum : int32_t { SYS_CSR = 0, SYS_STENCIL = 1, SYS_PACKED = 2 };
+
+// Flags of a launch sequence (which kernels the table's systems need).
+enum : int32_t { FLAG_STENCIL = 1, FLAG_PACKED = 2, FLAG_NO_CSR = 4 };
 
 struct BatchSys {
     PcgArgs a;
@@ -475,7 +551,42 @@ bpcg_spmv_dot_kernel(const BatchSys* tab, const int32_t* blk2sys, int ph) {
         const int64_t row = t0 + grp;
         if (row >= t.hi) continue;
         const float prow = p[row];  // read with the row's other loads, not after its store
-        const float sum = row_dot<G, NT>(rowptr, cols, vals, p, row, lane);
+        const float sum = row_dot<G, NT>(rowptr[row], rowptr[row + 1], cols, vals, p, lane);
+        if (lane == 0) {
+            q[row] = sum;
+            pq += (double)prow * sum;
+        }
+    }
+    store_partials2(pq, 0.0, partials, nullptr, lb);
+}
+
+// q = A p and per-block p.q partials for the packed-column CSR systems of the table.
+template <bool IL>
+__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(7)))
+bpcg_spmv_packed_dot_kernel(const BatchSys* tab, const int32_t* blk2sys, int ph) {
+    constexpr int G = 8;
+    int lb;
+    const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
+    if (!B || B->kind != SYS_PACKED) return;
+    const PcgArgs& A = B->a;
+    if (A.st->slot[ph].done) return;
+    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    constexpr int GROUPS = PCG_THREADS / G;
+    const RowTiles t = (IL && B->nblk % NUM_XCD == 0) ? tiles_interleaved(A.M, B->nblk, blockIdx.x - B->blk0, GROUPS)
+                                                      : tiles_contiguous(A.M, A.chunk, lb, GROUPS);
+    const NGS_GLOBAL int32_t* __restrict__ rowptr = as_global(A.rowptr);
+    const NGS_GLOBAL uint16_t* __restrict__ codes = as_global(A.codes);
+    const NGS_GLOBAL int32_t* __restrict__ pbase = as_global(A.pbase);
+    const NGS_GLOBAL float* __restrict__ vals = as_global(A.vals);
+    const NGS_GLOBAL float* __restrict__ p = as_global(A.p);
+    NGS_GLOBAL float* __restrict__ q = as_global(A.q);
+    double* partials = A.partials;
+    double pq = 0;
+    for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
+        const int64_t row = t0 + grp;
+        if (row >= t.hi) continue;
+        const float prow = p[row];
+        const float sum = row_dot_packed<G>(rowptr[row], rowptr[row + 1], codes, pbase, vals, p, row, lane);
         if (lane == 0) {
             q[row] = sum;
             pq += (double)prow * sum;
@@ -678,12 +789,18 @@ static void dispatch_spmv(F&& f) {
 static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, int total_blocks, int flags, int ph,
                                    hipStream_t st) {
     dim3 grid(total_blocks), block(PCG_THREADS);
-    if (flags & 1) bpcg_points_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
-    dispatch_spmv([&](auto g, auto nt, auto il) {
-        bpcg_spmv_dot_kernel<decltype(g)::value, decltype(nt)::value, decltype(il)::value>
-            <<<grid, block, 0, st>>>(tab, blk2sys, ph);
-    });
-    if (flags & 1) bpcg_stencil_dot_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
+    if (flags & FLAG_STENCIL) bpcg_points_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
+    if (!(flags & FLAG_NO_CSR)) {
+        dispatch_spmv([&](auto g, auto nt, auto il) {
+            bpcg_spmv_dot_kernel<decltype(g)::value, decltype(nt)::value, decltype(il)::value>
+                <<<grid, block, 0, st>>>(tab, blk2sys, ph);
+        });
+    }
+    if (flags & FLAG_PACKED) {
+        if (g_spmv_il) bpcg_spmv_packed_dot_kernel<true><<<grid, block, 0, st>>>(tab, blk2sys, ph);
+        else bpcg_spmv_packed_dot_kernel<false><<<grid, block, 0, st>>>(tab, blk2sys, ph);
+    }
+    if (flags & FLAG_STENCIL) bpcg_stencil_dot_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
     bpcg_update_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
     bpcg_direction_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
 }
@@ -742,7 +859,9 @@ static int pcg_solve_entry(BatchSys E, int64_t M, double tol, int32_t max_iter, 
     write_entry_kernel<<<1, 1, 0, work>>>(reinterpret_cast<BatchSys*>(tab), E);
     fill_map_kernel<<<grid1d(A.nblocks, 256), 256, 0, work>>>(map, A.nblocks);
     NGS_CHECK_LAUNCH();
-    const int flags = E.kind == SYS_STENCIL ? 1 : 0;
+    const int flags = E.kind == SYS_STENCIL ? (FLAG_STENCIL | FLAG_NO_CSR)
+                      : E.kind == SYS_PACKED ? (FLAG_PACKED | FLAG_NO_CSR)
+                                             : 0;
     int status = NGS_OK;
     int32_t it = 0, done = 0;
     while (true) {
@@ -793,6 +912,29 @@ int ngs_pcg_solve(const int32_t* rowptr, const int32_t* cols, const float* vals,
     return pcg_solve_entry(E, M, tol, max_iter, chunk, workspace, s, iters, rel_resid);
 }
 
+int ngs_pcg_solve_packed(const int32_t* rowptr, const uint16_t* codes, const int32_t* pbase, const float* vals,
+                         const float* inv_diag, const float* b, float* x, int64_t M, double tol, int32_t max_iter,
+                         int32_t chunk, void* workspace, ngs_stream_t s, int32_t* iters, double* rel_resid) {
+    NGS_REQUIRE(M >= 0 && workspace, "ngs_pcg_solve_packed: bad arguments");
+    NGS_REQUIRE(tol >= 0, "ngs_pcg_solve_packed: negative tolerance");
+    if (M == 0) {
+        if (iters) *iters = 0;
+        if (rel_resid) *rel_resid = 0;
+        return NGS_OK;
+    }
+    NGS_REQUIRE(rowptr && codes && pbase && vals && inv_diag && b && x, "ngs_pcg_solve_packed: null array");
+    BatchSys E{};
+    E.kind = SYS_PACKED;
+    E.a.rowptr = rowptr;
+    E.a.codes = codes;
+    E.a.pbase = pbase;
+    E.a.vals = vals;
+    E.a.dinv = inv_diag;
+    E.a.b = b;
+    E.a.x = x;
+    return pcg_solve_entry(E, M, tol, max_iter, chunk, workspace, s, iters, rel_resid);
+}
+
 int ngs_pcg_solve_stencil(ngs_stencil op, const float* inv_diag, const float* b, float* x, double tol, int32_t max_iter,
                           int32_t chunk, void* workspace, ngs_stream_t s, int32_t* iters, double* rel_resid) {
     const int64_t M = op.level.M;
@@ -825,6 +967,20 @@ int ngs_spmv(const int32_t* rowptr, const int32_t* cols, const float* vals, cons
             <<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, x, y, M, chunk);
     });
     g_spmv_il = il_saved;
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_spmv_packed(const int32_t* rowptr, const uint16_t* codes, const int32_t* pbase, const float* vals, const float* x,
+                    float* y, int64_t M, ngs_stream_t s) {
+    if (M <= 0) return NGS_OK;
+    NGS_REQUIRE(rowptr && codes && pbase && vals && x && y, "ngs_spmv_packed: null array");
+    int64_t chunk;
+    int nb = pcg_blocks(M, &chunk);
+    if (g_spmv_il && nb % NUM_XCD == 0)
+        spmv_packed_kernel<8, true><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, codes, pbase, vals, x, y, M, chunk);
+    else
+        spmv_packed_kernel<8, false><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, codes, pbase, vals, x, y, M, chunk);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
@@ -910,6 +1066,27 @@ int ngs_pcg_batch_prepare(const int32_t* rowptr, const int32_t* cols, const floa
     E.kind = SYS_CSR;
     E.a.rowptr = rowptr;
     E.a.cols = cols;
+    E.a.vals = vals;
+    E.a.dinv = inv_diag;
+    E.a.b = b;
+    E.a.x = x;
+    int r = batch_prepare(&E, M, tol, max_iter, workspace, blk0, nblk, s);
+    if (r == NGS_OK) *reinterpret_cast<BatchSys*>(entry) = E;
+    return r;
+}
+
+int ngs_pcg_batch_prepare_packed(const int32_t* rowptr, const uint16_t* codes, const int32_t* pbase, const float* vals,
+                                 const float* inv_diag, const float* b, float* x, int64_t M, double tol,
+                                 int32_t max_iter, void* workspace, int32_t blk0, int32_t nblk, void* entry,
+                                 ngs_stream_t s) {
+    NGS_REQUIRE(M > 0 && workspace && entry && rowptr && codes && pbase && vals && inv_diag && b && x,
+                "ngs_pcg_batch_prepare_packed: bad arguments");
+    NGS_REQUIRE(nblk == ngs_pcg_batch_blocks(M), "ngs_pcg_batch_prepare_packed: nblk must be ngs_pcg_batch_blocks(M)");
+    BatchSys E{};
+    E.kind = SYS_PACKED;
+    E.a.rowptr = rowptr;
+    E.a.codes = codes;
+    E.a.pbase = pbase;
     E.a.vals = vals;
     E.a.dinv = inv_diag;
     E.a.b = b;

@@ -74,6 +74,12 @@ _SIGNATURES = {
     "ngs_set_tables": ([P, P, c_int32], c_int32),
     "ngs_lhs_row_counts": ([Level, P, P], c_int32),
     "ngs_lhs_assemble": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P, P], c_int32),
+    "ngs_lhs_assemble_packed": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P, P, P, P, P], c_int32),
+    "ngs_pcg_solve_packed": ([P, P, P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
+                              ctypes.POINTER(c_double)], c_int32),
+    "ngs_pcg_batch_prepare_packed": ([P, P, P, P, P, P, P, c_int64, c_double, c_int32, P, c_int32, c_int32, P, P],
+                                     c_int32),
+    "ngs_spmv_packed": ([P, P, P, P, P, P, c_int64, P], c_int32),
     "ngs_rhs_cross_stiffness": ([Level, Level, P, P, P], c_int32),
     "ngs_rhs_normals": ([Level, Level, P, P, P], c_int32),
     "ngs_basis_splat": ([Grid, P, P, c_int64, c_float, c_double, P, P], c_int32),

@@ -42,6 +42,19 @@ class _Job:
         self.ready.record(torch.cuda.current_stream())
 
 
+def launch_flags(systems):
+    """ngs_pcg_batch_iterate flags for a set of systems: which SpMV kernels they need."""
+    stencil = packed = plain = False
+    for s in systems:
+        if isinstance(s, _solver.StencilSystem):
+            stencil = True
+        elif s.packed:
+            packed = True
+        else:
+            plain = True
+    return (1 if stencil else 0) | (2 if packed else 0) | (0 if plain else 4)
+
+
 class BatchedPCG:
     """Pool of concurrently iterating PCG systems (one batched launch per kernel per iteration)."""
 
@@ -121,6 +134,10 @@ class BatchedPCG:
                 call("ngs_pcg_batch_prepare_stencil", s.struct(), ptr(s.inv_diag), ptr(job.b), ptr(job.x),
                      float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
                      self.stream.cuda_stream)
+            elif s.packed:
+                call("ngs_pcg_batch_prepare_packed", ptr(s.rowptr), ptr(s.codes), ptr(s.pbase), ptr(s.vals),
+                     ptr(s.inv_diag), ptr(job.b), ptr(job.x), s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0,
+                     job.nblk, entry.data_ptr(), self.stream.cuda_stream)
             else:
                 call("ngs_pcg_batch_prepare", ptr(s.rowptr), ptr(s.cols), ptr(s.vals), ptr(s.inv_diag), ptr(job.b),
                      ptr(job.x), s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
@@ -146,7 +163,7 @@ class BatchedPCG:
                         self.dev_table[:n * self.eb].copy_(self.host_table[:n * self.eb], non_blocking=True)
                         self.dev_map.copy_(self.host_map, non_blocking=True)
                         self._dirty = False
-                    flags = int(any(isinstance(j.system, _solver.StencilSystem) for j in self.active))
+                    flags = launch_flags(j.system for j in self.active)
                     call("ngs_pcg_batch_iterate", ptr(self.dev_table), ptr(self.dev_map), self.cap, self.chunk,
                          int(self.use_graph), flags, self.stream.cuda_stream)
                     self.launches += 1

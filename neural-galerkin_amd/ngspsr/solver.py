@@ -34,12 +34,19 @@ def parse_solver(spec: str):
 
 
 class CSRSystem:
-    """Assembled level system: CSR (rows padded to multiples of 4 entries), inverse diagonal."""
+    """Assembled level system: CSR (rows padded to multiples of 4 entries), inverse diagonal, and
+    optionally the packed-column form of the same matrix (16-bit codes + per-row plane bases,
+    ngs_lhs_assemble_packed), which the solver then uses."""
 
-    def __init__(self, rowptr, cols, vals, inv_diag):
+    def __init__(self, rowptr, cols, vals, inv_diag, codes=None, pbase=None):
         self.rowptr, self.cols, self.vals, self.inv_diag = rowptr, cols, vals, inv_diag
+        self.codes, self.pbase = codes, pbase
         self.M = inv_diag.shape[0]
         self.device = vals.device
+
+    @property
+    def packed(self):
+        return self.codes is not None
 
     @property
     def nnz_stored(self):
@@ -52,8 +59,12 @@ class CSRSystem:
     def matvec(self, x):
         y = torch.empty(self.M, dtype=torch.float32, device=self.device)
         if self.M:
-            call("ngs_spmv", ptr(self.rowptr), ptr(self.cols), ptr(self.vals), ptr(x.float().contiguous()), ptr(y),
-                 self.M, stream())
+            if self.packed:
+                call("ngs_spmv_packed", ptr(self.rowptr), ptr(self.codes), ptr(self.pbase), ptr(self.vals),
+                     ptr(x.float().contiguous()), ptr(y), self.M, stream())
+            else:
+                call("ngs_spmv", ptr(self.rowptr), ptr(self.cols), ptr(self.vals), ptr(x.float().contiguous()), ptr(y),
+                     self.M, stream())
         return y
 
     def residual_f64(self, b64, x64):
@@ -203,6 +214,10 @@ def pcg(system, b, tol=1e-5, maxiter=-1, chunk=32, x_out=None, raise_on_cap=True
         st = lib().ngs_pcg_solve_stencil(system.struct(), ptr(system.inv_diag), ptr(b), ptr(x), float(tol),
                                          int(maxiter), int(chunk), ptr(ws), stream(), ctypes.byref(it),
                                          ctypes.byref(rr))
+    elif system.packed:
+        st = lib().ngs_pcg_solve_packed(ptr(system.rowptr), ptr(system.codes), ptr(system.pbase), ptr(system.vals),
+                                        ptr(system.inv_diag), ptr(b), ptr(x), M, float(tol), int(maxiter), int(chunk),
+                                        ptr(ws), stream(), ctypes.byref(it), ctypes.byref(rr))
     else:
         st = lib().ngs_pcg_solve(ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(system.inv_diag),
                                  ptr(b), ptr(x), M, float(tol), int(maxiter), int(chunk), ptr(ws), stream(),
