@@ -53,23 +53,33 @@ def _event_time(fn, reps):
     return s.elapsed_time(e) / 1e3 / reps
 
 
-def spmv_roofline(system, reps=50, packed=False):
-    """Average duration of the CSR SpMV kernel on a level system and its algorithmic bytes:
-    int32 columns (spmv_kernel): 8 nnz + 4 (M+1) + 8 M; packed columns (spmv_packed_kernel):
-    6 nnz + 20 M (plane bases) + 4 (M+1) + 8 M."""
+def spmv_roofline(system, reps=50):
+    """Average duration of the SpMV kernel of an assembled level system and its bytes per launch.
+    CSR (spmv_kernel): algorithmic 8 nnz + 4 (M+1) + 4 M (x) + 4 M (y).
+    Run form (spmv_runs_kernel): algorithmic 4 nnz (values) + 4 R (run words of the R non-empty
+    runs) + 4 M (x) + 4 M (y), zero slots and empty runs excluded; the layout itself streams all
+    125 value slots and 25 run words per row, 600 M + 8 M bytes ("layout_bytes").
+    Returns (seconds, algorithmic bytes, nnz, extra dict)."""
     import torch
     from ngspsr._lib import call, ptr, stream
+    from ngspsr.solver import RunSystem
     x = torch.randn(system.M, device=system.device)
     y = torch.empty_like(x)
-    if packed:
-        t = _event_time(lambda: call("ngs_spmv_packed", ptr(system.rowptr), ptr(system.codes), ptr(system.pbase),
-                                     ptr(system.vals), ptr(x), ptr(y), system.M, stream()), reps)
-    else:
-        t = _event_time(lambda: call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x),
-                                     ptr(y), system.M, stream()), reps)
+    M = system.M
     nnz = system.nnz()
-    algo = (6 * nnz + 20 * system.M if packed else 8 * nnz) + 4 * (system.M + 1) + 8 * system.M
-    return t, algo, nnz
+    if isinstance(system, RunSystem):
+        t = _event_time(lambda: call("ngs_spmv_runs", ptr(system.runs), ptr(system.rvals), ptr(x), ptr(y), M,
+                                     stream()), reps)
+        runs = int((system.counts() > 0).sum().item())
+        algo = 4 * nnz + 4 * runs + 8 * M
+        extra = {"nonempty_runs": runs, "layout_bytes": 600 * M + 8 * M,
+                 "layout_GB/s": round((600 * M + 8 * M) / t / 1e9, 1),
+                 "csr_equivalent_bytes": 8 * nnz + 4 * (M + 1) + 8 * M,
+                 "csr_equivalent_GB/s": round((8 * nnz + 4 * (M + 1) + 8 * M) / t / 1e9, 1)}
+        return t, algo, nnz, extra
+    t = _event_time(lambda: call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x),
+                                 ptr(y), M, stream()), reps)
+    return t, 8 * nnz + 4 * (M + 1) + 8 * M, nnz, {}
 
 
 def stencil_roofline(system, reps=50):
@@ -131,30 +141,30 @@ COPY_BYTES = 1 << 30
 
 def pmc_probe(res):
     """Kernels for the HBM counter runs: a 1 GiB device copy (its FETCH_SIZE / WRITE_SIZE readings
-    calibrate the counters against 1 GiB read + 1 GiB written) and the finest-level CSR SpMV."""
+    calibrate the counters against 1 GiB read + 1 GiB written), then the finest-level SpMV in run
+    form and in CSR form (the same matrix), 5 launches each."""
     import torch
     from ngspsr._lib import call, ptr, stream
-    from ngspsr.solver import StencilSystem
+    from ngspsr.solver import RunSystem, StencilSystem
     system = res.reconstructor.systems[0][0]
-    csr = system.to_csr() if isinstance(system, StencilSystem) else system
+    csr = system.to_csr() if isinstance(system, (StencilSystem, RunSystem)) else system
     a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=csr.device)
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     x = torch.randn(csr.M, device=csr.device)
     y = torch.empty_like(x)
+    if isinstance(system, RunSystem):
+        for _ in range(5):
+            call("ngs_spmv_runs", ptr(system.runs), ptr(system.rvals), ptr(x), ptr(y), system.M, stream())
     for _ in range(5):
         call("ngs_spmv", ptr(csr.rowptr), ptr(csr.cols), ptr(csr.vals), ptr(x), ptr(y), csr.M, stream())
-    if csr.packed:
-        for _ in range(5):
-            call("ngs_spmv_packed", ptr(csr.rowptr), ptr(csr.codes), ptr(csr.pbase), ptr(csr.vals), ptr(x), ptr(y),
-                 csr.M, stream())
     torch.cuda.synchronize()
     print(json.dumps({"pmc_probe": True, "M": csr.M, "nnz": csr.nnz(), "copy_bytes": COPY_BYTES}), flush=True)
 
 
 def measured_traffic(kernel):
-    """HBM bytes per launch of an SpMV kernel ("spmv_kernel" or "spmv_packed_kernel") at the default
+    """HBM bytes per launch of an SpMV kernel ("spmv_kernel" or "spmv_runs_kernel") at the default
     workload, from the committed counter summary (profiles/), if present."""
     path = os.path.join(ROOT, "profiles", "traffic_spmv_ellipsoids2M_512.json")
     if not os.path.exists(path):
@@ -292,25 +302,17 @@ def main():
     info = res.solve_info
     total_iters = int(sum(i.iters for i in info.values()))
     rel = max(float(i.rel_resid) for i in info.values())
-    from ngspsr.solver import StencilSystem
+    from ngspsr.solver import RunSystem, StencilSystem
     system, rhs = res.reconstructor.systems[0]
-    forms = {int(d): ("stencil" if isinstance(s_, StencilSystem) else "csr")
+    forms = {int(d): ("stencil" if isinstance(s_, StencilSystem) else "runs" if isinstance(s_, RunSystem) else "csr")
              for d, (s_, _) in sorted(res.reconstructor.systems.items())}
-    csr = system.to_csr() if isinstance(system, StencilSystem) else system
-    t_spmv, algo, nnz = spmv_roofline(csr)
+    csr = system.to_csr() if isinstance(system, (StencilSystem, RunSystem)) else system
+    t_spmv, algo, nnz, _ = spmv_roofline(csr)
     t_iter = pcg_iteration_rate(system)
-    spmv_obj = {"kernel": "spmv_kernel<8,0,1> (finest-level CSR SpMV, int32 columns, the solve_pcg_diag operator)",
+    spmv_obj = {"kernel": "spmv_kernel<8,0,1> (finest-level CSR SpMV, int32 columns, the solve_pcg_diag layout)",
                 "bound": "hbm", "achieved": round(algo / t_spmv / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": measured_traffic("spmv_kernel"),
                 "launch_us": round(1e6 * t_spmv, 2), "algorithmic_bytes": int(algo)}
-    packed_obj = None
-    if getattr(csr, "packed", False):
-        t_pk, algo_pk, _ = spmv_roofline(csr, packed=True)
-        packed_obj = {"kernel": "spmv_packed_kernel<8,1> (finest-level CSR SpMV, packed 16-bit columns; the PCG's "
-                                "operator)", "bound": "hbm", "achieved": round(algo_pk / t_pk / 1e9, 1),
-                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(algo_pk / t_pk / 1e9 / HBM_PEAK_GBS, 4),
-                      "traffic": measured_traffic("spmv_packed_kernel"), "launch_us": round(1e6 * t_pk, 2),
-                      "algorithmic_bytes": int(algo_pk)}
     if isinstance(system, StencilSystem):
         t_pts, b_pts, t_rows, b_rows = stencil_roofline(system)
         iter_bytes = b_pts + b_rows + 44 * system.M
@@ -322,9 +324,14 @@ def main():
                                "algorithmic_bytes": int(b_pts), "achieved": round(b_pts / t_pts / 1e9, 1)},
                 "operator_us": round(1e6 * (t_pts + t_rows), 2),
                 "csr_spmv_us_same_system": round(1e6 * t_spmv, 2)}
-    elif packed_obj is not None:
-        iter_bytes = packed_obj["algorithmic_bytes"] + 44 * system.M
-        roof = dict(packed_obj)
+    elif isinstance(system, RunSystem):
+        t_run, algo_run, _, extra = spmv_roofline(system)
+        iter_bytes = algo_run + 44 * system.M
+        roof = {"kernel": "spmv_runs_kernel (finest-level SpMV, run form: one run word per bitmap column, "
+                          "values plane-major)", "bound": "hbm", "achieved": round(algo_run / t_run / 1e9, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(algo_run / t_run / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic("spmv_runs_kernel"), "launch_us": round(1e6 * t_run, 2),
+                "algorithmic_bytes": int(algo_run), **extra, "csr_spmv_us_same_matrix": round(1e6 * t_spmv, 2)}
     else:
         iter_bytes = algo + 44 * system.M
         roof = dict(spmv_obj)
@@ -340,8 +347,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "precision": "FP32 level operators (stencil form where cheaper per iteration, CSR elsewhere) and PCG "
-                     "vectors; FP64 dot products, residual and refinement",
+        "precision": "FP32 level operators (assembled, run form) and PCG vectors; FP64 dot products, residual "
+                     "and refinement",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "points": cfg["n"], "voxel_size": cfg["vs"], "levels": cfg["depth"],
                    "solver": cfg["solver"], "finest_voxels": int(system.M), "finest_nnz": int(nnz),

@@ -221,15 +221,14 @@ int ngs_lhs_assemble(ngs_level lv, ngs_point_bins pts, const float *coarse, floa
                      const int32_t *rowptr, int32_t *cols, float *vals, float *inv_diag, double *rhs,
                      ngs_stream_t s);
 
-/* ngs_lhs_assemble plus the packed-column form of the same matrix (no reference counterpart; the
- * reference stores int32 columns, solver.py:40-48): for entry k of row i with neighbour offset
- * (dx, dy, dz), codes[k] = (dx + 2) << 13 | (column - plane base of dx), where the plane base is the
- * row's first column with that dx; pbase[8 i + d] = plane base of dx = d - 2 (0 if absent), pbase
- * entries 5..7 = 0. *overflow (device int32, caller-zeroed) is set when an offset needs more than
- * 13 bits; the packed arrays are then unusable and the int32 columns remain valid. */
-int ngs_lhs_assemble_packed(ngs_level lv, ngs_point_bins pts, const float *coarse, float sf, float delta,
-                            const int32_t *rowptr, int32_t *cols, uint16_t *codes, int32_t *pbase, int32_t *overflow,
-                            float *vals, float *inv_diag, double *rhs, ngs_stream_t s);
+/* ngs_lhs_assemble into the run form of the same matrix instead of CSR (no reference counterpart;
+ * the reference stores int32 columns, solver.py:40-48). Row i's neighbours in the bitmap column
+ * (dx, dy), p = 5 (dx + 2) + (dy + 2), have consecutive ranks start .. start + n - 1 (n <= 5):
+ * runs[p M + i] = start | n << 28 (n = 0: start = i), rvals[(5 p + j) M + i] = A(i, start + j) for
+ * j < n, 0 otherwise; 25 int32 and 125 float per row, plane-major. inv_diag and rhs as
+ * ngs_lhs_assemble, bit for bit (the values equal the CSR entries). M <= 2^28. */
+int ngs_lhs_assemble_runs(ngs_level lv, ngs_point_bins pts, const float *coarse, float sf, float delta,
+                          int32_t *runs, float *rvals, float *inv_diag, double *rhs, ngs_stream_t s);
 
 /* Stiffness part of the cross-level correction (reconstructor.py:153-168, COO of
  * get_neighbours(d, dd, 5), integrate_deriv_deriv_product bezier_tensor.py:75-96):
@@ -294,11 +293,11 @@ int ngs_pcg_solve(const int32_t *rowptr, const int32_t *cols, const float *vals,
                   const float *b, float *x, int64_t M, double tol, int32_t max_iter, int32_t chunk,
                   void *workspace, ngs_stream_t s, int32_t *iters, double *rel_resid);
 
-/* ngs_pcg_solve on the packed-column form of the matrix (ngs_lhs_assemble_packed): same iteration,
- * 6 instead of 8 bytes per stored entry in the SpMV. */
-int ngs_pcg_solve_packed(const int32_t *rowptr, const uint16_t *codes, const int32_t *pbase, const float *vals,
-                         const float *inv_diag, const float *b, float *x, int64_t M, double tol, int32_t max_iter,
-                         int32_t chunk, void *workspace, ngs_stream_t s, int32_t *iters, double *rel_resid);
+/* ngs_pcg_solve on the run form of the matrix (ngs_lhs_assemble_runs): same iteration and stop rule;
+ * the SpMV reads one run word per bitmap column instead of one column index per entry. */
+int ngs_pcg_solve_runs(const int32_t *runs, const float *rvals, const float *inv_diag, const float *b, float *x,
+                       int64_t M, double tol, int32_t max_iter, int32_t chunk, void *workspace, ngs_stream_t s,
+                       int32_t *iters, double *rel_resid);
 
 /* ngs_pcg_solve on the stencil form of the system (same iteration, same stop rule; the SpMV is the
  * point pass + row pass of ngs_stencil_apply). */
@@ -325,28 +324,26 @@ int ngs_pcg_batch_prepare(const int32_t *rowptr, const int32_t *cols, const floa
                           int32_t blk0, int32_t nblk, void *entry, ngs_stream_t s);
 /* Set the first block of a prepared host table entry (a multiple of 8). */
 int ngs_pcg_batch_place(void *entry, int32_t blk0);
-/* The same as ngs_pcg_batch_prepare for a system in packed-column form. */
-int ngs_pcg_batch_prepare_packed(const int32_t *rowptr, const uint16_t *codes, const int32_t *pbase, const float *vals,
-                                 const float *inv_diag, const float *b, float *x, int64_t M, double tol,
-                                 int32_t max_iter, void *workspace, int32_t blk0, int32_t nblk, void *entry,
-                                 ngs_stream_t s);
+/* The same as ngs_pcg_batch_prepare for a system in run form. */
+int ngs_pcg_batch_prepare_runs(const int32_t *runs, const float *rvals, const float *inv_diag, const float *b,
+                               float *x, int64_t M, double tol, int32_t max_iter, void *workspace, int32_t blk0,
+                               int32_t nblk, void *entry, ngs_stream_t s);
 /* The same as ngs_pcg_batch_prepare for a system in stencil form. */
 int ngs_pcg_batch_prepare_stencil(ngs_stencil op, const float *inv_diag, const float *b, float *x, double tol,
                                   int32_t max_iter, void *workspace, int32_t blk0, int32_t nblk, void *entry,
                                   ngs_stream_t s);
 /* `iters` (even) iterations of every active system in one launch per kernel (graph-replayed when
  * use_graph). flags: bit 0 the table may hold stencil systems (adds their point and row passes),
- * bit 1 it may hold packed-column systems (adds their SpMV), bit 2 it holds no int32-column CSR
- * systems (skips that SpMV). */
+ * bit 1 it may hold run-form systems (adds their SpMV), bit 2 it holds no CSR systems (skips
+ * that SpMV). */
 int ngs_pcg_batch_iterate(const void *table, const int32_t *blk2sys, int32_t total_blocks, int32_t iters,
                           int32_t use_graph, int32_t flags, ngs_stream_t s);
 /* (iter, done) of each system of the host copy of the table into states (2 nsys int32, host);
  * synchronises s. done = 1 converged, 2 iteration cap. */
 int ngs_pcg_batch_status(const void *host_table, int32_t nsys, int32_t *states, ngs_stream_t s);
 
-/* y = A x (FP32) on the packed-column form, exposed for measurement. */
-int ngs_spmv_packed(const int32_t *rowptr, const uint16_t *codes, const int32_t *pbase, const float *vals,
-                    const float *x, float *y, int64_t M, ngs_stream_t s);
+/* y = A x (FP32) on the run form (terms in increasing column order), exposed for measurement. */
+int ngs_spmv_runs(const int32_t *runs, const float *rvals, const float *x, float *y, int64_t M, ngs_stream_t s);
 
 /* y = A x (FP32), the SpMV of the solver, exposed for measurement. */
 int ngs_spmv(const int32_t *rowptr, const int32_t *cols, const float *vals, const float *x, float *y,
@@ -356,6 +353,9 @@ int ngs_spmv(const int32_t *rowptr, const int32_t *cols, const float *vals, cons
  * (device double). */
 int ngs_residual_f64(const int32_t *rowptr, const int32_t *cols, const float *vals, const double *b,
                      const double *x, double *r, int64_t M, double *norms, void *workspace, ngs_stream_t s);
+/* The same on the run form. */
+int ngs_residual_f64_runs(const int32_t *runs, const float *rvals, const double *b, const double *x, double *r,
+                          int64_t M, double *norms, void *workspace, ngs_stream_t s);
 
 /* ------------------------------------------------------------------------------------- mesh */
 
@@ -396,6 +396,14 @@ int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t *leaves, int64_t 
 /* Dual-corner positions (primal + stride) * voxel_size / n_upsample (reconstructor.py:368-373). */
 int ngs_dual_corner_positions(const int32_t *primal, int64_t n, int32_t stride, float scale, float *pos,
                               ngs_stream_t s);
+/* chi of one level at the dual corners of leaves (reconstructor.py:368-375, evaluate_chi at the
+ * corner positions): out[l * nu^3 + e] = (accumulate ? out : 0) + the ngs_eval_field value (unit
+ * scale) at the position ngs_dual_corner_positions writes for corner e of leaf l, bit for bit; for
+ * n_upsample 2 the 27-voxel windows of a leaf's corners are read together. leaf_stride: stride of
+ * the leaves' level. */
+int ngs_eval_leaf_corners(ngs_grid g, const float *field, const int32_t *leaves, int64_t n, int32_t leaf_stride,
+                          int32_t n_upsample, float scale, float inv_voxel_size, int32_t accumulate, float *out,
+                          ngs_stream_t s);
 /* Dual marching cubes (csrc/marching_cubes/dual_mc.cu:33-189): per-cube vertex counts, then
  * triangles (T x 3 x 3 float) and vertex keys (T x 3, uint64 = larger id << 32 | smaller id). */
 int ngs_mc_count(const int32_t *cubes, int64_t k, const float *values, int32_t *vcount, ngs_stream_t s);

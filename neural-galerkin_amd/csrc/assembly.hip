@@ -42,19 +42,21 @@ __global__ void lhs_count_kernel(ngs_level lv, int32_t* __restrict__ row_nnz) {
 // around the voxel (points in bin order, basis values and weights stored in bin order), then the
 // row is written in increasing column order. Cells and neighbours are found one bitmap column
 // (3 or 5 consecutive lattice points along z) at a time.
-// Packed columns (when codes != null): entry code = (dx + 2) << 13 | (column - plane base of dx),
-// with the row's five plane bases (first column of each dx group, or 0) in pbase[8 i .. 8 i + 4];
-// an offset that does not fit 13 bits sets *overflow.
-constexpr int PACK_OFFSET_BITS = 13;
+// RUNS: the row goes to the run form instead of CSR. The present neighbours of voxel i in the
+// bitmap column (dx, dy) (its five z-neighbours) have consecutive ranks start .. start + n - 1;
+// runs[p M + i] = start | n << 28 (n = 0: start = i) for p = 5 (dx + 2) + (dy + 2), and
+// rvals[(5 p + j) M + i] the value of column start + j (0 for j >= n). Both arrays are
+// plane-major, so the run and value loads of consecutive rows are contiguous.
+constexpr int RUN_COUNT_SHIFT = 28;
 
-template <bool HAS_ALPHA, bool HAS_COARSE>
+template <bool HAS_ALPHA, bool HAS_COARSE, bool RUNS>
 __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_point_bins pts,
                                                           const float* __restrict__ coarse, float sf, float sf_delta,
                                                           const float* __restrict__ F1, const float* __restrict__ G1,
                                                           const int32_t* __restrict__ rowptr, int32_t* __restrict__ cols,
                                                           float* __restrict__ vals, float* __restrict__ inv_diag,
-                                                          double* __restrict__ rhs, uint16_t* __restrict__ codes,
-                                                          int32_t* __restrict__ pbase, int32_t* __restrict__ overflow) {
+                                                          double* __restrict__ rhs, int32_t* __restrict__ runs,
+                                                          float* __restrict__ rvals) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= lv.M) return;
     const int s = lv.stride;
@@ -116,42 +118,38 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
         }
     }
     // Emit the row in increasing column order (lexicographic offsets over sorted voxels).
-    int32_t base = rowptr[i];
     int cnt = 0;
     const float fs = (float)s, inv_s = 1.0f / (float)s;
     const ngs_grid& g = lv.grid;
     const int l2 = g.log2_stride;
     const int lx = (cx >> l2) - g.origin[0], ly = (cy >> l2) - g.origin[1], lz = (cz >> l2) - g.origin[2];
-    int32_t plane[5] = {-1, -1, -1, -1, -1};
-    bool ovf = false;
-    // Entries go through a 4-deep shift register and leave as aligned 16-byte stores (rows start at
-    // multiples of 4): a quarter of the store instructions of per-entry stores.
+    // CSR: entries go through a 4-deep shift register and leave as aligned 16-byte stores (rows
+    // start at multiples of 4): a quarter of the store instructions of per-entry stores.
     typedef float f4 __attribute__((ext_vector_type(4)));
     typedef int i4 __attribute__((ext_vector_type(4)));
-    typedef unsigned short us4 __attribute__((ext_vector_type(4)));
+    const int32_t base = RUNS ? 0 : rowptr[i];
     float vb0 = 0.f, vb1 = 0.f, vb2 = 0.f, vb3 = 0.f;
     int32_t cb0 = 0, cb1 = 0, cb2 = 0, cb3 = 0;
-    uint32_t ub0 = 0, ub1 = 0, ub2 = 0, ub3 = 0;
-    auto emit = [&](float v, int32_t col, uint32_t code) {
+    auto emit = [&](float v, int32_t col) {
         vb0 = vb1; vb1 = vb2; vb2 = vb3; vb3 = v;
         cb0 = cb1; cb1 = cb2; cb2 = cb3; cb3 = col;
-        ub0 = ub1; ub1 = ub2; ub2 = ub3; ub3 = code;
         if ((cnt & 3) == 3) {
             const int64_t k = (int64_t)base + cnt - 3;
             *reinterpret_cast<f4*>(vals + k) = f4{vb0, vb1, vb2, vb3};
             *reinterpret_cast<i4*>(cols + k) = i4{cb0, cb1, cb2, cb3};
-            if (codes)
-                *reinterpret_cast<us4*>(codes + k) =
-                    us4{(unsigned short)ub0, (unsigned short)ub1, (unsigned short)ub2, (unsigned short)ub3};
         }
         ++cnt;
     };
+    const int64_t M = lv.M;
 #pragma unroll
     for (int dx = -2; dx <= 2; ++dx) {
 #pragma unroll
         for (int dy = -2; dy <= 2; ++dy) {
             int32_t j;
             const uint32_t m = grid_column(g, lx + dx, ly + dy, lz - 2, 5, &j);
+            const int pl = (dx + 2) * 5 + (dy + 2);
+            if (RUNS) runs[pl * M + i] = (m ? j : (int32_t)i) | (__popc(m) << RUN_COUNT_SHIFT);
+            int q = 0;
 #pragma unroll
             for (int dz = -2; dz <= 2; ++dz) {
                 if (!((m >> (dz + 2)) & 1u)) continue;
@@ -163,35 +161,16 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                     const float K = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
                     v = K + sf * acc[(dx + 2) * 25 + (dy + 2) * 5 + (dz + 2)];
                 }
-                uint32_t code = 0;
-                if (codes) {
-                    if (plane[dx + 2] < 0) plane[dx + 2] = j;
-                    const int32_t off = j - plane[dx + 2];
-                    ovf |= off >= (1 << PACK_OFFSET_BITS);
-                    code = ((dx + 2) << PACK_OFFSET_BITS) | (off & ((1 << PACK_OFFSET_BITS) - 1));
-                }
                 if (dx == 0 && dy == 0 && dz == 0) inv_diag[i] = 1.0f / v;
-                emit(v, j++, code);
+                if (RUNS) rvals[(5 * pl + q++) * M + i] = v;
+                else emit(v, j++);
             }
+            if (RUNS)
+                for (; q < 5; ++q) rvals[(5 * pl + q) * M + i] = 0.f;
         }
     }
-    while (cnt & 3) {
-        // padding: the row itself (dx = 0 group, always present) with a zero value
-        uint32_t code = 0;
-        if (codes) {
-            const int32_t off = (int32_t)i - plane[2];
-            ovf |= off >= (1 << PACK_OFFSET_BITS);
-            code = (2 << PACK_OFFSET_BITS) | (off & ((1 << PACK_OFFSET_BITS) - 1));
-        }
-        emit(0.f, (int32_t)i, code);
-    }
-    if (codes) {
-#pragma unroll
-        for (int d = 0; d < 5; ++d) pbase[8 * i + d] = plane[d] < 0 ? 0 : plane[d];
-#pragma unroll
-        for (int d = 5; d < 8; ++d) pbase[8 * i + d] = 0;
-        if (ovf) atomicOr(overflow, 1);
-    }
+    if (!RUNS)
+        while (cnt & 3) emit(0.f, (int32_t)i);  // padding: the row itself with a zero value
     rhs[i] += (double)(sf_delta * rs) - (double)(sf * rc);
 }
 
@@ -609,24 +588,28 @@ int ngs_stencil_row_counts(ngs_level lv, int32_t* row_nnz, ngs_stream_t s) {
 }
 
 static int lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, float sf, float delta,
-                        const int32_t* rowptr, int32_t* cols, float* vals, float* inv_diag, double* rhs,
-                        uint16_t* codes, int32_t* pbase, int32_t* overflow, ngs_stream_t s) {
+                        const int32_t* rowptr, int32_t* cols, float* vals, float* inv_diag, double* rhs, int32_t* runs,
+                        float* rvals, ngs_stream_t s) {
     if (lv.M <= 0) return NGS_OK;
     Tables T;
     NGS_REQUIRE(tables_for(1, &T), "ngs_lhs_assemble: integral tables not registered");
     const float sfd = sf * delta;
     dim3 grid(grid1d(lv.M, 64)), block(64);
     hipStream_t st = S(s);
-#define NGS_ASM(A, C)                                                                                              \
-    lhs_assemble_kernel<A, C><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, rowptr, cols, vals, inv_diag, \
-                                                      rhs, codes, pbase, overflow)
+#define NGS_ASM(A, C, R)                                                                                           \
+    lhs_assemble_kernel<A, C, R><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, rowptr, cols, vals,    \
+                                                         inv_diag, rhs, runs, rvals)
+#define NGS_ASM_R(A, C)          \
+    if (runs) NGS_ASM(A, C, true); \
+    else NGS_ASM(A, C, false)
     if (pts.alpha) {
-        if (coarse) NGS_ASM(true, true);
-        else NGS_ASM(true, false);
+        if (coarse) NGS_ASM_R(true, true);
+        else NGS_ASM_R(true, false);
     } else {
-        if (coarse) NGS_ASM(false, true);
-        else NGS_ASM(false, false);
+        if (coarse) NGS_ASM_R(false, true);
+        else NGS_ASM_R(false, false);
     }
+#undef NGS_ASM_R
 #undef NGS_ASM
     NGS_CHECK_LAUNCH();
     return NGS_OK;
@@ -634,14 +617,15 @@ static int lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, f
 
 int ngs_lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, float sf, float delta,
                      const int32_t* rowptr, int32_t* cols, float* vals, float* inv_diag, double* rhs, ngs_stream_t s) {
-    return lhs_assemble(lv, pts, coarse, sf, delta, rowptr, cols, vals, inv_diag, rhs, nullptr, nullptr, nullptr, s);
+    NGS_REQUIRE(lv.M <= 0 || (rowptr && cols && vals && inv_diag && rhs), "ngs_lhs_assemble: null array");
+    return lhs_assemble(lv, pts, coarse, sf, delta, rowptr, cols, vals, inv_diag, rhs, nullptr, nullptr, s);
 }
 
-int ngs_lhs_assemble_packed(ngs_level lv, ngs_point_bins pts, const float* coarse, float sf, float delta,
-                            const int32_t* rowptr, int32_t* cols, uint16_t* codes, int32_t* pbase, int32_t* overflow,
-                            float* vals, float* inv_diag, double* rhs, ngs_stream_t s) {
-    NGS_REQUIRE(codes && pbase && overflow, "ngs_lhs_assemble_packed: null packed outputs");
-    return lhs_assemble(lv, pts, coarse, sf, delta, rowptr, cols, vals, inv_diag, rhs, codes, pbase, overflow, s);
+int ngs_lhs_assemble_runs(ngs_level lv, ngs_point_bins pts, const float* coarse, float sf, float delta, int32_t* runs,
+                          float* rvals, float* inv_diag, double* rhs, ngs_stream_t s) {
+    NGS_REQUIRE(lv.M <= 0 || (runs && rvals && inv_diag && rhs), "ngs_lhs_assemble_runs: null array");
+    NGS_REQUIRE(lv.M <= ((int64_t)1 << RUN_COUNT_SHIFT), "ngs_lhs_assemble_runs: too many rows for run words");
+    return lhs_assemble(lv, pts, coarse, sf, delta, nullptr, nullptr, nullptr, inv_diag, rhs, runs, rvals, s);
 }
 
 int ngs_stencil_laplacian(int32_t stride, float* lap, ngs_stream_t s) {

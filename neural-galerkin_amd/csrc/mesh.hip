@@ -224,6 +224,121 @@ __global__ void corner_pos_kernel(const int32_t* __restrict__ prim, int64_t n, i
     pos[i] = (float)(prim[i] + s) * scale;
 }
 
+// Field of one level at the dual corners of leaves: corner e of leaf l (e in [0, nu)^3, leaf-major,
+// the order of ngs_expand_primal) sits at ((leaf * nu + e * s_leaf + s_leaf) * scale) / voxel_size
+// per axis, computed in the same float operations as corner_pos_kernel followed by eval_field_kernel
+// (unit field scale), and summed over the 27 voxels in the same order, so the values equal
+// evaluating the written positions. For nu = 2, when on every axis the two corners of a leaf fall in
+// the same or adjacent cells of the field level, the leaf's 4x4x4 coefficient window is read once
+// (16 bitmap columns) for all eight corners instead of 9 columns per corner; an absent voxel adds
+// +0, which leaves the sum unchanged.
+__global__ void eval_leaf_corners_kernel(ngs_grid g, const float* __restrict__ field, const int32_t* __restrict__ leaves,
+                                         int64_t n, int s_leaf, int nu, float scale, float inv_vs, int accumulate,
+                                         float* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int l2 = g.log2_stride;
+    const int sd = 1 << l2;
+    const float inv_sd = 1.0f / (float)sd;
+    const NGS_GLOBAL int32_t* lf = as_global(leaves) + 3 * l;
+    const NGS_GLOBAL float* f = as_global(field);
+    const int nu3 = nu * nu * nu;
+    const int64_t o0 = l * nu3;
+    if (nu == 2) {
+        int cell[3][2];
+        float b[3][2][3];
+        bool window = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int p = lf[a] * 2;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const float pos = (float)(p + e * s_leaf + s_leaf) * scale;
+                axis_basis(pos * inv_vs, sd, inv_sd, &cell[a][e], b[a][e]);
+            }
+            const int dc = cell[a][1] - cell[a][0];
+            window = window && (dc == 0 || dc == sd);
+        }
+        if (window) {
+            // lattice points cell[a][0] / sd - 1 + j, j in [0, 4), of the leaf's window; per axis and
+            // corner the basis values padded to 4 (the corner's 3 voxels start at j = 0 or j = 1)
+            const int wx = (cell[0][0] >> l2) - 1 - g.origin[0], wy = (cell[1][0] >> l2) - 1 - g.origin[1];
+            const int wz = (cell[2][0] >> l2) - 1 - g.origin[2];
+            float w[3][2][4];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const bool sh = cell[a][1] != cell[a][0];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const bool se = e == 1 && sh;
+                    w[a][e][0] = se ? 0.f : b[a][e][0];
+                    w[a][e][1] = se ? b[a][e][0] : b[a][e][1];
+                    w[a][e][2] = se ? b[a][e][1] : b[a][e][2];
+                    w[a][e][3] = se ? b[a][e][2] : 0.f;
+                }
+            }
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = 0.f;
+            // one bitmap column per (x, y) of the window; every corner's terms are added in its own
+            // (x, y, z) order, and the padded and absent ones add +0
+#pragma unroll
+            for (int jx = 0; jx < 4; ++jx)
+#pragma unroll
+                for (int jy = 0; jy < 4; ++jy) {
+                    int32_t r;
+                    const uint32_t m = grid_column(g, wx + jx, wy + jy, wz, 4, &r);
+                    float c[4];
+#pragma unroll
+                    for (int jz = 0; jz < 4; ++jz) {
+                        const bool present = (m >> jz) & 1u;
+                        c[jz] = present ? f[r] : 0.f;
+                        r += present ? 1 : 0;
+                    }
+#pragma unroll
+                    for (int ex = 0; ex < 2; ++ex)
+#pragma unroll
+                        for (int ey = 0; ey < 2; ++ey) {
+                            const float wxy = w[0][ex][jx] * w[1][ey][jy];
+#pragma unroll
+                            for (int ez = 0; ez < 2; ++ez)
+#pragma unroll
+                                for (int jz = 0; jz < 4; ++jz)
+                                    v[(ex * 2 + ey) * 2 + ez] += (wxy * w[2][ez][jz]) * c[jz];
+                        }
+                }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) out[o0 + e] = (accumulate ? out[o0 + e] : 0.f) + v[e];
+            return;
+        }
+    }
+    // general case: each corner on its own, as eval_field_kernel
+    for (int e = 0; e < nu3; ++e) {
+        const int ee[3] = {e / (nu * nu), (e / nu) % nu, e % nu};
+        int c[3];
+        float bb[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float pos = (float)(lf[a] * nu + ee[a] * s_leaf + s_leaf) * scale;
+            axis_basis(pos * inv_vs, sd, inv_sd, &c[a], bb[a]);
+        }
+        const int lx = (c[0] >> l2) - g.origin[0], ly = (c[1] >> l2) - g.origin[1], lz = (c[2] >> l2) - g.origin[2];
+        float v = 0.f;
+        for (int kx = 0; kx < 3; ++kx)
+            for (int ky = 0; ky < 3; ++ky) {
+                int32_t id;
+                const uint32_t m = grid_column(g, lx + kx - 1, ly + ky - 1, lz - 1, 3, &id);
+                if (!m) continue;
+                for (int kz = 0; kz < 3; ++kz) {
+                    if (!((m >> kz) & 1u)) continue;
+                    v += (bb[0][kx] * bb[1][ky] * bb[2][kz]) * f[id++];
+                }
+            }
+        out[o0 + e] = (accumulate ? out[o0 + e] : 0.f) + v;
+    }
+}
+
 __device__ __forceinline__ int cube_case(const int32_t* ids, const float* __restrict__ val) {
     int c = 0;
 #pragma unroll
@@ -538,6 +653,18 @@ int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t* leaves, int64_t 
     if (tot <= 0) return NGS_OK;
     dual_accumulate_leaves_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, leaves, n, n_upsample, level, id_base, acc,
                                                                       count);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_eval_leaf_corners(ngs_grid g, const float* field, const int32_t* leaves, int64_t n, int32_t leaf_stride,
+                          int32_t n_upsample, float scale, float inv_voxel_size, int32_t accumulate, float* out,
+                          ngs_stream_t s) {
+    NGS_REQUIRE(n_upsample >= 1 && leaf_stride >= 1 && field && out && (n == 0 || leaves),
+                "ngs_eval_leaf_corners: bad arguments");
+    if (n <= 0) return NGS_OK;
+    eval_leaf_corners_kernel<<<grid1d(n, 128), 128, 0, S(s)>>>(g, field, leaves, n, leaf_stride, n_upsample, scale,
+                                                               inv_voxel_size, accumulate, out);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -28,6 +28,8 @@ namespace ngs {
 
 constexpr int PCG_THREADS = 256;
 constexpr int NUM_XCD = 8;
+constexpr int RUN_COUNT_SHIFT = 28;  // run word: start | n << 28
+constexpr int32_t RUN_START_MASK = (1 << RUN_COUNT_SHIFT) - 1;
 
 struct PcgSlot {
     int32_t iter;  // iterations completed
@@ -45,8 +47,8 @@ struct PcgState {
 struct PcgArgs {
     const int32_t* rowptr;
     const int32_t* cols;
-    const uint16_t* codes;  // packed columns (kind SYS_PACKED): (dx + 2) << 13 | offset
-    const int32_t* pbase;   // packed columns: 8 int32 per row, the five plane bases
+    const int32_t* runs;    // run form (kind SYS_RUNS): 25 plane-major run words start | n << 28
+    const float* rvals;     // run form: 125 plane-major value slots per row
     const float* vals;
     const float* dinv;
     const float* b;
@@ -168,56 +170,40 @@ __device__ __forceinline__ float row_dot(int32_t k0, int32_t k1, const NGS_GLOBA
     return sum;
 }
 
-// Row dot of the packed-column CSR: the same schedule as row_dot, with each 16-bit column code
-// decoded against the row's five plane bases (column = base[code >> 13] + (code & 0x1fff)).
-template <int G>
-__device__ __forceinline__ float row_dot_packed(int32_t k0, int32_t k1, const NGS_GLOBAL uint16_t* __restrict__ codes,
-                                                const NGS_GLOBAL int32_t* __restrict__ pbase,
-                                                const NGS_GLOBAL float* __restrict__ vals,
-                                                const NGS_GLOBAL float* __restrict__ x, int64_t row, int lane) {
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    typedef int i4 __attribute__((ext_vector_type(4)));
-    typedef unsigned short us4 __attribute__((ext_vector_type(4)));
-    constexpr int U = 128 / (4 * G);
-    const i4 b03 = *reinterpret_cast<const NGS_GLOBAL i4*>(pbase + 8 * row);
-    const int32_t b4 = pbase[8 * row + 4];
-    float sum = 0.f;
-    auto decode = [&](unsigned code) -> int32_t {
-        const unsigned d = code >> 13;
-        const int32_t base = d == 0 ? b03.x : d == 1 ? b03.y : d == 2 ? b03.z : d == 3 ? b03.w : b4;
-        return base + (int32_t)(code & 0x1fffu);
-    };
-    if (k1 > k0) {
-        const int32_t klast = k1 - 4;
-        f4 v[U];
-        us4 c[U];
+// Row `row` of the run form times x: the 25 bitmap columns (dx, dy) in order, each a run of n <= 5
+// consecutive columns start .. start + n - 1 with its values in slots 0 .. n - 1 (the rest hold 0),
+// so the terms are summed in increasing column order. One lane per row: the run words and value
+// slots of consecutive rows are contiguous (one 256-byte segment per wave and load), and the x
+// reads of a run are consecutive words. The loads of 5 columns are issued before their gathers.
+template <typename T, typename XT>
+__device__ __forceinline__ T run_row_dot(const NGS_GLOBAL int32_t* __restrict__ runs,
+                                         const NGS_GLOBAL float* __restrict__ rvals, const NGS_GLOBAL XT* __restrict__ x,
+                                         int64_t M, int64_t row) {
+    T sum = 0;
+    constexpr int B = 5;
+#pragma unroll 1
+    for (int p0 = 0; p0 < 25; p0 += B) {
+        int32_t w[B];
+        float v[B][5];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int32_t k = min(k0 + 4 * lane + 4 * G * u, klast);
-            v[u] = *reinterpret_cast<const NGS_GLOBAL f4*>(vals + k);
-            c[u] = *reinterpret_cast<const NGS_GLOBAL us4*>(codes + k);
+        for (int b = 0; b < B; ++b) {
+            w[b] = runs[(p0 + b) * M + row];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) v[b][j] = rvals[(5 * (p0 + b) + j) * M + row];
         }
-        float xs[U][4];
+        XT xv[B][5];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            xs[u][0] = x[decode(c[u].x)];
-            xs[u][1] = x[decode(c[u].y)];
-            xs[u][2] = x[decode(c[u].z)];
-            xs[u][3] = x[decode(c[u].w)];
+        for (int b = 0; b < B; ++b) {
+            const int32_t start = w[b] & RUN_START_MASK;
+            const int n = (int)((uint32_t)w[b] >> RUN_COUNT_SHIFT);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) xv[b][j] = j < n ? x[start + j] : (XT)0;
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const float part = v[u].x * xs[u][0] + v[u].y * xs[u][1] + v[u].z * xs[u][2] + v[u].w * xs[u][3];
-            sum += (k0 + 4 * lane + 4 * G * u < k1) ? part : 0.f;
-        }
-        for (int32_t k = k0 + 128 + 4 * lane; k < k1; k += 4 * G) {
-            const f4 vv = *reinterpret_cast<const NGS_GLOBAL f4*>(vals + k);
-            const us4 cc = *reinterpret_cast<const NGS_GLOBAL us4*>(codes + k);
-            sum += vv.x * x[decode(cc.x)] + vv.y * x[decode(cc.y)] + vv.z * x[decode(cc.z)] + vv.w * x[decode(cc.w)];
-        }
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) sum += (T)v[b][j] * (T)xv[b][j];
     }
-#pragma unroll
-    for (int d = G / 2; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
     return sum;
 }
 
@@ -309,23 +295,16 @@ __global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_e
     }
 }
 
-template <int G, bool IL>
-__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(7)))
-spmv_packed_kernel(const int32_t* __restrict__ rowptr, const uint16_t* __restrict__ codes,
-                   const int32_t* __restrict__ pbase, const float* __restrict__ vals, const float* __restrict__ x,
-                   float* __restrict__ y, int64_t M, int64_t chunk) {
-    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
-    constexpr int GROUPS = PCG_THREADS / G;
-    const RowTiles t = IL ? tiles_interleaved(M, gridDim.x, blockIdx.x, GROUPS)
-                          : tiles_contiguous(M, chunk, logical_block(), GROUPS);
-    const NGS_GLOBAL int32_t* rp = as_global(rowptr);
-    for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
-        const int64_t row = t0 + grp;
-        if (row >= t.hi) continue;
-        float sum = row_dot_packed<G>(rp[row], rp[row + 1], as_global(codes), as_global(pbase), as_global(vals),
-                                      as_global(x), row, lane);
-        if (lane == 0) y[row] = sum;
-    }
+// y = A x for the run form: each block's contiguous row range (XCD-contiguous block order), one
+// row per thread.
+__global__ void __launch_bounds__(PCG_THREADS) spmv_runs_kernel(const int32_t* __restrict__ runs,
+                                                                const float* __restrict__ rvals,
+                                                                const float* __restrict__ x, float* __restrict__ y,
+                                                                int64_t M, int64_t chunk) {
+    const int64_t r0 = (int64_t)logical_block() * chunk;
+    const int64_t r1 = min(M, r0 + chunk);
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS)
+        y[row] = run_row_dot<float, float>(as_global(runs), as_global(rvals), as_global(x), M, row);
 }
 
 // r = b - A x in FP64 (FP32 entries, FP64 x and products), 8 lanes per row with 16-byte loads of
@@ -390,6 +369,28 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_kernel(const int32_t
             rr += ri * ri;
             bb += bi * bi;
         }
+    }
+    store_partials2(rr, bb, partials, partials + gridDim.x, lb);
+}
+
+// r = b - A x in FP64 for the run form (FP32 values, FP64 x and sums in the same column order).
+__global__ void __launch_bounds__(PCG_THREADS) residual_f64_runs_kernel(const int32_t* __restrict__ runs,
+                                                                        const float* __restrict__ rvals,
+                                                                        const double* __restrict__ b,
+                                                                        const double* __restrict__ x,
+                                                                        double* __restrict__ r, int64_t M,
+                                                                        int64_t chunk, double* partials) {
+    const int lb = logical_block();
+    const int64_t r0 = (int64_t)lb * chunk;
+    const int64_t r1 = min(M, r0 + chunk);
+    double rr = 0, bb = 0;
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
+        const double bi = b[row];
+        const double ri = x ? bi - run_row_dot<double, double>(as_global(runs), as_global(rvals), as_global(x), M, row)
+                            : bi;
+        r[row] = ri;
+        rr += ri * ri;
+        bb += bi * bi;
     }
     store_partials2(rr, bb, partials, partials + gridDim.x, lb);
 }
@@ -499,10 +500,10 @@ static HostState& host_state() {
 // contiguous range of blocks each one owns; every kernel looks up its system from its block index,
 // so one launch (and one captured graph, whose arguments are the fixed table pointers) serves any
 // mix of systems. Systems join and leave between launches by rewriting the table.
-enum : int32_t { SYS_CSR = 0, SYS_STENCIL = 1, SYS_PACKED = 2 };
+enum : int32_t { SYS_CSR = 0, SYS_STENCIL = 1, SYS_RUNS = 2 };
 
 // Flags of a launch sequence (which kernels the table's systems need).
-enum : int32_t { FLAG_STENCIL = 1, FLAG_PACKED = 2, FLAG_NO_CSR = 4 };
+enum : int32_t { FLAG_STENCIL = 1, FLAG_RUNS = 2, FLAG_NO_CSR = 4 };
 
 struct BatchSys {
     PcgArgs a;
@@ -560,39 +561,28 @@ bpcg_spmv_dot_kernel(const BatchSys* tab, const int32_t* blk2sys, int ph) {
     store_partials2(pq, 0.0, partials, nullptr, lb);
 }
 
-// q = A p and per-block p.q partials for the packed-column CSR systems of the table.
-template <bool IL>
-__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(7)))
-bpcg_spmv_packed_dot_kernel(const BatchSys* tab, const int32_t* blk2sys, int ph) {
-    constexpr int G = 8;
+// q = A p and per-block p.q partials for the run-form systems of the table.
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_runs_dot_kernel(const BatchSys* tab, const int32_t* blk2sys,
+                                                                    int ph) {
     int lb;
     const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
-    if (!B || B->kind != SYS_PACKED) return;
+    if (!B || B->kind != SYS_RUNS) return;
     const PcgArgs& A = B->a;
     if (A.st->slot[ph].done) return;
-    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
-    constexpr int GROUPS = PCG_THREADS / G;
-    const RowTiles t = (IL && B->nblk % NUM_XCD == 0) ? tiles_interleaved(A.M, B->nblk, blockIdx.x - B->blk0, GROUPS)
-                                                      : tiles_contiguous(A.M, A.chunk, lb, GROUPS);
-    const NGS_GLOBAL int32_t* __restrict__ rowptr = as_global(A.rowptr);
-    const NGS_GLOBAL uint16_t* __restrict__ codes = as_global(A.codes);
-    const NGS_GLOBAL int32_t* __restrict__ pbase = as_global(A.pbase);
-    const NGS_GLOBAL float* __restrict__ vals = as_global(A.vals);
+    const int64_t r0 = (int64_t)lb * A.chunk;
+    const int64_t r1 = min(A.M, r0 + A.chunk);
+    const NGS_GLOBAL int32_t* __restrict__ runs = as_global(A.runs);
+    const NGS_GLOBAL float* __restrict__ rvals = as_global(A.rvals);
     const NGS_GLOBAL float* __restrict__ p = as_global(A.p);
     NGS_GLOBAL float* __restrict__ q = as_global(A.q);
-    double* partials = A.partials;
     double pq = 0;
-    for (int64_t t0 = t.first; t0 < t.end; t0 += t.step) {
-        const int64_t row = t0 + grp;
-        if (row >= t.hi) continue;
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
         const float prow = p[row];
-        const float sum = row_dot_packed<G>(rowptr[row], rowptr[row + 1], codes, pbase, vals, p, row, lane);
-        if (lane == 0) {
-            q[row] = sum;
-            pq += (double)prow * sum;
-        }
+        const float sum = run_row_dot<float, float>(runs, rvals, p, A.M, row);
+        q[row] = sum;
+        pq += (double)prow * sum;
     }
-    store_partials2(pq, 0.0, partials, nullptr, lb);
+    store_partials2(pq, 0.0, A.partials, nullptr, lb);
 }
 
 // The same for the stencil systems of the table (row pass; the point pass ran before).
@@ -796,10 +786,7 @@ static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, 
                 <<<grid, block, 0, st>>>(tab, blk2sys, ph);
         });
     }
-    if (flags & FLAG_PACKED) {
-        if (g_spmv_il) bpcg_spmv_packed_dot_kernel<true><<<grid, block, 0, st>>>(tab, blk2sys, ph);
-        else bpcg_spmv_packed_dot_kernel<false><<<grid, block, 0, st>>>(tab, blk2sys, ph);
-    }
+    if (flags & FLAG_RUNS) bpcg_runs_dot_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
     if (flags & FLAG_STENCIL) bpcg_stencil_dot_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
     bpcg_update_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
     bpcg_direction_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
@@ -860,7 +847,7 @@ static int pcg_solve_entry(BatchSys E, int64_t M, double tol, int32_t max_iter, 
     fill_map_kernel<<<grid1d(A.nblocks, 256), 256, 0, work>>>(map, A.nblocks);
     NGS_CHECK_LAUNCH();
     const int flags = E.kind == SYS_STENCIL ? (FLAG_STENCIL | FLAG_NO_CSR)
-                      : E.kind == SYS_PACKED ? (FLAG_PACKED | FLAG_NO_CSR)
+                      : E.kind == SYS_RUNS ? (FLAG_RUNS | FLAG_NO_CSR)
                                              : 0;
     int status = NGS_OK;
     int32_t it = 0, done = 0;
@@ -912,23 +899,22 @@ int ngs_pcg_solve(const int32_t* rowptr, const int32_t* cols, const float* vals,
     return pcg_solve_entry(E, M, tol, max_iter, chunk, workspace, s, iters, rel_resid);
 }
 
-int ngs_pcg_solve_packed(const int32_t* rowptr, const uint16_t* codes, const int32_t* pbase, const float* vals,
-                         const float* inv_diag, const float* b, float* x, int64_t M, double tol, int32_t max_iter,
-                         int32_t chunk, void* workspace, ngs_stream_t s, int32_t* iters, double* rel_resid) {
-    NGS_REQUIRE(M >= 0 && workspace, "ngs_pcg_solve_packed: bad arguments");
-    NGS_REQUIRE(tol >= 0, "ngs_pcg_solve_packed: negative tolerance");
+int ngs_pcg_solve_runs(const int32_t* runs, const float* rvals, const float* inv_diag, const float* b, float* x,
+                       int64_t M, double tol, int32_t max_iter, int32_t chunk, void* workspace, ngs_stream_t s,
+                       int32_t* iters, double* rel_resid) {
+    NGS_REQUIRE(M >= 0 && workspace, "ngs_pcg_solve_runs: bad arguments");
+    NGS_REQUIRE(tol >= 0, "ngs_pcg_solve_runs: negative tolerance");
     if (M == 0) {
         if (iters) *iters = 0;
         if (rel_resid) *rel_resid = 0;
         return NGS_OK;
     }
-    NGS_REQUIRE(rowptr && codes && pbase && vals && inv_diag && b && x, "ngs_pcg_solve_packed: null array");
+    NGS_REQUIRE(runs && rvals && inv_diag && b && x, "ngs_pcg_solve_runs: null array");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_pcg_solve_runs: too many rows for run words");
     BatchSys E{};
-    E.kind = SYS_PACKED;
-    E.a.rowptr = rowptr;
-    E.a.codes = codes;
-    E.a.pbase = pbase;
-    E.a.vals = vals;
+    E.kind = SYS_RUNS;
+    E.a.runs = runs;
+    E.a.rvals = rvals;
     E.a.dinv = inv_diag;
     E.a.b = b;
     E.a.x = x;
@@ -971,16 +957,13 @@ int ngs_spmv(const int32_t* rowptr, const int32_t* cols, const float* vals, cons
     return NGS_OK;
 }
 
-int ngs_spmv_packed(const int32_t* rowptr, const uint16_t* codes, const int32_t* pbase, const float* vals, const float* x,
-                    float* y, int64_t M, ngs_stream_t s) {
+int ngs_spmv_runs(const int32_t* runs, const float* rvals, const float* x, float* y, int64_t M, ngs_stream_t s) {
     if (M <= 0) return NGS_OK;
-    NGS_REQUIRE(rowptr && codes && pbase && vals && x && y, "ngs_spmv_packed: null array");
+    NGS_REQUIRE(runs && rvals && x && y, "ngs_spmv_runs: null array");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_spmv_runs: too many rows for run words");
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
-    if (g_spmv_il && nb % NUM_XCD == 0)
-        spmv_packed_kernel<8, true><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, codes, pbase, vals, x, y, M, chunk);
-    else
-        spmv_packed_kernel<8, false><<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, codes, pbase, vals, x, y, M, chunk);
+    spmv_runs_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(runs, rvals, x, y, M, chunk);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
@@ -994,6 +977,21 @@ int ngs_residual_f64(const int32_t* rowptr, const int32_t* cols, const float* va
     char* w = reinterpret_cast<char*>(workspace);
     double* partials = reinterpret_cast<double*>(w + al256(sizeof(PcgState)));
     residual_f64_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(rowptr, cols, vals, b, x, r, M, chunk, partials);
+    norms_finish_kernel<<<1, PCG_THREADS, 0, S(s)>>>(partials, nb, norms);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_residual_f64_runs(const int32_t* runs, const float* rvals, const double* b, const double* x, double* r,
+                          int64_t M, double* norms, void* workspace, ngs_stream_t s) {
+    NGS_REQUIRE(workspace && b && r && norms && (!x || (runs && rvals)), "ngs_residual_f64_runs: bad arguments");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_residual_f64_runs: too many rows for run words");
+    if (M <= 0) return NGS_OK;
+    int64_t chunk;
+    int nb = pcg_blocks(M, &chunk);
+    char* w = reinterpret_cast<char*>(workspace);
+    double* partials = reinterpret_cast<double*>(w + al256(sizeof(PcgState)));
+    residual_f64_runs_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(runs, rvals, b, x, r, M, chunk, partials);
     norms_finish_kernel<<<1, PCG_THREADS, 0, S(s)>>>(partials, nb, norms);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
@@ -1075,19 +1073,17 @@ int ngs_pcg_batch_prepare(const int32_t* rowptr, const int32_t* cols, const floa
     return r;
 }
 
-int ngs_pcg_batch_prepare_packed(const int32_t* rowptr, const uint16_t* codes, const int32_t* pbase, const float* vals,
-                                 const float* inv_diag, const float* b, float* x, int64_t M, double tol,
-                                 int32_t max_iter, void* workspace, int32_t blk0, int32_t nblk, void* entry,
-                                 ngs_stream_t s) {
-    NGS_REQUIRE(M > 0 && workspace && entry && rowptr && codes && pbase && vals && inv_diag && b && x,
-                "ngs_pcg_batch_prepare_packed: bad arguments");
-    NGS_REQUIRE(nblk == ngs_pcg_batch_blocks(M), "ngs_pcg_batch_prepare_packed: nblk must be ngs_pcg_batch_blocks(M)");
+int ngs_pcg_batch_prepare_runs(const int32_t* runs, const float* rvals, const float* inv_diag, const float* b,
+                               float* x, int64_t M, double tol, int32_t max_iter, void* workspace, int32_t blk0,
+                               int32_t nblk, void* entry, ngs_stream_t s) {
+    NGS_REQUIRE(M > 0 && workspace && entry && runs && rvals && inv_diag && b && x,
+                "ngs_pcg_batch_prepare_runs: bad arguments");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_pcg_batch_prepare_runs: too many rows for run words");
+    NGS_REQUIRE(nblk == ngs_pcg_batch_blocks(M), "ngs_pcg_batch_prepare_runs: nblk must be ngs_pcg_batch_blocks(M)");
     BatchSys E{};
-    E.kind = SYS_PACKED;
-    E.a.rowptr = rowptr;
-    E.a.codes = codes;
-    E.a.pbase = pbase;
-    E.a.vals = vals;
+    E.kind = SYS_RUNS;
+    E.a.runs = runs;
+    E.a.rvals = rvals;
     E.a.dinv = inv_diag;
     E.a.b = b;
     E.a.x = x;

@@ -52,8 +52,10 @@ def reconstruct(xyz, normal, voxel_size: float, depth: int = 4, min_density: flo
     :param solver: solver string ("pcg@tol=1e-6@maxiter=10000", "mixed@tol=1e-8", ...); the default
         keeps the reference's PCG tolerance (1e-5 of ||b||) with an iteration cap
     :param xyz_depth: optional imposed per-point levels (skips the density-derived assignment)
-    :param operator: level operator form, "csr" (assembled matrix, as the reference), "stencil"
-        (applied from the voxel bitmap and the binned points) or "auto" (cheaper per iteration)
+    :param operator: level operator form, "runs" (assembled matrix, one run of consecutive columns
+        per bitmap column of the stencil), "csr" (assembled matrix in the reference's CSR layout),
+        "stencil" (applied from the voxel bitmap and the binned points) or "auto" (cheapest per
+        iteration: the run form at the sizes measured)
     :return: (vertices (V, 3), triangles (T, 3)) as numpy arrays for numpy input, device tensors
         otherwise; with return_result, a ReconstructionResult holding the solution field too.
     """

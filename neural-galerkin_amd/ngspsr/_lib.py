@@ -74,12 +74,12 @@ _SIGNATURES = {
     "ngs_set_tables": ([P, P, c_int32], c_int32),
     "ngs_lhs_row_counts": ([Level, P, P], c_int32),
     "ngs_lhs_assemble": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P, P], c_int32),
-    "ngs_lhs_assemble_packed": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P, P, P, P, P], c_int32),
-    "ngs_pcg_solve_packed": ([P, P, P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
-                              ctypes.POINTER(c_double)], c_int32),
-    "ngs_pcg_batch_prepare_packed": ([P, P, P, P, P, P, P, c_int64, c_double, c_int32, P, c_int32, c_int32, P, P],
-                                     c_int32),
-    "ngs_spmv_packed": ([P, P, P, P, P, P, c_int64, P], c_int32),
+    "ngs_lhs_assemble_runs": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P], c_int32),
+    "ngs_pcg_solve_runs": ([P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
+                            ctypes.POINTER(c_double)], c_int32),
+    "ngs_pcg_batch_prepare_runs": ([P, P, P, P, P, c_int64, c_double, c_int32, P, c_int32, c_int32, P, P], c_int32),
+    "ngs_spmv_runs": ([P, P, P, P, c_int64, P], c_int32),
+    "ngs_residual_f64_runs": ([P, P, P, P, P, c_int64, P, P, P], c_int32),
     "ngs_rhs_cross_stiffness": ([Level, Level, P, P, P], c_int32),
     "ngs_rhs_normals": ([Level, Level, P, P, P], c_int32),
     "ngs_basis_splat": ([Grid, P, P, c_int64, c_float, c_double, P, P], c_int32),
@@ -116,6 +116,7 @@ _SIGNATURES = {
     "ngs_dual_accumulate_leaves": ([Grid, P, c_int64, c_int32, c_int32, c_int32, P, P, P], c_int32),
     "ngs_dual_compact": ([P, P, c_int64, P, P, P, P, P], c_int32),
     "ngs_dual_corner_positions": ([P, c_int64, c_int32, c_float, P, P], c_int32),
+    "ngs_eval_leaf_corners": ([Grid, P, P, c_int64, c_int32, c_int32, c_float, c_float, c_int32, P, P], c_int32),
     "ngs_mc_count": ([P, c_int64, P, P, P], c_int32),
     "ngs_mc_emit": ([P, c_int64, P, P, P, P, P, P], c_int32),
     "ngs_dedup_workspace_bytes": ([c_int64], c_int64),

@@ -75,66 +75,70 @@ class Reconstructor:
         self.sample_weight = None
         self.fixed_level_set = None
         self.branch = HashTree.DECODER
-        self.operator = "auto"  # level operator form: "auto", "csr" or "stencil"
-        # CSR levels also get the packed-column form for the solver (6 instead of 8 bytes per entry).
-        # Off by default: on MI355X the finest-level SpMV is bound by its dependent-load latency, not by
-        # HBM bytes, so the packed form gains no time and its 16-bit stores slow the assembly.
-        self.pack_columns = False
+        self.operator = "auto"  # level operator form: "auto", "runs", "csr" or "stencil"
         ensure_device_tables(hash_tree.device)
 
     def set_fixed_level_set(self, value):
         self.fixed_level_set = value
 
-    # Per-application time model of the two operator forms (picoseconds, fitted to MI355X kernel
-    # timings on 0.05M-5.6M-row levels): CSR ~2.5 ps per stored entry (~90 entries per row); the
-    # stencil ~122 ps per row for the Laplacian part, ~3.5 ps per (row, point) visit of the
-    # screening part (27 cells around each row) and ~50 ps per point for the point pass.
+    # Per-application time model of the operator forms (picoseconds, fitted to MI355X kernel timings
+    # on 0.05M-5.6M-row levels): the run form ~115 ps per row (600 bytes per row at ~5.3 TB/s); CSR
+    # ~2.5 ps per stored entry (~90 entries per row); the stencil ~122 ps per row for the Laplacian
+    # part, ~3.5 ps per (row, point) visit of the screening part (27 cells around each row) and
+    # ~50 ps per point for the point pass. The run form indexes rows with 28 bits.
+    RUNS_PS_PER_ROW = 115.0
     CSR_PS_PER_ROW = 2.5 * 90
     STENCIL_PS_PER_ROW = 122.0
     STENCIL_PS_PER_VISIT = 3.5
     STENCIL_PS_PER_POINT = 50.0
+    RUNS_MAX_ROWS = 1 << 28
 
     def operator_form(self, M, n_points, n_cells=0):
-        if self.operator in ("csr", "stencil"):
-            return self.operator
-        visits = 27.0 * n_points / max(n_cells, 1) if n_points else 0.0
-        stencil = (self.STENCIL_PS_PER_ROW + self.STENCIL_PS_PER_VISIT * visits) * M + self.STENCIL_PS_PER_POINT * n_points
-        return "stencil" if stencil < self.CSR_PS_PER_ROW * M else "csr"
+        if self.operator in ("runs", "csr", "stencil"):
+            form = self.operator
+        else:
+            visits = 27.0 * n_points / max(n_cells, 1) if n_points else 0.0
+            stencil = ((self.STENCIL_PS_PER_ROW + self.STENCIL_PS_PER_VISIT * visits) * M
+                       + self.STENCIL_PS_PER_POINT * n_points)
+            form = "stencil" if stencil < self.RUNS_PS_PER_ROW * M else "runs"
+        if form == "runs" and M > self.RUNS_MAX_ROWS:
+            form = "csr"
+        return form
 
     def assemble_level(self, d, bins, coarse, sf, delta, rhs, form=None):
         """Level-d system (reconstructor.py:143-183); the screening RHS terms are added to rhs.
-        form: "csr" (assembled, ngs_lhs_assemble), "stencil" (ngs_stencil_*), None = operator_form."""
+        form: "runs" (ngs_lhs_assemble_runs), "csr" (ngs_lhs_assemble), "stencil" (ngs_stencil_*),
+        None = operator_form."""
         tree = self.hash_tree
         lv = tree.level(d)
         M = lv.M
+        dev = tree.device
         if form is None:
             form = self.operator_form(M, bins.n, getattr(bins, "ncells", 0))
         if form == "stencil":
             system = _solver.StencilSystem(tree.grids[d], tree.strides[d], bins, sf)
             system.diag_rhs(coarse, delta, rhs)
             return system
-        row_nnz = torch.empty(M, dtype=torch.int32, device=tree.device)
+        inv_diag = torch.empty(M, dtype=torch.float32, device=dev)
+        if form == "runs":
+            runs = torch.empty((25, M), dtype=torch.int32, device=dev)
+            rvals = torch.empty((125, M), dtype=torch.float32, device=dev)
+            call("ngs_lhs_assemble_runs", lv, bins.struct(), ptr(coarse), float(sf), float(delta), ptr(runs),
+                 ptr(rvals), ptr(inv_diag), ptr(rhs), stream())
+            return _solver.RunSystem(runs, rvals, inv_diag)
+        if form != "csr":
+            raise ValueError(f"unknown operator form {form!r}")
+        row_nnz = torch.empty(M, dtype=torch.int32, device=dev)
         call("ngs_lhs_row_counts", lv, ptr(row_nnz), stream())
-        rowptr = torch.empty(M + 1, dtype=torch.int32, device=tree.device)
-        ws = Scratch.get(lib().ngs_scan_workspace_bytes(M), tree.device)
+        rowptr = torch.empty(M + 1, dtype=torch.int32, device=dev)
+        ws = Scratch.get(lib().ngs_scan_workspace_bytes(M), dev)
         call("ngs_scan_i32", ptr(row_nnz), M, ptr(rowptr), ptr(ws), stream())
         nnz = int(rowptr[M].item())
-        cols = torch.empty(nnz, dtype=torch.int32, device=tree.device)
-        vals = torch.empty(nnz, dtype=torch.float32, device=tree.device)
-        inv_diag = torch.empty(M, dtype=torch.float32, device=tree.device)
-        if not self.pack_columns:
-            call("ngs_lhs_assemble", lv, bins.struct(), ptr(coarse), float(sf), float(delta), ptr(rowptr), ptr(cols),
-                 ptr(vals), ptr(inv_diag), ptr(rhs), stream())
-            return _solver.CSRSystem(rowptr, cols, vals, inv_diag)
-        # packed columns for the solver (6 instead of 8 bytes per entry per SpMV)
-        codes = torch.empty(nnz, dtype=torch.int16, device=tree.device)
-        pbase = torch.empty(8 * M, dtype=torch.int32, device=tree.device)
-        overflow = torch.zeros(1, dtype=torch.int32, device=tree.device)
-        call("ngs_lhs_assemble_packed", lv, bins.struct(), ptr(coarse), float(sf), float(delta), ptr(rowptr),
-             ptr(cols), ptr(codes), ptr(pbase), ptr(overflow), ptr(vals), ptr(inv_diag), ptr(rhs), stream())
-        if int(overflow.item()):
-            return _solver.CSRSystem(rowptr, cols, vals, inv_diag)
-        return _solver.CSRSystem(rowptr, cols, vals, inv_diag, codes, pbase)
+        cols = torch.empty(nnz, dtype=torch.int32, device=dev)
+        vals = torch.empty(nnz, dtype=torch.float32, device=dev)
+        call("ngs_lhs_assemble", lv, bins.struct(), ptr(coarse), float(sf), float(delta), ptr(rowptr), ptr(cols),
+             ptr(vals), ptr(inv_diag), ptr(rhs), stream())
+        return _solver.CSRSystem(rowptr, cols, vals, inv_diag)
 
     def solve_multigrid(self, start_depth, end_depth, normal_data: dict, screen_alpha=0.0, screen_xyz=None,
                         screen_delta=0.1, solver: str = "pcg", verbose: bool = False, solve_fn=None):
@@ -313,12 +317,27 @@ class Reconstructor:
             if primal[d].shape[0]:
                 call("ngs_dual_corner_positions", ptr(primal[d]), primal[d].shape[0], tree.strides[d], scale,
                      ptr(pos[bases_[d]:]), stream())
-        values = self.evaluate_chi(pos, max_points=max_points)
+        values = self.leaf_corner_chi(leaves, bases_, nprim, nu, scale) - self.get_mean_chi()
         v, f = self.dual_marching_cubes(cubes, pos, values)
         if return_info:
             return v, f, dict(leaves=leaves, primal=primal, centres=cg.coords, acc=acc, count=cnt, cubes=cubes, pos=pos,
                               values=values)
         return v, f
+
+    def leaf_corner_chi(self, leaves, bases_, nprim, nu, scale):
+        """Raw chi at the dual corners of every leaf (the values evaluate_raw_chi gives at the
+        positions ngs_dual_corner_positions writes, level by level in the same order)."""
+        tree = self.hash_tree
+        out = torch.zeros(nprim, dtype=torch.float32, device=tree.device)
+        for d, sol in self.solutions.items():
+            if sol.numel() == 0:
+                continue
+            for dl, lf in leaves.items():
+                if lf.shape[0] == 0:
+                    continue
+                call("ngs_eval_leaf_corners", tree.grids[d].struct(), ptr(sol), ptr(lf), lf.shape[0], tree.strides[dl],
+                     nu, scale, tree.inv_vs, 1, ptr(out[bases_[dl]:]), stream())
+        return out
 
     @staticmethod
     def dual_marching_cubes(cubes, pos, values):

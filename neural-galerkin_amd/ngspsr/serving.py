@@ -44,15 +44,15 @@ class _Job:
 
 def launch_flags(systems):
     """ngs_pcg_batch_iterate flags for a set of systems: which SpMV kernels they need."""
-    stencil = packed = plain = False
+    stencil = runs = plain = False
     for s in systems:
         if isinstance(s, _solver.StencilSystem):
             stencil = True
-        elif s.packed:
-            packed = True
+        elif isinstance(s, _solver.RunSystem):
+            runs = True
         else:
             plain = True
-    return (1 if stencil else 0) | (2 if packed else 0) | (0 if plain else 4)
+    return (1 if stencil else 0) | (2 if runs else 0) | (0 if plain else 4)
 
 
 class BatchedPCG:
@@ -134,10 +134,10 @@ class BatchedPCG:
                 call("ngs_pcg_batch_prepare_stencil", s.struct(), ptr(s.inv_diag), ptr(job.b), ptr(job.x),
                      float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
                      self.stream.cuda_stream)
-            elif s.packed:
-                call("ngs_pcg_batch_prepare_packed", ptr(s.rowptr), ptr(s.codes), ptr(s.pbase), ptr(s.vals),
-                     ptr(s.inv_diag), ptr(job.b), ptr(job.x), s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0,
-                     job.nblk, entry.data_ptr(), self.stream.cuda_stream)
+            elif isinstance(s, _solver.RunSystem):
+                call("ngs_pcg_batch_prepare_runs", ptr(s.runs), ptr(s.rvals), ptr(s.inv_diag), ptr(job.b), ptr(job.x),
+                     s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
+                     self.stream.cuda_stream)
             else:
                 call("ngs_pcg_batch_prepare", ptr(s.rowptr), ptr(s.cols), ptr(s.vals), ptr(s.inv_diag), ptr(job.b),
                      ptr(job.x), s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),

@@ -34,19 +34,12 @@ def parse_solver(spec: str):
 
 
 class CSRSystem:
-    """Assembled level system: CSR (rows padded to multiples of 4 entries), inverse diagonal, and
-    optionally the packed-column form of the same matrix (16-bit codes + per-row plane bases,
-    ngs_lhs_assemble_packed), which the solver then uses."""
+    """Assembled level system: CSR (rows padded to multiples of 4 entries) and inverse diagonal."""
 
-    def __init__(self, rowptr, cols, vals, inv_diag, codes=None, pbase=None):
+    def __init__(self, rowptr, cols, vals, inv_diag):
         self.rowptr, self.cols, self.vals, self.inv_diag = rowptr, cols, vals, inv_diag
-        self.codes, self.pbase = codes, pbase
         self.M = inv_diag.shape[0]
         self.device = vals.device
-
-    @property
-    def packed(self):
-        return self.codes is not None
 
     @property
     def nnz_stored(self):
@@ -59,12 +52,8 @@ class CSRSystem:
     def matvec(self, x):
         y = torch.empty(self.M, dtype=torch.float32, device=self.device)
         if self.M:
-            if self.packed:
-                call("ngs_spmv_packed", ptr(self.rowptr), ptr(self.codes), ptr(self.pbase), ptr(self.vals),
-                     ptr(x.float().contiguous()), ptr(y), self.M, stream())
-            else:
-                call("ngs_spmv", ptr(self.rowptr), ptr(self.cols), ptr(self.vals), ptr(x.float().contiguous()), ptr(y),
-                     self.M, stream())
+            call("ngs_spmv", ptr(self.rowptr), ptr(self.cols), ptr(self.vals), ptr(x.float().contiguous()), ptr(y),
+                 self.M, stream())
         return y
 
     def residual_f64(self, b64, x64):
@@ -82,6 +71,87 @@ class CSRSystem:
         import scipy.sparse as sp
         rp = self.rowptr.cpu().numpy()
         a = sp.csr_matrix((self.vals.cpu().numpy().astype(np.float64), self.cols.cpu().numpy(), rp),
+                          shape=(self.M, self.M))
+        a.eliminate_zeros()
+        return a
+
+
+class RunSystem:
+    """Assembled level system in run form (ngs_lhs_assemble_runs): for each row and each of the 25
+    bitmap columns (dx, dy) of its 5x5x5 neighbourhood, one run word (first column | count << 28)
+    and five value slots, both plane-major: runs (25, M) int32, rvals (125, M) float32. The values
+    are the CSR entries bit for bit; the SpMV reads 600 bytes per row with every load contiguous
+    across consecutive rows."""
+
+    COUNT_SHIFT = 28
+
+    def __init__(self, runs, rvals, inv_diag):
+        self.runs, self.rvals, self.inv_diag = runs, rvals, inv_diag
+        self.M = inv_diag.shape[0]
+        self.device = inv_diag.device
+
+    def counts(self):
+        """(25, M) run lengths."""
+        return (self.runs >> self.COUNT_SHIFT) & 7
+
+    def nnz(self):
+        """Stored entries (the CSR entries of the same matrix, without padding)."""
+        return int(self.counts().sum(dtype=torch.int64).item())
+
+    def matvec(self, x):
+        y = torch.empty(self.M, dtype=torch.float32, device=self.device)
+        if self.M:
+            call("ngs_spmv_runs", ptr(self.runs), ptr(self.rvals), ptr(x.float().contiguous()), ptr(y), self.M,
+                 stream())
+        return y
+
+    def residual_f64(self, b64, x64):
+        """(r = b - A x in FP64, ||r||^2, ||b||^2); x64 None means x = 0."""
+        r = torch.empty(self.M, dtype=torch.float64, device=self.device)
+        norms = torch.empty(2, dtype=torch.float64, device=self.device)
+        ws = _lib.Scratch.get(lib().ngs_pcg_workspace_bytes(self.M), self.device, slot=1)
+        call("ngs_residual_f64_runs", ptr(self.runs), ptr(self.rvals), ptr(b64.contiguous()),
+             None if x64 is None else ptr(x64.contiguous()), ptr(r), self.M, ptr(norms), ptr(ws), stream())
+        nv = norms.cpu()
+        return r, float(nv[0]), float(nv[1])
+
+    def coo(self):
+        """(rows, cols, vals) of the stored entries, rows ascending and columns ascending per row."""
+        M, dev = self.M, self.device
+        n = self.counts()
+        start = self.runs & ((1 << self.COUNT_SHIFT) - 1)
+        j = torch.arange(5, dtype=torch.int32, device=dev).view(1, 5, 1)
+        # (row, plane, slot) order is increasing column order within a row
+        present = (j < n.view(25, 1, M)).permute(2, 0, 1)
+        cols = (start.view(25, 1, M) + j).permute(2, 0, 1)[present]
+        rows = torch.arange(M, dtype=torch.int32, device=dev).view(1, 1, M).expand(25, 5, M).permute(2, 0, 1)[present]
+        vals = self.rvals.view(25, 5, M).permute(2, 0, 1)[present]
+        return rows, cols, vals
+
+    def to_csr(self):
+        """The same matrix as the CSR of ngs_lhs_assemble: rows in column order, padded to multiples
+        of 4 entries with the row's own column and value 0 (the CSR kernels read 4 entries at a time)."""
+        M, dev = self.M, self.device
+        r, c, v = self.coo()
+        counts = torch.bincount(r, minlength=M)
+        padded = (counts + 3) // 4 * 4
+        rowptr = torch.zeros(M + 1, dtype=torch.int64, device=dev)
+        rowptr[1:] = torch.cumsum(padded, 0)
+        first = torch.cumsum(counts, 0) - counts  # index of each row's first entry in (r, c, v)
+        rl = r.long()
+        pos = rowptr[:-1][rl] + (torch.arange(r.shape[0], device=dev) - first[rl])
+        total = int(rowptr[M].item())
+        cols = torch.repeat_interleave(torch.arange(M, dtype=torch.int32, device=dev), padded)
+        vals = torch.zeros(total, dtype=torch.float32, device=dev)
+        cols[pos] = c
+        vals[pos] = v
+        return CSRSystem(rowptr.to(torch.int32), cols, vals, self.inv_diag)
+
+    def to_scipy(self):
+        import numpy as np
+        import scipy.sparse as sp
+        r, c, v = self.coo()
+        a = sp.csr_matrix((v.cpu().numpy().astype(np.float64), (r.cpu().numpy(), c.cpu().numpy())),
                           shape=(self.M, self.M))
         a.eliminate_zeros()
         return a
@@ -214,10 +284,10 @@ def pcg(system, b, tol=1e-5, maxiter=-1, chunk=32, x_out=None, raise_on_cap=True
         st = lib().ngs_pcg_solve_stencil(system.struct(), ptr(system.inv_diag), ptr(b), ptr(x), float(tol),
                                          int(maxiter), int(chunk), ptr(ws), stream(), ctypes.byref(it),
                                          ctypes.byref(rr))
-    elif system.packed:
-        st = lib().ngs_pcg_solve_packed(ptr(system.rowptr), ptr(system.codes), ptr(system.pbase), ptr(system.vals),
-                                        ptr(system.inv_diag), ptr(b), ptr(x), M, float(tol), int(maxiter), int(chunk),
-                                        ptr(ws), stream(), ctypes.byref(it), ctypes.byref(rr))
+    elif isinstance(system, RunSystem):
+        st = lib().ngs_pcg_solve_runs(ptr(system.runs), ptr(system.rvals), ptr(system.inv_diag), ptr(b), ptr(x), M,
+                                      float(tol), int(maxiter), int(chunk), ptr(ws), stream(), ctypes.byref(it),
+                                      ctypes.byref(rr))
     else:
         st = lib().ngs_pcg_solve(ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(system.inv_diag),
                                  ptr(b), ptr(x), M, float(tol), int(maxiter), int(chunk), ptr(ws), stream(),

@@ -49,3 +49,16 @@ def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_LIB", None)
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         _lib.lib()
+
+
+def test_run_form_rejects_rows_beyond_run_words():
+    """Run words index rows with 28 bits: the run-form entry points refuse more than 2^28 rows
+    before any device work (argument checks only; the pointers are never dereferenced)."""
+    from ngspsr import _lib
+    lib = _lib.lib()
+    fake = ctypes.c_void_p(256)
+    M = (1 << 28) + 1
+    assert lib.ngs_spmv_runs(fake, fake, fake, fake, M, None) == 1
+    assert b"too many rows" in lib.ngs_last_error()
+    assert lib.ngs_residual_f64_runs(fake, fake, fake, fake, fake, M, fake, fake, None) == 1
+    assert b"too many rows" in lib.ngs_last_error()

@@ -203,3 +203,16 @@ def test_dual_leaf_forms_equal_per_cell(pair, nu):
     assert torch.equal(cnt, info["count"])
     full = cnt == 8
     assert torch.equal(acc[full], info["acc"][full])
+
+
+@pytest.mark.parametrize("nu", [1, 2, 3])
+def test_leaf_corner_chi_equals_pointwise(pair, nu):
+    """chi at the dual corners, evaluated per leaf (one coefficient window per leaf for nu = 2),
+    equals evaluate_chi at the written corner positions bit for bit."""
+    print(f"[case] leaf corner chi nu={nu}", flush=True)
+    rec = pair["res"].reconstructor
+    _, _, info = rec.extract_multiscale_mesh(n_upsample=nu, return_info=True)
+    ref = rec.evaluate_chi(info["pos"])
+    torch.cuda.synchronize()
+    assert info["values"].shape == ref.shape
+    assert torch.equal(info["values"], ref), int((info["values"] != ref).sum())
