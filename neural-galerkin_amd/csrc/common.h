@@ -100,9 +100,16 @@ __device__ __forceinline__ bool grid_test(const ngs_grid& g, int cx, int cy, int
     return (as_global(g.bits)[lin >> 5] >> (lin & 31)) & 1u;
 }
 
+// Sets the bits `mask` of word w. Bits only go from 0 to 1 while a grid is marked, so a plain read
+// that already shows them set makes the atomic unnecessary (a stale read only costs the atomic);
+// many points mark the same coarse-level voxels.
+__device__ __forceinline__ void grid_mark_word(const ngs_grid& g, int64_t w, uint32_t mask) {
+    if ((g.bits[w] & mask) != mask) atomicOr(&g.bits[w], mask);
+}
+
 __device__ __forceinline__ void grid_mark(const ngs_grid& g, int cx, int cy, int cz) {
     int64_t lin;
-    if (grid_cell(g, cx, cy, cz, &lin)) atomicOr(&g.bits[lin >> 5], 1u << (lin & 31));
+    if (grid_cell(g, cx, cy, cz, &lin)) grid_mark_word(g, lin >> 5, 1u << (lin & 31));
 }
 
 // Presence mask of the n (<= 31) consecutive lattice points (lx, ly, lz0 + k), k < n, of g (lattice

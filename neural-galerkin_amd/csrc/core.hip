@@ -302,10 +302,19 @@ __global__ void mark_corners_kernel(ngs_grid g, const float* __restrict__ xyz, i
     float loc[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) trilinear_axis(xyz[3 * i + a] * inv_vs, s, inv_s, &dc[a], &sg[a], &loc[a]);
+    // the two z-corners of each (x, y) usually share a bitmap word: one atomic for both
 #pragma unroll
-    for (int o = 0; o < 8; ++o) {
-        int nx = (o >> 2) & 1, ny = (o >> 1) & 1, nz = o & 1;
-        grid_mark(g, dc[0] + nx * sg[0] * s, dc[1] + ny * sg[1] * s, dc[2] + nz * sg[2] * s);
+    for (int o = 0; o < 4; ++o) {
+        const int nx = (o >> 1) & 1, ny = o & 1;
+        const int cx = dc[0] + nx * sg[0] * s, cy = dc[1] + ny * sg[1] * s;
+        int64_t l0, l1;
+        const bool v0 = grid_cell(g, cx, cy, dc[2], &l0), v1 = grid_cell(g, cx, cy, dc[2] + sg[2] * s, &l1);
+        if (v0 && v1 && (l0 >> 5) == (l1 >> 5)) {
+            grid_mark_word(g, l0 >> 5, (1u << (l0 & 31)) | (1u << (l1 & 31)));
+        } else {
+            if (v0) grid_mark_word(g, l0 >> 5, 1u << (l0 & 31));
+            if (v1) grid_mark_word(g, l1 >> 5, 1u << (l1 & 31));
+        }
     }
 }
 

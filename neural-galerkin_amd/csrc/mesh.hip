@@ -438,13 +438,13 @@ __global__ void mark_dual_leaves_kernel(ngs_grid g, const int32_t* __restrict__ 
         if (!grid_cell(g, x, y, z0 + oz * s, &lin)) continue;
         int64_t w = lin >> 5;
         if (w != cur_word) {
-            if (mask) atomicOr(&g.bits[cur_word], mask);
+            if (mask) grid_mark_word(g, cur_word, mask);
             cur_word = w;
             mask = 0;
         }
         mask |= 1u << (lin & 31);
     }
-    if (mask) atomicOr(&g.bits[cur_word], mask);
+    if (mask) grid_mark_word(g, cur_word, mask);
 }
 
 // Dual-cube assembly for leaves of stride 1 (the finest level): one thread per (leaf, lattice point

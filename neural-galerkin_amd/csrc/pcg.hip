@@ -295,15 +295,18 @@ __global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_e
     }
 }
 
-// y = A x for the run form: each block's contiguous row range (XCD-contiguous block order), one
-// row per thread.
+// Rows of the run-form kernels: one row per thread, grid-stride over the system's blocks (all blocks
+// sweep the rows together, block b taking rows b*256 .. b*256+255 of every 256*nb stretch), which
+// measured faster than per-block contiguous ranges for this access pattern.
+__device__ __forceinline__ int64_t run_rows_first(int local) { return (int64_t)local * PCG_THREADS + threadIdx.x; }
+
+// y = A x for the run form.
 __global__ void __launch_bounds__(PCG_THREADS) spmv_runs_kernel(const int32_t* __restrict__ runs,
                                                                 const float* __restrict__ rvals,
                                                                 const float* __restrict__ x, float* __restrict__ y,
-                                                                int64_t M, int64_t chunk) {
-    const int64_t r0 = (int64_t)logical_block() * chunk;
-    const int64_t r1 = min(M, r0 + chunk);
-    for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS)
+                                                                int64_t M) {
+    const int64_t step = (int64_t)gridDim.x * PCG_THREADS;
+    for (int64_t row = run_rows_first(blockIdx.x); row < M; row += step)
         y[row] = run_row_dot<float, float>(as_global(runs), as_global(rvals), as_global(x), M, row);
 }
 
@@ -379,12 +382,10 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_runs_kernel(const in
                                                                         const double* __restrict__ b,
                                                                         const double* __restrict__ x,
                                                                         double* __restrict__ r, int64_t M,
-                                                                        int64_t chunk, double* partials) {
-    const int lb = logical_block();
-    const int64_t r0 = (int64_t)lb * chunk;
-    const int64_t r1 = min(M, r0 + chunk);
+                                                                        double* partials) {
+    const int64_t step = (int64_t)gridDim.x * PCG_THREADS;
     double rr = 0, bb = 0;
-    for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
+    for (int64_t row = run_rows_first(blockIdx.x); row < M; row += step) {
         const double bi = b[row];
         const double ri = x ? bi - run_row_dot<double, double>(as_global(runs), as_global(rvals), as_global(x), M, row)
                             : bi;
@@ -392,7 +393,7 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_runs_kernel(const in
         rr += ri * ri;
         bb += bi * bi;
     }
-    store_partials2(rr, bb, partials, partials + gridDim.x, lb);
+    store_partials2(rr, bb, partials, partials + gridDim.x, blockIdx.x);
 }
 
 // One block: norms[0..1] = sums of the two partial arrays of a residual kernel (nb entries each).
@@ -569,14 +570,13 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_runs_dot_kernel(const BatchS
     if (!B || B->kind != SYS_RUNS) return;
     const PcgArgs& A = B->a;
     if (A.st->slot[ph].done) return;
-    const int64_t r0 = (int64_t)lb * A.chunk;
-    const int64_t r1 = min(A.M, r0 + A.chunk);
+    const int64_t step = (int64_t)B->nblk * PCG_THREADS;
     const NGS_GLOBAL int32_t* __restrict__ runs = as_global(A.runs);
     const NGS_GLOBAL float* __restrict__ rvals = as_global(A.rvals);
     const NGS_GLOBAL float* __restrict__ p = as_global(A.p);
     NGS_GLOBAL float* __restrict__ q = as_global(A.q);
     double pq = 0;
-    for (int64_t row = r0 + threadIdx.x; row < r1; row += PCG_THREADS) {
+    for (int64_t row = run_rows_first(blockIdx.x - B->blk0); row < A.M; row += step) {
         const float prow = p[row];
         const float sum = run_row_dot<float, float>(runs, rvals, p, A.M, row);
         q[row] = sum;
@@ -963,7 +963,7 @@ int ngs_spmv_runs(const int32_t* runs, const float* rvals, const float* x, float
     NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_spmv_runs: too many rows for run words");
     int64_t chunk;
     int nb = pcg_blocks(M, &chunk);
-    spmv_runs_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(runs, rvals, x, y, M, chunk);
+    spmv_runs_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(runs, rvals, x, y, M);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
@@ -991,7 +991,7 @@ int ngs_residual_f64_runs(const int32_t* runs, const float* rvals, const double*
     int nb = pcg_blocks(M, &chunk);
     char* w = reinterpret_cast<char*>(workspace);
     double* partials = reinterpret_cast<double*>(w + al256(sizeof(PcgState)));
-    residual_f64_runs_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(runs, rvals, b, x, r, M, chunk, partials);
+    residual_f64_runs_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(runs, rvals, b, x, r, M, partials);
     norms_finish_kernel<<<1, PCG_THREADS, 0, S(s)>>>(partials, nb, norms);
     NGS_CHECK_LAUNCH();
     return NGS_OK;

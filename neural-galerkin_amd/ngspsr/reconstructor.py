@@ -70,6 +70,7 @@ class Reconstructor:
         self.hash_tree = hash_tree
         self.basis = basis or BezierTensorBasis()
         self.solutions = {}
+        self._points_chi = None
         self.systems = {}
         self.solve_info = {}
         self.sample_weight = None
@@ -156,6 +157,8 @@ class Reconstructor:
         if isinstance(screen_delta, torch.Tensor):
             raise NotImplementedError("per-point screen_delta is not supported on this path")
         self.solutions = {}
+        self._points_chi = None
+        running, added = None, []
         xyz = screen_xyz.float().contiguous() if should_screen else None
         bins = {}
         for d in range(end_depth, start_depth + 1):
@@ -172,10 +175,16 @@ class Reconstructor:
             coarser = [dd for dd in range(start_depth, d, -1) if tree.get_coords_size(None, dd) > 0]
             coarse = None
             if should_screen and coarser:
-                coarse = torch.zeros(xyz.shape[0], dtype=torch.float32, device=tree.device)
+                # field of the coarser levels at the points, accumulated level by level from the coarsest
+                # (the order of evaluate_raw_chi): each level is added once, when it is first needed
+                if running is None:
+                    running = torch.zeros(xyz.shape[0], dtype=torch.float32, device=tree.device)
                 for dd in coarser:
-                    call("ngs_eval_field", tree.grids[dd].struct(), ptr(self.solutions[dd]), ptr(xyz), xyz.shape[0],
-                         tree.inv_vs, 1.0, 1, ptr(coarse), None, stream())
+                    if dd not in added:
+                        call("ngs_eval_field", tree.grids[dd].struct(), ptr(self.solutions[dd]), ptr(xyz),
+                             xyz.shape[0], tree.inv_vs, 1.0, 1, ptr(running), None, stream())
+                        added.append(dd)
+                coarse = running
             for dd in coarser:
                 call("ngs_rhs_cross_stiffness", tree.level(d), tree.level(dd), ptr(self.solutions[dd]), ptr(rhs),
                      stream())
@@ -187,7 +196,28 @@ class Reconstructor:
             if verbose:
                 r = system.matvec(x).double() - rhs
                 print(f"Solving complete at level {d}, residual = {torch.linalg.norm(r).item()}.")
+        if running is not None and self._same_points(xyz):
+            # completed with the remaining levels, the running sum is chi at the input points
+            for dd, sol in self.solutions.items():
+                if sol.numel() and dd not in added:
+                    call("ngs_eval_field", tree.grids[dd].struct(), ptr(sol), ptr(xyz), xyz.shape[0], tree.inv_vs,
+                         1.0, 1, ptr(running), None, stream())
+            self._points_chi = (self._solutions_key(), running)
         return self.solutions
+
+    def _same_points(self, xyz):
+        q = self.hash_tree.xyz
+        return xyz is not None and xyz.data_ptr() == q.data_ptr() and xyz.shape == q.shape and xyz.dtype == q.dtype
+
+    def _solutions_key(self):
+        # the tensors themselves (kept alive by the key, so identity cannot be reused) and their
+        # in-place version counters
+        return [(d, s, s._version) for d, s in self.solutions.items()]
+
+    def _key_matches(self, key):
+        cur = list(self.solutions.items())
+        return len(cur) == len(key) and all(d == kd and s is ks and s._version == kv
+                                            for (d, s), (kd, ks, kv) in zip(cur, key))
 
     # -------------------------------------------------------------------------- field
     def evaluate_raw_chi(self, xyz, compute_grad=False, depths=None):
@@ -209,7 +239,11 @@ class Reconstructor:
         """reconstructor.py:228-236: level set = sample-weighted mean of chi at the input points."""
         if self.fixed_level_set is not None:
             return float(self.fixed_level_set)
-        chi = self.evaluate_raw_chi(self.hash_tree.xyz)
+        cached = getattr(self, "_points_chi", None)
+        if cached is not None and self._key_matches(cached[0]):
+            chi = cached[1]  # raw chi at the input points, summed during the solve in the same order
+        else:
+            chi = self.evaluate_raw_chi(self.hash_tree.xyz)
         sums = torch.empty(2, dtype=torch.float64, device=chi.device)
         w = None if self.sample_weight is None else self.sample_weight.float().contiguous()
         call("ngs_weighted_sum", ptr(chi), ptr(w), chi.shape[0], ptr(sums), stream())
