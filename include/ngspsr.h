@@ -382,9 +382,11 @@ int ngs_mark_dual_centres(ngs_grid g, const int32_t *primal, int64_t n, int32_t 
 int ngs_dual_accumulate(ngs_grid centres, const int32_t *primal, int64_t n, int32_t stride, int32_t level,
                         int32_t id_base, int32_t *acc, int32_t *count, ngs_stream_t s);
 /* Keep complete dual cubes (count == 8) in order: cubes (K x 8) corner ids; flags (dc) and
- * offsets (dc + 1, offsets[dc] = K) are written on the way. */
-int ngs_dual_compact(const int32_t *acc, const int32_t *count, int64_t dc, int32_t *cubes, int32_t *flags,
-                     int32_t *offsets, void *workspace, ngs_stream_t s);
+ * offsets (dc + 1, offsets[dc] = K) are written on the way. With values (the field at the primal
+ * corners, nullable), only complete cubes whose corner values change sign are kept (the others
+ * yield no triangles). acc is read only for complete centres. */
+int ngs_dual_compact(const int32_t *acc, const int32_t *count, const float *values, int64_t dc, int32_t *cubes,
+                     int32_t *flags, int32_t *offsets, void *workspace, ngs_stream_t s);
 /* Leaf forms of the two calls above (same sets, fewer atomics): the dual centres of all sub-cells
  * of each leaf (lattice leaf*n + o*stride, o in [0, n]^3), and the dual-cube assembly of stride-1
  * leaves (one count update per leaf and lattice point; identical to the per-sub-cell form whenever no

@@ -197,16 +197,31 @@ __global__ void dual_accumulate_kernel(ngs_grid cen, const int32_t* __restrict__
             for (int sz = 0; sz < 2; ++sz) {
                 if (!((side[2] >> sz) & 1)) continue;
                 int sl = octant_slot(sx, sy, sz);
-                acc[((int64_t)dc * 8 + sl) * 2] = pid;
-                acc[((int64_t)dc * 8 + sl) * 2 + 1] = level;
+                *reinterpret_cast<int2*>(acc + ((int64_t)dc * 8 + sl) * 2) = make_int2(pid, level);
             }
         }
     }
 }
 
-__global__ void complete_flags_kernel(const int32_t* __restrict__ cnt, int64_t n, int32_t* __restrict__ flag) {
+// Complete cubes (count 8); with values, only those whose corners change sign (the cubes marching
+// cubes emits triangles for: cases other than 0 and 255).
+__global__ void complete_flags_kernel(const int32_t* __restrict__ cnt, const int32_t* __restrict__ acc,
+                                      const float* __restrict__ val, int64_t n, int32_t* __restrict__ flag) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flag[i] = cnt[i] == 8 ? 1 : 0;
+    if (i >= n) return;
+    int f = cnt[i] == 8 ? 1 : 0;
+    if (f && val) {
+        const int4* a = reinterpret_cast<const int4*>(acc + i * 16);
+        int c = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int4 q = a[k];
+            if (val[q.x] < 0.f) c |= 1 << (2 * k);
+            if (val[q.z] < 0.f) c |= 2 << (2 * k);
+        }
+        f = (c != 0 && c != 255) ? 1 : 0;
+    }
+    flag[i] = f;
 }
 
 __global__ void compact_cubes_kernel(const int32_t* __restrict__ acc, const int32_t* __restrict__ flag,
@@ -480,8 +495,7 @@ __global__ void dual_accumulate_leaves_kernel(ngs_grid cen, const int32_t* __res
             for (int k = 0; k < nc[2]; ++k) {
                 int sl = octant_slot(sd[0][i], sd[1][j], sd[2][k]);
                 int e = (ce[0][i] * nu + ce[1][j]) * nu + ce[2][k];
-                acc[((int64_t)dc * 8 + sl) * 2] = (int32_t)(base + e);
-                acc[((int64_t)dc * 8 + sl) * 2 + 1] = level;
+                *reinterpret_cast<int2*>(acc + ((int64_t)dc * 8 + sl) * 2) = make_int2((int32_t)(base + e), level);
             }
 }
 
@@ -548,11 +562,11 @@ int ngs_dual_accumulate(ngs_grid centres, const int32_t* primal, int64_t n, int3
     return NGS_OK;
 }
 
-int ngs_dual_compact(const int32_t* acc, const int32_t* count, int64_t dc, int32_t* cubes, int32_t* flags, int32_t* offsets,
-                     void* workspace, ngs_stream_t s) {
+int ngs_dual_compact(const int32_t* acc, const int32_t* count, const float* values, int64_t dc, int32_t* cubes,
+                     int32_t* flags, int32_t* offsets, void* workspace, ngs_stream_t s) {
     NGS_REQUIRE(workspace && flags && offsets, "ngs_dual_compact: bad arguments");
     if (dc <= 0) return NGS_OK;
-    complete_flags_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(count, dc, flags);
+    complete_flags_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(count, acc, values, dc, flags);
     NGS_CHECK_LAUNCH();
     int r = scan_i32_internal(flags, dc, offsets, workspace, S(s));
     if (r != NGS_OK) return r;

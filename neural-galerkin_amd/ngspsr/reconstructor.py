@@ -326,7 +326,8 @@ class Reconstructor:
                 call("ngs_mark_dual_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], tree.strides[d], nu,
                      stream())
         DC = cg.finalize()
-        acc = torch.zeros((DC, 8, 2), dtype=torch.int32, device=dev)
+        # (id, level) per octant; read only for complete centres, whose eight slots are all written
+        acc = torch.empty((DC, 8, 2), dtype=torch.int32, device=dev)
         cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
         for d in range(L):
             if not primal[d].shape[0]:
@@ -337,13 +338,6 @@ class Reconstructor:
             else:
                 call("ngs_dual_accumulate", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d], d,
                      bases_[d], ptr(acc), ptr(cnt), stream())
-        cubes = torch.empty((DC, 8), dtype=torch.int32, device=dev)
-        flags = torch.empty(DC, dtype=torch.int32, device=dev)
-        offs = torch.empty(DC + 1, dtype=torch.int32, device=dev)
-        ws = Scratch.get(lib().ngs_scan_workspace_bytes(DC), dev)
-        call("ngs_dual_compact", ptr(acc), ptr(cnt), DC, ptr(cubes), ptr(flags), ptr(offs), ptr(ws), stream())
-        K = int(offs[DC].item()) if DC else 0
-        cubes = cubes[:K]
         # dual corners = (primal + stride) * voxel_size / n_upsample, and chi there
         pos = torch.empty((nprim, 3), dtype=torch.float32, device=dev)
         scale = float(np.float32(tree.voxel_size / nu))
@@ -352,6 +346,15 @@ class Reconstructor:
                 call("ngs_dual_corner_positions", ptr(primal[d]), primal[d].shape[0], tree.strides[d], scale,
                      ptr(pos[bases_[d]:]), stream())
         values = self.leaf_corner_chi(leaves, bases_, nprim, nu, scale) - self.get_mean_chi()
+        # complete cubes whose corner values change sign (the only ones with triangles), in centre order
+        cubes = torch.empty((DC, 8), dtype=torch.int32, device=dev)
+        flags = torch.empty(DC, dtype=torch.int32, device=dev)
+        offs = torch.empty(DC + 1, dtype=torch.int32, device=dev)
+        ws = Scratch.get(lib().ngs_scan_workspace_bytes(DC), dev)
+        call("ngs_dual_compact", ptr(acc), ptr(cnt), ptr(values), DC, ptr(cubes), ptr(flags), ptr(offs), ptr(ws),
+             stream())
+        K = int(offs[DC].item()) if DC else 0
+        cubes = cubes[:K]
         v, f = self.dual_marching_cubes(cubes, pos, values)
         if return_info:
             return v, f, dict(leaves=leaves, primal=primal, centres=cg.coords, acc=acc, count=cnt, cubes=cubes, pos=pos,
