@@ -378,14 +378,17 @@ int ngs_expand_primal(const int32_t *leaves, int64_t n, int32_t n_upsample, int3
 int ngs_mark_dual_centres(ngs_grid g, const int32_t *primal, int64_t n, int32_t stride, ngs_stream_t s);
 /* Dual-cube assembly (csrc/marching_cubes/dual_mc_inds.cu:45-182): for each primal sub-cell and
  * each lattice point on its boundary that is a dual centre, register (id_base + cell, level) in
- * the octant slots the cell covers; acc (DC x 8 x 2) int32, count (DC) int32. */
+ * the octant slots the cell covers; acc (DC x 8 x 2) int32, count (DC) int32 (caller-zeroed).
+ * count bits 0..15: slots registered; bits 16..31: slots whose cell has a negative value, when
+ * values (the field at the primal cells' dual corners, indexed by id) is not NULL. The centre
+ * grid has stride 1. */
 int ngs_dual_accumulate(ngs_grid centres, const int32_t *primal, int64_t n, int32_t stride, int32_t level,
-                        int32_t id_base, int32_t *acc, int32_t *count, ngs_stream_t s);
-/* Keep complete dual cubes (count == 8) in order: cubes (K x 8) corner ids; flags (dc) and
- * offsets (dc + 1, offsets[dc] = K) are written on the way. With values (the field at the primal
- * corners, nullable), only complete cubes whose corner values change sign are kept (the others
- * yield no triangles). acc is read only for complete centres. */
-int ngs_dual_compact(const int32_t *acc, const int32_t *count, const float *values, int64_t dc, int32_t *cubes,
+                        int32_t id_base, const float *values, int32_t *acc, int32_t *count, ngs_stream_t s);
+/* Keep complete dual cubes (8 slots registered) in order: cubes (K x 8) corner ids; flags (dc)
+ * and offsets (dc + 1, offsets[dc] = K) are written on the way. active_only: keep only complete
+ * cubes whose corner values change sign (needs counts accumulated with values; the others yield
+ * no triangles). acc is read only for the kept centres. */
+int ngs_dual_compact(const int32_t *acc, const int32_t *count, int32_t active_only, int64_t dc, int32_t *cubes,
                      int32_t *flags, int32_t *offsets, void *workspace, ngs_stream_t s);
 /* Leaf forms of the two calls above (same sets, fewer atomics): the dual centres of all sub-cells
  * of each leaf (lattice leaf*n + o*stride, o in [0, n]^3), and the dual-cube assembly of stride-1
@@ -394,7 +397,7 @@ int ngs_dual_compact(const int32_t *acc, const int32_t *count, const float *valu
 int ngs_mark_dual_leaves(ngs_grid g, const int32_t *leaves, int64_t n, int32_t stride, int32_t n_upsample,
                          ngs_stream_t s);
 int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t *leaves, int64_t n, int32_t n_upsample, int32_t level,
-                               int32_t id_base, int32_t *acc, int32_t *count, ngs_stream_t s);
+                               int32_t id_base, const float *values, int32_t *acc, int32_t *count, ngs_stream_t s);
 /* Dual-corner positions (primal + stride) * voxel_size / n_upsample (reconstructor.py:368-373). */
 int ngs_dual_corner_positions(const int32_t *primal, int64_t n, int32_t stride, float scale, float *pos,
                               ngs_stream_t s);

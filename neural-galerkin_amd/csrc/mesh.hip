@@ -140,88 +140,86 @@ __global__ void mark_dual_kernel(ngs_grid g, const int32_t* __restrict__ prim, i
     grid_mark(g, prim[3 * p] + ((o >> 2) & 1) * s, prim[3 * p + 1] + ((o >> 1) & 1) * s, prim[3 * p + 2] + (o & 1) * s);
 }
 
-// Boundary lattice point t of a side-s cell: 8 corners, then 12 (s-1) edge interiors (edges
-// along axis e/4 at the (e&1, e>>1&1) corner of the other two axes), then 6 (s-1)^2 face
-// interiors (face f normal to axis f/2 on side f&1).
-__device__ __forceinline__ void boundary_point(int t, int s, int loc[3]) {
-    if (t < 8) {
-        loc[0] = ((t >> 2) & 1) * s;
-        loc[1] = ((t >> 1) & 1) * s;
-        loc[2] = (t & 1) * s;
-        return;
-    }
-    t -= 8;
-    const int e_n = 12 * (s - 1);
-    if (t < e_n) {
-        int e = t / (s - 1), u = t % (s - 1) + 1;
-        int ax = e / 4, a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
-        loc[ax] = u;
-        loc[a1] = (e & 1) * s;
-        loc[a2] = ((e >> 1) & 1) * s;
-        return;
-    }
-    t -= e_n;
-    const int fs = (s - 1) * (s - 1);
-    int f = t / fs, u = t % fs;
-    int ax = f / 2, a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
-    loc[ax] = (f & 1) * s;
-    loc[a1] = u % (s - 1) + 1;
-    loc[a2] = u / (s - 1) + 1;
-}
+// Count word of a dual centre: registrations (octant slots filled) in bits 0..15 and, when the
+// primal values are known, how many of those slots hold a cell whose value is negative in bits
+// 16..31, so completeness and the sign change of a cube are read from one word.
+constexpr int COUNT_BITS = 16;
+constexpr int32_t COUNT_MASK = (1 << COUNT_BITS) - 1;
 
+// One thread per (primal sub-cell, column (x, y) of its side-s boundary lattice): side columns
+// (x or y on the cell's boundary) hold s + 1 boundary points along z, interior columns only z = 0
+// and z = s. The column's dual centres come from bitmap windows of up to 31 points.
 __global__ void dual_accumulate_kernel(ngs_grid cen, const int32_t* __restrict__ prim, int64_t n, int s, int level,
-                                       int id_base, int32_t* __restrict__ acc, int32_t* __restrict__ cnt) {
-    const int per = 6 * s * s + 2;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * per) return;
-    int64_t p = t / per;
-    int loc[3];
-    boundary_point((int)(t % per), s, loc);
-    int dc = grid_find(cen, prim[3 * p] + loc[0], prim[3 * p + 1] + loc[1], prim[3 * p + 2] + loc[2]);
-    if (dc < 0) return;
-    // sides of the dual centre covered by the cell, per axis: bit0 negative side, bit1 positive
-    int side[3];
-    int nsl = 1;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        side[a] = loc[a] == 0 ? 2 : (loc[a] == s ? 1 : 3);
-        nsl *= (side[a] == 3) ? 2 : 1;
-    }
-    int old = atomicAdd(&cnt[dc], nsl);
-    if (old >= 8) return;
+                                       int id_base, const float* __restrict__ values, int32_t* __restrict__ acc,
+                                       int32_t* __restrict__ cnt) {
+    const int cols = (s + 1) * (s + 1);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * cols) return;
+    const int64_t p = t / cols;
+    const int c = (int)(t % cols);
+    const int lx = c / (s + 1), ly = c % (s + 1);
+    const bool side_col = lx == 0 || lx == s || ly == 0 || ly == s;
+    const int gx = prim[3 * p] + lx - cen.origin[0], gy = prim[3 * p + 1] + ly - cen.origin[1];
+    const int gz = prim[3 * p + 2] - cen.origin[2];
     const int pid = id_base + (int)p;
-    for (int sx = 0; sx < 2; ++sx) {
-        if (!((side[0] >> sx) & 1)) continue;
-        for (int sy = 0; sy < 2; ++sy) {
-            if (!((side[1] >> sy) & 1)) continue;
-            for (int sz = 0; sz < 2; ++sz) {
-                if (!((side[2] >> sz) & 1)) continue;
-                int sl = octant_slot(sx, sy, sz);
-                *reinterpret_cast<int2*>(acc + ((int64_t)dc * 8 + sl) * 2) = make_int2(pid, level);
+    const int32_t neg = (values && values[pid] < 0.f) ? 1 : 0;
+    for (int z0 = 0; z0 <= s; z0 += 31) {
+        const int len = min(31, s + 1 - z0);
+        int32_t r;
+        const uint32_t full = grid_column(cen, gx, gy, gz + z0, len, &r);
+        uint32_t m = full;
+        if (!side_col) m &= (z0 == 0 ? 1u : 0u) | (z0 + len - 1 == s ? 1u << (len - 1) : 0u);
+        while (m) {
+            // up to 8 points at a time: their count updates are issued together, then the slots
+            constexpr int K = 8;
+            int dcs[K], zs[K], old[K];
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const int k = m ? __ffs(m) - 1 : -1;
+                zs[q] = k;
+                dcs[q] = k >= 0 ? r + __popc(full & ((1u << k) - 1u)) : -1;
+                m &= m ? m - 1 : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                if (dcs[q] < 0) continue;
+                const int lz = z0 + zs[q];
+                const int nsl = (lx == 0 || lx == s ? 1 : 2) * (ly == 0 || ly == s ? 1 : 2) * (lz == 0 || lz == s ? 1 : 2);
+                old[q] = atomicAdd(&cnt[dcs[q]], nsl | ((neg * nsl) << COUNT_BITS));
+            }
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                if (dcs[q] < 0 || (old[q] & COUNT_MASK) >= 8) continue;
+                const int loc[3] = {lx, ly, z0 + zs[q]};
+                // sides of the dual centre covered by the cell, per axis: bit0 negative side, bit1 positive
+                int side[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) side[a] = loc[a] == 0 ? 2 : (loc[a] == s ? 1 : 3);
+                for (int sx = 0; sx < 2; ++sx) {
+                    if (!((side[0] >> sx) & 1)) continue;
+                    for (int sy = 0; sy < 2; ++sy) {
+                        if (!((side[1] >> sy) & 1)) continue;
+                        for (int sz = 0; sz < 2; ++sz) {
+                            if (!((side[2] >> sz) & 1)) continue;
+                            const int sl = octant_slot(sx, sy, sz);
+                            *reinterpret_cast<int2*>(acc + ((int64_t)dcs[q] * 8 + sl) * 2) = make_int2(pid, level);
+                        }
+                    }
+                }
             }
         }
     }
 }
 
-// Complete cubes (count 8); with values, only those whose corners change sign (the cubes marching
-// cubes emits triangles for: cases other than 0 and 255).
-__global__ void complete_flags_kernel(const int32_t* __restrict__ cnt, const int32_t* __restrict__ acc,
-                                      const float* __restrict__ val, int64_t n, int32_t* __restrict__ flag) {
+// Complete cubes (8 registrations); with active_only, only those whose corner values change sign
+// (negative-slot count strictly between 0 and 8: the cubes marching cubes emits triangles for).
+__global__ void complete_flags_kernel(const int32_t* __restrict__ cnt, int64_t n, int active_only,
+                                      int32_t* __restrict__ flag) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int f = cnt[i] == 8 ? 1 : 0;
-    if (f && val) {
-        const int4* a = reinterpret_cast<const int4*>(acc + i * 16);
-        int c = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int4 q = a[k];
-            if (val[q.x] < 0.f) c |= 1 << (2 * k);
-            if (val[q.z] < 0.f) c |= 2 << (2 * k);
-        }
-        f = (c != 0 && c != 255) ? 1 : 0;
-    }
-    flag[i] = f;
+    const int32_t c = cnt[i];
+    const int neg = (int)((uint32_t)c >> COUNT_BITS);
+    flag[i] = ((c & COUNT_MASK) == 8 && (!active_only || (neg > 0 && neg < 8))) ? 1 : 0;
 }
 
 __global__ void compact_cubes_kernel(const int32_t* __restrict__ acc, const int32_t* __restrict__ flag,
@@ -465,17 +463,19 @@ __global__ void mark_dual_leaves_kernel(ngs_grid g, const int32_t* __restrict__ 
 // Dual-cube assembly for leaves of stride 1 (the finest level): one thread per (leaf, lattice point
 // o in [0, nu]^3) registers every sub-cell of the leaf that has the point on its boundary, with one
 // aggregated count update. Per axis, o = 0 is covered by sub-cell 0 from the positive side, o = nu
-// by sub-cell nu-1 from the negative side, and 0 < o < nu by sub-cells o-1 (negative) and o (positive).
+// by sub-cell nu-1 from the negative side, and 0 < o < nu by sub-cells o-1 (negative) and o
+// (positive). With values, the number of registered sub-cells with a negative value goes to the
+// high half of the count word.
 __global__ void dual_accumulate_leaves_kernel(ngs_grid cen, const int32_t* __restrict__ leaves, int64_t n, int nu,
-                                              int level, int id_base, int32_t* __restrict__ acc,
-                                              int32_t* __restrict__ cnt) {
+                                              int level, int id_base, const float* __restrict__ values,
+                                              int32_t* __restrict__ acc, int32_t* __restrict__ cnt) {
     const int per = (nu + 1) * (nu + 1) * (nu + 1);
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * per) return;
-    int64_t l = t / per;
-    int r = (int)(t % per);
-    int o[3] = {r / ((nu + 1) * (nu + 1)), (r / (nu + 1)) % (nu + 1), r % (nu + 1)};
-    int dc = grid_find(cen, leaves[3 * l] * nu + o[0], leaves[3 * l + 1] * nu + o[1], leaves[3 * l + 2] * nu + o[2]);
+    const int64_t l = t / per;
+    const int r = (int)(t % per);
+    const int o[3] = {r / ((nu + 1) * (nu + 1)), (r / (nu + 1)) % (nu + 1), r % (nu + 1)};
+    const int dc = grid_find(cen, leaves[3 * l] * nu + o[0], leaves[3 * l + 1] * nu + o[1], leaves[3 * l + 2] * nu + o[2]);
     if (dc < 0) return;
     // per axis: up to two (sub-cell index, side) pairs; side 1 = cell on the positive side
     int ce[3][2], sd[3][2], nc[3];
@@ -487,14 +487,20 @@ __global__ void dual_accumulate_leaves_kernel(ngs_grid cen, const int32_t* __res
         if (o[a] < nu) { ce[a][nc[a]] = o[a]; sd[a][nc[a]] = 1; ++nc[a]; }
         nsl *= nc[a];
     }
-    int old = atomicAdd(&cnt[dc], nsl);
-    if (old >= 8) return;
     const int64_t base = (int64_t)id_base + l * (int64_t)(nu * nu * nu);
+    int neg = 0;
+    if (values)
+        for (int i = 0; i < nc[0]; ++i)
+            for (int j = 0; j < nc[1]; ++j)
+                for (int k = 0; k < nc[2]; ++k)
+                    neg += values[base + (ce[0][i] * nu + ce[1][j]) * nu + ce[2][k]] < 0.f ? 1 : 0;
+    const int old = atomicAdd(&cnt[dc], nsl | (neg << COUNT_BITS));
+    if ((old & COUNT_MASK) >= 8) return;
     for (int i = 0; i < nc[0]; ++i)
         for (int j = 0; j < nc[1]; ++j)
             for (int k = 0; k < nc[2]; ++k) {
-                int sl = octant_slot(sd[0][i], sd[1][j], sd[2][k]);
-                int e = (ce[0][i] * nu + ce[1][j]) * nu + ce[2][k];
+                const int sl = octant_slot(sd[0][i], sd[1][j], sd[2][k]);
+                const int e = (ce[0][i] * nu + ce[1][j]) * nu + ce[2][k];
                 *reinterpret_cast<int2*>(acc + ((int64_t)dc * 8 + sl) * 2) = make_int2((int32_t)(base + e), level);
             }
 }
@@ -553,20 +559,21 @@ int ngs_mark_dual_centres(ngs_grid g, const int32_t* primal, int64_t n, int32_t 
 }
 
 int ngs_dual_accumulate(ngs_grid centres, const int32_t* primal, int64_t n, int32_t stride, int32_t level, int32_t id_base,
-                        int32_t* acc, int32_t* count, ngs_stream_t s) {
-    NGS_REQUIRE(stride >= 1, "ngs_dual_accumulate: bad stride");
+                        const float* values, int32_t* acc, int32_t* count, ngs_stream_t s) {
+    NGS_REQUIRE(stride >= 1 && centres.log2_stride == 0, "ngs_dual_accumulate: bad stride");
     if (n <= 0) return NGS_OK;
-    int64_t tot = n * (6LL * stride * stride + 2);
-    dual_accumulate_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, primal, n, stride, level, id_base, acc, count);
+    const int64_t tot = n * (int64_t)(stride + 1) * (stride + 1);
+    dual_accumulate_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, primal, n, stride, level, id_base, values, acc,
+                                                                count);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
 
-int ngs_dual_compact(const int32_t* acc, const int32_t* count, const float* values, int64_t dc, int32_t* cubes,
+int ngs_dual_compact(const int32_t* acc, const int32_t* count, int32_t active_only, int64_t dc, int32_t* cubes,
                      int32_t* flags, int32_t* offsets, void* workspace, ngs_stream_t s) {
     NGS_REQUIRE(workspace && flags && offsets, "ngs_dual_compact: bad arguments");
     if (dc <= 0) return NGS_OK;
-    complete_flags_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(count, acc, values, dc, flags);
+    complete_flags_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(count, dc, active_only ? 1 : 0, flags);
     NGS_CHECK_LAUNCH();
     int r = scan_i32_internal(flags, dc, offsets, workspace, S(s));
     if (r != NGS_OK) return r;
@@ -661,12 +668,12 @@ int ngs_mark_dual_leaves(ngs_grid g, const int32_t* leaves, int64_t n, int32_t s
 }
 
 int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t* leaves, int64_t n, int32_t n_upsample, int32_t level,
-                               int32_t id_base, int32_t* acc, int32_t* count, ngs_stream_t s) {
-    NGS_REQUIRE(n_upsample >= 1, "ngs_dual_accumulate_leaves: bad n_upsample");
-    int64_t tot = n * (n_upsample + 1) * (n_upsample + 1) * (n_upsample + 1);
+                               int32_t id_base, const float* values, int32_t* acc, int32_t* count, ngs_stream_t s) {
+    NGS_REQUIRE(n_upsample >= 1 && centres.log2_stride == 0, "ngs_dual_accumulate_leaves: bad n_upsample or centre grid");
+    const int64_t tot = n * (n_upsample + 1) * (n_upsample + 1) * (n_upsample + 1);
     if (tot <= 0) return NGS_OK;
-    dual_accumulate_leaves_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, leaves, n, n_upsample, level, id_base, acc,
-                                                                      count);
+    dual_accumulate_leaves_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, leaves, n, n_upsample, level, id_base,
+                                                                      values, acc, count);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

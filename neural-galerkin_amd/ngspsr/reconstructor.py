@@ -326,18 +326,6 @@ class Reconstructor:
                 call("ngs_mark_dual_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], tree.strides[d], nu,
                      stream())
         DC = cg.finalize()
-        # (id, level) per octant; read only for complete centres, whose eight slots are all written
-        acc = torch.empty((DC, 8, 2), dtype=torch.int32, device=dev)
-        cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
-        for d in range(L):
-            if not primal[d].shape[0]:
-                continue
-            if tree.strides[d] == 1:
-                call("ngs_dual_accumulate_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], nu, d, bases_[d],
-                     ptr(acc), ptr(cnt), stream())
-            else:
-                call("ngs_dual_accumulate", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d], d,
-                     bases_[d], ptr(acc), ptr(cnt), stream())
         # dual corners = (primal + stride) * voxel_size / n_upsample, and chi there
         pos = torch.empty((nprim, 3), dtype=torch.float32, device=dev)
         scale = float(np.float32(tree.voxel_size / nu))
@@ -346,13 +334,25 @@ class Reconstructor:
                 call("ngs_dual_corner_positions", ptr(primal[d]), primal[d].shape[0], tree.strides[d], scale,
                      ptr(pos[bases_[d]:]), stream())
         values = self.leaf_corner_chi(leaves, bases_, nprim, nu, scale) - self.get_mean_chi()
+        # (id, level) per octant, read only for the kept centres (all eight slots written); the count
+        # word also counts the registered cells with a negative value
+        acc = torch.empty((DC, 8, 2), dtype=torch.int32, device=dev)
+        cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
+        for d in range(L):
+            if not primal[d].shape[0]:
+                continue
+            if tree.strides[d] == 1:
+                call("ngs_dual_accumulate_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], nu, d, bases_[d],
+                     ptr(values), ptr(acc), ptr(cnt), stream())
+            else:
+                call("ngs_dual_accumulate", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d], d,
+                     bases_[d], ptr(values), ptr(acc), ptr(cnt), stream())
         # complete cubes whose corner values change sign (the only ones with triangles), in centre order
         cubes = torch.empty((DC, 8), dtype=torch.int32, device=dev)
         flags = torch.empty(DC, dtype=torch.int32, device=dev)
         offs = torch.empty(DC + 1, dtype=torch.int32, device=dev)
         ws = Scratch.get(lib().ngs_scan_workspace_bytes(DC), dev)
-        call("ngs_dual_compact", ptr(acc), ptr(cnt), ptr(values), DC, ptr(cubes), ptr(flags), ptr(offs), ptr(ws),
-             stream())
+        call("ngs_dual_compact", ptr(acc), ptr(cnt), 1, DC, ptr(cubes), ptr(flags), ptr(offs), ptr(ws), stream())
         K = int(offs[DC].item()) if DC else 0
         cubes = cubes[:K]
         v, f = self.dual_marching_cubes(cubes, pos, values)
