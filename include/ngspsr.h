@@ -221,6 +221,11 @@ int ngs_lhs_assemble(ngs_level lv, ngs_point_bins pts, const float *coarse, floa
                      const int32_t *rowptr, int32_t *cols, float *vals, float *inv_diag, double *rhs,
                      ngs_stream_t s);
 
+/* Assembly knob for measurement: lanes per row (1, 4, 16; 0 = automatic: 4 or 16 lanes split the
+ * points of a row on levels with few rows and many points per row). Applies to ngs_lhs_assemble,
+ * ngs_lhs_assemble_runs and ngs_stencil_diag_rhs alike, so their diagonals stay bit-identical. */
+int ngs_assembly_config(int32_t lanes_per_row);
+
 /* ngs_lhs_assemble into the run form of the same matrix instead of CSR (no reference counterpart;
  * the reference stores int32 columns, solver.py:40-48). Row i's neighbours in the bitmap column
  * (dx, dy), p = 5 (dx + 2) + (dy + 2), have consecutive ranks start .. start + n - 1 (n <= 5):

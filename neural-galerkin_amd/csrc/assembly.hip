@@ -49,7 +49,30 @@ __global__ void lhs_count_kernel(ngs_level lv, int32_t* __restrict__ row_nnz) {
 // plane-major, so the run and value loads of consecutive rows are contiguous.
 constexpr int RUN_COUNT_SHIFT = 28;
 
-template <bool HAS_ALPHA, bool HAS_COARSE, bool RUNS>
+// Lanes per row of the assembly and of stencil_diag_rhs_kernel (the two must agree for their
+// diagonals and RHS terms to be bit-identical). A coarse level can have too few rows to fill the
+// GPU while each row visits hundreds of points (~27 n / M), so G lanes split a row's points (lane j
+// takes points j, j + G, ... of every cell) and their partial sums are combined by an xor
+// butterfly. The split costs registers (fewer waves per SIMD), so it is used only when the rows
+// alone give fewer than 128k threads. Measured at 2M points: 55k rows, ~1000 visits per row:
+// 3.5 ms with 1 lane per row, 2.0 with 4, 1.4 with 16; 261k rows: 1.4, 1.9, 3.2.
+static int g_assembly_lanes = 0;  // 0: automatic (ngs_assembly_config)
+
+inline int assembly_lanes(int64_t n_points, int64_t M) {
+    if (g_assembly_lanes) return g_assembly_lanes;
+    constexpr int64_t FILL = 128 * 1024;
+    const double visits = M > 0 ? 27.0 * (double)n_points / (double)M : 0.0;
+    return (M >= FILL || visits < 64.0) ? 1 : 16;
+}
+
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+template <bool HAS_ALPHA, bool HAS_COARSE, bool RUNS, int G>
 __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_point_bins pts,
                                                           const float* __restrict__ coarse, float sf, float sf_delta,
                                                           const float* __restrict__ F1, const float* __restrict__ G1,
@@ -57,7 +80,9 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                                                           float* __restrict__ vals, float* __restrict__ inv_diag,
                                                           double* __restrict__ rhs, int32_t* __restrict__ runs,
                                                           float* __restrict__ rvals) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = t / G;  // the G lanes of a row are consecutive lanes of one wave
+    const int gl = (int)(t % G);
     if (i >= lv.M) return;
     const int s = lv.stride;
     const int cx = lv.coords[3 * i], cy = lv.coords[3 * i + 1], cz = lv.coords[3 * i + 2];
@@ -82,7 +107,7 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                 if (!((cm >> (oz + 1)) & 1u)) continue;
                 const int k0 = cstart[cid], k1 = cstart[cid + 1];
                 ++cid;
-                for (int k = k0; k < k1; ++k) {
+                for (int k = k0 + gl; k < k1; k += G) {
                     const float* bp = pb_all + 9 * (int64_t)k;
                     float bx[3] = {bp[0], bp[1], bp[2]};
                     float by[3] = {bp[3], bp[4], bp[5]};
@@ -116,6 +141,13 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                 }
             }
         }
+    }
+    if (G > 1) {
+#pragma unroll
+        for (int q = 0; q < 125; ++q) acc[q] = group_sum<G>(acc[q]);
+        rs = group_sum<G>(rs);
+        if (HAS_COARSE) rc = group_sum<G>(rc);
+        if (gl != 0) return;
     }
     // Emit the row in increasing column order (lexicographic offsets over sorted voxels).
     int cnt = 0;
@@ -205,10 +237,13 @@ __global__ void bin_points_kernel(const int32_t* __restrict__ perm, int64_t n, c
 // Diagonal and screening RHS of row i for the stencil form: the same products, in the same
 // order, as the diagonal entry and rs/rc of lhs_assemble_kernel (cells in (ox, oy, oz) order,
 // points in bin order).
+template <int G>
 __global__ void __launch_bounds__(256) stencil_diag_rhs_kernel(ngs_stencil op, const float* __restrict__ coarse,
                                                                float sf_delta,
                                                                float* __restrict__ inv_diag, double* __restrict__ rhs) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = t / G;
+    const int gl = (int)(t % G);
     if (i >= op.level.M) return;
     const int s = op.level.stride;
     const int cx = op.level.coords[3 * i], cy = op.level.coords[3 * i + 1], cz = op.level.coords[3 * i + 2];
@@ -223,7 +258,7 @@ __global__ void __launch_bounds__(256) stencil_diag_rhs_kernel(ngs_stencil op, c
                     const int cid = grid_find(op.cells, cx + ox * s, cy + oy * s, cz + oz * s);
                     if (cid < 0) continue;
                     const int k0 = op.cell_start[cid], k1 = op.cell_start[cid + 1];
-                    for (int k = k0; k < k1; ++k) {
+                    for (int k = k0 + gl; k < k1; k += G) {
                         const float* bp = op.basis + 9 * (int64_t)k;
                         const float bx = bp[1 - ox], by = bp[4 - oy], bz = bp[7 - oz];
                         const float a = op.alpha ? op.alpha[k] : op.alpha_scalar;
@@ -240,6 +275,12 @@ __global__ void __launch_bounds__(256) stencil_diag_rhs_kernel(ngs_stencil op, c
                 }
             }
         }
+    }
+    if (G > 1) {
+        dd = group_sum<G>(dd);
+        rs = group_sum<G>(rs);
+        rc = group_sum<G>(rc);
+        if (gl != 0) return;
     }
     float v;
     {
@@ -594,14 +635,23 @@ static int lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, f
     Tables T;
     NGS_REQUIRE(tables_for(1, &T), "ngs_lhs_assemble: integral tables not registered");
     const float sfd = sf * delta;
-    dim3 grid(grid1d(lv.M, 64)), block(64);
+    const int lanes = assembly_lanes(pts.n, lv.M);
+    dim3 grid(grid1d(lv.M * lanes, 64)), block(64);
     hipStream_t st = S(s);
-#define NGS_ASM(A, C, R)                                                                                           \
-    lhs_assemble_kernel<A, C, R><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, rowptr, cols, vals,    \
-                                                         inv_diag, rhs, runs, rvals)
-#define NGS_ASM_R(A, C)          \
-    if (runs) NGS_ASM(A, C, true); \
-    else NGS_ASM(A, C, false)
+#define NGS_ASM_G(A, C, R, LN)                                                                                      \
+    lhs_assemble_kernel<A, C, R, LN><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, rowptr, cols, vals, \
+                                                             inv_diag, rhs, runs, rvals)
+#define NGS_ASM(A, C, R)                        \
+    do {                                        \
+        if (lanes == 16) NGS_ASM_G(A, C, R, 16);  \
+        else if (lanes == 4) NGS_ASM_G(A, C, R, 4); \
+        else NGS_ASM_G(A, C, R, 1);               \
+    } while (0)
+#define NGS_ASM_R(A, C)                \
+    do {                               \
+        if (runs) NGS_ASM(A, C, true);  \
+        else NGS_ASM(A, C, false);      \
+    } while (0)
     if (pts.alpha) {
         if (coarse) NGS_ASM_R(true, true);
         else NGS_ASM_R(true, false);
@@ -611,7 +661,15 @@ static int lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, f
     }
 #undef NGS_ASM_R
 #undef NGS_ASM
+#undef NGS_ASM_G
     NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_assembly_config(int32_t lanes_per_row) {
+    NGS_REQUIRE(lanes_per_row == 0 || lanes_per_row == 1 || lanes_per_row == 4 || lanes_per_row == 16,
+                "ngs_assembly_config: lanes_per_row must be 0, 1, 4 or 16");
+    g_assembly_lanes = lanes_per_row;
     return NGS_OK;
 }
 
@@ -651,7 +709,11 @@ int ngs_stencil_diag_rhs(ngs_stencil op, const float* coarse, float delta, float
     NGS_REQUIRE(inv_diag && rhs && op.level.coords && op.lap && (op.n == 0 || (op.basis && op.perm && op.cell_start)),
                 "ngs_stencil_diag_rhs: bad arguments");
     const float sfd = op.sf * delta;
-    stencil_diag_rhs_kernel<<<grid1d(op.level.M, 256), 256, 0, S(s)>>>(op, coarse, sfd, inv_diag, rhs);
+    const int lanes = assembly_lanes(op.n, op.level.M);
+    const unsigned grid = grid1d(op.level.M * lanes, 256);
+    if (lanes == 16) stencil_diag_rhs_kernel<16><<<grid, 256, 0, S(s)>>>(op, coarse, sfd, inv_diag, rhs);
+    else if (lanes == 4) stencil_diag_rhs_kernel<4><<<grid, 256, 0, S(s)>>>(op, coarse, sfd, inv_diag, rhs);
+    else stencil_diag_rhs_kernel<1><<<grid, 256, 0, S(s)>>>(op, coarse, sfd, inv_diag, rhs);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -74,6 +74,7 @@ _SIGNATURES = {
     "ngs_set_tables": ([P, P, c_int32], c_int32),
     "ngs_lhs_row_counts": ([Level, P, P], c_int32),
     "ngs_lhs_assemble": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P, P], c_int32),
+    "ngs_assembly_config": ([c_int32], c_int32),
     "ngs_lhs_assemble_runs": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P], c_int32),
     "ngs_pcg_solve_runs": ([P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
                             ctypes.POINTER(c_double)], c_int32),
