@@ -382,13 +382,14 @@ int ngs_expand_primal(const int32_t *leaves, int64_t n, int32_t n_upsample, int3
 /* Mark the 8 corners of every primal sub-cell (reconstructor.py:350-358). */
 int ngs_mark_dual_centres(ngs_grid g, const int32_t *primal, int64_t n, int32_t stride, ngs_stream_t s);
 /* Dual-cube assembly (csrc/marching_cubes/dual_mc_inds.cu:45-182): for each primal sub-cell and
- * each lattice point on its boundary that is a dual centre, register (id_base + cell, level) in
- * the octant slots the cell covers; acc (DC x 8 x 2) int32, count (DC) int32 (caller-zeroed).
+ * each lattice point on its boundary that is a dual centre, register id_base + cell in the octant
+ * slots the cell covers (the reference also records the level, which nothing downstream reads);
+ * acc (DC x 8) int32, count (DC) int32 (caller-zeroed).
  * count bits 0..15: slots registered; bits 16..31: slots whose cell has a negative value, when
  * values (the field at the primal cells' dual corners, indexed by id) is not NULL. The centre
  * grid has stride 1. */
-int ngs_dual_accumulate(ngs_grid centres, const int32_t *primal, int64_t n, int32_t stride, int32_t level,
-                        int32_t id_base, const float *values, int32_t *acc, int32_t *count, ngs_stream_t s);
+int ngs_dual_accumulate(ngs_grid centres, const int32_t *primal, int64_t n, int32_t stride, int32_t id_base,
+                        const float *values, int32_t *acc, int32_t *count, ngs_stream_t s);
 /* Keep complete dual cubes (8 slots registered) in order: cubes (K x 8) corner ids; flags (dc)
  * and offsets (dc + 1, offsets[dc] = K) are written on the way. active_only: keep only complete
  * cubes whose corner values change sign (needs counts accumulated with values; the others yield
@@ -401,7 +402,7 @@ int ngs_dual_compact(const int32_t *acc, const int32_t *count, int32_t active_on
  * dual centre is over-registered). */
 int ngs_mark_dual_leaves(ngs_grid g, const int32_t *leaves, int64_t n, int32_t stride, int32_t n_upsample,
                          ngs_stream_t s);
-int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t *leaves, int64_t n, int32_t n_upsample, int32_t level,
+int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t *leaves, int64_t n, int32_t n_upsample,
                                int32_t id_base, const float *values, int32_t *acc, int32_t *count, ngs_stream_t s);
 /* Dual-corner positions (primal + stride) * voxel_size / n_upsample (reconstructor.py:368-373). */
 int ngs_dual_corner_positions(const int32_t *primal, int64_t n, int32_t stride, float scale, float *pos,
@@ -414,16 +415,22 @@ int ngs_dual_corner_positions(const int32_t *primal, int64_t n, int32_t stride, 
 int ngs_eval_leaf_corners(ngs_grid g, const float *field, const int32_t *leaves, int64_t n, int32_t leaf_stride,
                           int32_t n_upsample, float scale, float inv_voxel_size, int32_t accumulate, float *out,
                           ngs_stream_t s);
-/* Dual marching cubes (csrc/marching_cubes/dual_mc.cu:33-189): per-cube vertex counts, then
- * triangles (T x 3 x 3 float) and vertex keys (T x 3, uint64 = larger id << 32 | smaller id). */
+/* Dual marching cubes (csrc/marching_cubes/dual_mc.cu:33-189): per-cube vertex counts, then per
+ * triangle vertex (T x 3, cube order then case-table order) its key, larger corner id << id_bits |
+ * smaller id (ids < 2^id_bits), and a tag, vertex index | (the interpolation's first corner is the
+ * larger id) << 31. */
 int ngs_mc_count(const int32_t *cubes, int64_t k, const float *values, int32_t *vcount, ngs_stream_t s);
-int ngs_mc_emit(const int32_t *cubes, int64_t k, const float *values, const float *pos, const int32_t *voffset,
-                float *tris, uint64_t *vkeys, ngs_stream_t s);
-/* Vertex de-duplication (reconstructor.py:379-383): sorted unique keys, faces (T*3) = rank of
- * each key, vertex positions from the first occurrence. *nverts (device int64). */
+int ngs_mc_emit(const int32_t *cubes, int64_t k, const float *values, const int32_t *voffset, int32_t id_bits,
+                uint64_t *vkeys, uint32_t *vtags, ngs_stream_t s);
+/* Vertex de-duplication (reconstructor.py:379-383): sorted unique keys; faces (T*3) = rank of each
+ * key; the vertex of a key is interpolated (dual_mc.cu:20-31 sdf_interp) on its edge, in the corner
+ * order of the key's first occurrence, from the corner positions pos (3 per id) and values — the
+ * position the reference's per-triangle vertex has at that occurrence. *nverts (device int64).
+ * The keys are radix-sorted on their 2 id_bits low bits. */
 int64_t ngs_dedup_workspace_bytes(int64_t nkeys);
-int ngs_dedup_vertices(const uint64_t *vkeys, const float *tris, int64_t nkeys, int64_t *faces, float *verts,
-                       int64_t *nverts, void *workspace, int64_t workspace_bytes, ngs_stream_t s);
+int ngs_dedup_vertices(const uint64_t *vkeys, const uint32_t *vtags, int64_t nkeys, int32_t id_bits, const float *pos,
+                       const float *values, int64_t *faces, float *verts, int64_t *nverts, void *workspace,
+                       int64_t workspace_bytes, ngs_stream_t s);
 
 /* ------------------------------------------------------------------------------------ misc */
 int ngs_f64_to_f32(const double *in, float *out, int64_t n, ngs_stream_t s);

@@ -149,7 +149,7 @@ constexpr int32_t COUNT_MASK = (1 << COUNT_BITS) - 1;
 // One thread per (primal sub-cell, column (x, y) of its side-s boundary lattice): side columns
 // (x or y on the cell's boundary) hold s + 1 boundary points along z, interior columns only z = 0
 // and z = s. The column's dual centres come from bitmap windows of up to 31 points.
-__global__ void dual_accumulate_kernel(ngs_grid cen, const int32_t* __restrict__ prim, int64_t n, int s, int level,
+__global__ void dual_accumulate_kernel(ngs_grid cen, const int32_t* __restrict__ prim, int64_t n, int s,
                                        int id_base, const float* __restrict__ values, int32_t* __restrict__ acc,
                                        int32_t* __restrict__ cnt) {
     const int cols = (s + 1) * (s + 1);
@@ -202,7 +202,7 @@ __global__ void dual_accumulate_kernel(ngs_grid cen, const int32_t* __restrict__
                         for (int sz = 0; sz < 2; ++sz) {
                             if (!((side[2] >> sz) & 1)) continue;
                             const int sl = octant_slot(sx, sy, sz);
-                            *reinterpret_cast<int2*>(acc + ((int64_t)dcs[q] * 8 + sl) * 2) = make_int2(pid, level);
+                            acc[(int64_t)dcs[q] * 8 + sl] = pid;
                         }
                     }
                 }
@@ -228,7 +228,7 @@ __global__ void compact_cubes_kernel(const int32_t* __restrict__ acc, const int3
     if (i >= n || !flag[i]) return;
     int64_t o = off[i];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) cubes[o * 8 + k] = acc[(i * 8 + k) * 2];
+    for (int k = 0; k < 8; ++k) cubes[o * 8 + k] = acc[i * 8 + k];
 }
 
 __global__ void corner_pos_kernel(const int32_t* __restrict__ prim, int64_t n, int s, float scale, float* __restrict__ pos) {
@@ -381,9 +381,12 @@ __device__ __forceinline__ void sdf_interp(const float* p1, const float* p2, flo
     out[2] = p1[2] * w1 + p2[2] * w2;
 }
 
+// Per triangle vertex: key (larger id << id_bits | smaller id) and tag = vertex index | (ia > ib) << 31,
+// where (ia, ib) are the edge's corners in sdf_interp argument order. Positions are computed later,
+// once per distinct key, from its first occurrence (the tag's orientation bit restores the order).
 __global__ void mc_emit_kernel(const int32_t* __restrict__ cubes, int64_t k, const float* __restrict__ val,
-                               const float* __restrict__ pos, const int32_t* __restrict__ voff, float* __restrict__ tris,
-                               uint64_t* __restrict__ keys) {
+                               const int32_t* __restrict__ voff, int id_bits, uint64_t* __restrict__ keys,
+                               uint32_t* __restrict__ tags) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k) return;
     const int32_t* ids = cubes + 8 * i;
@@ -393,23 +396,11 @@ __global__ void mc_emit_kernel(const int32_t* __restrict__ cubes, int64_t k, con
     for (int n = 0; n < 15; ++n, ++v) {
         int e = (int)((w >> (4 * n)) & 0xF);
         if (e == 0xF) break;
-        int a = c_edge[e][0], b = c_edge[e][1];
-        int ia = ids[a], ib = ids[b];
-        float p1[3] = {pos[3 * (int64_t)ia], pos[3 * (int64_t)ia + 1], pos[3 * (int64_t)ia + 2]};
-        float p2[3] = {pos[3 * (int64_t)ib], pos[3 * (int64_t)ib + 1], pos[3 * (int64_t)ib + 2]};
-        float out[3];
-        sdf_interp(p1, p2, val[ia], val[ib], out);
-        tris[3 * v] = out[0];
-        tris[3 * v + 1] = out[1];
-        tris[3 * v + 2] = out[2];
-        uint32_t hi = (uint32_t)max(ia, ib), lo = (uint32_t)min(ia, ib);
-        keys[v] = ((uint64_t)hi << 32) | lo;
+        const int ia = ids[c_edge[e][0]], ib = ids[c_edge[e][1]];
+        const uint32_t hi = (uint32_t)max(ia, ib), lo = (uint32_t)min(ia, ib);
+        keys[v] = ((uint64_t)hi << id_bits) | lo;
+        tags[v] = (uint32_t)v | (ia > ib ? 0x80000000u : 0u);
     }
-}
-
-__global__ void iota_kernel(int32_t* p, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = (int32_t)i;
 }
 
 __global__ void run_heads_kernel(const uint64_t* __restrict__ k, int64_t n, int32_t* __restrict__ head) {
@@ -418,18 +409,27 @@ __global__ void run_heads_kernel(const uint64_t* __restrict__ k, int64_t n, int3
 }
 
 __global__ void dedup_scatter_kernel(const int32_t* __restrict__ head, const int32_t* __restrict__ off,
-                                     const int32_t* __restrict__ idx, const float* __restrict__ tris, int64_t n,
+                                     const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ stags, int64_t n,
+                                     int id_bits, const float* __restrict__ pos, const float* __restrict__ val,
                                      int64_t* __restrict__ faces, float* __restrict__ verts) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // off[i] = number of run heads strictly before i; the run of element i has rank off[i+1]-1
-    int32_t rank = off[i] + head[i] - 1;
-    int32_t src = idx[i];
-    faces[src] = rank;
+    const int32_t rank = off[i] + head[i] - 1;
+    const uint32_t tag = stags[i];
+    faces[tag & 0x7FFFFFFFu] = rank;
     if (head[i]) {
-        verts[3 * (int64_t)rank] = tris[3 * (int64_t)src];
-        verts[3 * (int64_t)rank + 1] = tris[3 * (int64_t)src + 1];
-        verts[3 * (int64_t)rank + 2] = tris[3 * (int64_t)src + 2];
+        // the first occurrence (the sort is stable and tags ascend in emission order)
+        const uint64_t key = skeys[i];
+        const int32_t hi = (int32_t)(key >> id_bits), lo = (int32_t)(key & ((1ull << id_bits) - 1));
+        const int32_t ia = (tag >> 31) ? hi : lo, ib = (tag >> 31) ? lo : hi;
+        const float p1[3] = {pos[3 * (int64_t)ia], pos[3 * (int64_t)ia + 1], pos[3 * (int64_t)ia + 2]};
+        const float p2[3] = {pos[3 * (int64_t)ib], pos[3 * (int64_t)ib + 1], pos[3 * (int64_t)ib + 2]};
+        float out[3];
+        sdf_interp(p1, p2, val[ia], val[ib], out);
+        verts[3 * (int64_t)rank] = out[0];
+        verts[3 * (int64_t)rank + 1] = out[1];
+        verts[3 * (int64_t)rank + 2] = out[2];
     }
 }
 
@@ -467,7 +467,7 @@ __global__ void mark_dual_leaves_kernel(ngs_grid g, const int32_t* __restrict__ 
 // (positive). With values, the number of registered sub-cells with a negative value goes to the
 // high half of the count word.
 __global__ void dual_accumulate_leaves_kernel(ngs_grid cen, const int32_t* __restrict__ leaves, int64_t n, int nu,
-                                              int level, int id_base, const float* __restrict__ values,
+                                              int id_base, const float* __restrict__ values,
                                               int32_t* __restrict__ acc, int32_t* __restrict__ cnt) {
     const int per = (nu + 1) * (nu + 1) * (nu + 1);
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -501,7 +501,7 @@ __global__ void dual_accumulate_leaves_kernel(ngs_grid cen, const int32_t* __res
             for (int k = 0; k < nc[2]; ++k) {
                 const int sl = octant_slot(sd[0][i], sd[1][j], sd[2][k]);
                 const int e = (ce[0][i] * nu + ce[1][j]) * nu + ce[2][k];
-                *reinterpret_cast<int2*>(acc + ((int64_t)dc * 8 + sl) * 2) = make_int2((int32_t)(base + e), level);
+                acc[(int64_t)dc * 8 + sl] = (int32_t)(base + e);
             }
 }
 
@@ -558,12 +558,12 @@ int ngs_mark_dual_centres(ngs_grid g, const int32_t* primal, int64_t n, int32_t 
     return NGS_OK;
 }
 
-int ngs_dual_accumulate(ngs_grid centres, const int32_t* primal, int64_t n, int32_t stride, int32_t level, int32_t id_base,
+int ngs_dual_accumulate(ngs_grid centres, const int32_t* primal, int64_t n, int32_t stride, int32_t id_base,
                         const float* values, int32_t* acc, int32_t* count, ngs_stream_t s) {
     NGS_REQUIRE(stride >= 1 && centres.log2_stride == 0, "ngs_dual_accumulate: bad stride");
     if (n <= 0) return NGS_OK;
     const int64_t tot = n * (int64_t)(stride + 1) * (stride + 1);
-    dual_accumulate_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, primal, n, stride, level, id_base, values, acc,
+    dual_accumulate_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, primal, n, stride, id_base, values, acc,
                                                                 count);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
@@ -596,60 +596,62 @@ int ngs_mc_count(const int32_t* cubes, int64_t k, const float* values, int32_t* 
     return NGS_OK;
 }
 
-int ngs_mc_emit(const int32_t* cubes, int64_t k, const float* values, const float* pos, const int32_t* voffset, float* tris,
-                uint64_t* vkeys, ngs_stream_t s) {
+int ngs_mc_emit(const int32_t* cubes, int64_t k, const float* values, const int32_t* voffset, int32_t id_bits,
+                uint64_t* vkeys, uint32_t* vtags, ngs_stream_t s) {
+    NGS_REQUIRE(id_bits >= 1 && id_bits <= 31, "ngs_mc_emit: id_bits must be in [1, 31]");
     if (k <= 0) return NGS_OK;
-    mc_emit_kernel<<<grid1d(k, 128), 128, 0, S(s)>>>(cubes, k, values, pos, voffset, tris, vkeys);
+    mc_emit_kernel<<<grid1d(k, 128), 128, 0, S(s)>>>(cubes, k, values, voffset, id_bits, vkeys, vtags);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
 
-static int dedup_layout(int64_t n, size_t* tmp, size_t off[6], size_t* total) {
+static int dedup_layout(int64_t n, int key_bits, size_t* tmp, size_t off[5], size_t* total) {
     size_t t = 0;
-    int r = sort_pairs_u64(nullptr, nullptr, nullptr, nullptr, n, 64, nullptr, &t, nullptr);
+    int r = sort_pairs_u64(nullptr, nullptr, nullptr, nullptr, n, key_bits, nullptr, &t, nullptr);
     if (r != NGS_OK) return r;
     *tmp = t;
     off[0] = al256(t);                          // sorted keys
-    off[1] = off[0] + al256(8 * (size_t)n);     // iota
-    off[2] = off[1] + al256(4 * (size_t)n);     // sorted idx
-    off[3] = off[2] + al256(4 * (size_t)n);     // heads
-    off[4] = off[3] + al256(4 * (size_t)n);     // offsets (n + 1)
-    off[5] = off[4] + al256(4 * (size_t)(n + 1));
-    *total = off[5] + al256((size_t)ngs_scan_workspace_bytes(n));
+    off[1] = off[0] + al256(8 * (size_t)n);     // sorted tags
+    off[2] = off[1] + al256(4 * (size_t)n);     // heads
+    off[3] = off[2] + al256(4 * (size_t)n);     // offsets (n + 1)
+    off[4] = off[3] + al256(4 * (size_t)(n + 1));
+    *total = off[4] + al256((size_t)ngs_scan_workspace_bytes(n));
     return NGS_OK;
 }
 
 int64_t ngs_dedup_workspace_bytes(int64_t nkeys) {
-    size_t tmp, off[6], total;
-    if (dedup_layout(nkeys, &tmp, off, &total) != NGS_OK) return -1;
+    size_t tmp, off[5], total;
+    if (dedup_layout(nkeys, 64, &tmp, off, &total) != NGS_OK) return -1;
     return (int64_t)total;
 }
 
-int ngs_dedup_vertices(const uint64_t* vkeys, const float* tris, int64_t nkeys, int64_t* faces, float* verts, int64_t* nverts,
-                       void* workspace, int64_t workspace_bytes, ngs_stream_t s) {
-    NGS_REQUIRE(workspace, "ngs_dedup_vertices: null workspace");
+int ngs_dedup_vertices(const uint64_t* vkeys, const uint32_t* vtags, int64_t nkeys, int32_t id_bits, const float* pos,
+                       const float* values, int64_t* faces, float* verts, int64_t* nverts, void* workspace,
+                       int64_t workspace_bytes, ngs_stream_t s) {
+    NGS_REQUIRE(workspace && pos && values, "ngs_dedup_vertices: null argument");
+    NGS_REQUIRE(id_bits >= 1 && id_bits <= 31 && nkeys < ((int64_t)1 << 31), "ngs_dedup_vertices: bad sizes");
     if (nkeys <= 0) return NGS_OK;
-    size_t tmp, off[6], total;
-    int r = dedup_layout(nkeys, &tmp, off, &total);
+    size_t tmp, off[5], total;
+    int r = dedup_layout(nkeys, 2 * id_bits, &tmp, off, &total);
     if (r != NGS_OK) return r;
     NGS_REQUIRE((size_t)workspace_bytes >= total, "ngs_dedup_vertices: workspace too small");
     char* w = reinterpret_cast<char*>(workspace);
     hipStream_t st = S(s);
     uint64_t* kout = reinterpret_cast<uint64_t*>(w + off[0]);
-    int32_t* vin = reinterpret_cast<int32_t*>(w + off[1]);
-    int32_t* vout = reinterpret_cast<int32_t*>(w + off[2]);
-    int32_t* head = reinterpret_cast<int32_t*>(w + off[3]);
-    int32_t* offs = reinterpret_cast<int32_t*>(w + off[4]);
-    iota_kernel<<<grid1d(nkeys, 256), 256, 0, st>>>(vin, nkeys);
-    NGS_CHECK_LAUNCH();
+    uint32_t* tout = reinterpret_cast<uint32_t*>(w + off[1]);
+    int32_t* head = reinterpret_cast<int32_t*>(w + off[2]);
+    int32_t* offs = reinterpret_cast<int32_t*>(w + off[3]);
     size_t tb = tmp;
-    r = sort_pairs_u64(vkeys, kout, vin, vout, nkeys, 64, w, &tb, st);
+    // keys use 2 id_bits bits: the radix sort skips the empty high digits
+    r = sort_pairs_u64(vkeys, kout, reinterpret_cast<const int32_t*>(vtags), reinterpret_cast<int32_t*>(tout), nkeys,
+                       2 * id_bits, w, &tb, st);
     if (r != NGS_OK) return r;
     run_heads_kernel<<<grid1d(nkeys, 256), 256, 0, st>>>(kout, nkeys, head);
     NGS_CHECK_LAUNCH();
-    r = scan_i32_internal(head, nkeys, offs, w + off[5], st);
+    r = scan_i32_internal(head, nkeys, offs, w + off[4], st);
     if (r != NGS_OK) return r;
-    dedup_scatter_kernel<<<grid1d(nkeys, 256), 256, 0, st>>>(head, offs, vout, tris, nkeys, faces, verts);
+    dedup_scatter_kernel<<<grid1d(nkeys, 256), 256, 0, st>>>(head, offs, kout, tout, nkeys, id_bits, pos, values, faces,
+                                                              verts);
     NGS_CHECK_LAUNCH();
     if (nverts) {
         widen_kernel<<<1, 1, 0, st>>>(&offs[nkeys], nverts);
@@ -667,12 +669,12 @@ int ngs_mark_dual_leaves(ngs_grid g, const int32_t* leaves, int64_t n, int32_t s
     return NGS_OK;
 }
 
-int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t* leaves, int64_t n, int32_t n_upsample, int32_t level,
+int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t* leaves, int64_t n, int32_t n_upsample,
                                int32_t id_base, const float* values, int32_t* acc, int32_t* count, ngs_stream_t s) {
     NGS_REQUIRE(n_upsample >= 1 && centres.log2_stride == 0, "ngs_dual_accumulate_leaves: bad n_upsample or centre grid");
     const int64_t tot = n * (n_upsample + 1) * (n_upsample + 1) * (n_upsample + 1);
     if (tot <= 0) return NGS_OK;
-    dual_accumulate_leaves_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, leaves, n, n_upsample, level, id_base,
+    dual_accumulate_leaves_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, leaves, n, n_upsample, id_base,
                                                                       values, acc, count);
     NGS_CHECK_LAUNCH();
     return NGS_OK;

@@ -112,16 +112,16 @@ _SIGNATURES = {
     "ngs_compact_rows": ([P, c_int32, P, c_int64, P, P, P, P], c_int32),
     "ngs_expand_primal": ([P, c_int64, c_int32, c_int32, P, P], c_int32),
     "ngs_mark_dual_centres": ([Grid, P, c_int64, c_int32, P], c_int32),
-    "ngs_dual_accumulate": ([Grid, P, c_int64, c_int32, c_int32, c_int32, P, P, P, P], c_int32),
+    "ngs_dual_accumulate": ([Grid, P, c_int64, c_int32, c_int32, P, P, P, P], c_int32),
     "ngs_mark_dual_leaves": ([Grid, P, c_int64, c_int32, c_int32, P], c_int32),
-    "ngs_dual_accumulate_leaves": ([Grid, P, c_int64, c_int32, c_int32, c_int32, P, P, P, P], c_int32),
+    "ngs_dual_accumulate_leaves": ([Grid, P, c_int64, c_int32, c_int32, P, P, P, P], c_int32),
     "ngs_dual_compact": ([P, P, c_int32, c_int64, P, P, P, P, P], c_int32),
     "ngs_dual_corner_positions": ([P, c_int64, c_int32, c_float, P, P], c_int32),
     "ngs_eval_leaf_corners": ([Grid, P, P, c_int64, c_int32, c_int32, c_float, c_float, c_int32, P, P], c_int32),
     "ngs_mc_count": ([P, c_int64, P, P, P], c_int32),
-    "ngs_mc_emit": ([P, c_int64, P, P, P, P, P, P], c_int32),
+    "ngs_mc_emit": ([P, c_int64, P, P, c_int32, P, P, P], c_int32),
     "ngs_dedup_workspace_bytes": ([c_int64], c_int64),
-    "ngs_dedup_vertices": ([P, P, c_int64, P, P, P, P, c_int64, P], c_int32),
+    "ngs_dedup_vertices": ([P, P, c_int64, c_int32, P, P, P, P, P, P, c_int64, P], c_int32),
     "ngs_f64_to_f32": ([P, P, c_int64, P], c_int32),
     "ngs_weighted_sum": ([P, P, c_int64, P, P], c_int32),
 }

@@ -334,18 +334,18 @@ class Reconstructor:
                 call("ngs_dual_corner_positions", ptr(primal[d]), primal[d].shape[0], tree.strides[d], scale,
                      ptr(pos[bases_[d]:]), stream())
         values = self.leaf_corner_chi(leaves, bases_, nprim, nu, scale) - self.get_mean_chi()
-        # (id, level) per octant, read only for the kept centres (all eight slots written); the count
+        # primal id per octant, read only for the kept centres (all eight slots written); the count
         # word also counts the registered cells with a negative value
-        acc = torch.empty((DC, 8, 2), dtype=torch.int32, device=dev)
+        acc = torch.empty((DC, 8), dtype=torch.int32, device=dev)
         cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
         for d in range(L):
             if not primal[d].shape[0]:
                 continue
             if tree.strides[d] == 1:
-                call("ngs_dual_accumulate_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], nu, d, bases_[d],
+                call("ngs_dual_accumulate_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], nu, bases_[d],
                      ptr(values), ptr(acc), ptr(cnt), stream())
             else:
-                call("ngs_dual_accumulate", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d], d,
+                call("ngs_dual_accumulate", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d],
                      bases_[d], ptr(values), ptr(acc), ptr(cnt), stream())
         # complete cubes whose corner values change sign (the only ones with triangles), in centre order
         cubes = torch.empty((DC, 8), dtype=torch.int32, device=dev)
@@ -394,14 +394,17 @@ class Reconstructor:
         nv = int(voff[K].item())
         if nv == 0:
             return torch.zeros((0, 3), dtype=torch.float32, device=dev), torch.zeros((0, 3), dtype=torch.int64, device=dev)
-        tris = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+        # corner ids fit id_bits bits: keys are sorted on 2 id_bits bits
+        id_bits = max(1, (values.shape[0] - 1).bit_length())
         keys = torch.empty(nv, dtype=torch.int64, device=dev)
-        call("ngs_mc_emit", ptr(cubes), K, ptr(values), ptr(pos), ptr(voff), ptr(tris), ptr(keys), stream())
+        tags = torch.empty(nv, dtype=torch.int32, device=dev)
+        call("ngs_mc_emit", ptr(cubes), K, ptr(values), ptr(voff), id_bits, ptr(keys), ptr(tags), stream())
         faces = torch.empty(nv, dtype=torch.int64, device=dev)
         verts = torch.empty((nv, 3), dtype=torch.float32, device=dev)
         nvert = torch.empty(1, dtype=torch.int64, device=dev)
         wsb = lib().ngs_dedup_workspace_bytes(nv)
         ws2 = Scratch.get(wsb, dev, slot=4)
-        call("ngs_dedup_vertices", ptr(keys), ptr(tris), nv, ptr(faces), ptr(verts), ptr(nvert), ptr(ws2), wsb, stream())
+        call("ngs_dedup_vertices", ptr(keys), ptr(tags), nv, id_bits, ptr(pos.contiguous()), ptr(values), ptr(faces),
+             ptr(verts), ptr(nvert), ptr(ws2), wsb, stream())
         V = int(nvert.item())
         return verts[:V], faces.view(-1, 3)

@@ -197,18 +197,18 @@ def test_dual_leaf_forms_equal_per_cell(pair, nu):
             call("ngs_mark_dual_centres", cg.struct(), ptr(pr), pr.shape[0], tree.strides[d], stream())
     DC = cg.finalize()
     assert torch.equal(cg.coords, cg_leaf)
-    acc = torch.zeros((DC, 8, 2), dtype=torch.int32, device=dev)
+    acc = torch.zeros((DC, 8), dtype=torch.int32, device=dev)
     cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
     base = 0
     for d, pr in primal.items():
         if pr.shape[0]:
-            call("ngs_dual_accumulate", cg.struct(), ptr(pr), pr.shape[0], tree.strides[d], d, base, None, ptr(acc),
+            call("ngs_dual_accumulate", cg.struct(), ptr(pr), pr.shape[0], tree.strides[d], base, None, ptr(acc),
                  ptr(cnt), stream())
         base += pr.shape[0]
     # the mesh's count words also hold the negative-cell counts in their high half
     assert torch.equal(cnt, info["count"] & 0xFFFF)
     full_ = cnt == 8
-    ids = info["acc"][full_][:, :, 0].long()  # every slot of a complete centre is written
+    ids = info["acc"][full_].long()  # every slot of a complete centre is written
     assert int(ids.min()) >= 0 and int(ids.max()) < info["values"].shape[0]
     neg = (info["values"][ids] < 0).sum(1)
     assert torch.equal(info["count"][full_] >> 16, neg.int())
