@@ -427,10 +427,15 @@ __global__ void rhs_normals_same_kernel(ngs_level tgt, ngs_level dat, const floa
 // target cell is t - k s_t (k in [-1,1]^3) are those of the 3m-wide window starting at t - s_t on
 // each axis, and the data voxel at window position j has table row 3m - 1 - j (the row the
 // scatter kernel computes for the same pair). Window columns are read from the data bitmap; the
-// same FP32 terms are summed in FP64 in a fixed order (no atomics).
+// same FP32 terms are summed in FP64 in a fixed order (no atomics). G lanes share a target (lane j
+// takes window x-columns j, j + G, ...) when the targets are few and the windows wide; their FP64
+// partials are combined by an xor butterfly.
+template <int G>
 __global__ void rhs_normals_fine_gather_kernel(ngs_level tgt, ngs_level dat, const float* __restrict__ nrm, Tables T,
                                                double* __restrict__ rhs) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = t / G;  // the G lanes of a target are consecutive lanes of one wave
+    const int gl = (int)(t % G);
     if (i >= tgt.M) return;
     const int sd = dat.stride, st = tgt.stride, m = T.m, w3 = 3 * m;
     const ngs_grid& g = dat.grid;
@@ -440,7 +445,7 @@ __global__ void rhs_normals_fine_gather_kernel(ngs_level tgt, ngs_level dat, con
     const int lz0 = ((tgt.coords[3 * i + 2] - st) >> l2) - g.origin[2];
     const float fsd = (float)sd, imult = (float)sd / (float)st;
     double acc = 0.0;
-    for (int jx = 0; jx < w3; ++jx) {
+    for (int jx = gl; jx < w3; jx += G) {
         const int ix = w3 - 1 - jx;
         const float px = T.P[ix] * fsd, qx = T.Q[ix] * imult;
         for (int jy = 0; jy < w3; ++jy) {
@@ -467,6 +472,11 @@ __global__ void rhs_normals_fine_gather_kernel(ngs_level tgt, ngs_level dat, con
                 }
             }
         }
+    }
+    if (G > 1) {
+#pragma unroll
+        for (int d = G / 2; d > 0; d >>= 1) acc += __shfl_xor(acc, d, 64);
+        if (gl != 0) return;
     }
     rhs[i] += acc;
 }
@@ -739,9 +749,17 @@ int ngs_rhs_normals(ngs_level target, ngs_level data, const float* normals, doub
         NGS_REQUIRE(tables_for(m, &T), "ngs_rhs_normals: table multiple unavailable");
         // gather (window scan per target, no atomics) unless the data is much sparser than the
         // targets' windows, where the 27-target scatter touches less
-        if ((double)target.M * (3.0 * m) * (3.0 * m) < 4.0 * 27.0 * (double)data.M)
-            rhs_normals_fine_gather_kernel<<<grid1d(target.M, 128), 128, 0, S(s)>>>(target, data, normals, T, rhs);
-        else
+        if ((double)target.M * (3.0 * m) * (3.0 * m) < 4.0 * 27.0 * (double)data.M) {
+            // lanes per target: enough threads to fill the GPU, at most one per window x-column
+            int lanes = 1;
+            while (lanes < 16 && lanes * 2 <= 3 * m && target.M * lanes < 512 * 1024) lanes *= 2;
+            const unsigned grid = grid1d(target.M * lanes, 128);
+            if (lanes == 16) rhs_normals_fine_gather_kernel<16><<<grid, 128, 0, S(s)>>>(target, data, normals, T, rhs);
+            else if (lanes == 8) rhs_normals_fine_gather_kernel<8><<<grid, 128, 0, S(s)>>>(target, data, normals, T, rhs);
+            else if (lanes == 4) rhs_normals_fine_gather_kernel<4><<<grid, 128, 0, S(s)>>>(target, data, normals, T, rhs);
+            else if (lanes == 2) rhs_normals_fine_gather_kernel<2><<<grid, 128, 0, S(s)>>>(target, data, normals, T, rhs);
+            else rhs_normals_fine_gather_kernel<1><<<grid, 128, 0, S(s)>>>(target, data, normals, T, rhs);
+        } else
             rhs_normals_scatter_kernel<<<grid1d(data.M, 256), 256, 0, S(s)>>>(target, data, normals, T, rhs);
     } else {
         NGS_REQUIRE(tables_for(data.stride / target.stride, &T), "ngs_rhs_normals: table multiple unavailable");
