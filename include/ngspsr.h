@@ -407,6 +407,10 @@ int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t *leaves, int64_t 
 /* Dual-corner positions (primal + stride) * voxel_size / n_upsample (reconstructor.py:368-373). */
 int ngs_dual_corner_positions(const int32_t *primal, int64_t n, int32_t stride, float scale, float *pos,
                               ngs_stream_t s);
+/* The positions of ngs_dual_corner_positions for the sub-cells ngs_expand_primal would list (leaf-major,
+ * nu^3 per leaf), computed from the leaves directly, bit for bit. */
+int ngs_leaf_corner_positions(const int32_t *leaves, int64_t n, int32_t stride, int32_t n_upsample, float scale,
+                              float *pos, ngs_stream_t s);
 /* chi of one level at the dual corners of leaves (reconstructor.py:368-375, evaluate_chi at the
  * corner positions): out[l * nu^3 + e] = (accumulate ? out : 0) + the ngs_eval_field value (unit
  * scale) at the position ngs_dual_corner_positions writes for corner e of leaf l, bit for bit; for

@@ -237,6 +237,20 @@ __global__ void corner_pos_kernel(const int32_t* __restrict__ prim, int64_t n, i
     pos[i] = (float)(prim[i] + s) * scale;
 }
 
+// Dual-corner positions straight from the leaves: corner e of leaf l at (leaf * nu + e * s + s) * scale
+// per axis, the same integer and float operations as expand_kernel followed by corner_pos_kernel.
+__global__ void leaf_corner_pos_kernel(const int32_t* __restrict__ leaves, int64_t n, int nu, int s, float scale,
+                                       float* __restrict__ pos) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nu3 = nu * nu * nu;
+    if (t >= n * nu3) return;
+    const int64_t l = t / nu3;
+    const int e = (int)(t % nu3);
+    const int ee[3] = {e / (nu * nu), (e / nu) % nu, e % nu};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pos[3 * t + a] = (float)(leaves[3 * l + a] * nu + ee[a] * s + s) * scale;
+}
+
 // Field of one level at the dual corners of leaves: corner e of leaf l (e in [0, nu)^3, leaf-major,
 // the order of ngs_expand_primal) sits at ((leaf * nu + e * s_leaf + s_leaf) * scale) / voxel_size
 // per axis, computed in the same float operations as corner_pos_kernel followed by eval_field_kernel
@@ -676,6 +690,16 @@ int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t* leaves, int64_t 
     if (tot <= 0) return NGS_OK;
     dual_accumulate_leaves_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(centres, leaves, n, n_upsample, id_base,
                                                                       values, acc, count);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_leaf_corner_positions(const int32_t* leaves, int64_t n, int32_t stride, int32_t n_upsample, float scale,
+                              float* pos, ngs_stream_t s) {
+    NGS_REQUIRE(stride >= 1 && n_upsample >= 1 && pos && (n == 0 || leaves), "ngs_leaf_corner_positions: bad arguments");
+    const int64_t tot = n * n_upsample * n_upsample * n_upsample;
+    if (tot <= 0) return NGS_OK;
+    leaf_corner_pos_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(leaves, n, n_upsample, stride, scale, pos);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

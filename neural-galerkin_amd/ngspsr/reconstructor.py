@@ -306,16 +306,20 @@ class Reconstructor:
             leaves[d] = out[:k]
         L = max_d
         nu = n_upsample
-        primal, bases_, nprim = {}, {}, 0
+        # primal sub-cells (leaf-major, nu^3 per leaf); listed only where read: the dual-cube assembly
+        # of coarser levels (stride-1 leaves are assembled from the leaves themselves) and the report
+        primal, bases_, nsub, nprim = {}, {}, {}, 0
         for d in range(L):
             s = tree.strides[d]
             lf = leaves[d]
-            pr = torch.empty((lf.shape[0] * nu ** 3, 3), dtype=torch.int32, device=dev)
-            if lf.shape[0]:
-                call("ngs_expand_primal", ptr(lf), lf.shape[0], nu, s, ptr(pr), stream())
-            primal[d] = pr
+            nsub[d] = lf.shape[0] * nu ** 3
+            if s > 1 or return_info:
+                pr = torch.empty((nsub[d], 3), dtype=torch.int32, device=dev)
+                if lf.shape[0]:
+                    call("ngs_expand_primal", ptr(lf), lf.shape[0], nu, s, ptr(pr), stream())
+                primal[d] = pr
             bases_[d] = nprim
-            nprim += pr.shape[0]
+            nprim += nsub[d]
         # dual centres: sorted unique corners of all primal sub-cells (upsampled units)
         lo = [min(((o - 2) * tree.strides[d]) * nu for d in range(L) for o in [tree.grids[d].origin[a]]) for a in range(3)]
         hi = [max(((o + dm + 2) * tree.strides[d]) * nu for d in range(L)
@@ -325,13 +329,13 @@ class Reconstructor:
             if leaves[d].shape[0]:
                 call("ngs_mark_dual_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], tree.strides[d], nu,
                      stream())
-        DC = cg.finalize()
+        DC = cg.finalize(emit=return_info)  # the centres' coordinates are only reported, never read
         # dual corners = (primal + stride) * voxel_size / n_upsample, and chi there
         pos = torch.empty((nprim, 3), dtype=torch.float32, device=dev)
         scale = float(np.float32(tree.voxel_size / nu))
         for d in range(L):
-            if primal[d].shape[0]:
-                call("ngs_dual_corner_positions", ptr(primal[d]), primal[d].shape[0], tree.strides[d], scale,
+            if nsub[d]:
+                call("ngs_leaf_corner_positions", ptr(leaves[d]), leaves[d].shape[0], tree.strides[d], nu, scale,
                      ptr(pos[bases_[d]:]), stream())
         values = self.leaf_corner_chi(leaves, bases_, nprim, nu, scale) - self.get_mean_chi()
         # primal id per octant, read only for the kept centres (all eight slots written); the count
@@ -339,7 +343,7 @@ class Reconstructor:
         acc = torch.empty((DC, 8), dtype=torch.int32, device=dev)
         cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
         for d in range(L):
-            if not primal[d].shape[0]:
+            if not nsub[d]:
                 continue
             if tree.strides[d] == 1:
                 call("ngs_dual_accumulate_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], nu, bases_[d],
