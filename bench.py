@@ -190,9 +190,12 @@ def serving_scenes(n_req, seed0=1000):
     return out
 
 
-def run_serving(n_req=64, workers=8, rate=None):
-    """Plain sequential FP32 path vs the continuous-batching server (mixed precision) on the same
-    request stream; tolerance 1e-6 for both."""
+def run_serving(n_req=64):
+    """The same request stream served three ways, at relative residual 1e-6:
+    plain (one request at a time, `reconstruct` with FP32 PCG), batched FP32 (the continuous-batching
+    server, FP32 PCG solves batched in the pool), and plain vs batched with mixed precision (FP64
+    refinement; residual verified in FP64). Then the batched FP32 server under Poisson arrivals at
+    0.8x its measured capacity for latency."""
     import numpy as np
     import torch
     import ngspsr
@@ -201,35 +204,41 @@ def run_serving(n_req=64, workers=8, rate=None):
     scenes = serving_scenes(n_req)
     data = [(torch.from_numpy(x).to(dev), torch.from_numpy(n).to(dev), vs) for x, n, vs in scenes]
     pts = int(sum(x.shape[0] for x, _, _ in data))
-    # plain: one request at a time, FP32 PCG to 1e-6
-    for x, n, vs in data[:2]:
-        ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plain_lat = []
-    for x, n, vs in data:
-        t1 = time.perf_counter()
-        ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
+
+    def plain(solver):
+        for x, n, vs in data[:2]:
+            ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver=solver)
         torch.cuda.synchronize()
-        plain_lat.append(time.perf_counter() - t1)
-    plain_t = time.perf_counter() - t0
-    plain_rate = n_req / plain_t
-    # batched: saturating arrivals (all at t=0) for throughput
-    srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, mixed=True)
-    warm = [serving.Request(-1 - i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data[:workers])]
-    srv.serve(warm)
-    reqs = [serving.Request(i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data)]
-    t0 = time.perf_counter()
-    srv.serve(reqs, t0=t0)
-    torch.cuda.synchronize()
-    batched_t = max(r.finish for r in reqs) - t0
-    errs = [r for r in reqs if r.error is not None]
-    if errs:
-        raise errs[0].error
-    batched_rate = n_req / batched_t
-    res_max = max(r.max_rel_residual for r in reqs)
+        lat = []
+        t0 = time.perf_counter()
+        for x, n, vs in data:
+            t1 = time.perf_counter()
+            ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver=solver)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t1)
+        return n_req / (time.perf_counter() - t0), lat
+
+    def batched(mixed, workers, chunk):
+        srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, mixed=mixed, chunk=chunk)
+        srv.serve([serving.Request(-1 - i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data[:workers])])
+        reqs = [serving.Request(i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data)]
+        t0 = time.perf_counter()
+        srv.serve(reqs, t0=t0)
+        torch.cuda.synchronize()
+        errs = [r for r in reqs if r.error is not None]
+        if errs:
+            srv.close()
+            raise errs[0].error
+        return srv, n_req / (max(r.finish for r in reqs) - t0), reqs
+
+    plain_rate, plain_lat = plain("pcg@tol=1e-6@maxiter=20000")
+    plain_mixed_rate, _ = plain("mixed@tol=1e-6@inner_tol=1e-4")
+    srv_m, mixed_rate, reqs_m = batched(True, 8, 8)
+    mixed_res = max(r.max_rel_residual for r in reqs_m)
+    srv_m.close()
+    srv, batched_rate, reqs = batched(False, 16, 32)
     # latency at a Poisson arrival rate of 0.8x the measured batched capacity
-    lam = rate or 0.8 * batched_rate
+    lam = 0.8 * batched_rate
     rng = np.random.default_rng(7)
     arr = np.cumsum(rng.exponential(1.0 / lam, n_req))
     reqs2 = [serving.Request(i, x, n, vs, arrival=float(a)) for i, ((x, n, vs), a) in enumerate(zip(data, arr))]
@@ -239,13 +248,16 @@ def run_serving(n_req=64, workers=8, rate=None):
     pool = srv.pool
     srv.close()
     return {
-        "requests": n_req, "points_total": pts, "workers": workers,
+        "requests": n_req, "points_total": pts,
         "plain_fp32_solves_per_s": round(plain_rate, 3),
         "plain_latency_ms": {"p50": round(1e3 * float(np.percentile(plain_lat, 50)), 2),
                              "p99": round(1e3 * float(np.percentile(plain_lat, 99)), 2)},
-        "batched_mixed_solves_per_s": round(batched_rate, 3),
+        "batched_fp32_solves_per_s": round(batched_rate, 3), "batched_fp32_config": {"workers": 16, "chunk": 32},
         "batched_vs_plain": round(batched_rate / plain_rate, 3),
-        "batched_max_rel_residual": res_max,
+        "plain_mixed_solves_per_s": round(plain_mixed_rate, 3),
+        "batched_mixed_solves_per_s": round(mixed_rate, 3), "batched_mixed_config": {"workers": 8, "chunk": 8},
+        "batched_vs_plain_mixed": round(mixed_rate / plain_mixed_rate, 3),
+        "batched_mixed_max_fp64_rel_residual": mixed_res,
         "poisson_rate_per_s": round(lam, 3),
         "poisson_latency_ms": {"p50": round(lat["p50_ms"], 2), "p99": round(lat["p99_ms"], 2)},
         "pool_launches": pool.launches,
