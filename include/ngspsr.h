@@ -229,9 +229,10 @@ int ngs_assembly_config(int32_t lanes_per_row);
 /* ngs_lhs_assemble into the run form of the same matrix instead of CSR (no reference counterpart;
  * the reference stores int32 columns, solver.py:40-48). Row i's neighbours in the bitmap column
  * (dx, dy), p = 5 (dx + 2) + (dy + 2), have consecutive ranks start .. start + n - 1 (n <= 5):
- * runs[p M + i] = start | n << 28 (n = 0: start = i), rvals[(5 p + j) M + i] = A(i, start + j) for
- * j < n, 0 otherwise; 25 int32 and 125 float per row, plane-major. inv_diag and rhs as
- * ngs_lhs_assemble, bit for bit (the values equal the CSR entries). M <= 2^28. */
+ * runs[p P + i] = start | n << 28 (n = 0: start = i), rvals[(5 p + j) P + i] = A(i, start + j) for
+ * j < n, 0 otherwise; 25 int32 and 125 float per row, plane-major with plane stride
+ * P = M rounded up to a multiple of 64 (arrays of 25 P and 125 P elements; rows M..P-1 unused).
+ * inv_diag and rhs as ngs_lhs_assemble, bit for bit (the values equal the CSR entries). M <= 2^28. */
 int ngs_lhs_assemble_runs(ngs_level lv, ngs_point_bins pts, const float *coarse, float sf, float delta,
                           int32_t *runs, float *rvals, float *inv_diag, double *rhs, ngs_stream_t s);
 

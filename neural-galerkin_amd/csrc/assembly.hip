@@ -44,9 +44,9 @@ __global__ void lhs_count_kernel(ngs_level lv, int32_t* __restrict__ row_nnz) {
 // (3 or 5 consecutive lattice points along z) at a time.
 // RUNS: the row goes to the run form instead of CSR. The present neighbours of voxel i in the
 // bitmap column (dx, dy) (its five z-neighbours) have consecutive ranks start .. start + n - 1;
-// runs[p M + i] = start | n << 28 (n = 0: start = i) for p = 5 (dx + 2) + (dy + 2), and
-// rvals[(5 p + j) M + i] the value of column start + j (0 for j >= n). Both arrays are
-// plane-major, so the run and value loads of consecutive rows are contiguous.
+// runs[p P + i] = start | n << 28 (n = 0: start = i) for p = 5 (dx + 2) + (dy + 2), and
+// rvals[(5 p + j) P + i] the value of column start + j (0 for j >= n), P = run_plane_stride(M).
+// Both arrays are plane-major, so the run and value loads of consecutive rows are contiguous.
 constexpr int RUN_COUNT_SHIFT = 28;
 
 // Lanes per row of the assembly and of stencil_diag_rhs_kernel (the two must agree for their
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
         }
         ++cnt;
     };
-    const int64_t M = lv.M;
+    const int64_t P = run_plane_stride(lv.M);
 #pragma unroll
     for (int dx = -2; dx <= 2; ++dx) {
 #pragma unroll
@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
             int32_t j;
             const uint32_t m = grid_column(g, lx + dx, ly + dy, lz - 2, 5, &j);
             const int pl = (dx + 2) * 5 + (dy + 2);
-            if (RUNS) runs[pl * M + i] = (m ? j : (int32_t)i) | (__popc(m) << RUN_COUNT_SHIFT);
+            if (RUNS) runs[pl * P + i] = (m ? j : (int32_t)i) | (__popc(m) << RUN_COUNT_SHIFT);
             int q = 0;
 #pragma unroll
             for (int dz = -2; dz <= 2; ++dz) {
@@ -194,11 +194,11 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                     v = K + sf * acc[(dx + 2) * 25 + (dy + 2) * 5 + (dz + 2)];
                 }
                 if (dx == 0 && dy == 0 && dz == 0) inv_diag[i] = 1.0f / v;
-                if (RUNS) rvals[(5 * pl + q++) * M + i] = v;
+                if (RUNS) rvals[(5 * pl + q++) * P + i] = v;
                 else emit(v, j++);
             }
             if (RUNS)
-                for (; q < 5; ++q) rvals[(5 * pl + q) * M + i] = 0.f;
+                for (; q < 5; ++q) rvals[(5 * pl + q) * P + i] = 0.f;
         }
     }
     if (!RUNS)

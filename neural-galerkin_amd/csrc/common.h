@@ -133,6 +133,11 @@ __device__ __forceinline__ uint32_t grid_column(const ngs_grid& g, int lx, int l
     return m << (zlo - lz0);
 }
 
+// Plane stride of the run form (runs (25, P) int32, rvals (125, P) float32): M rounded up to 64
+// rows, so every plane starts on a 256-byte boundary and the loads of a wave's 64 consecutive rows
+// are single aligned 256-byte segments.
+__host__ __device__ __forceinline__ int64_t run_plane_stride(int64_t M) { return (M + 63) / 64 * 64; }
+
 // ----------------------------------------------------------------------------------- FNV keys
 // fold60(FNV-1a-64 over the uint32 words of a coordinate row); csrc/common/hashmap_cuda.cu:248-261.
 __device__ __forceinline__ uint64_t fnv_fold60(const int32_t* c, int dim) {

@@ -178,7 +178,7 @@ __device__ __forceinline__ float row_dot(int32_t k0, int32_t k1, const NGS_GLOBA
 template <typename T, typename XT>
 __device__ __forceinline__ T run_row_dot(const NGS_GLOBAL int32_t* __restrict__ runs,
                                          const NGS_GLOBAL float* __restrict__ rvals, const NGS_GLOBAL XT* __restrict__ x,
-                                         int64_t M, int64_t row) {
+                                         int64_t P, int64_t row) {
     T sum = 0;
     constexpr int B = 5;
 #pragma unroll 1
@@ -187,9 +187,9 @@ __device__ __forceinline__ T run_row_dot(const NGS_GLOBAL int32_t* __restrict__ 
         float v[B][5];
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-            w[b] = runs[(p0 + b) * M + row];
+            w[b] = runs[(p0 + b) * P + row];
 #pragma unroll
-            for (int j = 0; j < 5; ++j) v[b][j] = rvals[(5 * (p0 + b) + j) * M + row];
+            for (int j = 0; j < 5; ++j) v[b][j] = rvals[(5 * (p0 + b) + j) * P + row];
         }
         XT xv[B][5];
 #pragma unroll
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(PCG_THREADS) spmv_runs_kernel(const int32_t* _
                                                                 int64_t M) {
     const int64_t step = (int64_t)gridDim.x * PCG_THREADS;
     for (int64_t row = run_rows_first(blockIdx.x); row < M; row += step)
-        y[row] = run_row_dot<float, float>(as_global(runs), as_global(rvals), as_global(x), M, row);
+        y[row] = run_row_dot<float, float>(as_global(runs), as_global(rvals), as_global(x), run_plane_stride(M), row);
 }
 
 // r = b - A x in FP64 (FP32 entries, FP64 x and products), 8 lanes per row with 16-byte loads of
@@ -387,7 +387,8 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_runs_kernel(const in
     double rr = 0, bb = 0;
     for (int64_t row = run_rows_first(blockIdx.x); row < M; row += step) {
         const double bi = b[row];
-        const double ri = x ? bi - run_row_dot<double, double>(as_global(runs), as_global(rvals), as_global(x), M, row)
+        const double ri = x ? bi - run_row_dot<double, double>(as_global(runs), as_global(rvals), as_global(x),
+                                                               run_plane_stride(M), row)
                             : bi;
         r[row] = ri;
         rr += ri * ri;
@@ -578,7 +579,7 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_runs_dot_kernel(const BatchS
     double pq = 0;
     for (int64_t row = run_rows_first(blockIdx.x - B->blk0); row < A.M; row += step) {
         const float prow = p[row];
-        const float sum = run_row_dot<float, float>(runs, rvals, p, A.M, row);
+        const float sum = run_row_dot<float, float>(runs, rvals, p, run_plane_stride(A.M), row);
         q[row] = sum;
         pq += (double)prow * sum;
     }

@@ -122,8 +122,9 @@ class Reconstructor:
             return system
         inv_diag = torch.empty(M, dtype=torch.float32, device=dev)
         if form == "runs":
-            runs = torch.empty((25, M), dtype=torch.int32, device=dev)
-            rvals = torch.empty((125, M), dtype=torch.float32, device=dev)
+            P = _solver.RunSystem.plane_stride(M)
+            runs = torch.empty((25, P), dtype=torch.int32, device=dev)
+            rvals = torch.empty((125, P), dtype=torch.float32, device=dev)
             call("ngs_lhs_assemble_runs", lv, bins.struct(), ptr(coarse), float(sf), float(delta), ptr(runs),
                  ptr(rvals), ptr(inv_diag), ptr(rhs), stream())
             return _solver.RunSystem(runs, rvals, inv_diag)

@@ -79,20 +79,25 @@ class CSRSystem:
 class RunSystem:
     """Assembled level system in run form (ngs_lhs_assemble_runs): for each row and each of the 25
     bitmap columns (dx, dy) of its 5x5x5 neighbourhood, one run word (first column | count << 28)
-    and five value slots, both plane-major: runs (25, M) int32, rvals (125, M) float32. The values
-    are the CSR entries bit for bit; the SpMV reads 600 bytes per row with every load contiguous
-    across consecutive rows."""
+    and five value slots, both plane-major with the plane stride P = M rounded up to 64:
+    runs (25, P) int32, rvals (125, P) float32. The values are the CSR entries bit for bit; the SpMV
+    reads 600 bytes per row with every load an aligned contiguous segment across consecutive rows."""
 
     COUNT_SHIFT = 28
+
+    @staticmethod
+    def plane_stride(M):
+        return (M + 63) // 64 * 64
 
     def __init__(self, runs, rvals, inv_diag):
         self.runs, self.rvals, self.inv_diag = runs, rvals, inv_diag
         self.M = inv_diag.shape[0]
         self.device = inv_diag.device
+        assert runs.shape == (25, self.plane_stride(self.M)) and rvals.shape == (125, self.plane_stride(self.M))
 
     def counts(self):
         """(25, M) run lengths."""
-        return (self.runs >> self.COUNT_SHIFT) & 7
+        return (self.runs[:, :self.M] >> self.COUNT_SHIFT) & 7
 
     def nnz(self):
         """Stored entries (the CSR entries of the same matrix, without padding)."""
@@ -119,13 +124,13 @@ class RunSystem:
         """(rows, cols, vals) of the stored entries, rows ascending and columns ascending per row."""
         M, dev = self.M, self.device
         n = self.counts()
-        start = self.runs & ((1 << self.COUNT_SHIFT) - 1)
+        start = self.runs[:, :M] & ((1 << self.COUNT_SHIFT) - 1)
         j = torch.arange(5, dtype=torch.int32, device=dev).view(1, 5, 1)
         # (row, plane, slot) order is increasing column order within a row
-        present = (j < n.view(25, 1, M)).permute(2, 0, 1)
-        cols = (start.view(25, 1, M) + j).permute(2, 0, 1)[present]
+        present = (j < n.reshape(25, 1, M)).permute(2, 0, 1)
+        cols = (start.reshape(25, 1, M) + j).permute(2, 0, 1)[present]
         rows = torch.arange(M, dtype=torch.int32, device=dev).view(1, 1, M).expand(25, 5, M).permute(2, 0, 1)[present]
-        vals = self.rvals.view(25, 5, M).permute(2, 0, 1)[present]
+        vals = self.rvals[:, :M].reshape(25, 5, M).permute(2, 0, 1)[present]
         return rows, cols, vals
 
     def to_csr(self):
