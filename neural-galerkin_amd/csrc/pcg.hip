@@ -451,7 +451,9 @@ static int pcg_blocks(int64_t M, int64_t* chunk) {
     if (nb > 2048) nb = 2048;
     if (nb >= NUM_XCD) nb = (nb / NUM_XCD) * NUM_XCD;
     if (nb < 1) nb = 1;
-    *chunk = (M + nb - 1) / nb;
+    // whole 64-row groups per block: the vector loads and stores of each wave are aligned 256-byte
+    // segments (the last blocks may get fewer rows, or none)
+    *chunk = ((M + nb - 1) / nb + 63) / 64 * 64;
     return (int)nb;
 }
 

@@ -56,7 +56,8 @@ def test_stencil_equals_csr_operator(cuda, per_point_alpha):
         b64 = torch.randn(M, generator=g, dtype=torch.float64).to(cuda)
         rc, rrc, bbc = A.residual_f64(b64, x64)
         rs, rrs, bbs = S.residual_f64(b64, x64)
-        assert bbc == bbs
+        # ||b||^2 is summed over rows in each kernel's own row order
+        assert abs(bbc - bbs) <= 1e-13 * bbc
         # the CSR entries are FP32 sums over up to ~100 point terms (error up to ~100 ulp of the
         # entry); the FP64 stencil operator evaluates the same terms exactly
         bound64 = absA.matvec(x.abs()).double()
