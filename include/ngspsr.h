@@ -403,6 +403,10 @@ int ngs_dual_compact(const int32_t *acc, const int32_t *count, int32_t active_on
  * dual centre is over-registered). */
 int ngs_mark_dual_leaves(ngs_grid g, const int32_t *leaves, int64_t n, int32_t stride, int32_t n_upsample,
                          ngs_stream_t s);
+/* The dual centres of stride-1 leaves for n_upsample 2 (the set ngs_mark_dual_leaves marks), written
+ * one bitmap word at a time with no atomics: leaf_bitmap holds the leaves (stride-1 grid), every word
+ * of centres is overwritten, so call it before marking other levels' centres. */
+int ngs_mark_dual_leaf_words(ngs_grid centres, ngs_grid leaf_bitmap, ngs_stream_t s);
 int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t *leaves, int64_t n, int32_t n_upsample,
                                int32_t id_base, const float *values, int32_t *acc, int32_t *count, ngs_stream_t s);
 /* Dual-corner positions (primal + stride) * voxel_size / n_upsample (reconstructor.py:368-373). */

@@ -474,6 +474,59 @@ __global__ void mark_dual_leaves_kernel(ngs_grid g, const int32_t* __restrict__ 
     if (mask) grid_mark_word(g, cur_word, mask);
 }
 
+// Even bits of the result hold the bits of m (bit k -> bit 2k).
+__device__ __forceinline__ uint64_t spread_bits(uint32_t m) {
+    uint64_t x = m;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+// Dual centres of stride-1 leaves with n_upsample 2, one output bitmap word per thread and no
+// atomics: the centres are {2 L + o, o in [0, 2]^3} over the leaves L, so a centre column (x, y)
+// takes the z-bits of the leaf columns X with 2X <= x <= 2X + 2 (the same for y), each leaf bit Z
+// setting centre bits 2Z .. 2Z + 2. The word is written whole (it replaces any earlier content);
+// coarser leaves are marked afterwards with atomics.
+__global__ void mark_dual_leaf_words_kernel(ngs_grid cen, ngs_grid lg) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= cen.nwords) return;
+    const int64_t d2 = cen.dims[2], d12 = (int64_t)cen.dims[1] * d2;
+    const int64_t nbits = (int64_t)cen.dims[0] * d12;
+    int64_t lin = w * 32;
+    const int64_t end = min(lin + 32, nbits);
+    uint32_t out = 0;
+    while (lin < end) {
+        // one segment: bits of the word inside one centre column
+        const int x = (int)(lin / d12), y = (int)((lin / d2) % cen.dims[1]), z = (int)(lin % d2);
+        const int len = (int)min(end - lin, d2 - z);
+        const int cx = x + cen.origin[0], cy = y + cen.origin[1], cz = z + cen.origin[2];
+        // leaf rows covering centre z in [cz, cz + len): Z from floor((cz - 2) / 2) to floor((cz + len - 1) / 2)
+        const int z0 = (cz - 2) >> 1;
+        const int nz = ((cz + len - 1) >> 1) - z0 + 1;  // <= 18
+        uint64_t seg = 0;
+        for (int X = (cx - 1) >> 1; X <= (cx >> 1); ++X) {
+            if (2 * X > cx || 2 * X + 2 < cx) continue;
+            for (int Y = (cy - 1) >> 1; Y <= (cy >> 1); ++Y) {
+                if (2 * Y > cy || 2 * Y + 2 < cy) continue;
+                int32_t r;
+                const uint32_t m = grid_column(lg, X - lg.origin[0], Y - lg.origin[1], z0 - lg.origin[2], nz, &r);
+                if (!m) continue;
+                const uint64_t sp = spread_bits(m);
+                seg |= sp | (sp << 1) | (sp << 2);  // bit 2k + oz <-> centre z = 2 (z0 + k) + oz
+            }
+        }
+        // centre z = cz + i sits at bit 2 z0 + (bit index) => shift by cz - 2 z0 (2 or 3)
+        seg >>= (cz - 2 * z0);
+        const uint32_t segm = (uint32_t)seg & (len >= 32 ? 0xFFFFFFFFu : ((1u << len) - 1u));
+        out |= segm << (lin - w * 32);
+        lin += len;
+    }
+    cen.bits[w] = out;
+}
+
 // Dual-cube assembly for leaves of stride 1 (the finest level): one thread per (leaf, lattice point
 // o in [0, nu]^3) registers every sub-cell of the leaf that has the point on its boundary, with one
 // aggregated count update. Per axis, o = 0 is covered by sub-cell 0 from the positive side, o = nu
@@ -679,6 +732,16 @@ int ngs_mark_dual_leaves(ngs_grid g, const int32_t* leaves, int64_t n, int32_t s
     int64_t tot = n * (n_upsample + 1) * (n_upsample + 1);
     if (tot <= 0) return NGS_OK;
     mark_dual_leaves_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(g, leaves, n, stride, n_upsample);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_mark_dual_leaf_words(ngs_grid centres, ngs_grid leaf_bitmap, ngs_stream_t s) {
+    NGS_REQUIRE(centres.log2_stride == 0 && leaf_bitmap.log2_stride == 0 && centres.bits && leaf_bitmap.bits &&
+                    leaf_bitmap.rank,
+                "ngs_mark_dual_leaf_words: stride-1 grids required");
+    if (centres.nwords <= 0) return NGS_OK;
+    mark_dual_leaf_words_kernel<<<grid1d(centres.nwords, 256), 256, 0, S(s)>>>(centres, leaf_bitmap);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

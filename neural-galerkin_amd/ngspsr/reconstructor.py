@@ -326,7 +326,20 @@ class Reconstructor:
         hi = [max(((o + dm + 2) * tree.strides[d]) * nu for d in range(L)
                   for o, dm in [(tree.grids[d].origin[a], tree.grids[d].dims[a])]) for a in range(3)]
         cg = SparseGrid(lo, hi, 1, dev)
-        for d in range(L):
+        first = 0
+        if nu == 2 and tree.strides[0] == 1 and leaves[0].shape[0]:
+            # finest leaves: centre words computed from a leaf bitmap (no atomics), before the others
+            if 0 in parents:
+                po, pdm = parents[0].origin, parents[0].dims
+                llo, lhi = [2 * o for o in po], [2 * (o + dm - 1) + 1 for o, dm in zip(po, pdm)]
+            else:
+                g0 = tree.grids[0]
+                llo, lhi = list(g0.origin), [o + dm - 1 for o, dm in zip(g0.origin, g0.dims)]
+            lgrid = SparseGrid(llo, lhi, 1, dev)
+            lgrid.mark_coords(leaves[0])
+            call("ngs_mark_dual_leaf_words", cg.struct(), lgrid.struct(), stream())
+            first = 1
+        for d in range(first, L):
             if leaves[d].shape[0]:
                 call("ngs_mark_dual_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], tree.strides[d], nu,
                      stream())
