@@ -527,6 +527,53 @@ __global__ void mark_dual_leaf_words_kernel(ngs_grid cen, ngs_grid lg) {
     cen.bits[w] = out;
 }
 
+// Dual-cube assembly for the stride-1 leaves (n_upsample 2) gathered per dual centre, before any
+// other level: one thread per centre q (coordinates in rank order), per octant (sx, sy, sz) the
+// sub-cell c = q - 1 + s (upsampled units) of leaf c >> 1 covers it when that leaf is present. The
+// centre's count word (slots | negative-valued slots << 16) and its level-0 slots are written
+// directly; coarser levels add theirs with atomics afterwards. leaf_index maps a leaf's rank in the
+// leaf bitmap to its position in the leaf list (ids are id_base + 8 position + sub-cell).
+__global__ void dual_gather_leaf_cubes_kernel(const int32_t* __restrict__ centres, int64_t dcount, ngs_grid lg,
+                                              const int32_t* __restrict__ leaf_index, int id_base,
+                                              const float* __restrict__ values, int32_t* __restrict__ acc,
+                                              int32_t* __restrict__ cnt) {
+    const int64_t dc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (dc >= dcount) return;
+    const int q[3] = {centres[3 * dc], centres[3 * dc + 1], centres[3 * dc + 2]};
+    const int lz0 = (q[2] - 1) >> 1, nz = (q[2] >> 1) - lz0 + 1;  // leaf z of sz = 0 and sz = 1
+    int n = 0, neg = 0;
+    int32_t ids[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ids[k] = -1;
+#pragma unroll
+    for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+        for (int sy = 0; sy < 2; ++sy) {
+            const int cx = q[0] - 1 + sx, cy = q[1] - 1 + sy;
+            int32_t r;
+            const uint32_t m = grid_column(lg, (cx >> 1) - lg.origin[0], (cy >> 1) - lg.origin[1], lz0 - lg.origin[2],
+                                           nz, &r);
+#pragma unroll
+            for (int sz = 0; sz < 2; ++sz) {
+                const int cz = q[2] - 1 + sz;
+                const int k = (cz >> 1) - lz0;  // bit of the leaf in the column window
+                if (!((m >> k) & 1u)) continue;
+                const int32_t rank = r + __popc(m & ((1u << k) - 1u));
+                const int e = ((cx & 1) * 2 + (cy & 1)) * 2 + (cz & 1);
+                ids[octant_slot(sx, sy, sz)] = id_base + leaf_index[rank] * 8 + e;
+                ++n;
+            }
+        }
+    if (values) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) neg += (ids[k] >= 0 && values[ids[k]] < 0.f) ? 1 : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (ids[k] >= 0) acc[dc * 8 + k] = ids[k];
+    cnt[dc] = n | (neg << COUNT_BITS);
+}
+
 // Dual-cube assembly for leaves of stride 1 (the finest level): one thread per (leaf, lattice point
 // o in [0, nu]^3) registers every sub-cell of the leaf that has the point on its boundary, with one
 // aggregated count update. Per axis, o = 0 is covered by sub-cell 0 from the positive side, o = nu
@@ -742,6 +789,18 @@ int ngs_mark_dual_leaf_words(ngs_grid centres, ngs_grid leaf_bitmap, ngs_stream_
                 "ngs_mark_dual_leaf_words: stride-1 grids required");
     if (centres.nwords <= 0) return NGS_OK;
     mark_dual_leaf_words_kernel<<<grid1d(centres.nwords, 256), 256, 0, S(s)>>>(centres, leaf_bitmap);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_dual_gather_leaf_cubes(const int32_t* centre_coords, int64_t dc, ngs_grid leaf_bitmap, const int32_t* leaf_index,
+                               int32_t id_base, const float* values, int32_t* acc, int32_t* count, ngs_stream_t s) {
+    NGS_REQUIRE(leaf_bitmap.log2_stride == 0 && leaf_bitmap.bits && leaf_bitmap.rank && leaf_index && acc && count &&
+                    (dc == 0 || centre_coords),
+                "ngs_dual_gather_leaf_cubes: bad arguments");
+    if (dc <= 0) return NGS_OK;
+    dual_gather_leaf_cubes_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(centre_coords, dc, leaf_bitmap, leaf_index, id_base,
+                                                                    values, acc, count);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -327,6 +327,7 @@ class Reconstructor:
                   for o, dm in [(tree.grids[d].origin[a], tree.grids[d].dims[a])]) for a in range(3)]
         cg = SparseGrid(lo, hi, 1, dev)
         first = 0
+        lgrid = None
         if nu == 2 and tree.strides[0] == 1 and leaves[0].shape[0]:
             # finest leaves: centre words computed from a leaf bitmap (no atomics), before the others
             if 0 in parents:
@@ -343,7 +344,7 @@ class Reconstructor:
             if leaves[d].shape[0]:
                 call("ngs_mark_dual_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], tree.strides[d], nu,
                      stream())
-        DC = cg.finalize(emit=return_info)  # the centres' coordinates are only reported, never read
+        DC = cg.finalize(emit=return_info or lgrid is not None)  # coordinates read by the level-0 gather
         # dual corners = (primal + stride) * voxel_size / n_upsample, and chi there
         pos = torch.empty((nprim, 3), dtype=torch.float32, device=dev)
         scale = float(np.float32(tree.voxel_size / nu))
@@ -356,7 +357,17 @@ class Reconstructor:
         # word also counts the registered cells with a negative value
         acc = torch.empty((DC, 8), dtype=torch.int32, device=dev)
         cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
-        for d in range(L):
+        if lgrid is not None:
+            # finest leaves: gathered per centre (writes every count word), before the other levels
+            n0 = leaves[0].shape[0]
+            lgrid.finalize(emit=False)
+            ranks = torch.empty(n0, dtype=torch.int32, device=dev)
+            call("ngs_grid_rank_rows", lgrid.struct(), ptr(leaves[0]), n0, ptr(ranks), stream())
+            leaf_index = torch.empty(max(lgrid.count, 1), dtype=torch.int32, device=dev)
+            leaf_index[ranks.long()] = torch.arange(n0, dtype=torch.int32, device=dev)
+            call("ngs_dual_gather_leaf_cubes", ptr(cg.coords), DC, lgrid.struct(), ptr(leaf_index), bases_[0],
+                 ptr(values), ptr(acc), ptr(cnt), stream())
+        for d in range(first, L):
             if not nsub[d]:
                 continue
             if tree.strides[d] == 1:
