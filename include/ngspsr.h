@@ -407,13 +407,16 @@ int ngs_mark_dual_leaves(ngs_grid g, const int32_t *leaves, int64_t n, int32_t s
  * one bitmap word at a time with no atomics: leaf_bitmap holds the leaves (stride-1 grid), every word
  * of centres is overwritten, so call it before marking other levels' centres. */
 int ngs_mark_dual_leaf_words(ngs_grid centres, ngs_grid leaf_bitmap, ngs_stream_t s);
-/* The dual-cube assembly of the stride-1 leaves (n_upsample 2) gathered per dual centre with no atomics:
- * centre_coords (dc x 3, the centres in rank order, upsampled units), leaf_bitmap (finalized) holds the
- * leaves, leaf_index[rank in leaf_bitmap] = the leaf's position in the leaf list (ids id_base + 8
- * position + sub-cell, as ngs_dual_accumulate_leaves). Writes every centre's count word and its
- * level-0 slots, so it runs before the other levels' accumulation. */
-int ngs_dual_gather_leaf_cubes(const int32_t *centre_coords, int64_t dc, ngs_grid leaf_bitmap, const int32_t *leaf_index,
-                               int32_t id_base, const float *values, int32_t *acc, int32_t *count, ngs_stream_t s);
+/* The dual-cube assembly of all leaf levels for n_upsample 2, gathered per dual centre with no atomics
+ * (same slots and counts as the per-sub-cell accumulation when the leaves partition space, which the
+ * conformal leaves do): centre_coords (dc x 3, the centres in rank order, upsampled units); per level,
+ * fine to coarse (at most 8), a finalized bitmap of its leaves, a device map leaf_index[l][rank in
+ * that bitmap] = the leaf's position in the level's leaf list, and id_base[l] (ids id_base + 8
+ * position + sub-cell). leaf_bitmaps, leaf_index and id_base are host arrays. count: slots in bits
+ * 0..15, slots with a negative value in bits 16..31 (values nullable). */
+int ngs_dual_gather_cubes(const int32_t *centre_coords, int64_t dc, int32_t nlevels, const ngs_grid *leaf_bitmaps,
+                          const int32_t *const *leaf_index, const int32_t *id_base, const float *values, int32_t *acc,
+                          int32_t *count, ngs_stream_t s);
 int ngs_dual_accumulate_leaves(ngs_grid centres, const int32_t *leaves, int64_t n, int32_t n_upsample,
                                int32_t id_base, const float *values, int32_t *acc, int32_t *count, ngs_stream_t s);
 /* Dual-corner positions (primal + stride) * voxel_size / n_upsample (reconstructor.py:368-373). */

@@ -527,43 +527,80 @@ __global__ void mark_dual_leaf_words_kernel(ngs_grid cen, ngs_grid lg) {
     cen.bits[w] = out;
 }
 
-// Dual-cube assembly for the stride-1 leaves (n_upsample 2) gathered per dual centre, before any
-// other level: one thread per centre q (coordinates in rank order), per octant (sx, sy, sz) the
-// sub-cell c = q - 1 + s (upsampled units) of leaf c >> 1 covers it when that leaf is present. The
-// centre's count word (slots | negative-valued slots << 16) and its level-0 slots are written
-// directly; coarser levels add theirs with atomics afterwards. leaf_index maps a leaf's rank in the
-// leaf bitmap to its position in the leaf list (ids are id_base + 8 position + sub-cell).
-__global__ void dual_gather_leaf_cubes_kernel(const int32_t* __restrict__ centres, int64_t dcount, ngs_grid lg,
-                                              const int32_t* __restrict__ leaf_index, int id_base,
-                                              const float* __restrict__ values, int32_t* __restrict__ acc,
-                                              int32_t* __restrict__ cnt) {
+// Leaf levels of the gathered dual-cube assembly (n_upsample 2): per level the leaf bitmap (stride
+// s = 1 << log2_stride), the map from a leaf's bitmap rank to its position in the leaf list, and the
+// id of its first sub-cell.
+constexpr int MAX_LEAF_LEVELS = 8;
+struct LeafLevels {
+    ngs_grid g[MAX_LEAF_LEVELS];
+    const int32_t* index[MAX_LEAF_LEVELS];
+    int32_t base[MAX_LEAF_LEVELS];
+    int32_t n;
+};
+
+// Dual-cube assembly gathered per dual centre, n_upsample 2, no atomics. Sub-cells of a level-s leaf
+// L are [2 L + e s, 2 L + (e + 1) s] per axis in upsampled units (e in {0, 1}), so the octant
+// (sx, sy, sz) of centre q lies in the sub-cell with lower corner c = floor((q - 1 + s_a) / s) s per
+// axis, of leaf floor(c / 2s) s, sub-cell e = (c / s) mod 2. The leaves partition the covered space,
+// so each octant has at most one covering sub-cell, found at the finest level that has its leaf;
+// that is the set of registrations the per-sub-cell form makes (every centre lies on the boundary of
+// the sub-cells around it). Level 0 (stride 1) is read by bitmap columns. Writes each centre's count
+// word (slots | negative-valued slots << 16) and its slots.
+__global__ void dual_gather_cubes_kernel(const int32_t* __restrict__ centres, int64_t dcount, LeafLevels lv,
+                                         const float* __restrict__ values, int32_t* __restrict__ acc,
+                                         int32_t* __restrict__ cnt) {
     const int64_t dc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (dc >= dcount) return;
     const int q[3] = {centres[3 * dc], centres[3 * dc + 1], centres[3 * dc + 2]};
-    const int lz0 = (q[2] - 1) >> 1, nz = (q[2] >> 1) - lz0 + 1;  // leaf z of sz = 0 and sz = 1
-    int n = 0, neg = 0;
     int32_t ids[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) ids[k] = -1;
+    int n = 0;
+    // finest level by bitmap columns (when it is a stride-1 level)
+    int l0 = 0;
+    if (lv.n > 0 && lv.g[0].log2_stride == 0) {
+        l0 = 1;
+        const ngs_grid& lg = lv.g[0];
+        const int lz0 = (q[2] - 1) >> 1, nz = (q[2] >> 1) - lz0 + 1;  // leaf z of sz = 0 and sz = 1
 #pragma unroll
-    for (int sx = 0; sx < 2; ++sx)
+        for (int sx = 0; sx < 2; ++sx)
 #pragma unroll
-        for (int sy = 0; sy < 2; ++sy) {
-            const int cx = q[0] - 1 + sx, cy = q[1] - 1 + sy;
-            int32_t r;
-            const uint32_t m = grid_column(lg, (cx >> 1) - lg.origin[0], (cy >> 1) - lg.origin[1], lz0 - lg.origin[2],
-                                           nz, &r);
+            for (int sy = 0; sy < 2; ++sy) {
+                const int cx = q[0] - 1 + sx, cy = q[1] - 1 + sy;
+                int32_t r;
+                const uint32_t m = grid_column(lg, (cx >> 1) - lg.origin[0], (cy >> 1) - lg.origin[1],
+                                               lz0 - lg.origin[2], nz, &r);
 #pragma unroll
-            for (int sz = 0; sz < 2; ++sz) {
-                const int cz = q[2] - 1 + sz;
-                const int k = (cz >> 1) - lz0;  // bit of the leaf in the column window
-                if (!((m >> k) & 1u)) continue;
-                const int32_t rank = r + __popc(m & ((1u << k) - 1u));
-                const int e = ((cx & 1) * 2 + (cy & 1)) * 2 + (cz & 1);
-                ids[octant_slot(sx, sy, sz)] = id_base + leaf_index[rank] * 8 + e;
+                for (int sz = 0; sz < 2; ++sz) {
+                    const int cz = q[2] - 1 + sz;
+                    const int k = (cz >> 1) - lz0;  // bit of the leaf in the column window
+                    if (!((m >> k) & 1u)) continue;
+                    const int32_t rank = r + __popc(m & ((1u << k) - 1u));
+                    const int e = ((cx & 1) * 2 + (cy & 1)) * 2 + (cz & 1);
+                    ids[octant_slot(sx, sy, sz)] = lv.base[0] + lv.index[0][rank] * 8 + e;
+                    ++n;
+                }
+            }
+    }
+    // octants left: coarser levels, finest first
+    if (n < 8 && l0 < lv.n) {
+        for (int o = 0; o < 8; ++o) {
+            const int sx = (o >> 2) & 1, sy = (o >> 1) & 1, sz = o & 1;
+            const int sl = octant_slot(sx, sy, sz);
+            if (ids[sl] >= 0) continue;
+            for (int l = l0; l < lv.n; ++l) {
+                const int k = lv.g[l].log2_stride;
+                const int c[3] = {(q[0] - 1 + sx) >> k, (q[1] - 1 + sy) >> k, (q[2] - 1 + sz) >> k};  // in units of s
+                const int32_t rank = grid_find(lv.g[l], (c[0] >> 1) << k, (c[1] >> 1) << k, (c[2] >> 1) << k);
+                if (rank < 0) continue;
+                const int e = ((c[0] & 1) * 2 + (c[1] & 1)) * 2 + (c[2] & 1);
+                ids[sl] = lv.base[l] + lv.index[l][rank] * 8 + e;
                 ++n;
+                break;
             }
         }
+    }
+    int neg = 0;
     if (values) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) neg += (ids[k] >= 0 && values[ids[k]] < 0.f) ? 1 : 0;
@@ -793,14 +830,24 @@ int ngs_mark_dual_leaf_words(ngs_grid centres, ngs_grid leaf_bitmap, ngs_stream_
     return NGS_OK;
 }
 
-int ngs_dual_gather_leaf_cubes(const int32_t* centre_coords, int64_t dc, ngs_grid leaf_bitmap, const int32_t* leaf_index,
-                               int32_t id_base, const float* values, int32_t* acc, int32_t* count, ngs_stream_t s) {
-    NGS_REQUIRE(leaf_bitmap.log2_stride == 0 && leaf_bitmap.bits && leaf_bitmap.rank && leaf_index && acc && count &&
+int ngs_dual_gather_cubes(const int32_t* centre_coords, int64_t dc, int32_t nlevels, const ngs_grid* leaf_bitmaps,
+                          const int32_t* const* leaf_index, const int32_t* id_base, const float* values, int32_t* acc,
+                          int32_t* count, ngs_stream_t s) {
+    NGS_REQUIRE(nlevels >= 1 && nlevels <= MAX_LEAF_LEVELS && leaf_bitmaps && leaf_index && id_base && acc && count &&
                     (dc == 0 || centre_coords),
-                "ngs_dual_gather_leaf_cubes: bad arguments");
+                "ngs_dual_gather_cubes: bad arguments");
     if (dc <= 0) return NGS_OK;
-    dual_gather_leaf_cubes_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(centre_coords, dc, leaf_bitmap, leaf_index, id_base,
-                                                                    values, acc, count);
+    LeafLevels lv{};
+    lv.n = nlevels;
+    for (int l = 0; l < nlevels; ++l) {
+        NGS_REQUIRE(leaf_bitmaps[l].bits && leaf_bitmaps[l].rank && leaf_index[l], "ngs_dual_gather_cubes: null level");
+        NGS_REQUIRE(l == 0 || leaf_bitmaps[l].log2_stride > leaf_bitmaps[l - 1].log2_stride,
+                    "ngs_dual_gather_cubes: levels must go from fine to coarse");
+        lv.g[l] = leaf_bitmaps[l];
+        lv.index[l] = leaf_index[l];
+        lv.base[l] = id_base[l];
+    }
+    dual_gather_cubes_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(centre_coords, dc, lv, values, acc, count);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -115,7 +115,7 @@ _SIGNATURES = {
     "ngs_dual_accumulate": ([Grid, P, c_int64, c_int32, c_int32, P, P, P, P], c_int32),
     "ngs_mark_dual_leaves": ([Grid, P, c_int64, c_int32, c_int32, P], c_int32),
     "ngs_mark_dual_leaf_words": ([Grid, Grid, P], c_int32),
-    "ngs_dual_gather_leaf_cubes": ([P, c_int64, Grid, P, c_int32, P, P, P, P], c_int32),
+    "ngs_dual_gather_cubes": ([P, c_int64, c_int32, P, P, P, P, P, P, P], c_int32),
     "ngs_dual_accumulate_leaves": ([Grid, P, c_int64, c_int32, c_int32, P, P, P, P], c_int32),
     "ngs_dual_compact": ([P, P, c_int32, c_int64, P, P, P, P, P], c_int32),
     "ngs_dual_corner_positions": ([P, c_int64, c_int32, c_float, P, P], c_int32),

@@ -1,4 +1,6 @@
 """Screened Poisson solve and mesh extraction on the GPU (jspsr/core/reconstructor.py)."""
+import ctypes
+
 import numpy as np
 import torch
 
@@ -307,14 +309,16 @@ class Reconstructor:
             leaves[d] = out[:k]
         L = max_d
         nu = n_upsample
-        # primal sub-cells (leaf-major, nu^3 per leaf); listed only where read: the dual-cube assembly
-        # of coarser levels (stride-1 leaves are assembled from the leaves themselves) and the report
+        # n_upsample 2: the dual cubes are gathered per centre from per-level leaf bitmaps
+        gather = nu == 2 and L <= 8
+        # primal sub-cells (leaf-major, nu^3 per leaf); listed only where read: the per-sub-cell
+        # dual-cube assembly of coarser levels (without the gather) and the report
         primal, bases_, nsub, nprim = {}, {}, {}, 0
         for d in range(L):
             s = tree.strides[d]
             lf = leaves[d]
             nsub[d] = lf.shape[0] * nu ** 3
-            if s > 1 or return_info:
+            if (s > 1 and not gather) or return_info:
                 pr = torch.empty((nsub[d], 3), dtype=torch.int32, device=dev)
                 if lf.shape[0]:
                     call("ngs_expand_primal", ptr(lf), lf.shape[0], nu, s, ptr(pr), stream())
@@ -330,21 +334,14 @@ class Reconstructor:
         lgrid = None
         if nu == 2 and tree.strides[0] == 1 and leaves[0].shape[0]:
             # finest leaves: centre words computed from a leaf bitmap (no atomics), before the others
-            if 0 in parents:
-                po, pdm = parents[0].origin, parents[0].dims
-                llo, lhi = [2 * o for o in po], [2 * (o + dm - 1) + 1 for o, dm in zip(po, pdm)]
-            else:
-                g0 = tree.grids[0]
-                llo, lhi = list(g0.origin), [o + dm - 1 for o, dm in zip(g0.origin, g0.dims)]
-            lgrid = SparseGrid(llo, lhi, 1, dev)
-            lgrid.mark_coords(leaves[0])
+            lgrid = self._leaf_grid(0, leaves[0], parents)
             call("ngs_mark_dual_leaf_words", cg.struct(), lgrid.struct(), stream())
             first = 1
         for d in range(first, L):
             if leaves[d].shape[0]:
                 call("ngs_mark_dual_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], tree.strides[d], nu,
                      stream())
-        DC = cg.finalize(emit=return_info or lgrid is not None)  # coordinates read by the level-0 gather
+        DC = cg.finalize(emit=return_info or gather)  # coordinates read by the gathered assembly
         # dual corners = (primal + stride) * voxel_size / n_upsample, and chi there
         pos = torch.empty((nprim, 3), dtype=torch.float32, device=dev)
         scale = float(np.float32(tree.voxel_size / nu))
@@ -357,25 +354,42 @@ class Reconstructor:
         # word also counts the registered cells with a negative value
         acc = torch.empty((DC, 8), dtype=torch.int32, device=dev)
         cnt = torch.zeros(DC, dtype=torch.int32, device=dev)
-        if lgrid is not None:
-            # finest leaves: gathered per centre (writes every count word), before the other levels
-            n0 = leaves[0].shape[0]
-            lgrid.finalize(emit=False)
-            ranks = torch.empty(n0, dtype=torch.int32, device=dev)
-            call("ngs_grid_rank_rows", lgrid.struct(), ptr(leaves[0]), n0, ptr(ranks), stream())
-            leaf_index = torch.empty(max(lgrid.count, 1), dtype=torch.int32, device=dev)
-            leaf_index[ranks.long()] = torch.arange(n0, dtype=torch.int32, device=dev)
-            call("ngs_dual_gather_leaf_cubes", ptr(cg.coords), DC, lgrid.struct(), ptr(leaf_index), bases_[0],
-                 ptr(values), ptr(acc), ptr(cnt), stream())
-        for d in range(first, L):
-            if not nsub[d]:
-                continue
-            if tree.strides[d] == 1:
-                call("ngs_dual_accumulate_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], nu, bases_[d],
-                     ptr(values), ptr(acc), ptr(cnt), stream())
-            else:
-                call("ngs_dual_accumulate", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d],
-                     bases_[d], ptr(values), ptr(acc), ptr(cnt), stream())
+        if gather:
+            # n_upsample 2: every level's cubes gathered per centre from per-level leaf bitmaps (no atomics)
+            bitmaps, index, base = [], [], []
+            for d in range(L):
+                if not leaves[d].shape[0]:
+                    continue
+                if d == 0 and lgrid is not None:
+                    lg = lgrid
+                else:
+                    lg = self._leaf_grid(d, leaves[d], parents)
+                lg.finalize(emit=False)
+                n_l = leaves[d].shape[0]
+                ranks = torch.empty(n_l, dtype=torch.int32, device=dev)
+                call("ngs_grid_rank_rows", lg.struct(), ptr(leaves[d]), n_l, ptr(ranks), stream())
+                li = torch.empty(max(lg.count, 1), dtype=torch.int32, device=dev)
+                li[ranks.long()] = torch.arange(n_l, dtype=torch.int32, device=dev)
+                bitmaps.append(lg)
+                index.append(li)
+                base.append(bases_[d])
+            nl = len(bitmaps)
+            from ._lib import Grid
+            garr = (Grid * nl)(*[g.struct() for g in bitmaps])
+            iarr = (ctypes.c_void_p * nl)(*[t.data_ptr() for t in index])
+            barr = (ctypes.c_int32 * nl)(*base)
+            call("ngs_dual_gather_cubes", ptr(cg.coords), DC, nl, garr, iarr, barr, ptr(values), ptr(acc), ptr(cnt),
+                 stream())
+        else:
+            for d in range(L):
+                if not nsub[d]:
+                    continue
+                if tree.strides[d] == 1:
+                    call("ngs_dual_accumulate_leaves", cg.struct(), ptr(leaves[d]), leaves[d].shape[0], nu, bases_[d],
+                         ptr(values), ptr(acc), ptr(cnt), stream())
+                else:
+                    call("ngs_dual_accumulate", cg.struct(), ptr(primal[d]), primal[d].shape[0], tree.strides[d],
+                         bases_[d], ptr(values), ptr(acc), ptr(cnt), stream())
         # complete cubes whose corner values change sign (the only ones with triangles), in centre order
         cubes = torch.empty((DC, 8), dtype=torch.int32, device=dev)
         flags = torch.empty(DC, dtype=torch.int32, device=dev)
@@ -389,6 +403,21 @@ class Reconstructor:
             return v, f, dict(leaves=leaves, primal=primal, centres=cg.coords, acc=acc, count=cnt, cubes=cubes, pos=pos,
                               values=values)
         return v, f
+
+    def _leaf_grid(self, d, lf, parents):
+        """Bitmap of the level-d leaves (stride of level d): the children box of the level's parents,
+        or the level's own box for the last level."""
+        tree = self.hash_tree
+        s = tree.strides[d]
+        if d in parents:
+            po, pdm = parents[d].origin, parents[d].dims  # lattice units of stride 2s
+            lo, hi = [2 * o for o in po], [2 * (o + dm - 1) + 1 for o, dm in zip(po, pdm)]
+        else:
+            g = tree.grids[d]
+            lo, hi = list(g.origin), [o + dm - 1 for o, dm in zip(g.origin, g.dims)]
+        lg = SparseGrid(lo, hi, s, tree.device)
+        lg.mark_coords(lf)
+        return lg
 
     def leaf_corner_chi(self, leaves, bases_, nprim, nu, scale):
         """Raw chi at the dual corners of every leaf (the values evaluate_raw_chi gives at the
