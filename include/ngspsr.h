@@ -431,6 +431,12 @@ int ngs_leaf_corner_positions(const int32_t *leaves, int64_t n, int32_t stride, 
  * scale) at the position ngs_dual_corner_positions writes for corner e of leaf l, bit for bit; for
  * n_upsample 2 the 27-voxel windows of a leaf's corners are read together. leaf_stride: stride of
  * the leaves' level. */
+/* ngs_eval_leaf_corners for up to 8 field levels in one pass (grids, fields: host arrays of nlevels),
+ * summed in the given order: out = ((out or 0) + level 0 value) + level 1 value ..., the values of
+ * nlevels successive ngs_eval_leaf_corners calls. */
+int ngs_eval_leaf_corners_levels(int32_t nlevels, const ngs_grid *grids, const float *const *fields, const int32_t *leaves,
+                                 int64_t n, int32_t leaf_stride, int32_t n_upsample, float scale, float inv_voxel_size,
+                                 int32_t accumulate, float *out, ngs_stream_t s);
 int ngs_eval_leaf_corners(ngs_grid g, const float *field, const int32_t *leaves, int64_t n, int32_t leaf_stride,
                           int32_t n_upsample, float scale, float inv_voxel_size, int32_t accumulate, float *out,
                           ngs_stream_t s);

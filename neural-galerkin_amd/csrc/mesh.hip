@@ -259,12 +259,10 @@ __global__ void leaf_corner_pos_kernel(const int32_t* __restrict__ leaves, int64
 // the same or adjacent cells of the field level, the leaf's 4x4x4 coefficient window is read once
 // (16 bitmap columns) for all eight corners instead of 9 columns per corner; an absent voxel adds
 // +0, which leaves the sum unchanged.
-__global__ void eval_leaf_corners_kernel(ngs_grid g, const float* __restrict__ field, const int32_t* __restrict__ leaves,
-                                         int64_t n, int s_leaf, int nu, float scale, float inv_vs, int accumulate,
-                                         float* __restrict__ out) {
+__device__ __forceinline__ void eval_leaf_corners_one(const ngs_grid& g, const float* __restrict__ field,
+                                                      const int32_t* __restrict__ leaves, int64_t l, int s_leaf, int nu,
+                                                      float scale, float inv_vs, int accumulate, float* __restrict__ out) {
 #pragma clang fp contract(off)
-    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
     const int l2 = g.log2_stride;
     const int sd = 1 << l2;
     const float inv_sd = 1.0f / (float)sd;
@@ -364,6 +362,32 @@ __global__ void eval_leaf_corners_kernel(ngs_grid g, const float* __restrict__ f
             }
         out[o0 + e] = (accumulate ? out[o0 + e] : 0.f) + v;
     }
+}
+
+__global__ void eval_leaf_corners_kernel(ngs_grid g, const float* __restrict__ field, const int32_t* __restrict__ leaves,
+                                         int64_t n, int s_leaf, int nu, float scale, float inv_vs, int accumulate,
+                                         float* __restrict__ out) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) eval_leaf_corners_one(g, field, leaves, l, s_leaf, nu, scale, inv_vs, accumulate, out);
+}
+
+// Up to 8 field levels summed into the corner values in the given order, each exactly as one
+// eval_leaf_corners_kernel launch (out = out + level value): the same values as launching per level,
+// with the leaf reads and the output kept in cache between levels.
+constexpr int MAX_FIELD_LEVELS = 8;
+struct FieldLevels {
+    ngs_grid g[MAX_FIELD_LEVELS];
+    const float* field[MAX_FIELD_LEVELS];
+    int32_t n;
+};
+
+__global__ void eval_leaf_corners_levels_kernel(FieldLevels fl, const int32_t* __restrict__ leaves, int64_t n,
+                                                int s_leaf, int nu, float scale, float inv_vs, int accumulate,
+                                                float* __restrict__ out) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    for (int k = 0; k < fl.n; ++k)
+        eval_leaf_corners_one(fl.g[k], fl.field[k], leaves, l, s_leaf, nu, scale, inv_vs, accumulate || k > 0, out);
 }
 
 __device__ __forceinline__ int cube_case(const int32_t* ids, const float* __restrict__ val) {
@@ -869,6 +893,26 @@ int ngs_leaf_corner_positions(const int32_t* leaves, int64_t n, int32_t stride, 
     const int64_t tot = n * n_upsample * n_upsample * n_upsample;
     if (tot <= 0) return NGS_OK;
     leaf_corner_pos_kernel<<<grid1d(tot, 256), 256, 0, S(s)>>>(leaves, n, n_upsample, stride, scale, pos);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_eval_leaf_corners_levels(int32_t nlevels, const ngs_grid* grids, const float* const* fields, const int32_t* leaves,
+                                 int64_t n, int32_t leaf_stride, int32_t n_upsample, float scale, float inv_voxel_size,
+                                 int32_t accumulate, float* out, ngs_stream_t s) {
+    NGS_REQUIRE(nlevels >= 1 && nlevels <= MAX_FIELD_LEVELS && grids && fields && n_upsample >= 1 && leaf_stride >= 1 &&
+                    out && (n == 0 || leaves),
+                "ngs_eval_leaf_corners_levels: bad arguments");
+    if (n <= 0) return NGS_OK;
+    FieldLevels fl{};
+    fl.n = nlevels;
+    for (int k = 0; k < nlevels; ++k) {
+        NGS_REQUIRE(fields[k], "ngs_eval_leaf_corners_levels: null field");
+        fl.g[k] = grids[k];
+        fl.field[k] = fields[k];
+    }
+    eval_leaf_corners_levels_kernel<<<grid1d(n, 128), 128, 0, S(s)>>>(fl, leaves, n, leaf_stride, n_upsample, scale,
+                                                                      inv_voxel_size, accumulate, out);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -120,6 +120,8 @@ _SIGNATURES = {
     "ngs_dual_compact": ([P, P, c_int32, c_int64, P, P, P, P, P], c_int32),
     "ngs_dual_corner_positions": ([P, c_int64, c_int32, c_float, P, P], c_int32),
     "ngs_leaf_corner_positions": ([P, c_int64, c_int32, c_int32, c_float, P, P], c_int32),
+    "ngs_eval_leaf_corners_levels": ([c_int32, P, P, P, c_int64, c_int32, c_int32, c_float, c_float, c_int32, P, P],
+                                     c_int32),
     "ngs_eval_leaf_corners": ([Grid, P, P, c_int64, c_int32, c_int32, c_float, c_float, c_int32, P, P], c_int32),
     "ngs_mc_count": ([P, c_int64, P, P, P], c_int32),
     "ngs_mc_emit": ([P, c_int64, P, P, c_int32, P, P, P], c_int32),

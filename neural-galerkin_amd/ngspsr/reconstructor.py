@@ -5,7 +5,7 @@ import numpy as np
 import torch
 
 from . import solver as _solver
-from ._lib import PointBins, Scratch, call, lib, ptr, stream
+from ._lib import Grid, PointBins, Scratch, call, lib, ptr, stream
 from .bases import BezierTensorBasis, ensure_device_tables
 from .grid import SparseGrid
 from .hashtree import HashTree
@@ -374,7 +374,6 @@ class Reconstructor:
                 index.append(li)
                 base.append(bases_[d])
             nl = len(bitmaps)
-            from ._lib import Grid
             garr = (Grid * nl)(*[g.struct() for g in bitmaps])
             iarr = (ctypes.c_void_p * nl)(*[t.data_ptr() for t in index])
             barr = (ctypes.c_int32 * nl)(*base)
@@ -423,15 +422,17 @@ class Reconstructor:
         """Raw chi at the dual corners of every leaf (the values evaluate_raw_chi gives at the
         positions ngs_dual_corner_positions writes, level by level in the same order)."""
         tree = self.hash_tree
-        out = torch.zeros(nprim, dtype=torch.float32, device=tree.device)
-        for d, sol in self.solutions.items():
-            if sol.numel() == 0:
+        out = torch.empty(nprim, dtype=torch.float32, device=tree.device)
+        levels = [(d, sol) for d, sol in self.solutions.items() if sol.numel()]
+        if not levels:
+            return out.zero_()
+        garr = (Grid * len(levels))(*[tree.grids[d].struct() for d, _ in levels])
+        farr = (ctypes.c_void_p * len(levels))(*[sol.data_ptr() for _, sol in levels])
+        for dl, lf in leaves.items():
+            if lf.shape[0] == 0:
                 continue
-            for dl, lf in leaves.items():
-                if lf.shape[0] == 0:
-                    continue
-                call("ngs_eval_leaf_corners", tree.grids[d].struct(), ptr(sol), ptr(lf), lf.shape[0], tree.strides[dl],
-                     nu, scale, tree.inv_vs, 1, ptr(out[bases_[dl]:]), stream())
+            call("ngs_eval_leaf_corners_levels", len(levels), garr, farr, ptr(lf), lf.shape[0], tree.strides[dl], nu,
+                 scale, tree.inv_vs, 0, ptr(out[bases_[dl]:]), stream())
         return out
 
     @staticmethod
