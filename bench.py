@@ -105,6 +105,38 @@ def stencil_roofline(system, reps=50):
     return t_pts, b_pts, t_rows, b_rows
 
 
+def assembly_roofline(res, xyz, reps=10):
+    """Average duration of the finest-level assembly kernel (ngs_lhs_assemble_runs, screening with a
+    coarse field at the points) and its algorithmic bytes, the survey's per-level figure with the run
+    form's writes: 4 nnz (values) + 4 R (non-empty run words) + 4 M (inv_diag) + 8 M (rhs read) +
+    8 M (rhs write) + 12 M (coords) + per point 12 (cell) + 36 (basis) + 4 (coarse field), bin order.
+    The kernel is bound by its point-visit latency (27 rows visit every point), not by HBM."""
+    import torch
+    from ngspsr._lib import call, ptr, stream
+    from ngspsr.reconstructor import PointBinning
+    from ngspsr.solver import RunSystem
+    rec, tree = res.reconstructor, res.tree
+    M = tree.get_coords_size(None, 0)
+    bins = PointBinning(tree, xyz, 0, 4.0)
+    coarse = torch.randn(xyz.shape[0], device=xyz.device) * 0.1
+    P = RunSystem.plane_stride(M)
+    runs = torch.empty((25, P), dtype=torch.int32, device=xyz.device)
+    rvals = torch.empty((125, P), dtype=torch.float32, device=xyz.device)
+    inv_diag = torch.empty(M, dtype=torch.float32, device=xyz.device)
+    rhs = torch.zeros(M, dtype=torch.float64, device=xyz.device)
+    lv, pb = tree.level(0), bins.struct()
+    t = _event_time(lambda: call("ngs_lhs_assemble_runs", lv, pb, ptr(coarse), 1.0, 0.1, ptr(runs), ptr(rvals),
+                                 ptr(inv_diag), ptr(rhs), stream()), reps)
+    R = RunSystem(runs, rvals, inv_diag)
+    nnz, nruns = R.nnz(), int((R.counts() > 0).sum().item())
+    n = xyz.shape[0]
+    algo = 4 * nnz + 4 * nruns + (4 + 16 + 12) * M + 52 * n
+    return {"kernel": "lhs_assemble_kernel (finest level, run form, screening with a coarse field)", "bound": "latency",
+            "achieved": round(algo / t / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4), "launch_us": round(1e6 * t, 2),
+            "algorithmic_bytes": int(algo), "rows": int(M), "points": int(n)}
+
+
 def pcg_iteration_rate(system, iters=200):
     """Seconds per PCG iteration (3 kernels) on the finest system, tolerance 0 so every iteration runs."""
     import torch
@@ -374,6 +406,7 @@ def main():
                           "GB/s": round(iter_bytes / t_iter / 1e9, 1)},
         "roofline": roof,
         "csr_spmv": spmv_obj,
+        "assembly": assembly_roofline(res, xyz),
         "mesh": {"vertices": int(res.vertices.shape[0]), "triangles": int(res.triangles.shape[0])},
     }
     if not args.no_cpu:
