@@ -20,11 +20,11 @@ sys.path.insert(0, os.path.join(ROOT, "neural-galerkin_amd"))
 
 CONFIGS = {
     "c2": dict(workload="ellipsoids16-2M-vs512-L4-mixed1e-8", gen="ellipsoids", n=2_000_000, vs=1 / 512, depth=4,
-               min_density=1.0, solver="mixed@tol=1e-8@inner_tol=1e-4", sigma=0.002, kappa=200.0),
+               min_density=1.0, solver="refine@tol=1e-8@inner_tol=1e-4", sigma=0.002, kappa=200.0),
     "c1": dict(workload="torus-100k-vs128-L4-pcg1e-6", gen="torus", n=100_000, vs=1 / 128, depth=4, min_density=8.0,
                solver="pcg@tol=1e-6@maxiter=10000"),
     "c5": dict(workload="room40boxes-10M-vs1024-L4-mixed1e-7", gen="room", n=10_000_000, vs=1 / 1024, depth=4,
-               min_density=1.0, solver="mixed@tol=1e-7@inner_tol=1e-4", sigma=0.003),
+               min_density=1.0, solver="refine@tol=1e-7@inner_tol=1e-4", sigma=0.003),
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -251,7 +251,7 @@ def run_serving(n_req=64):
         return n_req / (time.perf_counter() - t0), lat
 
     def batched(mixed, workers, chunk):
-        srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, mixed=mixed, chunk=chunk)
+        srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, refine=mixed, chunk=chunk)
         srv.serve([serving.Request(-1 - i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data[:workers])])
         reqs = [serving.Request(i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data)]
         t0 = time.perf_counter()
@@ -264,7 +264,7 @@ def run_serving(n_req=64):
         return srv, n_req / (max(r.finish for r in reqs) - t0), reqs
 
     plain_rate, plain_lat = plain("pcg@tol=1e-6@maxiter=20000")
-    plain_mixed_rate, _ = plain("mixed@tol=1e-6@inner_tol=1e-4")
+    plain_mixed_rate, _ = plain("refine@tol=1e-6@inner_tol=1e-4")
     srv_m, mixed_rate, reqs_m = batched(True, 8, 8)
     mixed_res = max(r.max_rel_residual for r in reqs_m)
     srv_m.close()

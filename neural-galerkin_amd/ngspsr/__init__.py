@@ -49,7 +49,7 @@ def reconstruct(xyz, normal, voxel_size: float, depth: int = 4, min_density: flo
     :param min_density: density threshold of the adaptive levels
     :param screen_alpha: screening weight
     :param degree: basis degree (only 2, the reference's BezierTensorBasis, is available)
-    :param solver: solver string ("pcg@tol=1e-6@maxiter=10000", "mixed@tol=1e-8", ...); the default
+    :param solver: solver string ("pcg@tol=1e-6@maxiter=10000", "mixed", "cholmod", "refine@tol=1e-8", ...); the default
         keeps the reference's PCG tolerance (1e-5 of ||b||) with an iteration cap
     :param xyz_depth: optional imposed per-point levels (skips the density-derived assignment)
     :param operator: level operator form, "runs" (assembled matrix, one run of consecutive columns

@@ -224,14 +224,14 @@ class ReconstructionServer:
     """Continuous batching of reconstructions: admission by HBM footprint, concurrent pipelines,
     one shared batched PCG pool."""
 
-    def __init__(self, device="cuda:0", workers=8, hbm_budget=None, tol=1e-6, mixed=True, inner_tol=1e-4,
+    def __init__(self, device="cuda:0", workers=8, hbm_budget=None, tol=1e-6, refine=True, inner_tol=1e-4,
                  chunk=8, capacity_blocks=4096, use_graph=True):
         self.device = torch.device(device)
         torch.cuda.set_device(self.device)
         ensure_device_tables(self.device)
         free, total = torch.cuda.mem_get_info(self.device)
         self.budget = int(hbm_budget if hbm_budget is not None else 0.6 * free)
-        self.tol, self.mixed, self.inner_tol = tol, mixed, inner_tol
+        self.tol, self.refine, self.inner_tol = tol, refine, inner_tol
         self.pool = BatchedPCG(self.device, capacity_blocks=capacity_blocks, chunk=chunk, use_graph=use_graph)
         self.workers = workers
         self.in_use = 0
@@ -240,7 +240,7 @@ class ReconstructionServer:
 
     # ---------------------------------------------------------------------- solves via the pool
     def _solve(self, system, rhs):
-        if not self.mixed:
+        if not self.refine:
             x, info = self.pool.solve(system, rhs, tol=self.tol)
             return x, info
         b64 = rhs.double()

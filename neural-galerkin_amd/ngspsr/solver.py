@@ -2,11 +2,18 @@
 
 Solver strings keep the reference's mini-language, "pcg@tol=1e-6@maxiter=1000@res_fix=False"
 (solver.py:180-186), parsed without eval. Methods:
-  pcg    Jacobi-PCG in FP32 with FP64 reductions; stop when ||r|| <= tol ||b|| (recurrence residual)
-         or at maxiter (solve_pcg.cu:15-136 semantics; maxiter < 0 means uncapped in the reference,
-         capped here at 10^7 with an error).
-  mixed  FP64 iterative refinement around FP32 inner PCG solves: r = b - A x in FP64, inner
-         solve A e = r to inner_tol, x += e in FP64, until ||b - A x|| / ||b|| <= tol.
+  pcg      Jacobi-PCG in FP32 with FP64 reductions; stop when ||r|| <= tol ||b|| (recurrence residual)
+           or at maxiter (solve_pcg.cu:15-136 semantics; maxiter < 0 means uncapped in the reference,
+           capped here at 10^7 with an error).
+  mixed    the reference's "mixed" (solver.py:124-130,160-169): PCG with tol 1e-4 and at most 1000
+           iterations; when the absolute residual ||A x - b|| exceeds tol (1e-2) the solve falls
+           back to "cholmod".
+  cholmod  the reference's direct sparse solve (solve_cusparse.cu:15-42). There is no sparse
+           direct solver on this stack; it is replaced by "refine" to a relative residual of 1e-10,
+           which is as accurate as an FP32 direct solve of these well-conditioned systems.
+  refine   FP64 iterative refinement around FP32 inner PCG solves (no reference counterpart):
+           r = b - A x in FP64, inner solve A e = r to inner_tol, x += e in FP64, until
+           ||b - A x|| / ||b|| <= tol.
 """
 import ast
 import ctypes
@@ -17,8 +24,10 @@ from . import _lib
 from ._lib import NgsError, call, lib, ptr, stream
 
 DEFAULTS = {"pcg": {"tol": 1e-5, "maxiter": -1, "res_fix": False, "verbose": False, "chunk": 32},
-            "mixed": {"tol": 1e-8, "maxiter": 100000, "inner_tol": 1e-4, "max_outer": 20, "verbose": False,
-                      "chunk": 32}}
+            "mixed": {"tol": 1e-2, "fast_tol": 1e-4, "fast_maxiter": 1000, "verbose": False, "chunk": 32},
+            "cholmod": {"tol": 1e-10, "verbose": False, "chunk": 32},
+            "refine": {"tol": 1e-8, "maxiter": 100000, "inner_tol": 1e-4, "max_outer": 20, "verbose": False,
+                       "chunk": 32}}
 
 
 def parse_solver(spec: str):
@@ -269,6 +278,7 @@ class StencilSystem:
 class SolveInfo:
     def __init__(self, iters=0, rel_resid=0.0, outer=0, true_rel_resid=None):
         self.iters, self.rel_resid, self.outer, self.true_rel_resid = iters, rel_resid, outer, true_rel_resid
+        self.fallback = False  # "mixed": the direct-solve stand-in answered
 
     def __repr__(self):
         return (f"SolveInfo(iters={self.iters}, rel_resid={self.rel_resid:.3e}, outer={self.outer}, "
@@ -302,7 +312,7 @@ def pcg(system, b, tol=1e-5, maxiter=-1, chunk=32, x_out=None, raise_on_cap=True
     return x, SolveInfo(it.value, rr.value)
 
 
-def mixed(system, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, chunk=32):
+def refine(system, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, chunk=32):
     """FP64 iterative refinement with FP32 PCG inner solves. Returns (x64, SolveInfo)."""
     M = system.M
     b64 = b.double().contiguous()
@@ -325,14 +335,42 @@ def mixed(system, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, chu
     return x64, info
 
 
+def cholmod(system, b, tol=1e-10, chunk=32):
+    """Stand-in for the reference's direct solve (solve_cholmod, solver.py:147-157): refinement to a
+    relative residual of tol. Returns (x float32, SolveInfo)."""
+    x64, info = refine(system, b, tol=tol, chunk=chunk)
+    return x64.float(), info
+
+
+def mixed(system, b, tol=1e-2, fast_tol=1e-4, fast_maxiter=1000, chunk=32):
+    """The reference's "mixed" solve (solve_mixed with solve_pcg(maxiter 1000, tol 1e-4) and the
+    cholmod fallback, solver.py:124-130,160-169): PCG first; if ||A x - b|| > tol (absolute), the
+    direct-solve stand-in. SolveInfo.fallback tells which one answered. Returns (x float32, SolveInfo)."""
+    x, info = pcg(system, b, tol=fast_tol, maxiter=fast_maxiter, chunk=chunk, raise_on_cap=False)
+    if system.M == 0:
+        return x, info
+    _, rr, _ = system.residual_f64(b.double(), x.double())
+    if rr ** 0.5 > tol:
+        x, info2 = cholmod(system, b, chunk=chunk)
+        info2.iters += info.iters
+        info2.fallback = True
+        return x, info2
+    return x, info
+
+
 def solve(system, b, spec: str = "pcg"):
     """Dispatch on a solver string. Returns (x float32, SolveInfo)."""
     name, conf = parse_solver(spec)
     if name == "pcg":
         x, info = pcg(system, b, tol=conf["tol"], maxiter=conf["maxiter"], chunk=conf["chunk"])
+    elif name == "mixed":
+        x, info = mixed(system, b, tol=conf["tol"], fast_tol=conf["fast_tol"], fast_maxiter=conf["fast_maxiter"],
+                        chunk=conf["chunk"])
+    elif name == "cholmod":
+        x, info = cholmod(system, b, tol=conf["tol"], chunk=conf["chunk"])
     else:
-        x64, info = mixed(system, b, tol=conf["tol"], inner_tol=conf["inner_tol"], max_outer=conf["max_outer"],
-                          maxiter=conf["maxiter"], chunk=conf["chunk"])
+        x64, info = refine(system, b, tol=conf["tol"], inner_tol=conf["inner_tol"], max_outer=conf["max_outer"],
+                           maxiter=conf["maxiter"], chunk=conf["chunk"])
         x = x64.float()
     if conf.get("verbose"):
         print("PCG Iteration ended in", info.iters)

@@ -62,3 +62,20 @@ def test_run_form_rejects_rows_beyond_run_words():
     assert b"too many rows" in lib.ngs_last_error()
     assert lib.ngs_residual_f64_runs(fake, fake, fake, fake, fake, M, fake, fake, None) == 1
     assert b"too many rows" in lib.ngs_last_error()
+
+
+def test_solver_strings():
+    """The reference's solver mini-language (solver.py:180-186) and its modes; values are parsed as
+    literals, never evaluated."""
+    from ngspsr.solver import parse_solver
+    assert parse_solver("pcg@tol=1e-6@maxiter=1000") == ("pcg", {"tol": 1e-6, "maxiter": 1000, "res_fix": False,
+                                                                 "verbose": False, "chunk": 32})
+    name, conf = parse_solver("mixed")
+    assert name == "mixed" and (conf["tol"], conf["fast_tol"], conf["fast_maxiter"]) == (1e-2, 1e-4, 1000)
+    assert parse_solver("cholmod")[1]["tol"] == 1e-10
+    name, conf = parse_solver("refine@tol=1e-9@inner_tol=1e-3")
+    assert name == "refine" and conf["tol"] == 1e-9 and conf["inner_tol"] == 1e-3
+    with pytest.raises(NotImplementedError):
+        parse_solver("lu")
+    with pytest.raises(ValueError):
+        parse_solver("pcg@tol=__import__('os')")

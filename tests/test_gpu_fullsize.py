@@ -39,7 +39,7 @@ def test_full_size_properties(cuda, cfg):
         vs, mind, solver, tol = 1 / 128, 8.0, "pcg@tol=1e-6@maxiter=10000", 1e-6
     else:
         xyz, nrm = synthetic.ellipsoids(2000000, k=16, sigma=0.002, kappa=200.0, seed=0)
-        vs, mind, solver, tol = 1 / 512, 1.0, "mixed@tol=1e-8@inner_tol=1e-4", 1e-8
+        vs, mind, solver, tol = 1 / 512, 1.0, "refine@tol=1e-8@inner_tol=1e-4", 1e-8
     xyz_t, nrm_t = torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda)
     res = ngspsr.reconstruct(xyz_t, nrm_t, vs, depth=4, min_density=mind, solver=solver, return_result=True)
     tree = res.tree
@@ -68,7 +68,7 @@ def test_c1_solution_error_vs_fp64_oracle(cuda):
     xyz, nrm = synthetic.torus(100000, seed=0)
     xyz_t, nrm_t = torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda)
     out = {}
-    for name, solver in (("fp32", "pcg@tol=1e-6@maxiter=10000"), ("mixed", "mixed@tol=1e-8@inner_tol=1e-4")):
+    for name, solver in (("fp32", "pcg@tol=1e-6@maxiter=10000"), ("mixed", "refine@tol=1e-8@inner_tol=1e-4")):
         res = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 128, depth=4, min_density=8.0, solver=solver, return_result=True)
         system, rhs = res.reconstructor.systems[0]
         a = system.to_scipy()

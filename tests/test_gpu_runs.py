@@ -120,7 +120,7 @@ def test_runs_solve_matches_csr_solve(cuda):
         assert ir.rel_resid <= 1e-7 * 1.0001
         err = float((xr - xc).norm() / xc.norm())
         assert err < 1e-4, (d, err)
-        x64, info = solver.mixed(R, b, tol=1e-10, inner_tol=1e-4)
+        x64, info = solver.refine(R, b, tol=1e-10, inner_tol=1e-4)
         assert info.true_rel_resid <= 1e-10, (d, info)
         # the FP64 residual of the run form equals that of the CSR form (same FP32 entries)
         _, rr, bb = A.residual_f64(b.double(), x64)
@@ -158,3 +158,29 @@ def test_pool_mixes_runs_csr_and_stencil(cuda):
         torch.cuda.synchronize()
         assert ib.iters == info.iters
         assert torch.equal(xb, xs), float((xb - xs).abs().max())
+
+
+def test_reference_solver_modes(cuda):
+    """"mixed" is the reference's: PCG to 1e-4 (at most 1000 iterations), kept when the absolute
+    residual is at most 1e-2, otherwise the direct-solve stand-in ("cholmod": refinement to a
+    relative residual of 1e-10)."""
+    from ngspsr import solver
+    print("[case] mixed / cholmod modes", flush=True)
+    tree, recon, bins, xyz_t = _setup(cuda, False, seed=17)
+    M = tree.get_coords_size(None, 0)
+    A = recon.assemble_level(0, bins[0], None, 1.0, 0.1, torch.zeros(M, dtype=torch.float64, device=cuda), form="runs")
+    g = torch.Generator(device="cpu").manual_seed(2)
+    b = torch.randn(M, generator=g).to(cuda)
+    small = b * (1e-3 / float(b.norm()))  # ||b|| = 1e-3: PCG at 1e-4 leaves ||Ax - b|| ~1e-7
+    x, info = solver.solve(A, small, "mixed")
+    assert not info.fallback and 0 < info.iters <= 1000
+    _, rr, bb = A.residual_f64(small.double(), x.double())
+    assert rr ** 0.5 <= 1e-2 and (rr / bb) ** 0.5 <= 1e-4 * 1.01
+    large = b * (1e6 / float(b.norm()))  # ||b|| = 1e6: 1e-4 relative is far above 1e-2 absolute
+    x, info = solver.solve(A, large, "mixed")
+    assert info.fallback
+    _, rr, bb = A.residual_f64(large.double(), x.double())
+    assert (rr / bb) ** 0.5 < 1e-5  # FP32 result of a 1e-10 solve
+    xc, infoc = solver.solve(A, large, "cholmod")
+    assert infoc.true_rel_resid <= 1e-10
+    assert torch.equal(xc, x)

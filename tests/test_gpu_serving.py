@@ -53,7 +53,7 @@ def test_alone_equals_in_the_mix(cuda):
     print("[case] serving alone vs mix", flush=True)
     data = _requests(cuda)
     alone = []
-    srv = serving.ReconstructionServer(cuda, workers=1, tol=1e-6, mixed=True)
+    srv = serving.ReconstructionServer(cuda, workers=1, tol=1e-6, refine=True)
     try:
         for i, (x, n, vs) in enumerate(data):
             r = serving.Request(i, x, n, vs, depth=4, min_density=1.0)
@@ -62,7 +62,7 @@ def test_alone_equals_in_the_mix(cuda):
             alone.append(r)
     finally:
         srv.close()
-    srv = serving.ReconstructionServer(cuda, workers=4, tol=1e-6, mixed=True)
+    srv = serving.ReconstructionServer(cuda, workers=4, tol=1e-6, refine=True)
     try:
         mix = [serving.Request(i, x, n, vs, depth=4, min_density=1.0) for i, (x, n, vs) in enumerate(data)]
         srv.serve(mix)
@@ -84,7 +84,7 @@ def test_admission_by_footprint(cuda):
     print("[case] admission", flush=True)
     data = _requests(cuda, k=3)
     fp = max(serving.estimate_footprint(x.shape[0], vs) for x, _, vs in data)
-    srv = serving.ReconstructionServer(cuda, workers=4, hbm_budget=fp, tol=1e-6, mixed=False)
+    srv = serving.ReconstructionServer(cuda, workers=4, hbm_budget=fp, tol=1e-6, refine=False)
     peak = {"v": 0}
     orig = srv.run_request
 

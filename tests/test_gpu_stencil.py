@@ -86,7 +86,7 @@ def test_stencil_solve_matches_csr_solve(cuda):
         assert (rr / bb) ** 0.5 < 2e-6, d
         err = float((xs - xc).norm() / xc.norm())
         assert err < 1e-4, (d, err)
-        x64, info = solver.mixed(S, b, tol=1e-10, inner_tol=1e-4)
+        x64, info = solver.refine(S, b, tol=1e-10, inner_tol=1e-4)
         assert info.true_rel_resid <= 1e-10, (d, info)
 
 
@@ -128,9 +128,9 @@ def test_reconstruct_operator_forms_agree(cuda, operator):
     print(f"[case] reconstruct operator={operator}", flush=True)
     xyz, nrm = synthetic.torus(30000, seed=2)
     xyz_t, nrm_t = torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda)
-    ref = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 64, depth=4, min_density=4.0, solver="mixed@tol=1e-9",
+    ref = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 64, depth=4, min_density=4.0, solver="refine@tol=1e-9",
                              operator="csr", return_result=True)
-    got = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 64, depth=4, min_density=4.0, solver="mixed@tol=1e-9",
+    got = ngspsr.reconstruct(xyz_t, nrm_t, 1 / 64, depth=4, min_density=4.0, solver="refine@tol=1e-9",
                              operator=operator, return_result=True)
     # the mixed solver converges each form to its own FP64 operator: the CSR one uses the assembled
     # FP32 entries (FP32 sums over the points), the stencil one the exact terms; the two operators
