@@ -137,6 +137,9 @@ int ngs_splat(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size, con
  * the derivative with respect to q (finest-voxel units), as evaluate_derivative returns it. */
 int ngs_eval_field(ngs_grid g, const float *field, const float *xyz, int64_t n, float inv_voxel_size,
                    float scale, int32_t accumulate, float *out, float *grad_out, ngs_stream_t s);
+/* Support of one level's field at the queries (hashtree.py:186-189, compute_mask): mask[p] = 1 (uint8,
+ * caller-zeroed; other entries untouched) when any voxel of g is among the 27 that ngs_eval_field sums. */
+int ngs_field_support(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size, uint8_t *mask, ngs_stream_t s);
 /* xyz_depth of build_hierarchy_adaptive (_hash_backend.py:428-429) from densities. */
 int ngs_point_levels(const float *density, int64_t n, float min_density, float log_base,
                      int32_t depth, int32_t *levels, ngs_stream_t s);

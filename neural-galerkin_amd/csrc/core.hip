@@ -479,6 +479,30 @@ __global__ void eval_field_kernel(ngs_grid g, const float* __restrict__ field, c
     }
 }
 
+// Whether any voxel of g lies among the 27 around the cell of each query (the support test of
+// eval_field_kernel): mask[i] |= 1.
+__global__ void field_support_kernel(ngs_grid g, const float* __restrict__ xyz, int64_t n, float inv_vs,
+                                     uint8_t* __restrict__ mask) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = 1 << g.log2_stride;
+    const float inv_s = 1.0f / (float)s;
+    int cell[3];
+    float b[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) axis_basis(xyz[3 * i + a] * inv_vs, s, inv_s, &cell[a], b);
+    const int l2 = g.log2_stride;
+    const int lx = (cell[0] >> l2) - g.origin[0], ly = (cell[1] >> l2) - g.origin[1], lz = (cell[2] >> l2) - g.origin[2];
+    uint32_t any = 0;
+    for (int kx = 0; kx < 3; ++kx)
+        for (int ky = 0; ky < 3; ++ky) {
+            int32_t r;
+            any |= grid_column(g, lx + kx - 1, ly + ky - 1, lz - 1, 3, &r);
+        }
+    if (any) mask[i] = 1;
+}
+
 __global__ void levels_kernel(const float* __restrict__ rho, int64_t n, float inv_min, float inv_ln, int depth,
                               int32_t* __restrict__ lv) {
 #pragma clang fp contract(off)
@@ -742,6 +766,14 @@ int ngs_eval_field(ngs_grid g, const float* field, const float* xyz, int64_t n, 
                    int32_t accumulate, float* out, float* grad_out, ngs_stream_t s) {
     if (n <= 0) return NGS_OK;
     eval_field_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, field, xyz, n, inv_voxel_size, scale, accumulate, out, grad_out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_field_support(ngs_grid g, const float* xyz, int64_t n, float inv_voxel_size, uint8_t* mask, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    NGS_REQUIRE(mask && xyz, "ngs_field_support: null query or mask pointer");
+    field_support_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, xyz, n, inv_voxel_size, mask);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

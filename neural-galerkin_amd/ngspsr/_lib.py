@@ -67,6 +67,7 @@ _SIGNATURES = {
     "ngs_scan_i32": ([P, c_int64, P, P, P], c_int32),
     "ngs_splat": ([Grid, P, c_int64, c_float, P, c_int32, P, P, c_int32, c_double, P, P], c_int32),
     "ngs_eval_field": ([Grid, P, P, c_int64, c_float, c_float, c_int32, P, P, P], c_int32),
+    "ngs_field_support": ([Grid, P, c_int64, c_float, P, P], c_int32),
     "ngs_point_levels": ([P, c_int64, c_float, c_float, c_int32, P, P], c_int32),
     "ngs_point_basis": ([P, c_int64, c_float, c_int32, P, P, P], c_int32),
     "ngs_bucket_workspace_bytes": ([c_int64, c_int64], c_int64),

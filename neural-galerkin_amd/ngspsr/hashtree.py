@@ -183,9 +183,14 @@ class HashTree:
         f = feat.float().contiguous()
         out = torch.empty(n, dtype=torch.float32, device=self.device)
         grad = torch.empty((n, 3), dtype=torch.float32, device=self.device) if compute_grad else None
-        call("ngs_eval_field", self.grids[feat_depth].struct(), ptr(f), ptr(q), n, self.inv_vs, 1.0, 0, ptr(out),
-             ptr(grad), stream())
-        return grad if compute_grad else out
+        g = self.grids[feat_depth].struct()
+        call("ngs_eval_field", g, ptr(f), ptr(q), n, self.inv_vs, 1.0, 0, ptr(out), ptr(grad), stream())
+        val = grad if compute_grad else out
+        if compute_mask:
+            mask = torch.zeros(n, dtype=torch.bool, device=self.device)
+            call("ngs_field_support", g, ptr(q), n, self.inv_vs, ptr(mask), stream())
+            return val, mask
+        return val
 
     # ------------------------------------------------------------------ neighbour COO (API)
     def get_neighbours_data(self, source_coords, source_stride, target_depth, nn_kernel, branch=None,

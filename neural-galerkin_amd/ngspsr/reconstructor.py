@@ -223,20 +223,30 @@ class Reconstructor:
                                             for (d, s), (kd, ks, kv) in zip(cur, key))
 
     # -------------------------------------------------------------------------- field
-    def evaluate_raw_chi(self, xyz, compute_grad=False, depths=None):
+    def evaluate_raw_chi(self, xyz, compute_mask: bool = False, compute_grad: bool = False, depths=None):
+        """reconstructor.py:197-226: chi at metric positions xyz (N, 3), summed over the solved levels
+        in solve order (restricted to `depths` when given). With compute_grad, the gradient (N, 3) per
+        finest-voxel unit in place of chi (bezier_tensor.py:62-72). With compute_mask, also the (N,)
+        bool mask of queries that some level's basis covers (a voxel among the 27 around the query's
+        cell at any evaluated level; hashtree.py:186-190)."""
+        assert len(self.solutions) > 0, "Please run solver before evaluation."
         tree = self.hash_tree
         q = xyz.float().contiguous()
         n = q.shape[0]
         out = torch.zeros(n, dtype=torch.float32, device=tree.device)
         grad = torch.zeros((n, 3), dtype=torch.float32, device=tree.device) if compute_grad else None
+        mask = torch.zeros(n, dtype=torch.bool, device=tree.device) if compute_mask else None
         for d, sol in self.solutions.items():
             if depths is not None and d not in depths:
                 continue
             if sol.numel() == 0:
                 continue
-            call("ngs_eval_field", tree.grids[d].struct(), ptr(sol), ptr(q), n, tree.inv_vs, 1.0, 1, ptr(out), ptr(grad),
-                 stream())
-        return (out, grad) if compute_grad else out
+            g = tree.grids[d].struct()
+            call("ngs_eval_field", g, ptr(sol), ptr(q), n, tree.inv_vs, 1.0, 1, ptr(out), ptr(grad), stream())
+            if compute_mask:
+                call("ngs_field_support", g, ptr(q), n, tree.inv_vs, ptr(mask), stream())
+        val = grad if compute_grad else out
+        return (val, mask) if compute_mask else val
 
     def get_mean_chi(self):
         """reconstructor.py:228-236: level set = sample-weighted mean of chi at the input points."""
@@ -253,9 +263,22 @@ class Reconstructor:
         s = sums.cpu()
         return float(s[0] / s[1])
 
-    def evaluate_chi(self, xyz, max_points: int = -1):
+    def evaluate_chi(self, xyz, compute_mask: bool = False, max_points: int = -1, depths=None):
+        """reconstructor.py:238-255: chi minus the level set, evaluated in chunks of at most
+        max_points queries (all at once when max_points <= 0); with compute_mask also the cover mask."""
         mean = self.get_mean_chi()
-        return self.evaluate_raw_chi(xyz) - mean
+        n = xyz.shape[0]
+        step = n if max_points <= 0 else max_points
+        vals, masks = [], []
+        for a in range(0, max(n, 1), max(step, 1)):
+            r = self.evaluate_raw_chi(xyz[a:a + step], compute_mask=compute_mask, depths=depths)
+            vals.append(r[0] if compute_mask else r)
+            if compute_mask:
+                masks.append(r[1])
+        val = (vals[0] if len(vals) == 1 else torch.cat(vals)) - mean
+        if compute_mask:
+            return val, (masks[0] if len(masks) == 1 else torch.cat(masks))
+        return val
 
     # -------------------------------------------------------------------------- mesh
     def extract_multiscale_mesh(self, n_upsample: int = 1, max_depth: int = 100, expand: int = 0, trim: bool = False,

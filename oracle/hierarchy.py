@@ -206,14 +206,27 @@ class Hierarchy:
             return np.bincount(tid[found], weights=ad[found], minlength=M)
         return np.stack([np.bincount(tid[found], weights=ad[found, c], minlength=M) for c in range(ad.shape[1])], 1)
 
-    # hashtree.py:157-192 (Bezier basis, no features)
-    def evaluate_interpolated(self, query_pos, d, feat):
+    # hashtree.py:157-192 (Bezier basis, no features); the gradient is bezier_tensor.py:62-72 (per
+    # finest-voxel unit), the mask marks queries with any voxel in their 27-neighbourhood.
+    def evaluate_interpolated(self, query_pos, d, feat, compute_mask=False, compute_grad=False):
         pos = scaled_positions(query_pos, self.voxel_size) - np.float32(0.5)
-        sid, tid, nt, _ = coords_neighbours(self.index[d], self.strides[d], pos, 1, range_kernel(3),
-                                            float_source=True)
-        qc = -nt / self.strides[d]
-        val = bezier.B(qc[:, 0]) * bezier.B(qc[:, 1]) * bezier.B(qc[:, 2]) * np.asarray(feat, np.float64)[tid]
-        return np.bincount(sid, weights=val, minlength=pos.shape[0])
+        s = self.strides[d]
+        sid, tid, nt, _ = coords_neighbours(self.index[d], s, pos, 1, range_kernel(3), float_source=True)
+        qc = -nt / s
+        f = np.asarray(feat, np.float64)[tid]
+        n = pos.shape[0]
+        b = [bezier.B(qc[:, a]) for a in range(3)]
+        if compute_grad:
+            db = [bezier.dB(qc[:, a]) / s for a in range(3)]
+            terms = (db[0] * b[1] * b[2], b[0] * db[1] * b[2], b[0] * b[1] * db[2])
+            val = np.stack([np.bincount(sid, weights=t * f, minlength=n) for t in terms], 1)
+        else:
+            val = np.bincount(sid, weights=b[0] * b[1] * b[2] * f, minlength=n)
+        if compute_mask:
+            mask = np.zeros(n, bool)
+            mask[sid] = True
+            return val, mask
+        return val
 
     def point_basis(self, query_pos, d):
         """Screening pre-evaluation (reconstructor.py:111-120): (pts_ids, vx_ids, values)."""

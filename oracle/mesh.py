@@ -30,12 +30,22 @@ class RowLookup:
         return np.where(self.keys[pos] == k, self.order[pos], -1)
 
 
-def evaluate_raw_chi(h: Hierarchy, solutions: dict, xyz):
-    """reconstructor.py:197-226."""
-    val = np.zeros(np.asarray(xyz).shape[0])
+def evaluate_raw_chi(h: Hierarchy, solutions: dict, xyz, compute_mask=False, compute_grad=False, depths=None):
+    """reconstructor.py:197-226: chi (N,) or, with compute_grad, its gradient (N, 3); with
+    compute_mask also the mask of queries covered by some evaluated level."""
+    n = np.asarray(xyz).shape[0]
+    val = np.zeros((n, 3) if compute_grad else n)
+    mask = np.zeros(n, bool)
     for d, sol in solutions.items():
-        val += h.evaluate_interpolated(xyz, d, sol)
-    return val
+        if depths is not None and d not in depths:
+            continue
+        r = h.evaluate_interpolated(xyz, d, sol, compute_mask, compute_grad)
+        if compute_mask:
+            val += r[0]
+            mask |= r[1]
+        else:
+            val += r
+    return (val, mask) if compute_mask else val
 
 
 def mean_chi(h, solutions, xyz, sample_weight):

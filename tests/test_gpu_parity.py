@@ -227,3 +227,44 @@ def test_leaf_corner_chi_equals_pointwise(pair, nu):
     torch.cuda.synchronize()
     assert info["values"].shape == ref.shape
     assert torch.equal(info["values"], ref), int((info["values"] != ref).sum())
+
+
+def test_field_queries_match_oracle(pair):
+    """evaluate_raw_chi (value, gradient, cover mask, level subset), get_mean_chi and the chunked
+    evaluate_chi against the oracle's restatement on the GPU solutions (reconstructor.py:197-255)."""
+    print("[case] field queries", flush=True)
+    rec = pair["res"].reconstructor
+    h, mesh = pair["h"], pair["mesh"]
+    sols = {d: rec.solutions[d].cpu().numpy().astype(np.float64) for d in rec.solutions}
+    rng = np.random.default_rng(3)
+    q = rng.uniform(0.0, 1.0, (6000, 3)).astype(np.float32)  # covered and uncovered queries
+    qt = torch.from_numpy(q).cuda()
+    v, m = rec.evaluate_raw_chi(qt, compute_mask=True)
+    g, m_g = rec.evaluate_raw_chi(qt, compute_mask=True, compute_grad=True)
+    g_only = rec.evaluate_raw_chi(qt, compute_grad=True)
+    torch.cuda.synchronize()
+    ref_v, ref_m = mesh.evaluate_raw_chi(h, sols, q, compute_mask=True)
+    ref_g = mesh.evaluate_raw_chi(h, sols, q, compute_grad=True)
+    assert m.dtype == torch.bool and np.array_equal(m.cpu().numpy(), ref_m)
+    assert 0 < ref_m.sum() < ref_m.size
+    assert torch.equal(m, m_g) and torch.equal(g, g_only) and g.shape == (q.shape[0], 3)
+    # FP32 sums of at most 27 terms per level against FP64
+    np.testing.assert_allclose(v.cpu().numpy(), ref_v, atol=2e-6 * np.abs(ref_v).max())
+    np.testing.assert_allclose(g.cpu().numpy(), ref_g, atol=2e-6 * np.abs(ref_g).max())
+    assert np.all(ref_v[~ref_m] == 0) and torch.all(v[~m] == 0)
+    sub = [d for d in sols if d % 2 == 0]
+    v_sub = rec.evaluate_raw_chi(qt, depths=sub).cpu().numpy()
+    ref_sub = mesh.evaluate_raw_chi(h, sols, q, depths=sub)
+    np.testing.assert_allclose(v_sub, ref_sub, atol=2e-6 * np.abs(ref_sub).max())
+    # level set: sample-weighted mean of chi at the input points
+    w = rec.sample_weight.cpu().numpy().astype(np.float64)
+    ref_mean = mesh.mean_chi(h, sols, pair["xyz"], w)
+    chi_pts = mesh.evaluate_raw_chi(h, sols, pair["xyz"])
+    mean = rec.get_mean_chi()
+    assert abs(mean - ref_mean) <= 2e-6 * np.abs(chi_pts).max()
+    # chunked evaluation is pointwise: equal bit for bit to one launch
+    full = v - mean
+    c, cm = rec.evaluate_chi(qt, compute_mask=True, max_points=777)
+    assert torch.equal(c, full) and torch.equal(cm, m)
+    assert torch.equal(rec.evaluate_chi(qt, max_points=-1), full)
+    assert rec.evaluate_chi(qt[:0], max_points=100).shape == (0,)

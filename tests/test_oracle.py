@@ -241,6 +241,41 @@ def test_evaluate_interpolated_matches_all_voxel_sum():
         np.testing.assert_allclose(got, brute, atol=1e-9)
 
 
+def test_evaluate_interpolated_gradient_and_mask_brute_force():
+    """Gradient: central differences of the all-voxel sum (per finest-voxel unit of the query).
+    Mask: some voxel centre within 1.5 strides of the query's cell centre on every axis."""
+    rng = np.random.default_rng(11)
+    vs = 1 / 16
+    xyz = rng.uniform(0.3, 0.7, (60, 3)).astype(np.float32)
+    h = hierarchy.Hierarchy(2, vs)
+    for k in range(2):
+        h.set_coords(k, hierarchy.quantize_coords(xyz, vs, h.strides[k]))
+    q = rng.uniform(0.1, 0.9, (200, 3)).astype(np.float32)
+    for d in range(2):
+        f = rng.standard_normal(h.num_voxels(d))
+        s = h.strides[d]
+        cen = h.coords[d] + s / 2
+
+        def brute(qq):
+            u = (qq[:, None, :] - cen[None]) / s
+            return (bezier.B(u[..., 0]) * bezier.B(u[..., 1]) * bezier.B(u[..., 2])) @ f
+
+        g, m = h.evaluate_interpolated(q, d, f, compute_mask=True, compute_grad=True)
+        qq = q.astype(np.float64) / vs
+        eps = 1e-5
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = eps
+            fd = (brute(qq + e) - brute(qq - e)) / (2 * eps)
+            np.testing.assert_allclose(g[:, a], fd, atol=1e-5)
+        cell = np.floor(qq / s) * s + s / 2
+        near = np.all(np.abs(cell[:, None, :] - cen[None]) <= 1.01 * s, axis=2).any(1)
+        assert np.array_equal(m, near) and 0 < m.sum() < m.size
+        v, m2 = h.evaluate_interpolated(q, d, f, compute_mask=True)
+        np.testing.assert_allclose(v, brute(qq), atol=1e-9)
+        assert np.array_equal(m, m2)
+
+
 # ---------------------------------------------------------------------------------- assembly
 def test_screening_term_against_dense_brute_force():
     rng = np.random.default_rng(9)
