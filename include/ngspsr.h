@@ -104,6 +104,13 @@ int64_t ngs_scan_workspace_bytes(int64_t n);
 int ngs_grid_clear(ngs_grid g, ngs_stream_t s);
 /* Mark int32 coordinate rows (multiples of the grid stride). */
 int ngs_grid_mark_coords(ngs_grid g, const int32_t *coords, int64_t n, ngs_stream_t s);
+/* Mark c + (dx, dy, dz) * stride for every row c and offset in [-radius, radius]^3: the expansion of
+ * get_coords(expand = 2 radius + 1) (_hash_backend.py:132-136) up to the sort, which the bitmap gives. */
+int ngs_grid_mark_dilated(ngs_grid g, const int32_t *coords, int64_t n, int32_t radius, ngs_stream_t s);
+/* Sample positions of get_test_grid (hashtree.py:209-229): out[(l R^3 + (i R + j) R + k)] =
+ * (base[l] + (box_i, box_j, box_k)) * voxel_size with box = torch.linspace(0, stride, R). */
+int ngs_test_grid_positions(const int32_t *base, int64_t n, int32_t stride, int32_t resolution, float voxel_size,
+                            float *out, ngs_stream_t s);
 /* Mark the 8 trilinear corners at the grid stride of every point whose level <= max_level
  * (levels may be NULL: all points). Replaces _quantize_coords (_hash_backend.py:457-462, with
  * _trilinear_weights :469-510) up to the sort, which the bitmap order provides. */
@@ -450,6 +457,33 @@ int ngs_eval_leaf_corners(ngs_grid g, const float *field, const int32_t *leaves,
 int ngs_mc_count(const int32_t *cubes, int64_t k, const float *values, int32_t *vcount, ngs_stream_t s);
 int ngs_mc_emit(const int32_t *cubes, int64_t k, const float *values, const int32_t *voffset, int32_t id_bits,
                 uint64_t *vkeys, uint32_t *vtags, ngs_stream_t s);
+/* ---- Primal sparse marching cubes (marching_cubes_sparse_cuda, primal_mc.cu:176-234) ----
+ * Input: nleaf leaves, each with a base corner base[3l..3l+2] (integer-valued floats, in leaf units)
+ * and an (r+1)^3 block of samples sdf[l (r+1)^3 + (x (r+1) + y)(r+1) + z]. Sub-cube (rx, ry, rz) of a
+ * leaf spans samples rx..rx+1, ry..ry+1, rz..rz+1; sub-cubes are numbered (l, rx, ry, rz) z-fastest. */
+/* vcount[c] = 3 x triangles of sub-cube c (classify_voxels, primal_mc.cu:21-56). */
+int ngs_primal_mc_count(const float *sdf, int64_t nleaf, int32_t r, int32_t *vcount, ngs_stream_t s);
+/* Triangles of every sub-cube from vertex voffset[c] on (the exclusive scan of vcount), per triangle
+ * vertex v: tris[3v..] position (base + offset / r), vert_ids[4v..] = (global sub-lattice corner of
+ * the edge's lower end, edge axis), src_ids[2v..] = flat sample indices of the edge's ends in
+ * interpolation order, src_weight[v] = weight of the first end (meshing_cube, primal_mc.cu:59-174). */
+int ngs_primal_mc_emit(const float *base, const float *sdf, int64_t nleaf, int32_t r, const int32_t *voffset,
+                       float *tris, int32_t *vert_ids, int32_t *src_ids, float *src_weight, ngs_stream_t s);
+/* Vertex de-duplication of MarchingCubes.forward (ops.py:15-35): the distinct vert_ids rows in
+ * lexicographic order (unq, V x 4), faces[v] = row of triangle vertex v, and per distinct vertex the
+ * position, the src ids and the weight of its first occurrence in triangle order (the reference keeps
+ * an arbitrary one), plus compute_normal (ops.py:85-101) in normals. key_layout (host, 6 int32) =
+ * per-axis minimum of the vert_ids coordinates and per-axis bit widths of their span (sum + 2 <= 64).
+ * Outputs are sized n (V <= n); *nverts (device int64) = V. */
+int64_t ngs_primal_dedup_workspace_bytes(int64_t n);
+int ngs_primal_dedup(const int32_t *vert_ids, int64_t n, const int32_t *key_layout, const float *tris,
+                     const int32_t *src_ids, const float *src_weight, int64_t *faces, float *verts, float *normals,
+                     int32_t *unq, int32_t *vsrc, float *vw, int64_t *nverts, void *workspace, int64_t workspace_bytes,
+                     ngs_stream_t s);
+/* MarchingCubes.backward (ops.py:40-58): g = bandwidth (grad_vert . normal) per vertex, added as w g to
+ * sample vsrc[2v] and (1 - w) g to sample vsrc[2v+1] of grad_grid (caller-zeroed, atomics). */
+int ngs_primal_mc_backward(const float *grad_vert, const float *normals, const int32_t *vsrc, const float *vw,
+                           int64_t nverts, float bandwidth, float *grad_grid, ngs_stream_t s);
 /* Vertex de-duplication (reconstructor.py:379-383): sorted unique keys; faces (T*3) = rank of each
  * key; the vertex of a key is interpolated (dual_mc.cu:20-31 sdf_interp) on its edge, in the corner
  * order of the key's first occurrence, from the corner positions pos (3 per id) and values — the

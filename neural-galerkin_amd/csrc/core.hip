@@ -291,6 +291,45 @@ __global__ void mark_coords_kernel(ngs_grid g, const int32_t* __restrict__ c, in
     grid_mark(g, c[3 * i], c[3 * i + 1], c[3 * i + 2]);
 }
 
+// Marks c + (dx, dy, dz) * stride for every row c and every offset in [-radius, radius]^3; one thread
+// per (row, dx, dy), walking dz.
+__global__ void mark_dilated_kernel(ngs_grid g, const int32_t* __restrict__ c, int64_t n, int radius) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = 2 * radius + 1;
+    if (t >= n * w * w) return;
+    const int64_t i = t / (w * w);
+    const int j = (int)(t - i * w * w);
+    const int s = 1 << g.log2_stride;
+    const int x = c[3 * i] + (j / w - radius) * s, y = c[3 * i + 1] + (j % w - radius) * s;
+    for (int dz = -radius; dz <= radius; ++dz) grid_mark(g, x, y, c[3 * i + 2] + dz * s);
+}
+
+// Sample positions of get_test_grid (hashtree.py:209-229): (base + box) * voxel_size per leaf, box =
+// torch.linspace(0, stride, R) per axis in ATen's form (start + step i below the halfway index,
+// end - step (R - 1 - i) from it, step = (end - start) / (R - 1), each as one fused multiply-add);
+// rows (leaf, i, j, k) with k fastest.
+__device__ __forceinline__ float linspace_value(float end, int steps, int i) {
+    const float step = end / (float)(steps - 1);
+    return i < steps / 2 ? fmaf(step, (float)i, 0.0f) : fmaf(-step, (float)(steps - 1 - i), end);
+}
+
+__global__ void test_grid_kernel(const int32_t* __restrict__ base, int64_t n, int stride, int res, float vs,
+                                 float* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r3 = (int64_t)res * res * res;
+    if (t >= n * r3) return;
+    const int64_t l = t / r3;
+    const int sub = (int)(t - l * r3);
+    const int idx[3] = {sub / (res * res), (sub / res) % res, sub % res};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float b = (float)base[3 * l + a];
+        const float box = res == 1 ? 0.0f : linspace_value((float)stride, res, idx[a]);
+        out[3 * t + a] = (b + box) * vs;
+    }
+}
+
 __global__ void mark_corners_kernel(ngs_grid g, const float* __restrict__ xyz, int64_t n, float inv_vs,
                                     const int32_t* __restrict__ levels, int max_level) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -695,6 +734,27 @@ int ngs_grid_clear(ngs_grid g, ngs_stream_t s) {
 int ngs_grid_mark_coords(ngs_grid g, const int32_t* coords, int64_t n, ngs_stream_t s) {
     if (n <= 0) return NGS_OK;
     mark_coords_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, coords, n);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_grid_mark_dilated(ngs_grid g, const int32_t* coords, int64_t n, int32_t radius, ngs_stream_t s) {
+    NGS_REQUIRE(radius >= 0 && radius <= 64, "ngs_grid_mark_dilated: radius must be in [0, 64]");
+    const int64_t t = n * (2 * radius + 1) * (2 * radius + 1);
+    if (t <= 0) return NGS_OK;
+    NGS_REQUIRE(coords, "ngs_grid_mark_dilated: null coordinates");
+    mark_dilated_kernel<<<grid1d(t, 256), 256, 0, S(s)>>>(g, coords, n, radius);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_test_grid_positions(const int32_t* base, int64_t n, int32_t stride, int32_t resolution, float voxel_size,
+                            float* out, ngs_stream_t s) {
+    NGS_REQUIRE(resolution >= 1 && resolution <= 1024 && stride >= 1, "ngs_test_grid_positions: bad resolution");
+    const int64_t t = n * (int64_t)resolution * resolution * resolution;
+    if (t <= 0) return NGS_OK;
+    NGS_REQUIRE(base && out, "ngs_test_grid_positions: null argument");
+    test_grid_kernel<<<grid1d(t, 256), 256, 0, S(s)>>>(base, n, stride, resolution, voxel_size, out);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -681,8 +681,196 @@ __global__ void dual_accumulate_leaves_kernel(ngs_grid cen, const int32_t* __res
 }
 
 __global__ void widen_kernel(const int32_t* a, int64_t* b) { *b = *a; }
+__global__ void set_i64_kernel(int64_t* p, int64_t v) { *p = v; }
 
 static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+// ------------------------------------------------------------------------ primal sparse MC
+// Leaf l carries an (r+1)^3 block of samples, x-major: sdf[l (r+1)^3 + (x (r+1) + y)(r+1) + z], and a
+// base corner (integer-valued, in leaf units); sub-cube (rx, ry, rz) of the leaf spans samples
+// (rx..rx+1, ry..ry+1, rz..rz+1). One thread per sub-cube, z fastest, so a wave reads consecutive
+// samples. Corner k of a sub-cube sits at offset (k&1 ^ k>>1&1, k>>1&1, k>>2&1), as for dual cubes.
+__device__ __forceinline__ void corner_offset(int k, int* o) {
+    o[0] = (k & 1) ^ ((k >> 1) & 1);
+    o[1] = (k >> 1) & 1;
+    o[2] = (k >> 2) & 1;
+}
+
+__device__ __forceinline__ int primal_case(const float* __restrict__ sdf, int64_t leaf, int r, int rx, int ry, int rz,
+                                           float* v) {
+    const int r1 = r + 1;
+    const int64_t b = leaf * (int64_t)r1 * r1 * r1;
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int o[3];
+        corner_offset(k, o);
+        v[k] = sdf[b + ((int64_t)(rx + o[0]) * r1 + (ry + o[1])) * r1 + (rz + o[2])];
+        if (v[k] < 0.f) c |= 1 << k;
+    }
+    return c;
+}
+
+__global__ void primal_count_kernel(const float* __restrict__ sdf, int64_t nleaf, int r, int32_t* __restrict__ vc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r3 = (int64_t)r * r * r;
+    if (t >= nleaf * r3) return;
+    const int64_t leaf = t / r3;
+    const int sub = (int)(t - leaf * r3);
+    float v[8];
+    vc[t] = case_count(primal_case(sdf, leaf, r, sub / (r * r), (sub / r) % r, sub % r, v));
+}
+
+// sdf_interp of primal_mc.cu:7-19, also returning the weight of p1 (1 when snapped to p1, 0 when
+// snapped to p2).
+__device__ __forceinline__ float sdf_interp_w(const float* p1, const float* p2, float v1, float v2, float* out) {
+#pragma clang fp contract(off)
+    if (fabsf(0.0f - v1) < 1.0e-5f) { out[0] = p1[0]; out[1] = p1[1]; out[2] = p1[2]; return 1.0f; }
+    if (fabsf(0.0f - v2) < 1.0e-5f) { out[0] = p2[0]; out[1] = p2[1]; out[2] = p2[2]; return 0.0f; }
+    if (fabsf(v1 - v2) < 1.0e-5f) { out[0] = p1[0]; out[1] = p1[1]; out[2] = p1[2]; return 1.0f; }
+    float w2 = (0.0f - v1) / (v2 - v1);
+    float w1 = 1.0f - w2;
+    out[0] = p1[0] * w1 + p2[0] * w2;
+    out[1] = p1[1] * w1 + p2[1] * w2;
+    out[2] = p1[2] * w1 + p2[2] * w2;
+    return w1;
+}
+
+// Triangles of every sub-cube in (leaf, rx, ry, rz) order at voff[t] / 3 (primal_mc.cu:59-174): per
+// triangle vertex the position (base + offset / r), the edge id (global sub-lattice corner of the
+// edge's lower end and the edge axis), the flat sample indices of the edge's two ends (sdf_interp
+// argument order) and the weight of the first end.
+__global__ void primal_emit_kernel(const float* __restrict__ base, const float* __restrict__ sdf, int64_t nleaf, int r,
+                                   const int32_t* __restrict__ voff, float* __restrict__ tris,
+                                   int32_t* __restrict__ vid, int32_t* __restrict__ src, float* __restrict__ sw) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r3 = (int64_t)r * r * r;
+    if (t >= nleaf * r3) return;
+    const int64_t leaf = t / r3;
+    const int sub = (int)(t - leaf * r3);
+    const int rr[3] = {sub / (r * r), (sub / r) % r, sub % r};
+    float v[8];
+    const int c = primal_case(sdf, leaf, r, rr[0], rr[1], rr[2], v);
+    uint64_t w = c_mc_cases[c];
+    if ((w & 0xF) == 0xF) return;
+    const float sbs = 1.0f / (float)r;
+    const float b[3] = {base[3 * leaf], base[3 * leaf + 1], base[3 * leaf + 2]};
+    const int r1 = r + 1;
+    const int32_t sid0 = r1 * r1 * r1;
+    int64_t o = voff[t];
+    for (int n = 0; n < 15; ++n, ++o) {
+        const int e = (int)((w >> (4 * n)) & 0xF);
+        if (e == 0xF) break;
+        int oa[3], ob[3];
+        corner_offset(c_edge[e][0], oa);
+        corner_offset(c_edge[e][1], ob);
+        float p1[3], p2[3], q[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            p1[a] = b[a] + (float)(rr[a] + oa[a]) * sbs;
+            p2[a] = b[a] + (float)(rr[a] + ob[a]) * sbs;
+        }
+        const float w1 = sdf_interp_w(p1, p2, v[c_edge[e][0]], v[c_edge[e][1]], q);
+        int axis = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            tris[3 * o + a] = q[a];
+            vid[4 * o + a] = (int)b[a] * r + rr[a] + min(oa[a], ob[a]);
+            if (oa[a] != ob[a]) axis = a;
+        }
+        vid[4 * o + 3] = axis;
+        src[2 * o] = sid0 * (int32_t)leaf + ((rr[0] + oa[0]) * r1 + (rr[1] + oa[1])) * r1 + (rr[2] + oa[2]);
+        src[2 * o + 1] = sid0 * (int32_t)leaf + ((rr[0] + ob[0]) * r1 + (rr[1] + ob[1])) * r1 + (rr[2] + ob[2]);
+        sw[o] = w1;
+    }
+}
+
+// Order-preserving 64-bit key of an edge id row (x, y, z, axis): biased coordinates in bx, by, bz
+// bits above the 2-bit axis, so key order is the lexicographic order of the signed rows.
+struct EdgeKeyLayout {
+    int32_t org[3];
+    int sh[3];
+};
+
+__global__ void primal_pack_kernel(const int32_t* __restrict__ vid, int64_t n, EdgeKeyLayout kl,
+                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ tags) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = (uint64_t)(uint32_t)vid[4 * i + 3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) k |= (uint64_t)(uint32_t)(vid[4 * i + a] - kl.org[a]) << kl.sh[a];
+    keys[i] = k;
+    tags[i] = (uint32_t)i;
+}
+
+// Distinct edges in key order; every record of a vertex is copied from the edge's first occurrence
+// in triangle order (the stable sort keeps occurrences in emission order). start[rank] = position
+// of the edge's run in the sorted list.
+__global__ void primal_scatter_kernel(const int32_t* __restrict__ head, const int32_t* __restrict__ off,
+                                      const uint32_t* __restrict__ stags, int64_t n, const float* __restrict__ tris,
+                                      const int32_t* __restrict__ vid, const int32_t* __restrict__ src,
+                                      const float* __restrict__ sw, int64_t* __restrict__ faces,
+                                      float* __restrict__ verts, int32_t* __restrict__ unq, int32_t* __restrict__ vsrc,
+                                      float* __restrict__ vw, int32_t* __restrict__ start) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t rank = off[i] + head[i] - 1;
+    const int64_t tag = stags[i];
+    faces[tag] = rank;
+    if (!head[i]) return;
+    start[rank] = (int32_t)i;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) verts[3 * (int64_t)rank + a] = tris[3 * tag + a];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) unq[4 * (int64_t)rank + a] = vid[4 * tag + a];
+    vsrc[2 * (int64_t)rank] = src[2 * tag];
+    vsrc[2 * (int64_t)rank + 1] = src[2 * tag + 1];
+    vw[rank] = sw[tag];
+}
+
+// compute_normal (ops.py:85-101): per vertex the sum of the unnormalised normals (v1 - v0) x (v2 - v0)
+// of its triangles, in triangle order (the vertex's run of sorted occurrences), divided by its
+// length + 1e-6.
+__global__ void primal_normals_kernel(const int32_t* __restrict__ head, const uint32_t* __restrict__ stags,
+                                      const int32_t* __restrict__ start, const int64_t* __restrict__ nverts, int64_t n,
+                                      const int64_t* __restrict__ faces, const float* __restrict__ verts,
+                                      float* __restrict__ normals) {
+#pragma clang fp contract(off)
+    const int64_t rank = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (rank >= *nverts) return;
+    float s[3] = {0.f, 0.f, 0.f};
+    int64_t i = start[rank];
+    do {
+        const int64_t tri = stags[i] / 3;
+        const float* v0 = verts + 3 * faces[3 * tri];
+        const float* v1 = verts + 3 * faces[3 * tri + 1];
+        const float* v2 = verts + 3 * faces[3 * tri + 2];
+        const float a[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+        const float b[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+        s[0] += a[1] * b[2] - a[2] * b[1];
+        s[1] += a[2] * b[0] - a[0] * b[2];
+        s[2] += a[0] * b[1] - a[1] * b[0];
+        ++i;
+    } while (i < n && !head[i]);
+    const float len = sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + 1e-6f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) normals[3 * rank + a] = s[a] / len;
+}
+
+// MarchingCubes.backward (ops.py:40-58): g = bandwidth (grad_vert . normal); the first sample of the
+// vertex's edge receives w g, the second (1 - w) g.
+__global__ void primal_backward_kernel(const float* __restrict__ gv, const float* __restrict__ nrm,
+                                       const int32_t* __restrict__ vsrc, const float* __restrict__ vw, int64_t nv,
+                                       float bandwidth, float* __restrict__ grid) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const float g = bandwidth * (gv[3 * i] * nrm[3 * i] + gv[3 * i + 1] * nrm[3 * i + 1] + gv[3 * i + 2] * nrm[3 * i + 2]);
+    const float w = vw[i];
+    atomicAdd(&grid[vsrc[2 * i]], w * g);
+    atomicAdd(&grid[vsrc[2 * i + 1]], (1.0f - w) * g);
+}
 
 }  // namespace ngs
 
@@ -791,6 +979,120 @@ static int dedup_layout(int64_t n, int key_bits, size_t* tmp, size_t off[5], siz
     off[3] = off[2] + al256(4 * (size_t)n);     // offsets (n + 1)
     off[4] = off[3] + al256(4 * (size_t)(n + 1));
     *total = off[4] + al256((size_t)ngs_scan_workspace_bytes(n));
+    return NGS_OK;
+}
+
+static int primal_layout(int64_t n, size_t* tmp, size_t off[8], size_t* total) {
+    size_t t = 0;
+    int r = sort_pairs_u64(nullptr, nullptr, nullptr, nullptr, n, 64, nullptr, &t, nullptr);
+    if (r != NGS_OK) return r;
+    *tmp = t;
+    off[0] = al256(t);                          // keys
+    off[1] = off[0] + al256(8 * (size_t)n);     // tags
+    off[2] = off[1] + al256(4 * (size_t)n);     // sorted keys
+    off[3] = off[2] + al256(8 * (size_t)n);     // sorted tags
+    off[4] = off[3] + al256(4 * (size_t)n);     // heads
+    off[5] = off[4] + al256(4 * (size_t)n);     // offsets (n + 1)
+    off[6] = off[5] + al256(4 * (size_t)(n + 1));  // run starts
+    off[7] = off[6] + al256(4 * (size_t)n);     // scan workspace
+    *total = off[7] + al256((size_t)ngs_scan_workspace_bytes(n));
+    return NGS_OK;
+}
+
+int ngs_primal_mc_count(const float* sdf, int64_t nleaf, int32_t r, int32_t* vcount, ngs_stream_t s) {
+    NGS_REQUIRE(r >= 1 && r <= 1024, "ngs_primal_mc_count: resolution must be in [1, 1024]");
+    NGS_REQUIRE(nleaf >= 0 && nleaf * (int64_t)(r + 1) * (r + 1) * (r + 1) < ((int64_t)1 << 31),
+                "ngs_primal_mc_count: more than 2^31 samples");
+    const int64_t t = nleaf * (int64_t)r * r * r;
+    if (t <= 0) return NGS_OK;
+    NGS_REQUIRE(sdf && vcount, "ngs_primal_mc_count: null argument");
+    primal_count_kernel<<<grid1d(t, 256), 256, 0, S(s)>>>(sdf, nleaf, r, vcount);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_primal_mc_emit(const float* base, const float* sdf, int64_t nleaf, int32_t r, const int32_t* voffset,
+                       float* tris, int32_t* vert_ids, int32_t* src_ids, float* src_weight, ngs_stream_t s) {
+    NGS_REQUIRE(r >= 1 && r <= 1024, "ngs_primal_mc_emit: resolution must be in [1, 1024]");
+    NGS_REQUIRE(nleaf >= 0 && nleaf * (int64_t)(r + 1) * (r + 1) * (r + 1) < ((int64_t)1 << 31),
+                "ngs_primal_mc_emit: more than 2^31 samples");
+    const int64_t t = nleaf * (int64_t)r * r * r;
+    if (t <= 0) return NGS_OK;
+    NGS_REQUIRE(base && sdf && voffset && tris && vert_ids && src_ids && src_weight, "ngs_primal_mc_emit: null argument");
+    primal_emit_kernel<<<grid1d(t, 128), 128, 0, S(s)>>>(base, sdf, nleaf, r, voffset, tris, vert_ids, src_ids,
+                                                         src_weight);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int64_t ngs_primal_dedup_workspace_bytes(int64_t n) {
+    size_t tmp, off[8], total;
+    if (primal_layout(n, &tmp, off, &total) != NGS_OK) return -1;
+    return (int64_t)total;
+}
+
+int ngs_primal_dedup(const int32_t* vert_ids, int64_t n, const int32_t* key_layout, const float* tris,
+                     const int32_t* src_ids, const float* src_weight, int64_t* faces, float* verts, float* normals,
+                     int32_t* unq, int32_t* vsrc, float* vw, int64_t* nverts, void* workspace, int64_t workspace_bytes,
+                     ngs_stream_t s) {
+    NGS_REQUIRE(key_layout && nverts, "ngs_primal_dedup: null key layout or count");
+    NGS_REQUIRE(n >= 0 && n < ((int64_t)1 << 31), "ngs_primal_dedup: bad size");
+    EdgeKeyLayout kl;
+    const int bz = key_layout[5], by = key_layout[4], bx = key_layout[3];
+    NGS_REQUIRE(bx >= 0 && by >= 0 && bz >= 0 && bx <= 31 && by <= 31 && bz <= 31 && bx + by + bz + 2 <= 64,
+                "ngs_primal_dedup: key layout needs more than 64 bits");
+    for (int a = 0; a < 3; ++a) kl.org[a] = key_layout[a];
+    kl.sh[2] = 2;
+    kl.sh[1] = 2 + bz;
+    kl.sh[0] = 2 + bz + by;
+    if (n == 0) {
+        set_i64_kernel<<<1, 1, 0, S(s)>>>(nverts, 0);
+        NGS_CHECK_LAUNCH();
+        return NGS_OK;
+    }
+    NGS_REQUIRE(vert_ids && tris && src_ids && src_weight && faces && verts && normals && unq && vsrc && vw && workspace,
+                "ngs_primal_dedup: null argument");
+    size_t tmp, off[8], total;
+    int r = primal_layout(n, &tmp, off, &total);
+    if (r != NGS_OK) return r;
+    NGS_REQUIRE((size_t)workspace_bytes >= total, "ngs_primal_dedup: workspace too small");
+    char* w = reinterpret_cast<char*>(workspace);
+    hipStream_t st = S(s);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(w + off[0]);
+    uint32_t* tags = reinterpret_cast<uint32_t*>(w + off[1]);
+    uint64_t* kout = reinterpret_cast<uint64_t*>(w + off[2]);
+    uint32_t* tout = reinterpret_cast<uint32_t*>(w + off[3]);
+    int32_t* head = reinterpret_cast<int32_t*>(w + off[4]);
+    int32_t* offs = reinterpret_cast<int32_t*>(w + off[5]);
+    int32_t* start = reinterpret_cast<int32_t*>(w + off[6]);
+    primal_pack_kernel<<<grid1d(n, 256), 256, 0, st>>>(vert_ids, n, kl, keys, tags);
+    NGS_CHECK_LAUNCH();
+    size_t tb = tmp;
+    r = sort_pairs_u64(keys, kout, reinterpret_cast<const int32_t*>(tags), reinterpret_cast<int32_t*>(tout), n,
+                       bx + by + bz + 2, w, &tb, st);
+    if (r != NGS_OK) return r;
+    run_heads_kernel<<<grid1d(n, 256), 256, 0, st>>>(kout, n, head);
+    NGS_CHECK_LAUNCH();
+    r = scan_i32_internal(head, n, offs, w + off[7], st);
+    if (r != NGS_OK) return r;
+    primal_scatter_kernel<<<grid1d(n, 256), 256, 0, st>>>(head, offs, tout, n, tris, vert_ids, src_ids, src_weight,
+                                                          faces, verts, unq, vsrc, vw, start);
+    NGS_CHECK_LAUNCH();
+    widen_kernel<<<1, 1, 0, st>>>(&offs[n], nverts);
+    NGS_CHECK_LAUNCH();
+    // at most n vertices; threads beyond the count read it from the device and return
+    primal_normals_kernel<<<grid1d(n, 256), 256, 0, st>>>(head, tout, start, nverts, n, faces, verts, normals);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_primal_mc_backward(const float* grad_vert, const float* normals, const int32_t* vsrc, const float* vw,
+                           int64_t nverts, float bandwidth, float* grad_grid, ngs_stream_t s) {
+    if (nverts <= 0) return NGS_OK;
+    NGS_REQUIRE(grad_vert && normals && vsrc && vw && grad_grid, "ngs_primal_mc_backward: null argument");
+    primal_backward_kernel<<<grid1d(nverts, 256), 256, 0, S(s)>>>(grad_vert, normals, vsrc, vw, nverts, bandwidth,
+                                                                   grad_grid);
+    NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
 

@@ -68,6 +68,13 @@ _SIGNATURES = {
     "ngs_splat": ([Grid, P, c_int64, c_float, P, c_int32, P, P, c_int32, c_double, P, P], c_int32),
     "ngs_eval_field": ([Grid, P, P, c_int64, c_float, c_float, c_int32, P, P, P], c_int32),
     "ngs_field_support": ([Grid, P, c_int64, c_float, P, P], c_int32),
+    "ngs_grid_mark_dilated": ([Grid, P, c_int64, c_int32, P], c_int32),
+    "ngs_test_grid_positions": ([P, c_int64, c_int32, c_int32, c_float, P, P], c_int32),
+    "ngs_primal_mc_count": ([P, c_int64, c_int32, P, P], c_int32),
+    "ngs_primal_mc_emit": ([P, P, c_int64, c_int32, P, P, P, P, P, P], c_int32),
+    "ngs_primal_dedup_workspace_bytes": ([c_int64], c_int64),
+    "ngs_primal_dedup": ([P, c_int64, P, P, P, P, P, P, P, P, P, P, P, P, c_int64, P], c_int32),
+    "ngs_primal_mc_backward": ([P, P, P, P, c_int64, c_float, P, P], c_int32),
     "ngs_point_levels": ([P, c_int64, c_float, c_float, c_int32, P, P], c_int32),
     "ngs_point_basis": ([P, c_int64, c_float, c_int32, P, P, P], c_int32),
     "ngs_bucket_workspace_bytes": ([c_int64, c_int64], c_int64),

@@ -79,6 +79,53 @@ class HashTree:
         g = self.grids[depth]
         return 0 if g is None else g.count
 
+    def _coords_ext(self, depth, expand: int = 0, conforming: bool = False):
+        """_hash_backend.py:121-147: the level's voxels, dilated by the expand-cube of offsets
+        (expand >= 3, odd) and, with conforming, replaced by the 8 children of every distinct parent
+        at twice the stride (parents sorted, children in (x, y, z) bit order)."""
+        s = self.strides[depth]
+        g = self.grids[depth]
+        coords = g.coords
+        dev = coords.device
+        if expand >= 3:
+            if expand % 2 == 0:
+                raise ValueError("expand must be odd")
+            h = expand // 2
+            dg = SparseGrid([o - h for o in g.origin], [o + dm - 1 + h for o, dm in zip(g.origin, g.dims)], s, dev)
+            if coords.shape[0]:
+                call("ngs_grid_mark_dilated", dg.struct(), ptr(coords), coords.shape[0], h, stream())
+            dg.finalize()
+            coords = dg.coords
+            g = dg
+        if conforming:
+            pg = SparseGrid([(o * s) // (2 * s) - 1 for o in g.origin],
+                            [((o + dm) * s) // (2 * s) + 1 for o, dm in zip(g.origin, g.dims)], 2 * s, dev)
+            pg.mark_coords(coords)
+            pg.finalize()
+            npar = pg.count
+            out = torch.empty((npar * 8, 3), dtype=torch.int32, device=dev)
+            keep = torch.empty(npar * 8, dtype=torch.int32, device=dev)
+            if npar:
+                call("ngs_conformal_children", ptr(pg.coords), npar, s, SparseGrid.empty(dev).struct(), 0, ptr(out),
+                     ptr(keep), stream())
+            coords = out
+        return coords
+
+    def get_test_grid(self, resolution, depth, expand: int = 0, conforming: bool = False):
+        """hashtree.py:209-229: sample positions (N R^3, 3) of an R^3 lattice spanning every voxel of
+        the level (corners included, R = resolution) in metric units, and the voxel corners (N, 3)
+        int32; (None, corners) for resolution 0."""
+        s = self.strides[depth]
+        base = self._coords_ext(depth, expand, conforming)
+        if resolution == 0:
+            return None, base
+        n = base.shape[0]
+        q = torch.empty((n * resolution ** 3, 3), dtype=torch.float32, device=base.device)
+        if n:
+            call("ngs_test_grid_positions", ptr(base), n, s, int(resolution), float(self._voxel_size), ptr(q),
+                 stream())
+        return q, base
+
     def get_voxel_centers(self, depth, branch=DECODER, normalized=False):
         c = self.get_coords(branch, depth).float() + self.strides[depth] / 2.0
         return c if normalized else c * self._voxel_size

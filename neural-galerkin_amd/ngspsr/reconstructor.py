@@ -281,6 +281,24 @@ class Reconstructor:
         return val
 
     # -------------------------------------------------------------------------- mesh
+    def extract_mesh(self, base_coords, chi_field, chi_depth: int, build_o3d_mesh: bool = True):
+        """reconstructor.py:257-294: primal sparse marching cubes over the voxels `base_coords` of
+        level chi_depth, each carrying an R^3 block of chi samples (get_test_grid order), with the
+        MeshSDF backward to chi_field. Returns (vertices (V, 3), triangles (T, 3) int64, vertex
+        normals (V, 3)) in metric units, or with build_o3d_mesh a triangle mesh object (Open3D's when it
+        is installed, else ngspsr.io.TriangleMesh with the same attribute names)."""
+        from .ops import marching_cubes_op
+        scale = self.hash_tree.get_stride(None, chi_depth)
+        num_lif = base_coords.shape[0]
+        res = round((chi_field.shape[0] // max(num_lif, 1)) ** (1 / 3.0))
+        chi = chi_field.reshape(-1, res, res, res)
+        vertices, triangles, normals, _ = marching_cubes_op(base_coords.float() / scale, chi)
+        vertices = vertices * (scale * self.hash_tree.voxel_size)
+        if not build_o3d_mesh:
+            return vertices, triangles, normals
+        from .io import triangle_mesh
+        return triangle_mesh(vertices, triangles, normals)
+
     def extract_multiscale_mesh(self, n_upsample: int = 1, max_depth: int = 100, expand: int = 0, trim: bool = False,
                                 build_o3d_mesh: bool = False, max_points: int = -1, return_info=False):
         """reconstructor.py:296-397 (expand=0, trim=False): dual marching cubes over the conformal
