@@ -484,6 +484,28 @@ int ngs_primal_dedup(const int32_t *vert_ids, int64_t n, const int32_t *key_layo
  * sample vsrc[2v] and (1 - w) g to sample vsrc[2v+1] of grad_grid (caller-zeroed, atomics). */
 int ngs_primal_mc_backward(const float *grad_vert, const float *normals, const int32_t *vsrc, const float *vw,
                            int64_t nverts, float bandwidth, float *grad_grid, ngs_stream_t s);
+/* ---- Learned additive bases (jspsr/bases/additive.py) ----
+ * DDI forward (csrc/integration/ddi.cu:21-76): for pair n and axis v, p_mats[n, v] = sum_si sum_ti
+ * s[n, 3 si + v] t[n, 3 ti + v] f_f_i[si][ti][rel_pos[n, v]] and q_mats likewise with df_df_i.
+ * s, t are (n, 3 C) float32, rel_pos (n, 3) int64 in [0, table_len), tables (C, C, table_len). */
+int ngs_ddi_forward(const float *source_expanded, const float *target_expanded, const int64_t *rel_pos, int64_t n,
+                    int32_t n_components, const float *f_f_i, const float *df_df_i, int32_t table_len, float *p_mats,
+                    float *q_mats, ngs_stream_t s);
+/* DDI backward (ddi.cu:78-141): grad_s, grad_t (n, 3 C) float32, written (not accumulated). */
+int ngs_ddi_backward(const float *source_expanded, const float *target_expanded, const int64_t *rel_pos, int64_t n,
+                     int32_t n_components, const float *f_f_i, const float *df_df_i, int32_t table_len,
+                     const float *grad_p, const float *grad_q, float *grad_s, float *grad_t, ngs_stream_t s);
+/* EvalFunc / EvalDerivFunc forward (additive.py:71-122): funcs[n, v] = sum_c q_c(xyz[n, v])
+ * coeffs[feat_ids[n], 3 c + v], dfuncs (nullable) the same with q_c' / stride. components: 6 floats
+ * per component, two factors (kind, a0, a1): kind 0 = 1, 1 = x^a0, 2 = Bezier, 3 = sin(a0 x + a1). */
+int ngs_additive_eval(const float *xyz, const float *coeffs, const int64_t *feat_ids, int64_t n,
+                      const float *components, int32_t n_components, float stride, float *funcs, float *dfuncs,
+                      ngs_stream_t s);
+/* Their backward (additive.py:86-99,124-136): grad_coeffs[feat_ids[n], 3 c + v] += grad_funcs q_c +
+ * grad_dfuncs q_c' / stride (grad_dfuncs nullable; grad_coeffs caller-zeroed, atomics). */
+int ngs_additive_eval_backward(const float *xyz, const int64_t *feat_ids, int64_t n, const float *components,
+                               int32_t n_components, float stride, const float *grad_funcs, const float *grad_dfuncs,
+                               float *grad_coeffs, ngs_stream_t s);
 /* Vertex de-duplication (reconstructor.py:379-383): sorted unique keys; faces (T*3) = rank of each
  * key; the vertex of a key is interpolated (dual_mc.cu:20-31 sdf_interp) on its edge, in the corner
  * order of the key's first occurrence, from the corner positions pos (3 per id) and values — the

@@ -70,6 +70,10 @@ _SIGNATURES = {
     "ngs_field_support": ([Grid, P, c_int64, c_float, P, P], c_int32),
     "ngs_grid_mark_dilated": ([Grid, P, c_int64, c_int32, P], c_int32),
     "ngs_test_grid_positions": ([P, c_int64, c_int32, c_int32, c_float, P, P], c_int32),
+    "ngs_ddi_forward": ([P, P, P, c_int64, c_int32, P, P, c_int32, P, P, P], c_int32),
+    "ngs_ddi_backward": ([P, P, P, c_int64, c_int32, P, P, c_int32, P, P, P, P, P], c_int32),
+    "ngs_additive_eval": ([P, P, P, c_int64, P, c_int32, c_float, P, P, P], c_int32),
+    "ngs_additive_eval_backward": ([P, P, c_int64, P, c_int32, c_float, P, P, P, P], c_int32),
     "ngs_primal_mc_count": ([P, c_int64, c_int32, P, P], c_int32),
     "ngs_primal_mc_emit": ([P, P, c_int64, c_int32, P, P, P, P, P, P], c_int32),
     "ngs_primal_dedup_workspace_bytes": ([c_int64], c_int64),
