@@ -296,6 +296,55 @@ def run_serving(n_req=64):
     }
 
 
+def extensions(res, xyz):
+    """Timings of the field-query API, the primal marching cubes and the DDI kernel on the
+    benchmark's own solution, outside the timed reconstruction loop (HIP events)."""
+    import torch
+    from ngspsr import additive
+    from ngspsr._lib import call, ptr, stream
+    rec, tree = res.reconstructor, res.tree
+    dev = xyz.device
+    gen = torch.Generator(device=dev).manual_seed(0)
+    lo, hi = xyz.min(0).values, xyz.max(0).values
+    nq = 8_000_000
+    q = lo + (hi - lo) * torch.rand((nq, 3), device=dev, generator=gen)
+    levels = len(rec.solutions)
+    t_val = _event_time(lambda: rec.evaluate_raw_chi(q), 5)
+    t_grad = _event_time(lambda: rec.evaluate_raw_chi(q, compute_grad=True), 5)
+    out = {"field_query": {"queries": nq, "levels": levels, "chi_ms": round(1e3 * t_val, 3),
+                           "chi_Mqueries_per_s": round(nq / t_val / 1e6, 1), "gradient_ms": round(1e3 * t_grad, 3),
+                           "gradient_Mqueries_per_s": round(nq / t_grad / 1e6, 1)}}
+    del q
+    res_mc, depth_mc = 4, 1
+    t_grid = _event_time(lambda: tree.get_test_grid(res_mc, depth_mc), 3)
+    qg, base = tree.get_test_grid(res_mc, depth_mc)
+    t_chi = _event_time(lambda: rec.evaluate_chi(qg, max_points=2 ** 24), 3)
+    chi = rec.evaluate_chi(qg, max_points=2 ** 24)
+    t_mc = _event_time(lambda: rec.extract_mesh(base, chi, depth_mc, build_o3d_mesh=False), 3)
+    v, f, _ = rec.extract_mesh(base, chi, depth_mc, build_o3d_mesh=False)
+    out["primal_mesh"] = {"depth": depth_mc, "resolution": res_mc, "leaves": int(base.shape[0]),
+                          "samples": int(chi.numel()), "test_grid_ms": round(1e3 * t_grid, 3),
+                          "evaluate_chi_ms": round(1e3 * t_chi, 3), "extract_mesh_ms": round(1e3 * t_mc, 3),
+                          "vertices": int(v.shape[0]), "triangles": int(f.shape[0])}
+    del qg, chi, v, f
+    basis = additive.AdditiveBasis.get_poly_6()
+    npairs, nc = 4_000_000, len(basis.components)
+    s = torch.randn((npairs, 3 * nc), device=dev, generator=gen)
+    t = torch.randn((npairs, 3 * nc), device=dev, generator=gen)
+    rel = torch.randint(0, 5, (npairs, 3), device=dev, generator=gen)
+    ff = basis.ff_integral[0]["f_f"].data.to(dev)
+    dd = basis.ff_integral[0]["df_df"].data.to(dev)
+    pm = torch.empty((npairs, 3), device=dev)
+    qm = torch.empty((npairs, 3), device=dev)
+    t_ddi = _event_time(lambda: call("ngs_ddi_forward", ptr(s), ptr(t), ptr(rel), npairs, nc, ptr(ff), ptr(dd), 5,
+                                     ptr(pm), ptr(qm), stream()), 10)
+    bytes_pair = 2 * 12 * nc + 24 + 24  # s and t rows, rel, p and q
+    out["ddi"] = {"kernel": "ddi_forward_kernel", "pairs": npairs, "components": nc, "launch_us": round(1e6 * t_ddi, 2),
+                  "Gpairs_per_s": round(npairs / t_ddi / 1e9, 3),
+                  "GB/s": round(npairs * bytes_pair / t_ddi / 1e9, 1), "algorithmic_bytes_per_pair": bytes_pair}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +352,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extensions", action="store_true",
+                    help="skip the field-query, primal marching cubes and DDI timings")
     ap.add_argument("--serving", type=int, default=64, help="requests of the serving comparison (0: skip)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter probe: one reconstruction, then 3 device copies of 1 GiB (calibration) and 5 "
@@ -409,6 +460,8 @@ def main():
         "assembly": assembly_roofline(res, xyz),
         "mesh": {"vertices": int(res.vertices.shape[0]), "triangles": int(res.triangles.shape[0])},
     }
+    if not args.no_extensions:
+        out["extensions"] = extensions(res, xyz)
     if not args.no_cpu:
         per_iter, it = cpu_baseline(csr, rhs)
         cpu_rate = 1.0 / (per_iter * total_iters)

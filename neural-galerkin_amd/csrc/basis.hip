@@ -69,6 +69,111 @@ __global__ void ddi_backward_kernel(const float* __restrict__ s, const float* __
     }
 }
 
+// The same two kernels with the tables staged in LDS as (ff, dd) pairs in [r][si][ti] order, the
+// pair's coefficients of one axis in registers, and a grid-stride loop over pairs (every block loads
+// the tables once). Same products and summation order as above, so the results are identical.
+constexpr int DDI_MAXC = 16;
+constexpr int DDI_LDS_BYTES = 64 * 1024;
+
+__device__ __forceinline__ void ddi_stage_tables(const float* __restrict__ ff, const float* __restrict__ dd, int nc,
+                                                 int L, float2* tab) {
+    const int nn = nc * nc;
+    for (int k = threadIdx.x; k < L * nn; k += blockDim.x) {
+        const int r = k / nn, st = k - r * nn;
+        tab[k] = make_float2(ff[(int64_t)st * L + r], dd[(int64_t)st * L + r]);
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) ddi_forward_lds_kernel(const float* __restrict__ s, const float* __restrict__ t,
+                                                              const int64_t* __restrict__ rel, int64_t n, int nc,
+                                                              const float* __restrict__ ff,
+                                                              const float* __restrict__ dd, int L,
+                                                              float* __restrict__ p, float* __restrict__ q) {
+#pragma clang fp contract(off)
+    extern __shared__ float2 tab[];
+    ddi_stage_tables(ff, dd, nc, L, tab);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float* sr = s + i * 3 * nc;
+        const float* tr = t + i * 3 * nc;
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const float2* tb = tab + (int64_t)rel[3 * i + v] * nc * nc;
+            float tv[DDI_MAXC];
+#pragma unroll
+            for (int ti = 0; ti < DDI_MAXC; ++ti) tv[ti] = ti < nc ? tr[3 * ti + v] : 0.f;
+            float pa = 0.f, qa = 0.f;
+            for (int si = 0; si < nc; ++si) {
+                const float sv = sr[3 * si + v];
+#pragma unroll
+                for (int ti = 0; ti < DDI_MAXC; ++ti) {
+                    if (ti < nc) {
+                        const float st = sv * tv[ti];
+                        const float2 w = tb[si * nc + ti];
+                        pa += st * w.x;
+                        qa += st * w.y;
+                    }
+                }
+            }
+            p[3 * i + v] = pa;
+            q[3 * i + v] = qa;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) ddi_backward_lds_kernel(const float* __restrict__ s, const float* __restrict__ t,
+                                                               const int64_t* __restrict__ rel, int64_t n, int nc,
+                                                               const float* __restrict__ ff,
+                                                               const float* __restrict__ dd, int L,
+                                                               const float* __restrict__ gp,
+                                                               const float* __restrict__ gq, float* __restrict__ gs,
+                                                               float* __restrict__ gt) {
+#pragma clang fp contract(off)
+    extern __shared__ float2 tab[];
+    ddi_stage_tables(ff, dd, nc, L, tab);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float* sr = s + i * 3 * nc;
+        const float* tr = t + i * 3 * nc;
+        float* gsr = gs + i * 3 * nc;
+        float* gtr = gt + i * 3 * nc;
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const float2* tb = tab + (int64_t)rel[3 * i + v] * nc * nc;
+            const float a = gp[3 * i + v], b = gq[3 * i + v];
+            float tv[DDI_MAXC], gacc[DDI_MAXC];
+#pragma unroll
+            for (int k = 0; k < DDI_MAXC; ++k) {
+                tv[k] = k < nc ? tr[3 * k + v] : 0.f;
+                gacc[k] = 0.f;
+            }
+            for (int si = 0; si < nc; ++si) {
+                const float svs = sr[3 * si + v];
+                float acc = 0.f;
+#pragma unroll
+                for (int ti = 0; ti < DDI_MAXC; ++ti) {
+                    if (ti < nc) {
+                        const float2 w = tb[si * nc + ti];
+                        const float c = a * w.x + b * w.y;
+                        acc += c * tv[ti];
+                        gacc[ti] += c * svs;
+                    }
+                }
+                gsr[3 * si + v] = acc;
+            }
+#pragma unroll
+            for (int ti = 0; ti < DDI_MAXC; ++ti)
+                if (ti < nc) gtr[3 * ti + v] = gacc[ti];
+        }
+    }
+}
+
+static bool ddi_use_lds(int nc, int L) { return nc <= DDI_MAXC && (int64_t)nc * nc * L * 8 <= DDI_LDS_BYTES; }
+
+static unsigned ddi_grid(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
 // Elementary functions q(x) of additive_components.py, evaluated as the reference's torch code does
 // (no support mask except Bezier's). A component is the product of two factors (kind, a0, a1):
 // kind 0 = 1, 1 = x^a0, 2 = Bezier, 3 = sin(a0 x + a1).
@@ -180,8 +285,12 @@ int ngs_ddi_forward(const float* source_expanded, const float* target_expanded, 
     if (n <= 0) return NGS_OK;
     NGS_REQUIRE(source_expanded && target_expanded && rel_pos && f_f_i && df_df_i && p_mats && q_mats,
                 "ngs_ddi_forward: null argument");
-    ddi_forward_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(source_expanded, target_expanded, rel_pos, n, n_components,
-                                                         f_f_i, df_df_i, table_len, p_mats, q_mats);
+    if (ddi_use_lds(n_components, table_len))
+        ddi_forward_lds_kernel<<<ddi_grid(n), 256, (size_t)n_components * n_components * table_len * 8, S(s)>>>(
+            source_expanded, target_expanded, rel_pos, n, n_components, f_f_i, df_df_i, table_len, p_mats, q_mats);
+    else
+        ddi_forward_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(source_expanded, target_expanded, rel_pos, n,
+                                                             n_components, f_f_i, df_df_i, table_len, p_mats, q_mats);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
@@ -194,8 +303,14 @@ int ngs_ddi_backward(const float* source_expanded, const float* target_expanded,
     NGS_REQUIRE(source_expanded && target_expanded && rel_pos && f_f_i && df_df_i && grad_p && grad_q && grad_s &&
                     grad_t,
                 "ngs_ddi_backward: null argument");
-    ddi_backward_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(source_expanded, target_expanded, rel_pos, n, n_components,
-                                                          f_f_i, df_df_i, table_len, grad_p, grad_q, grad_s, grad_t);
+    if (ddi_use_lds(n_components, table_len))
+        ddi_backward_lds_kernel<<<ddi_grid(n), 256, (size_t)n_components * n_components * table_len * 8, S(s)>>>(
+            source_expanded, target_expanded, rel_pos, n, n_components, f_f_i, df_df_i, table_len, grad_p, grad_q,
+            grad_s, grad_t);
+    else
+        ddi_backward_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(source_expanded, target_expanded, rel_pos, n,
+                                                              n_components, f_f_i, df_df_i, table_len, grad_p, grad_q,
+                                                              grad_s, grad_t);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
