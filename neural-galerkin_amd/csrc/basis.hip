@@ -121,6 +121,60 @@ __global__ void __launch_bounds__(256) ddi_forward_lds_kernel(const float* __res
     }
 }
 
+// Forward with the coefficient rows of 256 consecutive pairs also staged in LDS: the rows are read
+// from global memory as one contiguous, coalesced range per tile (a pair's row is 12 C bytes, so
+// direct per-thread row reads touch a different cache line per lane). Same arithmetic and order.
+constexpr int DDI_TILE = 256;
+
+__global__ void __launch_bounds__(DDI_TILE) ddi_forward_tiled_kernel(
+    const float* __restrict__ s, const float* __restrict__ t, const int64_t* __restrict__ rel, int64_t n, int nc,
+    const float* __restrict__ ff, const float* __restrict__ dd, int L, float* __restrict__ p, float* __restrict__ q) {
+#pragma clang fp contract(off)
+    extern __shared__ float2 tab[];
+    ddi_stage_tables(ff, dd, nc, L, tab);
+    const int w = 3 * nc;
+    float* rs = reinterpret_cast<float*>(tab + (int64_t)L * nc * nc);
+    float* rt = rs + DDI_TILE * w;
+    const int64_t ntiles = (n + DDI_TILE - 1) / DDI_TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t base = tile * DDI_TILE;
+        const int rows = (int)min((int64_t)DDI_TILE, n - base);
+        __syncthreads();  // the previous tile's rows are no longer read
+        for (int k = threadIdx.x; k < rows * w; k += DDI_TILE) {
+            rs[k] = s[base * w + k];
+            rt[k] = t[base * w + k];
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < rows) {
+            const int64_t i = base + threadIdx.x;
+            const float* sr = rs + threadIdx.x * w;
+            const float* tr = rt + threadIdx.x * w;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const float2* tb = tab + (int64_t)rel[3 * i + v] * nc * nc;
+                float tv[DDI_MAXC];
+#pragma unroll
+                for (int ti = 0; ti < DDI_MAXC; ++ti) tv[ti] = ti < nc ? tr[3 * ti + v] : 0.f;
+                float pa = 0.f, qa = 0.f;
+                for (int si = 0; si < nc; ++si) {
+                    const float sv = sr[3 * si + v];
+#pragma unroll
+                    for (int ti = 0; ti < DDI_MAXC; ++ti) {
+                        if (ti < nc) {
+                            const float st = sv * tv[ti];
+                            const float2 wt = tb[si * nc + ti];
+                            pa += st * wt.x;
+                            qa += st * wt.y;
+                        }
+                    }
+                }
+                p[3 * i + v] = pa;
+                q[3 * i + v] = qa;
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) ddi_backward_lds_kernel(const float* __restrict__ s, const float* __restrict__ t,
                                                                const int64_t* __restrict__ rel, int64_t n, int nc,
                                                                const float* __restrict__ ff,
@@ -168,6 +222,7 @@ __global__ void __launch_bounds__(256) ddi_backward_lds_kernel(const float* __re
 }
 
 static bool ddi_use_lds(int nc, int L) { return nc <= DDI_MAXC && (int64_t)nc * nc * L * 8 <= DDI_LDS_BYTES; }
+static int64_t ddi_tiled_bytes(int nc, int L) { return (int64_t)nc * nc * L * 8 + 2LL * DDI_TILE * 3 * nc * 4; }
 
 static unsigned ddi_grid(int64_t n) {
     const int64_t b = (n + 255) / 256;
@@ -285,7 +340,10 @@ int ngs_ddi_forward(const float* source_expanded, const float* target_expanded, 
     if (n <= 0) return NGS_OK;
     NGS_REQUIRE(source_expanded && target_expanded && rel_pos && f_f_i && df_df_i && p_mats && q_mats,
                 "ngs_ddi_forward: null argument");
-    if (ddi_use_lds(n_components, table_len))
+    if (ddi_use_lds(n_components, table_len) && ddi_tiled_bytes(n_components, table_len) <= DDI_LDS_BYTES)
+        ddi_forward_tiled_kernel<<<ddi_grid(n), DDI_TILE, (size_t)ddi_tiled_bytes(n_components, table_len), S(s)>>>(
+            source_expanded, target_expanded, rel_pos, n, n_components, f_f_i, df_df_i, table_len, p_mats, q_mats);
+    else if (ddi_use_lds(n_components, table_len))
         ddi_forward_lds_kernel<<<ddi_grid(n), 256, (size_t)n_components * n_components * table_len * 8, S(s)>>>(
             source_expanded, target_expanded, rel_pos, n, n_components, f_f_i, df_df_i, table_len, p_mats, q_mats);
     else

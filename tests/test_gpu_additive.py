@@ -17,14 +17,15 @@ def _dir_check(f, x0, grad, rng, k=4, eps=1e-3):
         assert abs(got - fd) <= 1e-4 * (1 + abs(fd)), (got, fd)
 
 
-@pytest.mark.parametrize("m,nc", [(0, 4), (2, 4), (1, 17)])
+@pytest.mark.parametrize("m,nc", [(0, 4), (2, 4), (0, 12), (1, 17)])
 def test_ddi_forward_backward(cuda, m, nc):
-    """nc = 4 runs the LDS-table kernels, nc = 17 the kernels reading the tables from global memory."""
+    """nc = 4: tables and coefficient rows in LDS; nc = 12: tables only; nc = 17: tables read from
+    global memory."""
     from ngspsr import additive as A
     from oracle import additive as OA
     print(f"[case] ddi m={m} components={nc}", flush=True)
     comps = [A.Power(0), A.Power(2), A.Bezier(), A.SinPeaks(2)] if nc == 4 else \
-        [A.Power(p) for p in range(nc - 1)] + [A.Bezier()]
+        [A.Power(p % 7) if p < 7 else A.Product(A.Power(p % 7), A.Bezier()) for p in range(nc - 1)] + [A.Bezier()]
     b = A.AdditiveBasis(comps, [1.0] * nc)
     rng = np.random.default_rng(m)
     n, L = 3000, 3 * 2 ** m + 2
