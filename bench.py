@@ -125,7 +125,11 @@ def assembly_roofline(res, xyz, reps=10):
     inv_diag = torch.empty(M, dtype=torch.float32, device=xyz.device)
     rhs = torch.zeros(M, dtype=torch.float64, device=xyz.device)
     lv, pb = tree.level(0), bins.struct()
-    t = _event_time(lambda: call("ngs_lhs_assemble_runs", lv, pb, ptr(coarse), 1.0, 0.1, ptr(runs), ptr(rvals),
+    # the coarse field in bin order, as the reconstruction passes it
+    cb = torch.empty_like(coarse)
+    call("ngs_gather_f32", ptr(coarse), ptr(bins.perm), bins.n, ptr(cb), stream())
+    pb.perm = None
+    t = _event_time(lambda: call("ngs_lhs_assemble_runs", lv, pb, ptr(cb), 1.0, 0.1, ptr(runs), ptr(rvals),
                                  ptr(inv_diag), ptr(rhs), stream()), reps)
     R = RunSystem(runs, rvals, inv_diag)
     nnz, nruns = R.nnz(), int((R.counts() > 0).sum().item())

@@ -188,7 +188,8 @@ typedef struct {
 typedef struct {
     ngs_grid cells;          /* cells containing >= 1 point */
     const int32_t *cell_start; /* ncells + 1 */
-    const int32_t *perm;     /* n: original point id of each bin slot */
+    const int32_t *perm;     /* n: original point id of each bin slot; NULL for the assembly calls
+                              * means their per-point coarse field is given in bin order */
     const float *b;          /* n x 9 basis values at level d, bin order */
     const float *alpha;      /* n screening weights in bin order, or NULL with alpha_scalar */
     float alpha_scalar;
@@ -262,6 +263,8 @@ int ngs_rhs_normals(ngs_level target, ngs_level data, const float *normals, doub
 /* The 125 same-level stiffness coefficients K(o), o in [-2,2]^3 lexicographic, of stride `stride`
  * (the K of ngs_lhs_assemble, computed the same way from the m = 1 tables). */
 int ngs_stencil_laplacian(int32_t stride, float *lap, ngs_stream_t s);
+/* dst[k] = src[idx[k]] (the coarse field in bin order, for the assembly with perm = NULL). */
+int ngs_gather_f32(const float *src, const int32_t *idx, int64_t n, float *dst, ngs_stream_t s);
 /* Bin-order copies of per-point data: point_cell[k] = cell[perm[k]] (n x 3), basis[k] = b[perm[k]]
  * (n x 9), alpha_out[k] = alpha[perm[k]] when alpha is not NULL. */
 int ngs_bin_points(const int32_t *perm, int64_t n, const int32_t *cell, const float *b, const float *alpha,

@@ -121,7 +121,9 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                         w = a * (bx[1 - ox] * by[1 - oy] * bz[1 - oz]);
                     }
                     rs += w;
-                    if (HAS_COARSE) rc = fmaf(w, coarse[pts.perm[k]], rc);
+                    // coarse is in bin order when the bins carry no permutation (a contiguous stream
+                    // instead of one random gather per visit; 27 rows visit every point)
+                    if (HAS_COARSE) rc = fmaf(w, coarse[pts.perm ? pts.perm[k] : k], rc);
 #pragma unroll
                     for (int vx = -1; vx <= 1; ++vx) {
 #pragma unroll
@@ -219,6 +221,12 @@ __global__ void stencil_laplacian_kernel(const float* __restrict__ F1, const flo
         const float g0 = G1[dx + 2] * inv_s, g1 = G1[dy + 2] * inv_s, g2 = G1[dz + 2] * inv_s;
         lap[t] = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
     }
+}
+
+__global__ void gather_f32_kernel(const float* __restrict__ src, const int32_t* __restrict__ idx, int64_t n,
+                                  float* __restrict__ dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[idx[k]];
 }
 
 __global__ void bin_points_kernel(const int32_t* __restrict__ perm, int64_t n, const int32_t* __restrict__ cell,
@@ -701,6 +709,14 @@ int ngs_stencil_laplacian(int32_t stride, float* lap, ngs_stream_t s) {
     Tables T;
     NGS_REQUIRE(tables_for(1, &T), "ngs_stencil_laplacian: integral tables not registered");
     stencil_laplacian_kernel<<<1, 128, 0, S(s)>>>(T.F, T.G, stride, lap);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_gather_f32(const float* src, const int32_t* idx, int64_t n, float* dst, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    NGS_REQUIRE(src && idx && dst, "ngs_gather_f32: null argument");
+    gather_f32_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(src, idx, n, dst);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -74,6 +74,7 @@ _SIGNATURES = {
     "ngs_ddi_backward": ([P, P, P, c_int64, c_int32, P, P, c_int32, P, P, P, P, P], c_int32),
     "ngs_additive_eval": ([P, P, P, c_int64, P, c_int32, c_float, P, P, P], c_int32),
     "ngs_additive_eval_backward": ([P, P, c_int64, P, c_int32, c_float, P, P, P, P], c_int32),
+    "ngs_gather_f32": ([P, P, c_int64, P, P], c_int32),
     "ngs_primal_mc_count": ([P, c_int64, c_int32, P, P], c_int32),
     "ngs_primal_mc_emit": ([P, P, c_int64, c_int32, P, P, P, P, P, P], c_int32),
     "ngs_primal_dedup_workspace_bytes": ([c_int64], c_int64),

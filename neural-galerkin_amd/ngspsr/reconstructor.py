@@ -123,11 +123,17 @@ class Reconstructor:
             system.diag_rhs(coarse, delta, rhs)
             return system
         inv_diag = torch.empty(M, dtype=torch.float32, device=dev)
+        pb = bins.struct()
+        if coarse is not None and bins.perm is not None and bins.n:
+            # the coarse field in bin order: the assembly then streams it with the binned points
+            cb = torch.empty(bins.n, dtype=torch.float32, device=dev)
+            call("ngs_gather_f32", ptr(coarse), ptr(bins.perm), bins.n, ptr(cb), stream())
+            coarse, pb.perm = cb, None
         if form == "runs":
             P = _solver.RunSystem.plane_stride(M)
             runs = torch.empty((25, P), dtype=torch.int32, device=dev)
             rvals = torch.empty((125, P), dtype=torch.float32, device=dev)
-            call("ngs_lhs_assemble_runs", lv, bins.struct(), ptr(coarse), float(sf), float(delta), ptr(runs),
+            call("ngs_lhs_assemble_runs", lv, pb, ptr(coarse), float(sf), float(delta), ptr(runs),
                  ptr(rvals), ptr(inv_diag), ptr(rhs), stream())
             return _solver.RunSystem(runs, rvals, inv_diag)
         if form != "csr":
@@ -140,7 +146,7 @@ class Reconstructor:
         nnz = int(rowptr[M].item())
         cols = torch.empty(nnz, dtype=torch.int32, device=dev)
         vals = torch.empty(nnz, dtype=torch.float32, device=dev)
-        call("ngs_lhs_assemble", lv, bins.struct(), ptr(coarse), float(sf), float(delta), ptr(rowptr), ptr(cols),
+        call("ngs_lhs_assemble", lv, pb, ptr(coarse), float(sf), float(delta), ptr(rowptr), ptr(cols),
              ptr(vals), ptr(inv_diag), ptr(rhs), stream())
         return _solver.CSRSystem(rowptr, cols, vals, inv_diag)
 
