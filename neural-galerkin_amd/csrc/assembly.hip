@@ -73,7 +73,7 @@ __device__ __forceinline__ float group_sum(float v) {
 }
 
 template <bool HAS_ALPHA, bool HAS_COARSE, bool RUNS, int G>
-__global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_point_bins pts,
+__global__ void __launch_bounds__(64, 3) lhs_assemble_kernel(ngs_level lv, ngs_point_bins pts,
                                                           const float* __restrict__ coarse, float sf, float sf_delta,
                                                           const float* __restrict__ F1, const float* __restrict__ G1,
                                                           const int32_t* __restrict__ rowptr, int32_t* __restrict__ cols,
