@@ -365,8 +365,8 @@ def main():
                          "launches of the finest-level SpMV; run under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding of the path is described "
-                                   "in DESIGN.md and is not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; the path's multi-GPU sharding (by request for serving, "
+                                   "by spatial slabs for one large solve) is not launched", "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
     import torch
