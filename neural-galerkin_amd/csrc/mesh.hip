@@ -934,8 +934,8 @@ int ngs_dual_accumulate(ngs_grid centres, const int32_t* primal, int64_t n, int3
 
 int ngs_dual_compact(const int32_t* acc, const int32_t* count, int32_t active_only, int64_t dc, int32_t* cubes,
                      int32_t* flags, int32_t* offsets, void* workspace, ngs_stream_t s) {
-    NGS_REQUIRE(workspace && flags && offsets, "ngs_dual_compact: bad arguments");
     if (dc <= 0) return NGS_OK;
+    NGS_REQUIRE(workspace && flags && offsets, "ngs_dual_compact: bad arguments");
     complete_flags_kernel<<<grid1d(dc, 256), 256, 0, S(s)>>>(count, dc, active_only ? 1 : 0, flags);
     NGS_CHECK_LAUNCH();
     int r = scan_i32_internal(flags, dc, offsets, workspace, S(s));

@@ -68,6 +68,13 @@ def reconstruct(xyz, normal, voxel_size: float, depth: int = 4, min_density: flo
     nrm_t = _to_device(normal, device)
     assert xyz_t.shape[0] == nrm_t.shape[0] and xyz_t.shape[1] == nrm_t.shape[1] == 3, \
         "Input positions and normals should have same size (N, 3)!"
+    if xyz_t.shape[0] == 0:
+        # no points, no surface: an empty mesh of the usual types (nothing is launched)
+        v = torch.zeros((0, 3), dtype=torch.float32, device=device)
+        f = torch.zeros((0, 3), dtype=torch.int64, device=device)
+        if return_result:
+            raise ValueError("reconstruct: no input points, so there is no solution field to return")
+        return (v.cpu().numpy(), f.cpu().numpy()) if is_numpy else (v, f)
     with torch.cuda.device(device):
         tree = HashTree(xyz_t, voxel_size=voxel_size, depth=depth)
         tree.build_encoder_hierarchy_adaptive(min_density=min_density, xyz_depth=xyz_depth)

@@ -421,11 +421,12 @@ class Reconstructor:
                 index.append(li)
                 base.append(bases_[d])
             nl = len(bitmaps)
-            garr = (Grid * nl)(*[g.struct() for g in bitmaps])
-            iarr = (ctypes.c_void_p * nl)(*[t.data_ptr() for t in index])
-            barr = (ctypes.c_int32 * nl)(*base)
-            call("ngs_dual_gather_cubes", ptr(cg.coords), DC, nl, garr, iarr, barr, ptr(values), ptr(acc), ptr(cnt),
-                 stream())
+            if nl:  # no leaves at all (the finer levels empty, reconstructor.py:322-326): no cubes, empty mesh
+                garr = (Grid * nl)(*[g.struct() for g in bitmaps])
+                iarr = (ctypes.c_void_p * nl)(*[t.data_ptr() for t in index])
+                barr = (ctypes.c_int32 * nl)(*base)
+                call("ngs_dual_gather_cubes", ptr(cg.coords), DC, nl, garr, iarr, barr, ptr(values), ptr(acc),
+                     ptr(cnt), stream())
         else:
             for d in range(L):
                 if not nsub[d]:

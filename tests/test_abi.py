@@ -64,6 +64,18 @@ def test_run_form_rejects_rows_beyond_run_words():
     assert b"too many rows" in lib.ngs_last_error()
 
 
+def test_zero_size_calls_need_no_buffers():
+    """Zero-size calls return before any argument is dereferenced or any kernel is launched, so empty
+    tensors (whose device pointers are null) can be passed through (the mesh path of a cloud whose
+    finer levels are empty makes such calls)."""
+    from ngspsr import _lib
+    lib = _lib.lib()
+    assert lib.ngs_dual_compact(None, None, 1, 0, None, None, None, None, None) == 0
+    assert lib.ngs_primal_mc_count(None, 0, 3, None, None) == 0
+    assert lib.ngs_field_support(_lib.Grid(), None, 0, 1.0, None, None) == 0
+    assert lib.ngs_ddi_forward(None, None, None, 0, 4, None, None, 5, None, None, None) == 0
+
+
 def test_solver_strings():
     """The reference's solver mini-language (solver.py:180-186) and its modes; values are parsed as
     literals, never evaluated."""
