@@ -264,6 +264,12 @@ class ReconstructionServer:
         try:
             with torch.cuda.stream(s):
                 req.start = time.perf_counter()
+                if req.xyz.shape[0] == 0:
+                    # no points, no surface (as reconstruct): nothing is launched
+                    req.result = (torch.zeros((0, 3), dtype=torch.float32, device=self.device),
+                                  torch.zeros((0, 3), dtype=torch.int64, device=self.device))
+                    req.max_rel_residual = 0.0
+                    return
                 tree = HashTree(req.xyz, req.voxel_size, req.depth)
                 tree.build_encoder_hierarchy_adaptive(min_density=req.min_density)
                 w = 1.0 / tree.xyz_density

@@ -55,3 +55,30 @@ def test_degenerate_clouds(cuda, kind):
         # a closed surface: the mesh lies near the sphere of radius 0.3 about the centre
         d = np.linalg.norm(v - 0.5, axis=1)
         assert f.shape[0] > 100 and np.abs(d - 0.3).max() < 2 / 32
+
+
+def test_serving_mixes_degenerate_requests(cuda):
+    """An empty request and a low-density one (finer levels empty) in a stream with ordinary ones:
+    every request completes, the degenerate ones with the empty mesh reconstruct gives them."""
+    import ngspsr
+    from ngspsr import serving, synthetic
+    print("[case] serving with degenerate requests", flush=True)
+    xyz, nrm = synthetic.ellipsoids(20000, k=4, seed=9)
+    few_x, few_n = synthetic.sphere(12, seed=1)
+    reqs_data = [(xyz, nrm), (np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32)), (few_x, few_n),
+                 (xyz[::2].copy(), nrm[::2].copy())]
+    srv = serving.ReconstructionServer(cuda, workers=4, tol=1e-6, refine=True)
+    try:
+        reqs = [serving.Request(i, torch.from_numpy(x).to(cuda), torch.from_numpy(n).to(cuda), 1 / 64, depth=4,
+                                min_density=1.0 if i != 2 else 32.0) for i, (x, n) in enumerate(reqs_data)]
+        srv.serve(reqs)
+    finally:
+        srv.close()
+    for r in reqs:
+        assert r.error is None, (r.rid, r.error)
+        v, f = r.result
+        _check_mesh(v.cpu().numpy(), f.cpu().numpy())
+    assert reqs[1].result[0].shape == (0, 3)
+    ref_v, ref_f = ngspsr.reconstruct(few_x, few_n, 1 / 64, depth=4, min_density=32.0)
+    assert reqs[2].result[1].shape[0] == ref_f.shape[0]
+    assert reqs[0].result[1].shape[0] > 1000 and reqs[3].result[1].shape[0] > 1000
