@@ -61,6 +61,14 @@ def reconstruct(xyz, normal, voxel_size: float, depth: int = 4, min_density: flo
     """
     if degree != 2:
         raise NotImplementedError("only the degree-2 B-spline basis is available (the reference's BezierTensorBasis)")
+    if not (voxel_size > 0 and np.isfinite(voxel_size)):
+        raise ValueError(f"reconstruct: voxel_size must be positive and finite, got {voxel_size}")
+    if int(depth) != depth or depth < 1:
+        raise ValueError(f"reconstruct: depth must be a positive integer, got {depth}")
+    for name, a in (("xyz", xyz), ("normal", normal)):
+        finite = bool(torch.isfinite(a).all()) if isinstance(a, torch.Tensor) else bool(np.isfinite(np.asarray(a)).all())
+        if not finite:
+            raise ValueError(f"reconstruct: {name} holds non-finite values")
     is_numpy = not isinstance(xyz, torch.Tensor)
     if device is None:
         device = xyz.device if (isinstance(xyz, torch.Tensor) and xyz.is_cuda) else torch.device("cuda", torch.cuda.current_device())

@@ -76,6 +76,26 @@ def test_zero_size_calls_need_no_buffers():
     assert lib.ngs_ddi_forward(None, None, None, 0, 4, None, None, 5, None, None, None) == 0
 
 
+def test_reconstruct_rejects_bad_arguments_before_device_work():
+    """Argument errors are raised on the host before any device is selected (no GPU needed)."""
+    import numpy as np
+    import ngspsr
+    xyz = np.random.default_rng(0).uniform(0.2, 0.8, (50, 3)).astype(np.float32)
+    nrm = np.tile(np.float32([[0.0, 0.0, 1.0]]), (50, 1))
+    bad = xyz.copy()
+    bad[3, 1] = np.nan
+    with pytest.raises(ValueError, match="non-finite"):
+        ngspsr.reconstruct(bad, nrm, 1 / 32)
+    with pytest.raises(ValueError, match="non-finite"):
+        ngspsr.reconstruct(xyz, np.where(nrm == 1.0, np.inf, nrm), 1 / 32)
+    with pytest.raises(ValueError, match="voxel_size"):
+        ngspsr.reconstruct(xyz, nrm, 0.0)
+    with pytest.raises(ValueError, match="depth"):
+        ngspsr.reconstruct(xyz, nrm, 1 / 32, depth=0)
+    with pytest.raises(NotImplementedError):
+        ngspsr.reconstruct(xyz, nrm, 1 / 32, degree=3)
+
+
 def test_solver_strings():
     """The reference's solver mini-language (solver.py:180-186) and its modes; values are parsed as
     literals, never evaluated."""
