@@ -65,10 +65,13 @@ def reconstruct(xyz, normal, voxel_size: float, depth: int = 4, min_density: flo
         raise ValueError(f"reconstruct: voxel_size must be positive and finite, got {voxel_size}")
     if int(depth) != depth or depth < 1:
         raise ValueError(f"reconstruct: depth must be a positive integer, got {depth}")
-    for name, a in (("xyz", xyz), ("normal", normal)):
-        finite = bool(torch.isfinite(a).all()) if isinstance(a, torch.Tensor) else bool(np.isfinite(np.asarray(a)).all())
-        if not finite:
-            raise ValueError(f"reconstruct: {name} holds non-finite values")
+    if isinstance(xyz, torch.Tensor) and isinstance(normal, torch.Tensor) and xyz.device == normal.device:
+        finite = bool(torch.isfinite(xyz).all() & torch.isfinite(normal).all())  # one device read
+    else:
+        finite = all(bool(torch.isfinite(a).all()) if isinstance(a, torch.Tensor) else bool(np.isfinite(np.asarray(a)).all())
+                     for a in (xyz, normal))
+    if not finite:
+        raise ValueError("reconstruct: xyz or normal holds non-finite values")
     is_numpy = not isinstance(xyz, torch.Tensor)
     if device is None:
         device = xyz.device if (isinstance(xyz, torch.Tensor) and xyz.is_cuda) else torch.device("cuda", torch.cuda.current_device())
