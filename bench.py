@@ -192,6 +192,70 @@ def fp64_error(csr, rhs, x, tol=1e-12, max_iter=3000):
             "stored_fp32_solution_rel_residual": float(opcg.relative_residual(a, xg, b))}
 
 
+def cpu_stages():
+    """Per-stage wall-clock of the oracle pipeline (numpy, with the C/OpenMP Jacobi-PCG in FP32 with
+    FP32 dots as the reference solves, to 1e-6) on the 100k-point torus workload (128^3, 4 levels),
+    one run per stage (the whole pipeline takes minutes), beside the GPU's end-to-end time for the same
+    reconstruction. Prints one JSON line."""
+    import numpy as np
+    import torch
+    import ngspsr
+    sys.path.insert(0, ROOT)
+    from oracle import assembly, mesh, pcg
+    cfg = CONFIGS["c1"]
+    xyz, nrm = make_scene(cfg)
+    t = {}
+    t0 = time.perf_counter()
+    h, lv, rho = assembly.build_tree(xyz, cfg["vs"], cfg["depth"], cfg["min_density"])
+    t["voxelise_density_levels"] = time.perf_counter() - t0
+    print(f"[cpu stage] voxelise_density_levels: {t['voxelise_density_levels']:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    nd = assembly.splat_normals(h, xyz, nrm, lv, 1.0 / rho)
+    t["splat"] = time.perf_counter() - t0
+    print(f"[cpu stage] splat: {t['splat']:.1f} s", file=sys.stderr, flush=True)
+    pcg_s = [0.0]
+
+    def solve(a, b):
+        t1 = time.perf_counter()
+        out = pcg.solve_pcg(a, b, tol=1e-6, max_iter=10000, dtype=np.float32)
+        pcg_s[0] += time.perf_counter() - t1
+        print(f"[cpu stage] level of {a.shape[0]} rows solved in {out[1]} iterations", file=sys.stderr, flush=True)
+        return out
+
+    t0 = time.perf_counter()
+    res = assembly.solve_multigrid(h, nd, xyz, 4.0, 0.1, solve=solve)
+    t["assembly_rhs"] = time.perf_counter() - t0 - pcg_s[0]
+    t["pcg"] = pcg_s[0]
+    print(f"[cpu stage] pcg: {t['pcg']:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    v, f, _ = mesh.extract_multiscale_mesh(h, res.solutions, 1.0 / rho, xyz)
+    t["mesh"] = time.perf_counter() - t0
+    print(f"[cpu stage] mesh: {t['mesh']:.1f} s", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    xt, nt = torch.from_numpy(xyz).to(dev), torch.from_numpy(nrm).to(dev)
+
+    def gpu():
+        return ngspsr.reconstruct(xt, nt, cfg["vs"], depth=cfg["depth"], min_density=cfg["min_density"],
+                                  screen_alpha=4.0, solver=cfg["solver"])
+    for _ in range(3):
+        gpu()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        vg, fg = gpu()
+    torch.cuda.synchronize()
+    g_s = (time.perf_counter() - t0) / 10
+    total = sum(t.values())
+    print(json.dumps({"cpu_stages_s": {k: round(v, 3) for k, v in t.items()}, "cpu_total_s": round(total, 2),
+                      "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+                      "workload": cfg["workload"], "voxels_by_level": [int(h.num_voxels(d)) for d in range(h.depth)],
+                      "cpu_pcg_iterations": {int(d): int(i) for d, i in res.iters.items()},
+                      "cpu_mesh_triangles": int(f.shape[0]), "gpu_ms": round(1e3 * g_s, 2),
+                      "gpu_triangles": int(fg.shape[0]), "cpu_over_gpu": round(total / g_s, 1),
+                      "note": "one run per CPU stage; numpy stages single-threaded, PCG on the OpenMP threads"}),
+          flush=True)
+
+
 COPY_BYTES = 1 << 30
 
 
@@ -380,6 +444,8 @@ def main():
     ap.add_argument("--no-extensions", action="store_true",
                     help="skip the field-query, primal marching cubes and DDI timings")
     ap.add_argument("--serving", type=int, default=64, help="requests of the serving comparison (0: skip)")
+    ap.add_argument("--cpu-stages", action="store_true",
+                    help="per-stage CPU timing of the oracle pipeline on the 100k-point torus workload (minutes)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter probe: one reconstruction, then 3 device copies of 1 GiB (calibration) and 5 "
                          "launches of the finest-level SpMV; run under rocprofv3 --pmc")
@@ -402,6 +468,9 @@ def main():
         return ngspsr.reconstruct(xyz, nrm, cfg["vs"], depth=cfg["depth"], min_density=cfg["min_density"],
                                   screen_alpha=4.0, solver=cfg["solver"], return_result=True)
 
+    if args.cpu_stages:
+        cpu_stages()
+        return
     if args.pmc_probe:
         pmc_probe(step())
         return
