@@ -296,7 +296,15 @@ class Reconstructor:
         from .ops import marching_cubes_op
         scale = self.hash_tree.get_stride(None, chi_depth)
         num_lif = base_coords.shape[0]
-        res = round((chi_field.shape[0] // max(num_lif, 1)) ** (1 / 3.0))
+        if num_lif == 0:  # no voxels, no surface
+            dev = self.hash_tree.device
+            vertices = torch.zeros((0, 3), dtype=torch.float32, device=dev)
+            triangles = torch.zeros((0, 3), dtype=torch.int64, device=dev)
+            if not build_o3d_mesh:
+                return vertices, triangles, torch.zeros((0, 3), dtype=torch.float32, device=dev)
+            from .io import triangle_mesh
+            return triangle_mesh(vertices, triangles, vertices)
+        res = round((chi_field.shape[0] // num_lif) ** (1 / 3.0))
         chi = chi_field.reshape(-1, res, res, res)
         vertices, triangles, normals, _ = marching_cubes_op(base_coords.float() / scale, chi)
         vertices = vertices * (scale * self.hash_tree.voxel_size)

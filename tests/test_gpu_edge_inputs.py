@@ -51,6 +51,14 @@ def test_degenerate_clouds(cuda, kind):
     v, f = ngspsr.reconstruct(xyz, nrm, 1 / 32, depth=3)
     assert isinstance(v, np.ndarray) and isinstance(f, np.ndarray)
     _check_mesh(v, f)
+    # the learned caller's path on the same field: an empty voxel set gives an empty mesh
+    rec = res.reconstructor
+    q, base = res.tree.get_test_grid(4, 0)
+    vv, ff, nn = rec.extract_mesh(base[:0], rec.evaluate_chi(q[:0]), 0, build_o3d_mesh=False)
+    assert vv.shape == (0, 3) and ff.shape == (0, 3) and nn.shape == (0, 3)
+    if base.shape[0]:
+        vv, ff, nn = rec.extract_mesh(base, rec.evaluate_chi(q), 0, build_o3d_mesh=False)
+        _check_mesh(vv.cpu().numpy(), ff.cpu().numpy())
     if kind == "sphere-numpy-float64":
         # a closed surface: the mesh lies near the sphere of radius 0.3 about the centre
         d = np.linalg.norm(v - 0.5, axis=1)
