@@ -144,6 +144,11 @@ int ngs_splat(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size, con
  * the derivative with respect to q (finest-voxel units), as evaluate_derivative returns it. */
 int ngs_eval_field(ngs_grid g, const float *field, const float *xyz, int64_t n, float inv_voxel_size,
                    float scale, int32_t accumulate, float *out, float *grad_out, ngs_stream_t s);
+/* ngs_eval_field with accumulate = 1 and no gradient, for the points binned at the field's level by
+ * ngs_bin_points (point_cell, basis in bin order, perm): slot k adds to out[perm[k]]. Bit-identical to
+ * ngs_eval_field on the same points; the bitmap and coefficient reads follow the bins. */
+int ngs_eval_field_binned(ngs_grid g, const float *field, const int32_t *point_cell, const float *basis,
+                          const int32_t *perm, int64_t n, float scale, float *out, ngs_stream_t s);
 /* Support of one level's field at the queries (hashtree.py:186-189, compute_mask): mask[p] = 1 (uint8,
  * caller-zeroed; other entries untouched) when any voxel of g is among the 27 that ngs_eval_field sums. */
 int ngs_field_support(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size, uint8_t *mask, ngs_stream_t s);

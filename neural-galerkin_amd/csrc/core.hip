@@ -518,6 +518,36 @@ __global__ void eval_field_kernel(ngs_grid g, const float* __restrict__ field, c
     }
 }
 
+// eval_field_kernel (accumulate, no gradient) for points binned at the field's level: each point's cell
+// and basis values come from the bins (point_basis_kernel's, the values axis_basis gives here too), the
+// 27 terms are summed in the same order, and slot k adds to out[perm[k]] (original point order), so
+// the result is bit-identical while the bitmap and coefficient reads follow the bins.
+__global__ void eval_field_binned_kernel(ngs_grid g, const float* __restrict__ field, const int32_t* __restrict__ cell,
+                                         const float* __restrict__ b, const int32_t* __restrict__ perm, int64_t n,
+                                         float scale, float* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float* bk = b + 9 * k;
+    const int l2 = g.log2_stride;
+    const int lx = (cell[3 * k] >> l2) - g.origin[0], ly = (cell[3 * k + 1] >> l2) - g.origin[1],
+              lz = (cell[3 * k + 2] >> l2) - g.origin[2];
+    float v = 0.f;
+    for (int kx = 0; kx < 3; ++kx)
+        for (int ky = 0; ky < 3; ++ky) {
+            int32_t id;
+            const uint32_t m = grid_column(g, lx + kx - 1, ly + ky - 1, lz - 1, 3, &id);
+            if (!m) continue;
+            for (int kz = 0; kz < 3; ++kz) {
+                if (!((m >> kz) & 1u)) continue;
+                const float f = field[id++];
+                v += (bk[kx] * bk[3 + ky] * bk[6 + kz]) * f;
+            }
+        }
+    const int64_t p = perm[k];
+    out[p] = out[p] + v * scale;
+}
+
 // Whether any voxel of g lies among the 27 around the cell of each query (the support test of
 // eval_field_kernel): mask[i] |= 1.
 __global__ void field_support_kernel(ngs_grid g, const float* __restrict__ xyz, int64_t n, float inv_vs,
@@ -826,6 +856,15 @@ int ngs_eval_field(ngs_grid g, const float* field, const float* xyz, int64_t n, 
                    int32_t accumulate, float* out, float* grad_out, ngs_stream_t s) {
     if (n <= 0) return NGS_OK;
     eval_field_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, field, xyz, n, inv_voxel_size, scale, accumulate, out, grad_out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_eval_field_binned(ngs_grid g, const float* field, const int32_t* point_cell, const float* basis,
+                          const int32_t* perm, int64_t n, float scale, float* out, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    NGS_REQUIRE(field && point_cell && basis && perm && out, "ngs_eval_field_binned: null argument");
+    eval_field_binned_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, field, point_cell, basis, perm, n, scale, out);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

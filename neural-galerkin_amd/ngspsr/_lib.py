@@ -67,6 +67,7 @@ _SIGNATURES = {
     "ngs_scan_i32": ([P, c_int64, P, P, P], c_int32),
     "ngs_splat": ([Grid, P, c_int64, c_float, P, c_int32, P, P, c_int32, c_double, P, P], c_int32),
     "ngs_eval_field": ([Grid, P, P, c_int64, c_float, c_float, c_int32, P, P, P], c_int32),
+    "ngs_eval_field_binned": ([Grid, P, P, P, P, c_int64, c_float, P, P], c_int32),
     "ngs_field_support": ([Grid, P, c_int64, c_float, P, P], c_int32),
     "ngs_grid_mark_dilated": ([Grid, P, c_int64, c_int32, P], c_int32),
     "ngs_test_grid_positions": ([P, c_int64, c_int32, c_int32, c_float, P, P], c_int32),

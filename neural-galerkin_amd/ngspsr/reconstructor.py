@@ -150,6 +150,17 @@ class Reconstructor:
              ptr(vals), ptr(inv_diag), ptr(rhs), stream())
         return _solver.CSRSystem(rowptr, cols, vals, inv_diag)
 
+    def _add_field_at_points(self, d, sol, xyz, bins, out):
+        """out += level d's field at the screening points (ngs_eval_field's values bit for bit), read
+        through the level's point bins when they hold these points."""
+        tree = self.hash_tree
+        if bins is not None and bins.perm is not None and bins.n == xyz.shape[0]:
+            call("ngs_eval_field_binned", tree.grids[d].struct(), ptr(sol), ptr(bins.point_cell), ptr(bins.b),
+                 ptr(bins.perm), bins.n, 1.0, ptr(out), stream())
+        else:
+            call("ngs_eval_field", tree.grids[d].struct(), ptr(sol), ptr(xyz), xyz.shape[0], tree.inv_vs, 1.0, 1,
+                 ptr(out), None, stream())
+
     def solve_multigrid(self, start_depth, end_depth, normal_data: dict, screen_alpha=0.0, screen_xyz=None,
                         screen_delta=0.1, solver: str = "pcg", verbose: bool = False, solve_fn=None):
         """reconstructor.py:79-195: coarse-to-fine per-level solves; each level's RHS is corrected by
@@ -190,8 +201,7 @@ class Reconstructor:
                     running = torch.zeros(xyz.shape[0], dtype=torch.float32, device=tree.device)
                 for dd in coarser:
                     if dd not in added:
-                        call("ngs_eval_field", tree.grids[dd].struct(), ptr(self.solutions[dd]), ptr(xyz),
-                             xyz.shape[0], tree.inv_vs, 1.0, 1, ptr(running), None, stream())
+                        self._add_field_at_points(dd, self.solutions[dd], xyz, bins[dd], running)
                         added.append(dd)
                 coarse = running
             for dd in coarser:
@@ -209,8 +219,7 @@ class Reconstructor:
             # completed with the remaining levels, the running sum is chi at the input points
             for dd, sol in self.solutions.items():
                 if sol.numel() and dd not in added:
-                    call("ngs_eval_field", tree.grids[dd].struct(), ptr(sol), ptr(xyz), xyz.shape[0], tree.inv_vs,
-                         1.0, 1, ptr(running), None, stream())
+                    self._add_field_at_points(dd, sol, xyz, bins[dd], running)
             self._points_chi = (self._solutions_key(), running)
         return self.solutions
 

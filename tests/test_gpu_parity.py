@@ -268,3 +268,15 @@ def test_field_queries_match_oracle(pair):
     assert torch.equal(c, full) and torch.equal(cm, m)
     assert torch.equal(rec.evaluate_chi(qt, max_points=-1), full)
     assert rec.evaluate_chi(qt[:0], max_points=100).shape == (0,)
+
+
+def test_points_chi_cache_equals_evaluation(pair):
+    """The raw chi at the input points accumulated during the solve (read through the point bins)
+    equals evaluate_raw_chi at the same points bit for bit; the level set follows from it."""
+    print("[case] points chi cache", flush=True)
+    rec = pair["res"].reconstructor
+    key, cached = rec._points_chi
+    assert rec._key_matches(key)
+    direct = rec.evaluate_raw_chi(rec.hash_tree.xyz)
+    torch.cuda.synchronize()
+    assert torch.equal(cached, direct), int((cached != direct).sum())
