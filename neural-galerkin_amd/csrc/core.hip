@@ -646,9 +646,22 @@ __global__ void __launch_bounds__(256) weighted_sum_kernel(const float* __restri
         a += __shfl_xor(a, d, 64);
         c += __shfl_xor(c, d, 64);
     }
+    // the block's 4 waves through LDS, then 2 atomics per block (per-wave atomics on the same two
+    // words serialise)
+    __shared__ double sa[4], sc[4];
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&sums[0], a);
-        atomicAdd(&sums[1], c);
+        sa[threadIdx.x >> 6] = a;
+        sc[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = sa[0], tc = sc[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            ta += sa[k];
+            tc += sc[k];
+        }
+        atomicAdd(&sums[0], ta);
+        atomicAdd(&sums[1], tc);
     }
 }
 
@@ -974,7 +987,7 @@ int ngs_weighted_sum(const float* v, const float* w, int64_t n, double* sums, ng
     zero_f64<<<1, 64, 0, st>>>(sums, 2);
     if (n > 0) {
         unsigned blocks = grid1d(n, 256);
-        if (blocks > 1024) blocks = 1024;
+        if (blocks > 256) blocks = 256;
         weighted_sum_kernel<<<blocks, 256, 0, st>>>(v, w, n, sums);
     }
     NGS_CHECK_LAUNCH();
