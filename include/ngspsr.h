@@ -98,7 +98,8 @@ int ngs_hash_query(const ngs_hash_table *t, const uint64_t *queries, int64_t n, 
 
 /* Words needed by a grid with these dims (0 if the box exceeds the capacity limit). */
 int64_t ngs_grid_words(const int32_t dims[3]);
-/* Device workspace bytes of the exclusive scans used below (n = words or rows). */
+/* Device workspace bytes of the exclusive scans used below (n = words or rows): the single-pass scan's
+ * look-back state as the device asks for it, so it needs a current HIP device (-1 without one). */
 int64_t ngs_scan_workspace_bytes(int64_t n);
 
 int ngs_grid_clear(ngs_grid g, ngs_stream_t s);

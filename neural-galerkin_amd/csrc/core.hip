@@ -11,118 +11,21 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 // ========================================================================================
-// Scan (exclusive prefix sums) — three-phase: block totals, scan of totals, block downsweep.
+// Scan (exclusive prefix sums): single-pass rocPRIM scans.
 // ========================================================================================
-constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+int scan_counts_device(const int32_t* counts, int64_t n, int32_t* offsets, void* tmp, size_t* tmp_bytes, hipStream_t st);
+int scan_popc_device(const uint32_t* words, int64_t n, int32_t* rank, void* tmp, size_t* tmp_bytes, hipStream_t st);
 
-struct LoadPopc {
-    const uint32_t* w;
-    __device__ int64_t operator()(int64_t i) const { return __popc(w[i]); }
-};
-struct LoadI32 {
-    const int32_t* v;
-    __device__ int64_t operator()(int64_t i) const { return v[i]; }
-};
-
-// Block-wide exclusive scan of one value per thread (256 threads). Returns the exclusive
-// prefix and writes the block total to *total.
-__device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
-    __shared__ int64_t wave_sums[SCAN_THREADS / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int64_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int64_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) wave_sums[wave] = x;
-    __syncthreads();
-    int64_t off = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < SCAN_THREADS / 64; ++w) {
-        if (w < wave) off += wave_sums[w];
-        tot += wave_sums[w];
-    }
-    __syncthreads();
-    *total = tot;
-    return off + x - v;
-}
-
-template <class Load>
-__global__ void __launch_bounds__(SCAN_THREADS) scan_tile_totals(Load ld, int64_t n, int64_t* tile_sums) {
-    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    int64_t s = 0;
-#pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; ++j)
-        if (base + j < n) s += ld(base + j);
-    int64_t tot;
-    block_exclusive_scan(s, &tot);
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
-}
-
-// Single block: in-place exclusive scan of nt tile totals; tile_sums[nt] = grand total.
-__global__ void __launch_bounds__(SCAN_THREADS) scan_tile_prefix(int64_t* tile_sums, int64_t nt) {
-    int64_t carry = 0;
-    for (int64_t base = 0; base < nt; base += SCAN_THREADS) {
-        int64_t i = base + threadIdx.x;
-        int64_t v = i < nt ? tile_sums[i] : 0;
-        int64_t tot;
-        int64_t ex = block_exclusive_scan(v, &tot);
-        if (i < nt) tile_sums[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) tile_sums[nt] = carry;
-}
-
-template <class Load>
-__global__ void __launch_bounds__(SCAN_THREADS) scan_downsweep(Load ld, int64_t n, const int64_t* tile_sums,
-                                                               int32_t* out) {
-    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-    int64_t vals[SCAN_ITEMS];
-    int64_t s = 0;
-#pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; ++j) {
-        vals[j] = (base + j < n) ? ld(base + j) : 0;
-        s += vals[j];
-    }
-    int64_t tot;
-    int64_t run = block_exclusive_scan(s, &tot) + tile_sums[blockIdx.x];
-#pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; ++j) {
-        if (base + j < n) out[base + j] = (int32_t)run;
-        run += vals[j];
-    }
-}
-
-__global__ void write_total_i32(const int64_t* tile_sums, int64_t nt, int32_t* dst) { *dst = (int32_t)tile_sums[nt]; }
-__global__ void write_total_i64(const int64_t* tile_sums, int64_t nt, int64_t* dst) { *dst = tile_sums[nt]; }
-
-static int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
-
-template <class Load>
-static int run_scan(Load ld, int64_t n, int32_t* out, int64_t* total64, int32_t* total32, void* ws, hipStream_t st) {
-    int64_t nt = scan_tiles(n);
-    int64_t* tiles = reinterpret_cast<int64_t*>(ws);
-    if (nt > 0) {
-        scan_tile_totals<<<(unsigned)nt, SCAN_THREADS, 0, st>>>(ld, n, tiles);
-        NGS_CHECK_LAUNCH();
-    }
-    scan_tile_prefix<<<1, SCAN_THREADS, 0, st>>>(tiles, nt);
-    NGS_CHECK_LAUNCH();
-    if (nt > 0) {
-        scan_downsweep<<<(unsigned)nt, SCAN_THREADS, 0, st>>>(ld, n, tiles, out);
-        NGS_CHECK_LAUNCH();
-    }
-    if (total64) write_total_i64<<<1, 1, 0, st>>>(tiles, nt, total64);
-    if (total32) write_total_i32<<<1, 1, 0, st>>>(tiles, nt, total32);
-    NGS_CHECK_LAUNCH();
-    return NGS_OK;
-}
-
+// offsets[0..n] <- exclusive scan of counts[0..n), offsets[n] = total. ws holds ngs_scan_workspace_bytes(n) bytes.
 int scan_i32_internal(const int32_t* counts, int64_t n, int32_t* offsets, void* ws, hipStream_t st) {
-    return run_scan(LoadI32{counts}, n, offsets, nullptr, offsets + n, ws, st);
+    size_t need = 0;
+    int r = scan_counts_device(nullptr, n, nullptr, nullptr, &need, st);
+    if (r != NGS_OK) return r;
+    return scan_counts_device(counts, n, offsets, ws, &need, st);
+}
+
+__global__ void popc_total_kernel(const uint32_t* words, const int32_t* rank, int64_t n, int64_t* total) {
+    *total = n > 0 ? (int64_t)rank[n - 1] + __popc(words[n - 1]) : 0;
 }
 
 // ========================================================================================
@@ -781,7 +684,15 @@ int64_t ngs_grid_words(const int32_t dims[3]) {
     return (bits + 31) / 32;
 }
 
-int64_t ngs_scan_workspace_bytes(int64_t n) { return (scan_tiles(n) + 2) * 8; }
+int64_t ngs_scan_workspace_bytes(int64_t n) {
+    // the size rocPRIM asks for on the current device (-1 without one): the larger of the two scan forms
+    size_t a = 0, b = 0;
+    n = n < 0 ? 0 : n;
+    if (scan_counts_device(nullptr, n, nullptr, nullptr, &a, nullptr) != NGS_OK) return -1;
+    if (scan_popc_device(nullptr, n, nullptr, nullptr, &b, nullptr) != NGS_OK) return -1;
+    size_t m = a > b ? a : b;
+    return m <= 256 ? 256 : (int64_t)((m + 255) / 256 * 256);  // never 0: callers pass a non-null buffer
+}
 
 int ngs_grid_clear(ngs_grid g, ngs_stream_t s) {
     NGS_REQUIRE(g.bits, "ngs_grid_clear: null bits");
@@ -829,7 +740,17 @@ int ngs_grid_mark_corners(ngs_grid g, const float* xyz, int64_t n, float inv_vox
 
 int ngs_grid_finalize(ngs_grid g, int64_t* count, void* workspace, ngs_stream_t s) {
     NGS_REQUIRE(g.bits && g.rank && workspace, "ngs_grid_finalize: null pointer");
-    return run_scan(LoadPopc{g.bits}, g.nwords, g.rank, count, nullptr, workspace, S(s));
+    hipStream_t st = S(s);
+    size_t need = 0;
+    int r = scan_popc_device(nullptr, g.nwords, nullptr, nullptr, &need, st);
+    if (r != NGS_OK) return r;
+    r = scan_popc_device(g.bits, g.nwords, g.rank, workspace, &need, st);
+    if (r != NGS_OK) return r;
+    if (count) {
+        popc_total_kernel<<<1, 1, 0, st>>>(g.bits, g.rank, g.nwords, count);
+        NGS_CHECK_LAUNCH();
+    }
+    return NGS_OK;
 }
 
 int ngs_grid_emit(ngs_grid g, int32_t* coords_out, ngs_stream_t s) {
@@ -933,12 +854,14 @@ static int bucket_layout(int64_t n, int64_t ncells, size_t* sort_tmp, int64_t* o
     size_t tmp = 0;
     int r = sort_pairs_u32(nullptr, nullptr, nullptr, nullptr, n, 32, nullptr, &tmp, nullptr);
     if (r != NGS_OK) return r;
+    const int64_t scan_bytes = ngs_scan_workspace_bytes(ncells);
+    if (scan_bytes < 0) return NGS_ERR_HIP;
     auto al = [](int64_t x) { return (x + 255) / 256 * 256; };
     *sort_tmp = tmp;
     *off_keys = al((int64_t)tmp);
     *off_vin = *off_keys + al(n * 4);
     *off_scan = *off_vin + al(n * 4);
-    *off_counts = *off_scan + al(ngs_scan_workspace_bytes(ncells));
+    *off_counts = *off_scan + al(scan_bytes);
     *total = *off_counts + al(ncells * 4);
     return NGS_OK;
 }

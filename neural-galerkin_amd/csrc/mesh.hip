@@ -978,7 +978,9 @@ static int dedup_layout(int64_t n, int key_bits, size_t* tmp, size_t off[5], siz
     off[2] = off[1] + al256(4 * (size_t)n);     // heads
     off[3] = off[2] + al256(4 * (size_t)n);     // offsets (n + 1)
     off[4] = off[3] + al256(4 * (size_t)(n + 1));
-    *total = off[4] + al256((size_t)ngs_scan_workspace_bytes(n));
+    const int64_t scan_bytes = ngs_scan_workspace_bytes(n);
+    if (scan_bytes < 0) return NGS_ERR_HIP;
+    *total = off[4] + al256((size_t)scan_bytes);
     return NGS_OK;
 }
 
@@ -995,7 +997,9 @@ static int primal_layout(int64_t n, size_t* tmp, size_t off[8], size_t* total) {
     off[5] = off[4] + al256(4 * (size_t)n);     // offsets (n + 1)
     off[6] = off[5] + al256(4 * (size_t)(n + 1));  // run starts
     off[7] = off[6] + al256(4 * (size_t)n);     // scan workspace
-    *total = off[7] + al256((size_t)ngs_scan_workspace_bytes(n));
+    const int64_t scan_bytes = ngs_scan_workspace_bytes(n);
+    if (scan_bytes < 0) return NGS_ERR_HIP;
+    *total = off[7] + al256((size_t)scan_bytes);
     return NGS_OK;
 }
 

@@ -1,5 +1,9 @@
-// Stable radix sorts (rocPRIM) used for point bucketing and vertex de-duplication.
+// rocPRIM device primitives: stable radix sorts (point bucketing, vertex de-duplication) and
+// single-pass exclusive scans (decoupled look-back: one read of the input, one write of the output).
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include "common.h"
 
@@ -23,6 +27,38 @@ int sort_pairs_u64(const uint64_t* kin, uint64_t* kout, const int32_t* vin, int3
         return NGS_ERR_HIP;
     }
     return NGS_OK;
+}
+
+namespace {
+// Element i of the (n + 1)-long sequence counts[0], ..., counts[n - 1], 0: its exclusive scan ends with the total.
+struct PaddedCount {
+    const int32_t* v;
+    int64_t n;
+    __host__ __device__ int32_t operator()(int64_t i) const { return i < n ? v[i] : 0; }
+};
+struct WordPopc {
+    const uint32_t* w;
+    __host__ __device__ int32_t operator()(int64_t i) const { return __builtin_popcount(w[i]); }
+};
+
+template <class F>
+int scan_seq(F f, int64_t len, int32_t* out, void* tmp, size_t* tmp_bytes, hipStream_t st, const char* what) {
+    auto in = rocprim::make_transform_iterator(rocprim::counting_iterator<int64_t>(0), f);
+    hipError_t e = rocprim::exclusive_scan(tmp, *tmp_bytes, in, out, int32_t(0), (size_t)len, rocprim::plus<int32_t>(), st);
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return NGS_ERR_HIP;
+    }
+    return NGS_OK;
+}
+}  // namespace
+
+int scan_counts_device(const int32_t* counts, int64_t n, int32_t* offsets, void* tmp, size_t* tmp_bytes, hipStream_t st) {
+    return scan_seq(PaddedCount{counts, n}, n + 1, offsets, tmp, tmp_bytes, st, "exclusive_scan(i32)");
+}
+
+int scan_popc_device(const uint32_t* words, int64_t n, int32_t* rank, void* tmp, size_t* tmp_bytes, hipStream_t st) {
+    return scan_seq(WordPopc{words}, n, rank, tmp, tmp_bytes, st, "exclusive_scan(popc)");
 }
 
 }  // namespace ngs
