@@ -117,6 +117,14 @@ int ngs_test_grid_positions(const int32_t *base, int64_t n, int32_t stride, int3
  * _trilinear_weights :469-510) up to the sort, which the bitmap order provides. */
 int ngs_grid_mark_corners(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size,
                           const int32_t *levels, int32_t max_level, ngs_stream_t s);
+/* Byte-staged forms of ngs_grid_mark_coords and ngs_grid_mark_corners (same members, same reference
+ * lines): each member is recorded by a plain byte store marks[lattice index] = 1, then the bytes are
+ * ORed into the bitmap. marks: device buffer of nwords * 32 bytes, 16-byte aligned, all zero on entry
+ * and left all zero on return (the calls that share one marks buffer must be stream-ordered). Many
+ * points that share a coarse voxel then cost merged stores instead of serialised atomics. */
+int ngs_grid_mark_coords_staged(ngs_grid g, const int32_t *coords, int64_t n, uint8_t *marks, ngs_stream_t s);
+int ngs_grid_mark_corners_staged(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size,
+                                 const int32_t *levels, int32_t max_level, uint8_t *marks, ngs_stream_t s);
 /* Fill rank[] and write the member count to *count (device int64). */
 int ngs_grid_finalize(ngs_grid g, int64_t *count, void *workspace, ngs_stream_t s);
 /* Write the members in rank order (the reference's torch.unique(dim=0) order). */

@@ -194,6 +194,38 @@ __global__ void mark_coords_kernel(ngs_grid g, const int32_t* __restrict__ c, in
     grid_mark(g, c[3 * i], c[3 * i + 1], c[3 * i + 2]);
 }
 
+// Byte-staged marking: marks[lattice index] = 1 by a plain store. Stores of the same byte from many
+// lanes and waves merge in the caches, where a device-scope atomic OR per word serialises whenever
+// many points share a coarse voxel. fold_marks_kernel then ORs each word's 32 bytes into the bitmap
+// and clears them, so the byte map is all zero again after every staged call.
+__device__ __forceinline__ void stage_mark(const ngs_grid& g, uint8_t* marks, int cx, int cy, int cz) {
+    int64_t lin;
+    if (grid_cell(g, cx, cy, cz, &lin)) marks[lin] = 1;
+}
+
+__global__ void stage_coords_kernel(ngs_grid g, const int32_t* __restrict__ c, int64_t n, uint8_t* marks) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    stage_mark(g, marks, c[3 * i], c[3 * i + 1], c[3 * i + 2]);
+}
+
+// Bit k of the result = byte k of x, for bytes that are 0 or 1.
+__device__ __forceinline__ uint32_t pack4(uint32_t x) { return (x | (x >> 7) | (x >> 14) | (x >> 21)) & 0xFu; }
+
+__global__ void fold_marks_kernel(uint32_t* __restrict__ bits, uint8_t* __restrict__ marks, int64_t nwords) {
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    uint4* p = reinterpret_cast<uint4*>(marks + 32 * w);
+    const uint4 a = p[0], b = p[1];
+    const uint32_t m = pack4(a.x) | (pack4(a.y) << 4) | (pack4(a.z) << 8) | (pack4(a.w) << 12) | (pack4(b.x) << 16) |
+                       (pack4(b.y) << 20) | (pack4(b.z) << 24) | (pack4(b.w) << 28);
+    if (m) {
+        bits[w] |= m;
+        p[0] = make_uint4(0u, 0u, 0u, 0u);
+        p[1] = make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
 // Marks c + (dx, dy, dz) * stride for every row c and every offset in [-radius, radius]^3; one thread
 // per (row, dx, dy), walking dz.
 __global__ void mark_dilated_kernel(ngs_grid g, const int32_t* __restrict__ c, int64_t n, int radius) {
@@ -230,6 +262,25 @@ __global__ void test_grid_kernel(const int32_t* __restrict__ base, int64_t n, in
         const float b = (float)base[3 * l + a];
         const float box = res == 1 ? 0.0f : linspace_value((float)stride, res, idx[a]);
         out[3 * t + a] = (b + box) * vs;
+    }
+}
+
+// The 8 corners of mark_corners_kernel, byte-staged.
+__global__ void stage_corners_kernel(ngs_grid g, const float* __restrict__ xyz, int64_t n, float inv_vs,
+                                     const int32_t* __restrict__ levels, int max_level, uint8_t* marks) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (levels && levels[i] > max_level) return;
+    const int s = 1 << g.log2_stride;
+    const float inv_s = 1.0f / (float)s;
+    int dc[3], sg[3];
+    float loc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) trilinear_axis(xyz[3 * i + a] * inv_vs, s, inv_s, &dc[a], &sg[a], &loc[a]);
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        const int nx = (o >> 2) & 1, ny = (o >> 1) & 1, nz = o & 1;
+        stage_mark(g, marks, dc[0] + nx * sg[0] * s, dc[1] + ny * sg[1] * s, dc[2] + nz * sg[2] * s);
     }
 }
 
@@ -709,6 +760,21 @@ int ngs_grid_mark_coords(ngs_grid g, const int32_t* coords, int64_t n, ngs_strea
     return NGS_OK;
 }
 
+static int fold_marks(const ngs_grid& g, uint8_t* marks, hipStream_t st) {
+    fold_marks_kernel<<<grid1d(g.nwords, 256), 256, 0, st>>>(g.bits, marks, g.nwords);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_grid_mark_coords_staged(ngs_grid g, const int32_t* coords, int64_t n, uint8_t* marks, ngs_stream_t s) {
+    if (n <= 0 || g.nwords <= 0) return NGS_OK;
+    NGS_REQUIRE(coords && marks && g.bits, "ngs_grid_mark_coords_staged: null argument");
+    NGS_REQUIRE((reinterpret_cast<uintptr_t>(marks) & 15) == 0, "ngs_grid_mark_coords_staged: marks not 16-byte aligned");
+    stage_coords_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, coords, n, marks);
+    NGS_CHECK_LAUNCH();
+    return fold_marks(g, marks, S(s));
+}
+
 int ngs_grid_mark_dilated(ngs_grid g, const int32_t* coords, int64_t n, int32_t radius, ngs_stream_t s) {
     NGS_REQUIRE(radius >= 0 && radius <= 64, "ngs_grid_mark_dilated: radius must be in [0, 64]");
     const int64_t t = n * (2 * radius + 1) * (2 * radius + 1);
@@ -736,6 +802,16 @@ int ngs_grid_mark_corners(ngs_grid g, const float* xyz, int64_t n, float inv_vox
     mark_corners_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, xyz, n, inv_voxel_size, levels, max_level);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
+}
+
+int ngs_grid_mark_corners_staged(ngs_grid g, const float* xyz, int64_t n, float inv_voxel_size, const int32_t* levels,
+                                 int32_t max_level, uint8_t* marks, ngs_stream_t s) {
+    if (n <= 0 || g.nwords <= 0) return NGS_OK;
+    NGS_REQUIRE(xyz && marks && g.bits, "ngs_grid_mark_corners_staged: null argument");
+    NGS_REQUIRE((reinterpret_cast<uintptr_t>(marks) & 15) == 0, "ngs_grid_mark_corners_staged: marks not 16-byte aligned");
+    stage_corners_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, xyz, n, inv_voxel_size, levels, max_level, marks);
+    NGS_CHECK_LAUNCH();
+    return fold_marks(g, marks, S(s));
 }
 
 int ngs_grid_finalize(ngs_grid g, int64_t* count, void* workspace, ngs_stream_t s) {

@@ -59,6 +59,8 @@ _SIGNATURES = {
     "ngs_grid_clear": ([Grid, P], c_int32),
     "ngs_grid_mark_coords": ([Grid, P, c_int64, P], c_int32),
     "ngs_grid_mark_corners": ([Grid, P, c_int64, c_float, P, c_int32, P], c_int32),
+    "ngs_grid_mark_coords_staged": ([Grid, P, c_int64, P, P], c_int32),
+    "ngs_grid_mark_corners_staged": ([Grid, P, c_int64, c_float, P, c_int32, P, P], c_int32),
     "ngs_grid_finalize": ([Grid, P, P, P], c_int32),
     "ngs_grid_emit": ([Grid, P, P], c_int32),
     "ngs_grid_query": ([Grid, P, c_int64, P, c_int32, P, P], c_int32),
@@ -202,6 +204,25 @@ def stream():
     if _raw_stream is not None and _cur_device is not None:
         return ctypes.c_void_p(_raw_stream(_cur_device()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Marks:
+    """Per-thread, per-device all-zero byte maps for the byte-staged grid marking calls, which leave
+    the map all zero again (stream-ordered use, as Scratch)."""
+    _tls = threading.local()
+    CAP = 1 << 31  # bytes; larger boxes mark with atomics
+
+    @classmethod
+    def get(cls, nbytes, device):
+        maps = getattr(cls._tls, "maps", None)
+        if maps is None:
+            maps = cls._tls.maps = {}
+        key = str(device)
+        buf = maps.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.zeros(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+            maps[key] = buf
+        return buf
 
 
 class Scratch:

@@ -5,7 +5,7 @@ import math
 import torch
 
 from . import _lib
-from ._lib import Grid, HashTable, NgsError, Scratch, call, lib, ptr, stream
+from ._lib import Grid, HashTable, Marks, NgsError, Scratch, call, lib, ptr, stream
 
 
 def log2_int(s: int) -> int:
@@ -65,9 +65,31 @@ class SparseGrid:
         g.rank = self.rank.data_ptr()
         return g
 
+    def _marks(self, nmarks):
+        """The zeroed byte map of the staged marking calls, or None (atomic marking) where the fold,
+        which reads one byte per lattice point of the box, would cost more than the atomics it saves
+        (more than 64 lattice points per mark) or the box exceeds Marks.CAP lattice points."""
+        nb = self.nwords * 32
+        return Marks.get(nb, self.device) if nb <= min(Marks.CAP, 64 * nmarks) else None
+
     def mark_coords(self, coords):
         if coords.shape[0]:
-            call("ngs_grid_mark_coords", self.struct(), ptr(coords), coords.shape[0], stream())
+            m = self._marks(coords.shape[0])
+            if m is None:
+                call("ngs_grid_mark_coords", self.struct(), ptr(coords), coords.shape[0], stream())
+            else:
+                call("ngs_grid_mark_coords_staged", self.struct(), ptr(coords), coords.shape[0], ptr(m), stream())
+
+    def mark_corners(self, xyz, inv_vs, levels=None, max_level=0):
+        """The 8 trilinear corners at the grid stride of the points with level <= max_level."""
+        if xyz.shape[0]:
+            m = self._marks(8 * xyz.shape[0])
+            if m is None:
+                call("ngs_grid_mark_corners", self.struct(), ptr(xyz), xyz.shape[0], inv_vs, ptr(levels), max_level,
+                     stream())
+            else:
+                call("ngs_grid_mark_corners_staged", self.struct(), ptr(xyz), xyz.shape[0], inv_vs, ptr(levels),
+                     max_level, ptr(m), stream())
 
     def finalize(self, emit=True) -> int:
         cnt = torch.empty(1, dtype=torch.int64, device=self.device)

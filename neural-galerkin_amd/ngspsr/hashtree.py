@@ -158,8 +158,7 @@ class HashTree:
 
     def _voxelise(self, stride, inv_vs, qmin, qmax, levels=None, max_level=0):
         g = SparseGrid.for_points(qmin, qmax, stride, self.device)
-        call("ngs_grid_mark_corners", g.struct(), ptr(self.xyz), self.xyz.shape[0], inv_vs, ptr(levels), max_level,
-             stream())
+        g.mark_corners(self.xyz, inv_vs, levels, max_level)
         g.finalize()
         return g
 
