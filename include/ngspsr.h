@@ -440,7 +440,9 @@ int ngs_mark_dual_leaf_words(ngs_grid centres, ngs_grid leaf_bitmap, ngs_stream_
  * fine to coarse (at most 8), a finalized bitmap of its leaves, a device map leaf_index[l][rank in
  * that bitmap] = the leaf's position in the level's leaf list, and id_base[l] (ids id_base + 8
  * position + sub-cell). leaf_bitmaps, leaf_index and id_base are host arrays. count: slots in bits
- * 0..15, slots with a negative value in bits 16..31 (values nullable). */
+ * 0..15, slots with a negative value in bits 16..31 (values nullable). The 8 slot ids are written,
+ * and the negative slots counted, for complete centres only (8 slots; the only ones
+ * ngs_dual_compact reads); the acc rows of the other centres are left as they were. */
 int ngs_dual_gather_cubes(const int32_t *centre_coords, int64_t dc, int32_t nlevels, const ngs_grid *leaf_bitmaps,
                           const int32_t *const *leaf_index, const int32_t *id_base, const float *values, int32_t *acc,
                           int32_t *count, ngs_stream_t s);

@@ -624,14 +624,16 @@ __global__ void dual_gather_cubes_kernel(const int32_t* __restrict__ centres, in
             }
         }
     }
+    // only complete centres become cubes: the others write no slots and read no values
     int neg = 0;
-    if (values) {
+    if (n == 8) {
+        if (values) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) neg += (ids[k] >= 0 && values[ids[k]] < 0.f) ? 1 : 0;
+            for (int k = 0; k < 8; ++k) neg += values[ids[k]] < 0.f ? 1 : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[dc * 8 + k] = ids[k];
     }
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-        if (ids[k] >= 0) acc[dc * 8 + k] = ids[k];
     cnt[dc] = n | (neg << COUNT_BITS);
 }
 
