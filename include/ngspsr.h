@@ -147,6 +147,17 @@ int ngs_scan_i32(const int32_t *counts, int64_t n, int32_t *offsets, void *works
 int ngs_splat(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size, const float *data,
               int32_t ncomp, const float *scale, const int32_t *levels, int32_t level, double post,
               double *acc, ngs_stream_t s);
+/* The same splat as a gather (no FP64 atomics): points binned by the rank in g of the lower node of
+ * their trilinear stencil (a radix sort), then one thread per node of g sums its 8 neighbouring bins
+ * in FP64 with the same FP32 weights. node_coords: g's members in rank order (count x 3, as
+ * ngs_grid_emit writes them). A point whose lower node is not in g is splatted with atomics. The FP64
+ * sums of FP32 terms here are exact, so the result equals ngs_splat's. Workspace from
+ * ngs_splat_gather_workspace_bytes(n, count) (-1: sizes beyond 2^31 or no device). */
+int64_t ngs_splat_gather_workspace_bytes(int64_t n, int64_t count);
+int ngs_splat_gather(ngs_grid g, const int32_t *node_coords, int64_t count, const float *xyz, int64_t n,
+                     float inv_voxel_size, const float *data, int32_t ncomp, const float *scale,
+                     const int32_t *levels, int32_t level, double post, double *acc, void *workspace,
+                     int64_t workspace_bytes, ngs_stream_t s);
 /* Basis-interpolated field at points (hashtree.py:157-192, evaluate_interpolated with the Bezier
  * basis): out[p] = (accumulate ? out[p] : 0) + scale * sum_i field[i] B((q_p - c_i)/s) over the 27
  * voxels of g around the cell of q_p = xyz_p * inv_voxel_size. grad_out (nullable, n x 3) receives

@@ -448,6 +448,116 @@ __global__ void splat_kernel(ngs_grid g, const float* __restrict__ xyz, int64_t 
     }
 }
 
+// Gather form of the splat. Each point is binned by the rank of the lower node of its trilinear
+// stencil (min(dc, dc + sg s) per axis); one thread per node then sums, in FP64 registers, the points
+// of the 8 bins whose lower node is node - o s, o in {0, 1}^3, with the weights splat_kernel gives
+// them (same float operations). Points outside `level` get key `count` (sorted last, never read);
+// a point whose lower node is not in g is splatted with atomics here (the gather cannot reach it).
+__global__ void splat_key_kernel(ngs_grid g, const float* __restrict__ xyz, int64_t n, float inv_vs,
+                                 const float* __restrict__ data, int ncomp, const float* __restrict__ scale,
+                                 const int32_t* __restrict__ levels, int level, double post, double* __restrict__ acc,
+                                 uint32_t count, uint32_t* __restrict__ keys, int32_t* __restrict__ idx) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    idx[i] = (int32_t)i;
+    keys[i] = count;
+    if (levels && levels[i] != level) return;
+    const int s = 1 << g.log2_stride;
+    const float inv_s = 1.0f / (float)s;
+    int dc[3], sg[3];
+    float loc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) trilinear_axis(xyz[3 * i + a] * inv_vs, s, inv_s, &dc[a], &sg[a], &loc[a]);
+    const int32_t r = grid_find(g, sg[0] < 0 ? dc[0] - s : dc[0], sg[1] < 0 ? dc[1] - s : dc[1],
+                                sg[2] < 0 ? dc[2] - s : dc[2]);
+    if (r >= 0) {
+        keys[i] = (uint32_t)r;
+        return;
+    }
+    float row[3] = {1.f, 1.f, 1.f};
+    if (data) {
+        float sc = scale ? scale[i] : 1.0f;
+        for (int c = 0; c < ncomp; ++c) row[c] = data[i * ncomp + c] * sc;
+    }
+    for (int o = 0; o < 8; ++o) {
+        int nx = (o >> 2) & 1, ny = (o >> 1) & 1, nz = o & 1;
+        int id = grid_find(g, dc[0] + nx * sg[0] * s, dc[1] + ny * sg[1] * s, dc[2] + nz * sg[2] * s);
+        if (id < 0) continue;
+        float ax = nx ? loc[0] : 1.0f - loc[0];
+        float ay = ny ? loc[1] : 1.0f - loc[1];
+        float az = nz ? loc[2] : 1.0f - loc[2];
+        float w = ax * ay * az;
+        if (data) {
+            for (int c = 0; c < ncomp; ++c) atomicAdd(&acc[(int64_t)id * ncomp + c], (double)(w * row[c]) * post);
+        } else {
+            atomicAdd(&acc[id], (double)w * post);
+        }
+    }
+}
+
+// Bin bounds over the sorted keys: [bstart[k], bend[k]) holds the points of key k (both zeroed first).
+__global__ void splat_bins_kernel(const uint32_t* __restrict__ ks, int64_t n, uint32_t count, int32_t* __restrict__ bstart,
+                                  int32_t* __restrict__ bend) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = ks[i];
+    if (k >= count) return;
+    if (i == 0 || ks[i - 1] != k) bstart[k] = (int32_t)i;
+    if (i == n - 1 || ks[i + 1] != k) bend[k] = (int32_t)(i + 1);
+}
+
+__global__ void splat_gather_kernel(ngs_grid g, const int32_t* __restrict__ nodes, int64_t count,
+                                    const float* __restrict__ xyz, float inv_vs, const float* __restrict__ data,
+                                    int ncomp, const float* __restrict__ scale, const int32_t* __restrict__ order,
+                                    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bend, double post,
+                                    double* __restrict__ acc) {
+#pragma clang fp contract(off)
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= count) return;
+    const int s = 1 << g.log2_stride;
+    const float inv_s = 1.0f / (float)s;
+    const int N[3] = {nodes[3 * node], nodes[3 * node + 1], nodes[3 * node + 2]};
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (int o = 0; o < 8; ++o) {
+        const int32_t r = grid_find(g, N[0] - ((o >> 2) & 1) * s, N[1] - ((o >> 1) & 1) * s, N[2] - (o & 1) * s);
+        if (r < 0) continue;
+        const int32_t e = bend[r];
+        for (int32_t j = bstart[r]; j < e; ++j) {
+            const int32_t p = order[j];
+            int dc[3], sg[3];
+            float loc[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) trilinear_axis(xyz[3 * p + a] * inv_vs, s, inv_s, &dc[a], &sg[a], &loc[a]);
+            // which stencil node of p this node is, per axis (none: p's bin neighbours this node only)
+            int nn[3];
+            bool hit = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (N[a] == dc[a]) nn[a] = 0;
+                else if (sg[a] != 0 && N[a] == dc[a] + sg[a] * s) nn[a] = 1;
+                else hit = false;
+            }
+            if (!hit) continue;
+            float ax = nn[0] ? loc[0] : 1.0f - loc[0];
+            float ay = nn[1] ? loc[1] : 1.0f - loc[1];
+            float az = nn[2] ? loc[2] : 1.0f - loc[2];
+            float w = ax * ay * az;
+            if (data) {
+                float sc = scale ? scale[p] : 1.0f;
+                for (int c = 0; c < ncomp; ++c) sum[c] += (double)(w * (data[(int64_t)p * ncomp + c] * sc)) * post;
+            } else {
+                sum[0] += (double)w * post;
+            }
+        }
+    }
+    if (data) {
+        for (int c = 0; c < ncomp; ++c) acc[node * ncomp + c] += sum[c];
+    } else {
+        acc[node] += sum[0];
+    }
+}
+
 __global__ void eval_field_kernel(ngs_grid g, const float* __restrict__ field, const float* __restrict__ xyz, int64_t n,
                                   float inv_vs, float scale, int accumulate, float* __restrict__ out,
                                   float* __restrict__ grad) {
@@ -875,6 +985,69 @@ int ngs_splat(ngs_grid g, const float* xyz, int64_t n, float inv_voxel_size, con
     NGS_REQUIRE(!data || (ncomp >= 1 && ncomp <= 3), "ngs_splat: ncomp must be 1..3");
     if (n <= 0) return NGS_OK;
     splat_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, xyz, n, inv_voxel_size, data, ncomp, scale, levels, level, post, acc);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+static int splat_gather_layout(int64_t n, int64_t count, size_t* tmp, size_t off[6], size_t* total) {
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    const int bits = 32 - __builtin_clz((uint32_t)count);  // the keys run up to count
+    size_t t = 0;
+    int r = sort_pairs_u32(nullptr, nullptr, nullptr, nullptr, n, bits, nullptr, &t, nullptr);
+    if (r != NGS_OK) return r;
+    *tmp = t;
+    off[0] = al(t);                               // keys
+    off[1] = off[0] + al(4 * (size_t)n);          // sorted keys
+    off[2] = off[1] + al(4 * (size_t)n);          // point ids
+    off[3] = off[2] + al(4 * (size_t)n);          // point ids in key order
+    off[4] = off[3] + al(4 * (size_t)n);          // bin starts (count)
+    off[5] = off[4] + al(4 * (size_t)count);      // bin ends (count)
+    *total = off[5] + al(4 * (size_t)count);
+    return NGS_OK;
+}
+
+int64_t ngs_splat_gather_workspace_bytes(int64_t n, int64_t count) {
+    if (n <= 0 || count <= 0) return 256;
+    if (count >= ((int64_t)1 << 31) || n >= ((int64_t)1 << 31)) return -1;
+    size_t tmp, off[6], total;
+    if (splat_gather_layout(n, count, &tmp, off, &total) != NGS_OK) return -1;
+    return (int64_t)total;
+}
+
+int ngs_splat_gather(ngs_grid g, const int32_t* node_coords, int64_t count, const float* xyz, int64_t n,
+                     float inv_voxel_size, const float* data, int32_t ncomp, const float* scale, const int32_t* levels,
+                     int32_t level, double post, double* acc, void* workspace, int64_t workspace_bytes, ngs_stream_t s) {
+    NGS_REQUIRE(!data || (ncomp >= 1 && ncomp <= 3), "ngs_splat_gather: ncomp must be 1..3");
+    if (n <= 0 || count <= 0) return NGS_OK;
+    NGS_REQUIRE(count < ((int64_t)1 << 31) && n < ((int64_t)1 << 31), "ngs_splat_gather: more than 2^31 points or nodes");
+    NGS_REQUIRE(node_coords && xyz && acc && workspace && g.bits && g.rank, "ngs_splat_gather: null argument");
+    size_t tmp, off[6], total;
+    int r = splat_gather_layout(n, count, &tmp, off, &total);
+    if (r != NGS_OK) return r;
+    NGS_REQUIRE((size_t)workspace_bytes >= total, "ngs_splat_gather: workspace too small");
+    char* w = reinterpret_cast<char*>(workspace);
+    hipStream_t st = S(s);
+    uint32_t* keys = reinterpret_cast<uint32_t*>(w + off[0]);
+    uint32_t* ks = reinterpret_cast<uint32_t*>(w + off[1]);
+    int32_t* idx = reinterpret_cast<int32_t*>(w + off[2]);
+    int32_t* order = reinterpret_cast<int32_t*>(w + off[3]);
+    int32_t* bstart = reinterpret_cast<int32_t*>(w + off[4]);
+    int32_t* bend = reinterpret_cast<int32_t*>(w + off[5]);
+    splat_key_kernel<<<grid1d(n, 256), 256, 0, st>>>(g, xyz, n, inv_voxel_size, data, ncomp, scale, levels, level, post,
+                                                     acc, (uint32_t)count, keys, idx);
+    NGS_CHECK_LAUNCH();
+    const int bits = 32 - __builtin_clz((uint32_t)count);
+    size_t tb = tmp;
+    r = sort_pairs_u32(keys, ks, idx, order, n, bits, w, &tb, st);
+    if (r != NGS_OK) return r;
+    clear_words<<<grid1d(count, 256), 256, 0, st>>>(reinterpret_cast<uint32_t*>(bstart), count);
+    NGS_CHECK_LAUNCH();
+    clear_words<<<grid1d(count, 256), 256, 0, st>>>(reinterpret_cast<uint32_t*>(bend), count);
+    NGS_CHECK_LAUNCH();
+    splat_bins_kernel<<<grid1d(n, 256), 256, 0, st>>>(ks, n, (uint32_t)count, bstart, bend);
+    NGS_CHECK_LAUNCH();
+    splat_gather_kernel<<<grid1d(count, 256), 256, 0, st>>>(g, node_coords, count, xyz, inv_voxel_size, data, ncomp,
+                                                            scale, order, bstart, bend, post, acc);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -68,6 +68,9 @@ _SIGNATURES = {
     "ngs_point_bounds": ([P, c_int64, c_float, P, P], c_int32),
     "ngs_scan_i32": ([P, c_int64, P, P, P], c_int32),
     "ngs_splat": ([Grid, P, c_int64, c_float, P, c_int32, P, P, c_int32, c_double, P, P], c_int32),
+    "ngs_splat_gather_workspace_bytes": ([c_int64, c_int64], c_int64),
+    "ngs_splat_gather": ([Grid, P, c_int64, P, c_int64, c_float, P, c_int32, P, P, c_int32, c_double, P, P, c_int64, P],
+                         c_int32),
     "ngs_eval_field": ([Grid, P, P, c_int64, c_float, c_float, c_int32, P, P, P], c_int32),
     "ngs_eval_field_binned": ([Grid, P, P, P, P, c_int64, c_float, P, P], c_int32),
     "ngs_field_support": ([Grid, P, c_int64, c_float, P, P], c_int32),

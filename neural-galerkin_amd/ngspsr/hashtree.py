@@ -170,8 +170,7 @@ class HashTree:
         q4min, q4max = self._bounds(inv4)
         g4 = self._voxelise(1, inv4, q4min, q4max)
         w = torch.zeros(g4.count, dtype=torch.float64, device=self.device)
-        call("ngs_splat", g4.struct(), ptr(self.xyz), self.xyz.shape[0], inv4, None, 0, None, None, 0, 1.0, ptr(w),
-             stream())
+        self._splat(g4, self.xyz, inv4, None, 0, None, None, 0, w)
         wf = w.float()
         rho = torch.empty(self.xyz.shape[0], dtype=torch.float32, device=self.device)
         call("ngs_eval_field", g4.struct(), ptr(wf), ptr(self.xyz), self.xyz.shape[0], inv4, 1.0 / (enc * enc), 0,
@@ -205,20 +204,31 @@ class HashTree:
         return self._level_counts
 
     # ------------------------------------------------------------------ data on the tree
+    def _splat(self, g, xyz, inv_vs, data, nc, scale, levels, level, out):
+        """acc += the trilinear splat of the points on g: the binned gather (no FP64 atomics), or the
+        atomic scatter where g's members are not emitted or the sizes exceed the gather's."""
+        n = xyz.shape[0]
+        wsb = lib().ngs_splat_gather_workspace_bytes(n, g.count) if g.coords is not None else -1
+        if wsb < 0:
+            call("ngs_splat", g.struct(), ptr(xyz), n, inv_vs, ptr(data), nc, ptr(scale), ptr(levels), level, 1.0,
+                 ptr(out), stream())
+            return
+        ws = Scratch.get(wsb, self.device)
+        call("ngs_splat_gather", g.struct(), ptr(g.coords), g.count, ptr(xyz), n, inv_vs, ptr(data), nc, ptr(scale),
+             ptr(levels), level, 1.0, ptr(out), ptr(ws), wsb, stream())
+
     def splat_data(self, xyz, branch, data_depth, data=None, check_corr=True, scale=None, levels=None, level=0):
         """_hash_backend.py:332-356 (float64 accumulation, returned as float32)."""
         g = self.grids[data_depth]
         xyz = xyz.float().contiguous()
         if data is None:
             out = torch.zeros(g.count, dtype=torch.float64, device=self.device)
-            call("ngs_splat", g.struct(), ptr(xyz), xyz.shape[0], self.inv_vs, None, 0, ptr(scale), ptr(levels), level,
-                 1.0, ptr(out), stream())
+            self._splat(g, xyz, self.inv_vs, None, 0, scale, levels, level, out)
             return out.float()
         data = data.float().contiguous()
         nc = 1 if data.ndim == 1 else data.shape[1]
         out = torch.zeros((g.count, nc), dtype=torch.float64, device=self.device)
-        call("ngs_splat", g.struct(), ptr(xyz), xyz.shape[0], self.inv_vs, ptr(data), nc, ptr(scale), ptr(levels),
-             level, 1.0, ptr(out), stream())
+        self._splat(g, xyz, self.inv_vs, data, nc, scale, levels, level, out)
         return out.float().squeeze(1) if data.ndim == 1 else out.float()
 
     def evaluate_interpolated(self, query_pos, basis, basis_feat, feat_depth, feat, compute_mask=False,
