@@ -497,20 +497,31 @@ __global__ void splat_key_kernel(ngs_grid g, const float* __restrict__ xyz, int6
 }
 
 // Bin bounds over the sorted keys: [bstart[k], bend[k]) holds the points of key k (both zeroed first).
-__global__ void splat_bins_kernel(const uint32_t* __restrict__ ks, int64_t n, uint32_t count, int32_t* __restrict__ bstart,
-                                  int32_t* __restrict__ bend) {
+// The binned points' positions and scaled data rows are copied in bin order, so the gather below reads
+// each bin as one contiguous run.
+__global__ void splat_bins_kernel(const uint32_t* __restrict__ ks, const int32_t* __restrict__ order, int64_t n,
+                                  uint32_t count, const float* __restrict__ xyz, const float* __restrict__ data,
+                                  int ncomp, const float* __restrict__ scale, int32_t* __restrict__ bstart,
+                                  int32_t* __restrict__ bend, float* __restrict__ bxyz, float* __restrict__ brow) {
+#pragma clang fp contract(off)
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = ks[i];
     if (k >= count) return;
     if (i == 0 || ks[i - 1] != k) bstart[k] = (int32_t)i;
     if (i == n - 1 || ks[i + 1] != k) bend[k] = (int32_t)(i + 1);
+    const int64_t p = order[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) bxyz[3 * i + a] = xyz[3 * p + a];
+    if (data) {
+        const float sc = scale ? scale[p] : 1.0f;
+        for (int c = 0; c < ncomp; ++c) brow[3 * i + c] = data[p * ncomp + c] * sc;
+    }
 }
 
-__global__ void splat_gather_kernel(ngs_grid g, const int32_t* __restrict__ nodes, int64_t count,
-                                    const float* __restrict__ xyz, float inv_vs, const float* __restrict__ data,
-                                    int ncomp, const float* __restrict__ scale, const int32_t* __restrict__ order,
-                                    const int32_t* __restrict__ bstart, const int32_t* __restrict__ bend, double post,
+__global__ void splat_gather_kernel(ngs_grid g, const int32_t* __restrict__ nodes, int64_t count, float inv_vs,
+                                    int ncomp, const int32_t* __restrict__ bstart, const int32_t* __restrict__ bend,
+                                    const float* __restrict__ bxyz, const float* __restrict__ brow, double post,
                                     double* __restrict__ acc) {
 #pragma clang fp contract(off)
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -518,40 +529,45 @@ __global__ void splat_gather_kernel(ngs_grid g, const int32_t* __restrict__ node
     const int s = 1 << g.log2_stride;
     const float inv_s = 1.0f / (float)s;
     const int N[3] = {nodes[3 * node], nodes[3 * node + 1], nodes[3 * node + 2]};
+    const int lx = (N[0] >> g.log2_stride) - g.origin[0], ly = (N[1] >> g.log2_stride) - g.origin[1],
+              lz = (N[2] >> g.log2_stride) - g.origin[2];
     double sum[3] = {0.0, 0.0, 0.0};
-    for (int o = 0; o < 8; ++o) {
-        const int32_t r = grid_find(g, N[0] - ((o >> 2) & 1) * s, N[1] - ((o >> 1) & 1) * s, N[2] - (o & 1) * s);
-        if (r < 0) continue;
-        const int32_t e = bend[r];
-        for (int32_t j = bstart[r]; j < e; ++j) {
-            const int32_t p = order[j];
-            int dc[3], sg[3];
-            float loc[3];
+    // the 8 bins with lower node N - (ox, oy, oz) s: one column query per (ox, oy) covers oz = 1, 0
+    for (int oxy = 0; oxy < 4; ++oxy) {
+        int32_t r0;
+        const uint32_t m = grid_column(g, lx - (oxy >> 1), ly - (oxy & 1), lz - 1, 2, &r0);
+        for (int k = 0; k < 2; ++k) {
+            if (!((m >> k) & 1u)) continue;
+            const int32_t r = r0 + __popc(m & ((1u << k) - 1u));
+            const int32_t e = bend[r];
+            for (int32_t j = bstart[r]; j < e; ++j) {
+                int dc[3], sg[3];
+                float loc[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) trilinear_axis(xyz[3 * p + a] * inv_vs, s, inv_s, &dc[a], &sg[a], &loc[a]);
-            // which stencil node of p this node is, per axis (none: p's bin neighbours this node only)
-            int nn[3];
-            bool hit = true;
+                for (int a = 0; a < 3; ++a) trilinear_axis(bxyz[3 * j + a] * inv_vs, s, inv_s, &dc[a], &sg[a], &loc[a]);
+                // which stencil node of the point this node is, per axis (none: the bin only neighbours it)
+                int nn[3];
+                bool hit = true;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                if (N[a] == dc[a]) nn[a] = 0;
-                else if (sg[a] != 0 && N[a] == dc[a] + sg[a] * s) nn[a] = 1;
-                else hit = false;
-            }
-            if (!hit) continue;
-            float ax = nn[0] ? loc[0] : 1.0f - loc[0];
-            float ay = nn[1] ? loc[1] : 1.0f - loc[1];
-            float az = nn[2] ? loc[2] : 1.0f - loc[2];
-            float w = ax * ay * az;
-            if (data) {
-                float sc = scale ? scale[p] : 1.0f;
-                for (int c = 0; c < ncomp; ++c) sum[c] += (double)(w * (data[(int64_t)p * ncomp + c] * sc)) * post;
-            } else {
-                sum[0] += (double)w * post;
+                for (int a = 0; a < 3; ++a) {
+                    if (N[a] == dc[a]) nn[a] = 0;
+                    else if (sg[a] != 0 && N[a] == dc[a] + sg[a] * s) nn[a] = 1;
+                    else hit = false;
+                }
+                if (!hit) continue;
+                float ax = nn[0] ? loc[0] : 1.0f - loc[0];
+                float ay = nn[1] ? loc[1] : 1.0f - loc[1];
+                float az = nn[2] ? loc[2] : 1.0f - loc[2];
+                float w = ax * ay * az;
+                if (ncomp) {
+                    for (int c = 0; c < ncomp; ++c) sum[c] += (double)(w * brow[3 * j + c]) * post;
+                } else {
+                    sum[0] += (double)w * post;
+                }
             }
         }
     }
-    if (data) {
+    if (ncomp) {
         for (int c = 0; c < ncomp; ++c) acc[node * ncomp + c] += sum[c];
     } else {
         acc[node] += sum[0];
@@ -989,7 +1005,7 @@ int ngs_splat(ngs_grid g, const float* xyz, int64_t n, float inv_voxel_size, con
     return NGS_OK;
 }
 
-static int splat_gather_layout(int64_t n, int64_t count, size_t* tmp, size_t off[6], size_t* total) {
+static int splat_gather_layout(int64_t n, int64_t count, size_t* tmp, size_t off[8], size_t* total) {
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     const int bits = 32 - __builtin_clz((uint32_t)count);  // the keys run up to count
     size_t t = 0;
@@ -1002,14 +1018,16 @@ static int splat_gather_layout(int64_t n, int64_t count, size_t* tmp, size_t off
     off[3] = off[2] + al(4 * (size_t)n);          // point ids in key order
     off[4] = off[3] + al(4 * (size_t)n);          // bin starts (count)
     off[5] = off[4] + al(4 * (size_t)count);      // bin ends (count)
-    *total = off[5] + al(4 * (size_t)count);
+    off[6] = off[5] + al(4 * (size_t)count);      // positions in bin order (n x 3)
+    off[7] = off[6] + al(12 * (size_t)n);         // scaled data rows in bin order (n x 3)
+    *total = off[7] + al(12 * (size_t)n);
     return NGS_OK;
 }
 
 int64_t ngs_splat_gather_workspace_bytes(int64_t n, int64_t count) {
     if (n <= 0 || count <= 0) return 256;
     if (count >= ((int64_t)1 << 31) || n >= ((int64_t)1 << 31)) return -1;
-    size_t tmp, off[6], total;
+    size_t tmp, off[8], total;
     if (splat_gather_layout(n, count, &tmp, off, &total) != NGS_OK) return -1;
     return (int64_t)total;
 }
@@ -1021,7 +1039,7 @@ int ngs_splat_gather(ngs_grid g, const int32_t* node_coords, int64_t count, cons
     if (n <= 0 || count <= 0) return NGS_OK;
     NGS_REQUIRE(count < ((int64_t)1 << 31) && n < ((int64_t)1 << 31), "ngs_splat_gather: more than 2^31 points or nodes");
     NGS_REQUIRE(node_coords && xyz && acc && workspace && g.bits && g.rank, "ngs_splat_gather: null argument");
-    size_t tmp, off[6], total;
+    size_t tmp, off[8], total;
     int r = splat_gather_layout(n, count, &tmp, off, &total);
     if (r != NGS_OK) return r;
     NGS_REQUIRE((size_t)workspace_bytes >= total, "ngs_splat_gather: workspace too small");
@@ -1033,6 +1051,8 @@ int ngs_splat_gather(ngs_grid g, const int32_t* node_coords, int64_t count, cons
     int32_t* order = reinterpret_cast<int32_t*>(w + off[3]);
     int32_t* bstart = reinterpret_cast<int32_t*>(w + off[4]);
     int32_t* bend = reinterpret_cast<int32_t*>(w + off[5]);
+    float* bxyz = reinterpret_cast<float*>(w + off[6]);
+    float* brow = reinterpret_cast<float*>(w + off[7]);
     splat_key_kernel<<<grid1d(n, 256), 256, 0, st>>>(g, xyz, n, inv_voxel_size, data, ncomp, scale, levels, level, post,
                                                      acc, (uint32_t)count, keys, idx);
     NGS_CHECK_LAUNCH();
@@ -1044,10 +1064,11 @@ int ngs_splat_gather(ngs_grid g, const int32_t* node_coords, int64_t count, cons
     NGS_CHECK_LAUNCH();
     clear_words<<<grid1d(count, 256), 256, 0, st>>>(reinterpret_cast<uint32_t*>(bend), count);
     NGS_CHECK_LAUNCH();
-    splat_bins_kernel<<<grid1d(n, 256), 256, 0, st>>>(ks, n, (uint32_t)count, bstart, bend);
+    splat_bins_kernel<<<grid1d(n, 256), 256, 0, st>>>(ks, order, n, (uint32_t)count, xyz, data, ncomp, scale, bstart,
+                                                      bend, bxyz, brow);
     NGS_CHECK_LAUNCH();
-    splat_gather_kernel<<<grid1d(count, 256), 256, 0, st>>>(g, node_coords, count, xyz, inv_voxel_size, data, ncomp,
-                                                            scale, order, bstart, bend, post, acc);
+    splat_gather_kernel<<<grid1d(count, 256), 256, 0, st>>>(g, node_coords, count, inv_voxel_size, data ? ncomp : 0,
+                                                            bstart, bend, bxyz, brow, post, acc);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
