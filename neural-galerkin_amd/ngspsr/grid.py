@@ -91,6 +91,24 @@ class SparseGrid:
                 call("ngs_grid_mark_corners_staged", self.struct(), ptr(xyz), xyz.shape[0], inv_vs, ptr(levels),
                      max_level, ptr(m), stream())
 
+    @staticmethod
+    def finalize_all(grids, emit=True):
+        """finalize() of several grids with one device-to-host read of their member counts."""
+        if not grids:
+            return grids
+        dev = grids[0].device
+        cnt = torch.empty(len(grids), dtype=torch.int64, device=dev)
+        for i, g in enumerate(grids):
+            ws = Scratch.get(lib().ngs_scan_workspace_bytes(g.nwords), dev)
+            call("ngs_grid_finalize", g.struct(), ptr(cnt[i:i + 1]), ptr(ws), stream())
+        for g, c in zip(grids, cnt.tolist()):
+            g.count = int(c)
+            if emit:
+                g.coords = torch.empty((g.count, 3), dtype=torch.int32, device=g.device)
+                if g.count:
+                    call("ngs_grid_emit", g.struct(), ptr(g.coords), stream())
+        return grids
+
     def finalize(self, emit=True) -> int:
         cnt = torch.empty(1, dtype=torch.int64, device=self.device)
         ws = Scratch.get(lib().ngs_scan_workspace_bytes(self.nwords), self.device)

@@ -196,8 +196,13 @@ class HashTree:
         self.xyz_depth = lv
         self._level_counts = torch.bincount(lv.long(), minlength=self._depth).cpu().numpy()
         self._qmin, self._qmax = self._bounds(self.inv_vs)
+        grids = []
         for d in range(self._depth):
-            self.grids[d] = self._voxelise(self.strides[d], self.inv_vs, self._qmin, self._qmax, lv, d)
+            g = SparseGrid.for_points(self._qmin, self._qmax, self.strides[d], self.device)
+            g.mark_corners(self.xyz, self.inv_vs, lv, d)
+            grids.append(g)
+        for d, g in enumerate(SparseGrid.finalize_all(grids)):
+            self.grids[d] = g
         return self.xyz_depth
 
     def level_point_counts(self):

@@ -340,8 +340,8 @@ class Reconstructor:
             pg = SparseGrid([(o * s) // (2 * s) - 1 for o in g.origin],
                             [((o + dm) * s) // (2 * s) + 1 for o, dm in zip(g.origin, g.dims)], 2 * s, dev)
             pg.mark_coords(g.coords)
-            pg.finalize()
             parents[d] = pg
+        SparseGrid.finalize_all(list(parents.values()))
         leaves = {}
         max_d = L
         for d in range(L):

@@ -291,6 +291,12 @@ class ReconstructionServer:
         except Exception as e:  # reported per request
             req.error = e
         finally:
+            if req.error is not None:
+                # the thread's scratch and mark buffers serve its next request on another stream
+                try:
+                    s.synchronize()
+                except Exception:  # the request's error is already recorded
+                    pass
             req.finish = time.perf_counter()
             with self.lock:
                 self.in_use -= req.footprint
