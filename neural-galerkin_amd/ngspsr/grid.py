@@ -109,9 +109,14 @@ class SparseGrid:
                     call("ngs_grid_emit", g.struct(), ptr(g.coords), stream())
         return grids
 
-    def finalize(self, emit=True) -> int:
-        cnt = torch.empty(1, dtype=torch.int64, device=self.device)
+    def finalize(self, emit=True, count=True) -> int:
+        """Ranks, member count (count=False: ranks only, no device-to-host read) and, with emit,
+        the members in rank order."""
         ws = Scratch.get(lib().ngs_scan_workspace_bytes(self.nwords), self.device)
+        if not count:
+            call("ngs_grid_finalize", self.struct(), None, ptr(ws), stream())
+            return None
+        cnt = torch.empty(1, dtype=torch.int64, device=self.device)
         call("ngs_grid_finalize", self.struct(), ptr(cnt), ptr(ws), stream())
         self.count = int(cnt.item())
         if emit:

@@ -428,11 +428,11 @@ class Reconstructor:
                     lg = lgrid
                 else:
                     lg = self._leaf_grid(d, leaves[d], parents)
-                lg.finalize(emit=False)
+                lg.finalize(emit=False, count=False)  # the leaves are distinct: the rank of each is below n_l
                 n_l = leaves[d].shape[0]
                 ranks = torch.empty(n_l, dtype=torch.int32, device=dev)
                 call("ngs_grid_rank_rows", lg.struct(), ptr(leaves[d]), n_l, ptr(ranks), stream())
-                li = torch.empty(max(lg.count, 1), dtype=torch.int32, device=dev)
+                li = torch.empty(max(n_l, 1), dtype=torch.int32, device=dev)
                 li[ranks.long()] = torch.arange(n_l, dtype=torch.int32, device=dev)
                 bitmaps.append(lg)
                 index.append(li)
