@@ -321,24 +321,49 @@ __global__ void cross_stiffness_kernel(ngs_level fine, ngs_level coarse, const f
     const ngs_grid& g = coarse.grid;
     const int l2 = g.log2_stride;
     const int qx = (Q[0] >> l2) - g.origin[0], qy = (Q[1] >> l2) - g.origin[1], qz = (Q[2] >> l2) - g.origin[2];
+    // per-axis table factors, loaded once: z for all five offsets, x and y per column
+    float cz[5], gz[5];
+    bool okz[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        const int iz = (t - 2) * m + (Q[2] - c[2]) / sf_ + 2 * m;
+        okz[t] = iz >= 0 && iz < nrow;
+        {
+#pragma clang fp contract(off)
+            cz[t] = okz[t] ? T.F[iz] * fs : 0.f;
+            gz[t] = okz[t] ? T.G[iz] * inv_sc : 0.f;
+        }
+    }
     float acc = 0.f;
     for (int kx = -2; kx <= 2; ++kx) {
-        int ix = kx * m + (Q[0] - c[0]) / sf_ + 2 * m;
+        const int ix = kx * m + (Q[0] - c[0]) / sf_ + 2 * m;
+        if (ix < 0 || ix >= nrow) continue;
+        float c0, g0;
+        {
+#pragma clang fp contract(off)
+            c0 = T.F[ix] * fs;
+            g0 = T.G[ix] * inv_sc;
+        }
         for (int ky = -2; ky <= 2; ++ky) {
-            int iy = ky * m + (Q[1] - c[1]) / sf_ + 2 * m;
+            const int iy = ky * m + (Q[1] - c[1]) / sf_ + 2 * m;
+            if (iy < 0 || iy >= nrow) continue;
+            float c1, g1;
+            {
+#pragma clang fp contract(off)
+                c1 = T.F[iy] * fs;
+                g1 = T.G[iy] * inv_sc;
+            }
             int32_t j;
             const uint32_t msk = grid_column(g, qx + kx, qy + ky, qz - 2, 5, &j);
-            for (int kz = -2; kz <= 2; ++kz) {
-                if (!((msk >> (kz + 2)) & 1u)) continue;
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                if (!((msk >> t) & 1u)) continue;
                 const int32_t jj = j++;
-                int iz = kz * m + (Q[2] - c[2]) / sf_ + 2 * m;
-                if (ix < 0 || ix >= nrow || iy < 0 || iy >= nrow || iz < 0 || iz >= nrow) continue;
+                if (!okz[t]) continue;
                 float K;
                 {
 #pragma clang fp contract(off)
-                    const float c0 = T.F[ix] * fs, c1 = T.F[iy] * fs, c2 = T.F[iz] * fs;
-                    const float g0 = T.G[ix] * inv_sc, g1 = T.G[iy] * inv_sc, g2 = T.G[iz] * inv_sc;
-                    K = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
+                    K = g0 * c1 * cz[t] + c0 * g1 * cz[t] + c0 * c1 * gz[t];
                 }
                 acc += K * xc[jj];
             }
