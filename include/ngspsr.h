@@ -125,7 +125,7 @@ int ngs_grid_mark_corners(ngs_grid g, const float *xyz, int64_t n, float inv_vox
 int ngs_grid_mark_coords_staged(ngs_grid g, const int32_t *coords, int64_t n, uint8_t *marks, ngs_stream_t s);
 int ngs_grid_mark_corners_staged(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size,
                                  const int32_t *levels, int32_t max_level, uint8_t *marks, ngs_stream_t s);
-/* Fill rank[] and write the member count to *count (device int64). */
+/* Fill rank[] and write the member count to *count (device int64; NULL: ranks only). */
 int ngs_grid_finalize(ngs_grid g, int64_t *count, void *workspace, ngs_stream_t s);
 /* Write the members in rank order (the reference's torch.unique(dim=0) order). */
 int ngs_grid_emit(ngs_grid g, int32_t *coords_out, ngs_stream_t s);
