@@ -317,15 +317,27 @@ __global__ void emit_kernel(ngs_grid g, int32_t* __restrict__ out) {
     uint32_t bits = g.bits[w];
     if (!bits) return;
     int32_t r = g.rank[w];
-    const int64_t dyz = (int64_t)g.dims[1] * g.dims[2];
+    const int dy = g.dims[1], dz = g.dims[2];
+    const int64_t dyz = (int64_t)dy * dz;
+    // (x, y, z) of the word's bit 0 by division once per word, then advanced bit by bit (z fastest)
+    const int64_t lin0 = w * 32;
+    int x = (int)(lin0 / dyz);
+    const int64_t rem = lin0 - (int64_t)x * dyz;
+    int y = (int)(rem / dz);
+    int z = (int)(rem - (int64_t)y * dz);
+    int cur = 0;
     while (bits) {
-        int b = __ffs(bits) - 1;
+        const int b = __ffs(bits) - 1;
         bits &= bits - 1;
-        int64_t lin = w * 32 + b;
-        int x = (int)(lin / dyz);
-        int64_t rem = lin - (int64_t)x * dyz;
-        int y = (int)(rem / g.dims[2]);
-        int z = (int)(rem - (int64_t)y * g.dims[2]);
+        z += b - cur;
+        cur = b;
+        while (z >= dz) {
+            z -= dz;
+            if (++y == dy) {
+                y = 0;
+                ++x;
+            }
+        }
         out[3 * (int64_t)r] = (x + g.origin[0]) << g.log2_stride;
         out[3 * (int64_t)r + 1] = (y + g.origin[1]) << g.log2_stride;
         out[3 * (int64_t)r + 2] = (z + g.origin[2]) << g.log2_stride;
