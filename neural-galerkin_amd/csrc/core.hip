@@ -711,9 +711,16 @@ __global__ void point_basis_kernel(const float* __restrict__ xyz, int64_t n, flo
     }
 }
 
-__global__ void count_cells_kernel(const int32_t* __restrict__ rank, int64_t n, int32_t* __restrict__ counts) {
+// cell_start[k], k in [0, ncells], = the first sorted position whose key is >= k (keys at or above
+// ncells, the points outside every cell, count as ncells): each position writes the starts of the
+// cells between its predecessor's key and its own, so every entry is written once.
+__global__ void cell_starts_kernel(const uint32_t* __restrict__ ks, int64_t n, int64_t ncells,
+                                   int32_t* __restrict__ cell_start) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && rank[i] >= 0) atomicAdd(&counts[rank[i]], 1);
+    if (i > n) return;
+    const int64_t k = i < n ? (int64_t)min((uint64_t)ks[i], (uint64_t)ncells) : ncells + 1;
+    const int64_t prev = i > 0 ? (int64_t)min((uint64_t)ks[i - 1], (uint64_t)ncells) : -1;
+    for (int64_t c = prev + 1; c <= min(k, ncells); ++c) cell_start[c] = (int32_t)i;
 }
 
 __global__ void iota_i32(int32_t* p, int64_t n) {
@@ -1167,17 +1174,17 @@ int ngs_bucket_points(const int32_t* cell_rank, int64_t n, int64_t ncells, int32
     char* w = reinterpret_cast<char*>(workspace);
     uint32_t* kout = reinterpret_cast<uint32_t*>(w + ok);
     int32_t* vin = reinterpret_cast<int32_t*>(w + ov);
-    int32_t* counts = reinterpret_cast<int32_t*>(w + oc);
-    zero_i32<<<grid1d(ncells, 256), 256, 0, st>>>(counts, ncells);
-    count_cells_kernel<<<grid1d(n, 256), 256, 0, st>>>(cell_rank, n, counts);
     iota_i32<<<grid1d(n, 256), 256, 0, st>>>(vin, n);
     NGS_CHECK_LAUNCH();
-    r = scan_i32_internal(counts, ncells, cell_start, w + osc, st);
-    if (r != NGS_OK) return r;
     int bits = 1;
     while (((int64_t)1 << bits) < ncells + 1 && bits < 32) ++bits;
     size_t tb = tmp;
-    return sort_pairs_u32(reinterpret_cast<const uint32_t*>(cell_rank), kout, vin, perm, n, bits, w, &tb, st);
+    r = sort_pairs_u32(reinterpret_cast<const uint32_t*>(cell_rank), kout, vin, perm, n, bits, w, &tb, st);
+    if (r != NGS_OK) return r;
+    // the bounds of each cell's run in the sorted keys (no per-cell atomics, no scan)
+    cell_starts_kernel<<<grid1d(n + 1, 256), 256, 0, st>>>(kout, n, ncells, cell_start);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
 }
 
 int ngs_f64_to_f32(const double* in, float* out, int64_t n, ngs_stream_t s) {
