@@ -343,6 +343,21 @@ int ngs_pcg_solve_runs(const int32_t *runs, const float *rvals, const float *inv
                        int64_t M, double tol, int32_t max_iter, int32_t chunk, void *workspace, ngs_stream_t s,
                        int32_t *iters, double *rel_resid);
 
+/* ngs_pcg_solve_runs without a workspace argument, for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc). No reference
+ * counterpart. The workspace is a library-owned device array of NGS_PCG_NOALLOC_SCRATCH_BYTES (one
+ * copy per GPU, held until the process ends); systems above NGS_PCG_NOALLOC_MAX_ROWS rows are
+ * rejected with NGS_ERR_CAPACITY. The call allocates nothing: kernels are launched directly on s
+ * (no graph, no private stream or event) and the stream is synchronised between chunks. Same
+ * iteration, stop rule and results as ngs_pcg_solve_runs. Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other threads or streams) race; serialise them. The
+ * stream must belong to the current device. */
+#define NGS_PCG_NOALLOC_SCRATCH_BYTES (32ll << 20)
+#define NGS_PCG_NOALLOC_MAX_ROWS 2791360ll
+int ngs_pcg_solve_runs_noalloc(const int32_t *runs, const float *rvals, const float *inv_diag, const float *b,
+                               float *x, int64_t M, double tol, int32_t max_iter, int32_t chunk, ngs_stream_t s,
+                               int32_t *iters, double *rel_resid);
+
 /* ngs_pcg_solve on the stencil form of the system (same iteration, same stop rule; the SpMV is the
  * point pass + row pass of ngs_stencil_apply). */
 int ngs_pcg_solve_stencil(ngs_stencil op, const float *inv_diag, const float *b, float *x, double tol,

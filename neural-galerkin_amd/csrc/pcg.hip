@@ -21,6 +21,7 @@
 #include "common.h"
 #include "stencil.h"
 
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -922,6 +923,107 @@ int ngs_pcg_solve_runs(const int32_t* runs, const float* rvals, const float* inv
     E.a.b = b;
     E.a.x = x;
     return pcg_solve_entry(E, M, tol, max_iter, chunk, workspace, s, iters, rel_resid);
+}
+
+// ------------------------------------------------------------------ solve without allocation
+// The workspace of ngs_pcg_solve_runs_noalloc is a file-scope device array, one copy per GPU, held
+// until the process ends. On the host the array's name is only a stand-in symbol, never a device
+// address: the address of the current device's copy is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_ptr, and only that address reaches kernels.
+static __device__ __align__(256) char scratch[NGS_PCG_NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+static int scratch_address(void** out) {
+    int dev = -1;
+    NGS_CHECK_HIP(hipGetDevice(&dev));
+    NGS_REQUIRE(dev >= 0 && dev < SCRATCH_MAX_DEVICES, "ngs_pcg_solve_runs_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("ngs_pcg_solve_runs_noalloc: hipGetSymbolAddress of the scratch array failed: ") +
+                      hipGetErrorString(e));
+            return NGS_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return NGS_OK;
+}
+
+int ngs_pcg_solve_runs_noalloc(const int32_t* runs, const float* rvals, const float* inv_diag, const float* b,
+                               float* x, int64_t M, double tol, int32_t max_iter, int32_t chunk, ngs_stream_t s,
+                               int32_t* iters, double* rel_resid) {
+    NGS_REQUIRE(M >= 0, "ngs_pcg_solve_runs_noalloc: bad arguments");
+    NGS_REQUIRE(tol >= 0, "ngs_pcg_solve_runs_noalloc: negative tolerance");
+    if (M == 0) {
+        if (iters) *iters = 0;
+        if (rel_resid) *rel_resid = 0;
+        return NGS_OK;
+    }
+    NGS_REQUIRE(runs && rvals && inv_diag && b && x, "ngs_pcg_solve_runs_noalloc: null array");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_pcg_solve_runs_noalloc: too many rows for run words");
+    const int64_t need = pcg_ws_bytes(M);
+    if (need > (int64_t)NGS_PCG_NOALLOC_SCRATCH_BYTES) {
+        set_error("ngs_pcg_solve_runs_noalloc: " + std::to_string(M) + " rows need " + std::to_string(need) +
+                  " bytes of workspace, above the " + std::to_string((int64_t)NGS_PCG_NOALLOC_SCRATCH_BYTES) +
+                  "-byte scratch array (at most " + std::to_string((int64_t)NGS_PCG_NOALLOC_MAX_ROWS) +
+                  " rows); use ngs_pcg_solve_runs with a workspace");
+        return NGS_ERR_CAPACITY;
+    }
+    // the advertised row limit is exactly the largest system whose layout fits the array
+    NGS_REQUIRE(pcg_ws_bytes(NGS_PCG_NOALLOC_MAX_ROWS) <= (int64_t)NGS_PCG_NOALLOC_SCRATCH_BYTES &&
+                    pcg_ws_bytes(NGS_PCG_NOALLOC_MAX_ROWS + 1) > (int64_t)NGS_PCG_NOALLOC_SCRATCH_BYTES,
+                "ngs_pcg_solve_runs_noalloc: row limit does not match the workspace layout");
+    void* ws = nullptr;
+    if (int r = scratch_address(&ws)) return r;
+    if (chunk <= 0) chunk = 32;
+    chunk += chunk & 1;  // launch sequences have an even number of steps (state slot parity)
+    if (max_iter < 0) max_iter = 10000000;
+    hipStream_t st = S(s);
+    // Everything runs on the caller's stream, launched directly (no private stream, event or graph).
+    // The init kernels write every workspace word the iterations read before they read it: x, r, p,
+    // the r.z / b.b partials, the state of both slots, the table entry and the block map.
+    BatchSys E{};
+    E.kind = SYS_RUNS;
+    E.a.runs = runs;
+    E.a.rvals = rvals;
+    E.a.dinv = inv_diag;
+    E.a.b = b;
+    E.a.x = x;
+    void* tab = nullptr;
+    int32_t* map = nullptr;
+    PcgArgs A = E.a;
+    pcg_layout(M, &A, reinterpret_cast<char*>(ws), &tab, &map);
+    pcg_init_kernel<<<A.nblocks, PCG_THREADS, 0, st>>>(A);
+    pcg_init_finish_kernel<<<1, PCG_THREADS, 0, st>>>(A, tol, max_iter);
+    E.a = A;
+    E.blk0 = 0;
+    E.nblk = A.nblocks;
+    write_entry_kernel<<<1, 1, 0, st>>>(reinterpret_cast<BatchSys*>(tab), E);
+    fill_map_kernel<<<grid1d(A.nblocks, 256), 256, 0, st>>>(map, A.nblocks);
+    NGS_CHECK_LAUNCH();
+    PcgSlot slot{};
+    while (true) {
+        if (int r = ngs_pcg_batch_iterate(tab, map, A.nblocks, chunk, 0, FLAG_RUNS | FLAG_NO_CSR, s)) return r;
+        NGS_CHECK_HIP(hipMemcpyAsync(&slot, &A.st->slot[0], sizeof(PcgSlot), hipMemcpyDeviceToHost, st));
+        NGS_CHECK_HIP(hipStreamSynchronize(st));
+        if (slot.done) break;
+    }
+    double sc[2];
+    NGS_CHECK_HIP(hipMemcpyAsync(sc, &A.st->rr, sizeof(double) * 2, hipMemcpyDeviceToHost, st));  // rr, bb
+    NGS_CHECK_HIP(hipStreamSynchronize(st));
+    if (iters) *iters = slot.iter;
+    if (rel_resid) *rel_resid = sc[1] > 0 ? std::sqrt(sc[0] / sc[1]) : 0.0;
+    if (slot.done == 2) {
+        set_error("ngs_pcg_solve_runs_noalloc: iteration cap reached before the tolerance");
+        return NGS_ERR_NOT_CONVERGED;
+    }
+    return NGS_OK;
 }
 
 int ngs_pcg_solve_stencil(ngs_stencil op, const float* inv_diag, const float* b, float* x, double tol, int32_t max_iter,

@@ -97,6 +97,8 @@ _SIGNATURES = {
     "ngs_lhs_assemble_runs": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P], c_int32),
     "ngs_pcg_solve_runs": ([P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
                             ctypes.POINTER(c_double)], c_int32),
+    "ngs_pcg_solve_runs_noalloc": ([P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, ctypes.POINTER(c_int32),
+                                    ctypes.POINTER(c_double)], c_int32),
     "ngs_pcg_batch_prepare_runs": ([P, P, P, P, P, c_int64, c_double, c_int32, P, c_int32, c_int32, P, P], c_int32),
     "ngs_spmv_runs": ([P, P, P, P, c_int64, P], c_int32),
     "ngs_residual_f64_runs": ([P, P, P, P, P, c_int64, P, P, P], c_int32),
