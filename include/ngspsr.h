@@ -136,6 +136,21 @@ int ngs_grid_query(ngs_grid g, const int32_t *coords, int64_t n, const int32_t *
                    int64_t *out, ngs_stream_t s);
 /* min (3) and max (3) of q = xyz * inv_voxel_size over all points, into minmax[6] (device float). */
 int ngs_point_bounds(const float *xyz, int64_t n, float inv_voxel_size, float *minmax, ngs_stream_t s);
+/* Cells (n, 3) int32 of points at a power-of-two stride: floor(q * scale / stride) * stride, with
+ * q * scale in FP32. Replaces the floor divisions of build_hierarchy_dense and
+ * build_hierarchy_subdivide (_hash_backend.py:363,396; scale = 1 / voxel_size or 1). */
+int ngs_point_cells(const float *q, int64_t n, float scale, int32_t stride, int32_t *cells, ngs_stream_t s);
+/* SparseFeatureHierarchy.evaluate_voxel_status (_hash_backend.py:292-318): per coordinate, 0 when
+ * absent from grid g, 1 when present without children, 2 when present with children, i.e. when
+ * the coordinate is in finer_parents (the parents, at g's stride, of the next finer level; read
+ * only when has_finer). Output int64. */
+int ngs_voxel_status(ngs_grid g, ngs_grid finer_parents, int32_t has_finer, const int32_t *coords, int64_t n,
+                     int64_t *status, ngs_stream_t s);
+/* SparseFeatureHierarchy.split_data (_hash_backend.py:320-330): trilinear interpolation of the
+ * per-voxel rows data (M, nc) at the points (the splat's 8 corners and weights at g's stride;
+ * absent corners add nothing), out (n, nc) float32. */
+int ngs_split_data(ngs_grid g, const float *xyz, int64_t n, float inv_voxel_size, const float *data, int32_t nc,
+                   float *out, ngs_stream_t s);
 /* Exclusive scan of int32 counts into int32 offsets (n + 1 entries; offsets[n] = total). */
 int ngs_scan_i32(const int32_t *counts, int64_t n, int32_t *offsets, void *workspace, ngs_stream_t s);
 
@@ -416,6 +431,40 @@ int ngs_residual_f64(const int32_t *rowptr, const int32_t *cols, const float *va
 int ngs_residual_f64_runs(const int32_t *runs, const float *rvals, const double *b, const double *x, double *r,
                           int64_t M, double *norms, void *workspace, ngs_stream_t s);
 
+/* Sparse direct solve (replaces _sparse_solve.solve_cusparse, csrc/sparse_solve/solve_cusparse.cu:15-42,
+ * the sparse Cholesky behind the "cholmod" solver, jspsr/core/solver.py:147-157): a supernodal
+ * multifrontal Cholesky factorization in FP64 of a symmetric positive definite CSR matrix (the
+ * lower triangle is read; padded rows repeating their diagonal with value 0 are accepted).
+ * ngs_chol_analyze reads the pattern (device arrays, read on stream s, which it synchronises) and
+ * computes on the host a
+ * nested-dissection order, the supernodes and the front structures; the opaque handle owns only host
+ * memory (ngs_chol_free). The device workspace of ngs_chol_workspace_bytes holds the factor and
+ * index maps; ngs_chol_factor uploads the maps into it (once per workspace) and factors on the
+ * stream. A pivot at or below tol times its column's diagonal marks the matrix singular: the
+ * original row index of the first such pivot is stored in *singularity (-1 when there is none, as
+ * cuSOLVER reports) and the pivot is replaced by the diagonal so that the factorization completes.
+ * ngs_chol_solve solves A x = b with the factor of that workspace (b, x: FP32 or FP64, one of each
+ * non-NULL). stats of ngs_chol_info (8 int64): supernodes, tree heights, largest front, entries of
+ * L, pool doubles, index-list entries, parent-map entries, CSR entries. ngs_chol_permutation: the
+ * order (new -> original row). */
+typedef struct ngs_chol_s ngs_chol;
+int ngs_chol_analyze(const int32_t *rowptr, const int32_t *cols, int64_t M, ngs_stream_t s, ngs_chol **handle);
+/* The same analysis from a pattern in host memory. */
+int ngs_chol_analyze_host(const int32_t *rowptr, const int32_t *cols, int64_t M, ngs_chol **handle);
+/* The symbolic structure, for inspection: per supernode 8 int64 (first pivot column, pivot
+ * columns, front size, parent, pool offset, index-list offset, parent-map offset, height), the
+ * front index lists, the child-to-parent position maps and the CSR-entry-to-pool map (sizes from
+ * ngs_chol_info stats[0], [5], [6], [7]). */
+int ngs_chol_structure(const ngs_chol *handle, int64_t *nodes, int32_t *front_idx, int32_t *cmap, int64_t *amap);
+int64_t ngs_chol_workspace_bytes(const ngs_chol *handle);
+int ngs_chol_info(const ngs_chol *handle, int64_t *stats);
+int ngs_chol_permutation(const ngs_chol *handle, int32_t *perm);
+int ngs_chol_factor(ngs_chol *handle, const int32_t *rowptr, const int32_t *cols, const float *vals, double tol,
+                    void *workspace, ngs_stream_t s, int64_t *singularity);
+int ngs_chol_solve(const ngs_chol *handle, const void *workspace, const float *b, const double *b64, float *x,
+                   double *x64, ngs_stream_t s);
+void ngs_chol_free(ngs_chol *handle);
+
 /* ------------------------------------------------------------------------------------- mesh */
 
 /* Conformal leaf construction (reconstructor.py:314-336 with get_coords(conforming=True),
@@ -551,6 +600,15 @@ int ngs_additive_eval(const float *xyz, const float *coeffs, const int64_t *feat
 int ngs_additive_eval_backward(const float *xyz, const int64_t *feat_ids, int64_t n, const float *components,
                                int32_t n_components, float stride, const float *grad_funcs, const float *grad_dfuncs,
                                float *grad_coeffs, ngs_stream_t s);
+/* Replaces _sparse_op.screened_multiplication (csrc/sparse_op/sparse_product.cu:202-258, smult_kernel
+ * :100-137) for bases given by per-(point, voxel) values: out[k] += (va * vb) * w[p] (w NULL:
+ * w_scalar) for each point p, each entry (a_ids[t], a_vals[t]) with a_pt[t] = p of level a and each
+ * entry of p at level b (b_start: n_points + 1 offsets into b_ids, b_vals), k = position of column
+ * b_id in row a_id of the pair structure (rowptr, cols ascending within rows); pairs outside the
+ * structure are dropped. FP32 atomics into out (caller-zeroed or accumulating). */
+int ngs_screen_pairs(int64_t na, const int32_t *a_pt, const int32_t *a_ids, const float *a_vals,
+                     const int32_t *b_start, const int32_t *b_ids, const float *b_vals, const float *w,
+                     float w_scalar, const int32_t *rowptr, const int32_t *cols, float *out, ngs_stream_t s);
 /* Vertex de-duplication (reconstructor.py:379-383): sorted unique keys; faces (T*3) = rank of each
  * key; the vertex of a key is interpolated (dual_mc.cu:20-31 sdf_interp) on its edge, in the corner
  * order of the key's first occurrence, from the corner positions pos (3 per id) and values — the

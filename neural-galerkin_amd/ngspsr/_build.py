@@ -9,7 +9,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 OBJ = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libngspsr.so")
-SOURCES = ["core.hip", "assembly.hip", "pcg.hip", "mesh.hip", "sort.hip", "basis.hip"]
+SOURCES = ["core.hip", "assembly.hip", "pcg.hip", "mesh.hip", "sort.hip", "basis.hip", "chol.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -66,6 +66,9 @@ _SIGNATURES = {
     "ngs_grid_query": ([Grid, P, c_int64, P, c_int32, P, P], c_int32),
     "ngs_grid_rank_rows": ([Grid, P, c_int64, P, P], c_int32),
     "ngs_point_bounds": ([P, c_int64, c_float, P, P], c_int32),
+    "ngs_point_cells": ([P, c_int64, c_float, c_int32, P, P], c_int32),
+    "ngs_voxel_status": ([Grid, Grid, c_int32, P, c_int64, P, P], c_int32),
+    "ngs_split_data": ([Grid, P, c_int64, c_float, P, c_int32, P, P], c_int32),
     "ngs_scan_i32": ([P, c_int64, P, P, P], c_int32),
     "ngs_splat": ([Grid, P, c_int64, c_float, P, c_int32, P, P, c_int32, c_double, P, P], c_int32),
     "ngs_splat_gather_workspace_bytes": ([c_int64, c_int64], c_int64),
@@ -149,6 +152,16 @@ _SIGNATURES = {
     "ngs_dedup_workspace_bytes": ([c_int64], c_int64),
     "ngs_dedup_vertices": ([P, P, c_int64, c_int32, P, P, P, P, P, P, c_int64, P], c_int32),
     "ngs_f64_to_f32": ([P, P, c_int64, P], c_int32),
+    "ngs_chol_analyze": ([P, P, c_int64, P, ctypes.POINTER(c_void_p)], c_int32),
+    "ngs_chol_analyze_host": ([P, P, c_int64, ctypes.POINTER(c_void_p)], c_int32),
+    "ngs_chol_workspace_bytes": ([c_void_p], c_int64),
+    "ngs_chol_info": ([c_void_p, P], c_int32),
+    "ngs_chol_permutation": ([c_void_p, P], c_int32),
+    "ngs_chol_structure": ([c_void_p, P, P, P, P], c_int32),
+    "ngs_chol_factor": ([c_void_p, P, P, P, c_double, P, P, ctypes.POINTER(c_int64)], c_int32),
+    "ngs_chol_solve": ([c_void_p, P, P, P, P, P, P], c_int32),
+    "ngs_chol_free": ([c_void_p], None),
+    "ngs_screen_pairs": ([c_int64, P, P, P, P, P, P, P, c_float, P, P, P, P], c_int32),
     "ngs_weighted_sum": ([P, P, c_int64, P, P], c_int32),
 }
 

@@ -8,9 +8,11 @@ Solver strings keep the reference's mini-language, "pcg@tol=1e-6@maxiter=1000@re
   mixed    the reference's "mixed" (solver.py:124-130,160-169): PCG with tol 1e-4 and at most 1000
            iterations; when the absolute residual ||A x - b|| exceeds tol (1e-2) the solve falls
            back to "cholmod".
-  cholmod  the reference's direct sparse solve (solve_cusparse.cu:15-42). There is no sparse
-           direct solver on this stack; it is replaced by "refine" to a relative residual of 1e-10,
-           which is as accurate as an FP32 direct solve of these well-conditioned systems.
+  cholmod  the reference's direct sparse solve (solve_cusparse.cu:15-42): a supernodal multifrontal
+           Cholesky factorization in FP64 (nested-dissection order, fronts factored on the GPU), kept
+           on the system for later solves with the same matrix (the adjoint). Returns the FP64
+           solution; a pivot at or below tol (1e-6, relative to its diagonal) is reported as the
+           singularity, as cuSOLVER reports it.
   refine   FP64 iterative refinement around FP32 inner PCG solves (no reference counterpart):
            r = b - A x in FP64, inner solve A e = r to inner_tol, x += e in FP64, until
            ||b - A x|| / ||b|| <= tol.
@@ -25,7 +27,7 @@ from ._lib import NgsError, call, lib, ptr, stream
 
 DEFAULTS = {"pcg": {"tol": 1e-5, "maxiter": -1, "res_fix": False, "verbose": False, "chunk": 32},
             "mixed": {"tol": 1e-2, "fast_tol": 1e-4, "fast_maxiter": 1000, "verbose": False, "chunk": 32},
-            "cholmod": {"tol": 1e-10, "verbose": False, "chunk": 32},
+            "cholmod": {"tol": 1e-6, "verbose": False, "chunk": 32},
             "refine": {"tol": 1e-8, "maxiter": 100000, "inner_tol": 1e-4, "max_outer": 20, "verbose": False,
                        "chunk": 32}}
 
@@ -278,7 +280,9 @@ class StencilSystem:
 class SolveInfo:
     def __init__(self, iters=0, rel_resid=0.0, outer=0, true_rel_resid=None):
         self.iters, self.rel_resid, self.outer, self.true_rel_resid = iters, rel_resid, outer, true_rel_resid
-        self.fallback = False  # "mixed": the direct-solve stand-in answered
+        self.fallback = False  # "mixed": the direct solve answered
+        self.direct = False  # answered by the sparse Cholesky factorization
+        self.singularity = -1  # direct solve: row of the first singular pivot (-1: none)
 
     def __repr__(self):
         return (f"SolveInfo(iters={self.iters}, rel_resid={self.rel_resid:.3e}, outer={self.outer}, "
@@ -312,6 +316,19 @@ def pcg(system, b, tol=1e-5, maxiter=-1, chunk=32, x_out=None, raise_on_cap=True
     return x, SolveInfo(it.value, rr.value)
 
 
+FP32_REACH = 1e-6  # relative residual an FP32 Jacobi-PCG reaches on these systems before it stagnates
+
+
+def inner_tolerance(tol, inner_tol, outer):
+    """Relative tolerance of an inner FP32 solve of iterative refinement. Targets the FP32 solver
+    reaches on its own (tol >= 1e-6) are aimed at directly by the first inner solve, to half the
+    target so that its FP32 rounding still lands below it, and verified in FP64: one FP32 solve and
+    one FP64 residual. Tighter targets take inner_tol per step (each step gains that factor)."""
+    if outer == 0 and tol >= FP32_REACH:
+        return 0.5 * tol
+    return inner_tol
+
+
 def refine(system, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, chunk=32):
     """FP64 iterative refinement with FP32 PCG inner solves. Returns (x64, SolveInfo)."""
     M = system.M
@@ -324,7 +341,8 @@ def refine(system, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, ch
     nb = bb ** 0.5
     rel = (rr ** 0.5) / nb if nb > 0 else 0.0
     while rel > tol and info.outer < max_outer:
-        e, inner = pcg(system, r64.float(), tol=inner_tol, maxiter=maxiter, chunk=chunk, raise_on_cap=False)
+        e, inner = pcg(system, r64.float(), tol=inner_tolerance(tol, inner_tol, info.outer), maxiter=maxiter,
+                       chunk=chunk, raise_on_cap=False)
         x64 += e.double()
         info.iters += inner.iters
         info.outer += 1
@@ -335,11 +353,93 @@ def refine(system, b, tol=1e-8, inner_tol=1e-4, max_outer=20, maxiter=100000, ch
     return x64, info
 
 
-def cholmod(system, b, tol=1e-10, chunk=32):
-    """Stand-in for the reference's direct solve (solve_cholmod, solver.py:147-157): refinement to a
-    relative residual of tol. Returns (x float32, SolveInfo)."""
-    x64, info = refine(system, b, tol=tol, chunk=chunk)
-    return x64.float(), info
+class CholeskyFactor:
+    """Sparse Cholesky factor of a level system (ngs_chol_*): the analysis (order, supernodes, front
+    structures) on the host, the FP64 factor in a device workspace. solve(b) -> x (FP64)."""
+
+    def __init__(self, csr, tol=1e-6):
+        self.M = csr.M
+        self.device = csr.device
+        self.csr = csr
+        h = ctypes.c_void_p()
+        # the pattern read on the host (ordered after the stream that wrote it), analysed there
+        rp = csr.rowptr.to(torch.int32).cpu().contiguous()
+        cl = csr.cols.to(torch.int32).cpu().contiguous()
+        call("ngs_chol_analyze_host", ctypes.c_void_p(rp.data_ptr()), ctypes.c_void_p(cl.data_ptr()), self.M,
+             ctypes.byref(h))
+        self.handle = h
+        self.ws = torch.empty(max(int(lib().ngs_chol_workspace_bytes(h)), 256), dtype=torch.uint8, device=self.device)
+        sing = ctypes.c_int64(-1)
+        call("ngs_chol_factor", h, ptr(csr.rowptr), ptr(csr.cols), ptr(csr.vals), float(tol), ptr(self.ws), stream(),
+             ctypes.byref(sing))
+        self.singularity = int(sing.value)
+
+    def info(self):
+        st = (ctypes.c_int64 * 8)()
+        call("ngs_chol_info", self.handle, st)
+        keys = ("supernodes", "heights", "max_front", "factor_entries", "pool_doubles", "index_entries",
+                "map_entries", "csr_entries")
+        return dict(zip(keys, [int(v) for v in st]))
+
+    def solve(self, b):
+        x = torch.empty(self.M, dtype=torch.float64, device=self.device)
+        if self.M:
+            b = b.contiguous()
+            f64 = b.dtype == torch.float64
+            call("ngs_chol_solve", self.handle, ptr(self.ws), None if f64 else ptr(b.float()), ptr(b) if f64 else None,
+                 None, ptr(x), stream())
+        return x
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                lib().ngs_chol_free(h)
+            except Exception:  # interpreter shutdown
+                pass
+            self.handle = None
+
+
+def cholesky(system, tol=1e-6):
+    """The system's Cholesky factor, computed on first use and kept on the system (the adjoint and
+    later solves with the same matrix reuse it)."""
+    f = getattr(system, "_chol", None)
+    if f is None or f.tol != tol:
+        csr = system.to_csr() if isinstance(system, (RunSystem, StencilSystem)) else system
+        f = CholeskyFactor(csr, tol)
+        f.tol = tol
+        system._chol = f
+    return f
+
+
+def cholmod(system, b, tol=1e-6, chunk=32, refine_tol=1e-12, max_refine=4):
+    """The reference's direct solve (solve_cholmod, solver.py:147-157 -> solve_cusparse.cu:15-42): the
+    sparse Cholesky factor of the system (FP64, from its lower triangle), two triangular solves, then
+    iterative refinement against the full stored matrix (FP64 residual, correction through the
+    factor) until the relative residual is below refine_tol: the assembled FP32 rows are symmetric
+    only to rounding, so the factor alone stops near 1e-8. Returns (x64, SolveInfo) with the FP64
+    relative residual of x and, in SolveInfo.singularity, the first singular pivot's row (-1: none)."""
+    info = SolveInfo()
+    if system.M == 0:
+        return torch.zeros(0, dtype=torch.float64, device=system.device), info
+    f = cholesky(system, tol)
+    b64 = b.double().contiguous()
+    x = f.solve(b64)
+    r, rr, bb = system.residual_f64(b64, x)
+    rel = (rr / bb) ** 0.5 if bb > 0 else 0.0
+    while rel > refine_tol and info.outer < max_refine:
+        x += f.solve(r)
+        info.outer += 1
+        r, rr, _ = system.residual_f64(b64, x)
+        rel_new = (rr / bb) ** 0.5
+        if not rel_new < rel:  # no further gain (a singular or indefinite system)
+            rel = rel_new
+            break
+        rel = rel_new
+    info.rel_resid = info.true_rel_resid = rel
+    info.singularity = f.singularity
+    info.direct = True
+    return x, info
 
 
 def mixed(system, b, tol=1e-2, fast_tol=1e-4, fast_maxiter=1000, chunk=32):
@@ -354,12 +454,14 @@ def mixed(system, b, tol=1e-2, fast_tol=1e-4, fast_maxiter=1000, chunk=32):
         x, info2 = cholmod(system, b, chunk=chunk)
         info2.iters += info.iters
         info2.fallback = True
-        return x, info2
+        return x.float(), info2
     return x, info
 
 
 def solve(system, b, spec: str = "pcg"):
-    """Dispatch on a solver string. Returns (x float32, SolveInfo)."""
+    """Dispatch on a solver string. Returns (x, SolveInfo): x is float32, or float64 for "refine"
+    (the refinement's FP64 iterate, whose residual SolveInfo.rel_resid reports) and "cholmod" (the
+    FP64 direct solution)."""
     name, conf = parse_solver(spec)
     if name == "pcg":
         x, info = pcg(system, b, tol=conf["tol"], maxiter=conf["maxiter"], chunk=conf["chunk"])
@@ -369,9 +471,10 @@ def solve(system, b, spec: str = "pcg"):
     elif name == "cholmod":
         x, info = cholmod(system, b, tol=conf["tol"], chunk=conf["chunk"])
     else:
-        x64, info = refine(system, b, tol=conf["tol"], inner_tol=conf["inner_tol"], max_outer=conf["max_outer"],
-                           maxiter=conf["maxiter"], chunk=conf["chunk"])
-        x = x64.float()
+        # the FP64 iterate is the solution: its FP64 residual is the one SolveInfo reports (its FP32
+        # rounding would sit at the FP32 floor, ~1e-8 relative, above the tighter tolerances)
+        x, info = refine(system, b, tol=conf["tol"], inner_tol=conf["inner_tol"], max_outer=conf["max_outer"],
+                         maxiter=conf["maxiter"], chunk=conf["chunk"])
     if conf.get("verbose"):
         print("PCG Iteration ended in", info.iters)
     return x, info

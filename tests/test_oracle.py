@@ -463,3 +463,26 @@ def test_adjoint_gradient_matches_central_differences():
         sm_[i] -= hstep
         fd = (bs.loss(alpha, sp_, query, target)[0] - bs.loss(alpha, sm_, query, target)[0]) / (2 * hstep)
         assert abs(fd - gs[i]) <= 1e-6 * max(abs(fd), 1e-12) + 1e-13, (i, fd, gs[i])
+
+
+def test_direct_solve_closed_form_and_singularity():
+    """oracle.direct: the inverse of the 1-D Laplacian tridiag(-1, 2, -1) of size n is
+    (T^-1)_ij = min(i, j) (n + 1 - max(i, j)) / (n + 1) (1-based); the upper triangle is ignored;
+    the Neumann Laplacian (constant null space) reports its last pivot as singular."""
+    from oracle import direct
+    n = 30
+    T = 2 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    i = np.arange(1, n + 1)
+    inv = np.minimum.outer(i, i) * (n + 1 - np.maximum.outer(i, i)) / (n + 1)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal(n)
+    x, sing = direct.solve_spd(sp.csr_matrix(T), b)
+    assert sing == -1
+    np.testing.assert_allclose(x, inv @ b, rtol=1e-12, atol=1e-12)
+    garbage_upper = T + np.triu(rng.standard_normal((n, n)), 1)
+    x2, _ = direct.solve_spd(garbage_upper, b)
+    np.testing.assert_allclose(x2, x, rtol=1e-12, atol=1e-12)
+    N = T.copy()
+    N[0, 0] = N[-1, -1] = 1.0
+    _, sing = direct.solve_spd(N, b)
+    assert sing == n - 1
