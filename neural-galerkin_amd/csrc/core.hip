@@ -361,6 +361,66 @@ __global__ void grid_rank_rows_kernel(ngs_grid g, const int32_t* __restrict__ c,
     out[i] = grid_find(g, c[3 * i], c[3 * i + 1], c[3 * i + 2]);
 }
 
+// Cell of every point at stride s: floor(q scale / s) s per axis, q scale in FP32 (the reference
+// divides positions by the voxel size, hashtree.py build paths; _hash_backend.py:363,396). The
+// division by the power-of-two stride is exact, so floor(v / s) = floor(v * (1 / s)).
+__global__ void point_cells_kernel(const float* __restrict__ q, int64_t n3, float scale, int s, float inv_s,
+                                   int32_t* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n3) return;
+    const float v = q[t] * scale;
+    out[t] = (int)floorf(v * inv_s) * s;
+}
+
+// Status of coordinates in a level (_hash_backend.py:292-318): 0 absent, 1 present without
+// children, 2 present with children. A voxel of the level has children in the next finer level
+// exactly when its coordinate is the parent (at the level's stride) of some finer voxel, which is
+// what `finer_parents` holds.
+__global__ void voxel_status_kernel(ngs_grid g, ngs_grid finer_parents, int has_finer, const int32_t* __restrict__ c,
+                                    int64_t n, int64_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = c[3 * i], y = c[3 * i + 1], z = c[3 * i + 2];
+    int64_t st = 0;
+    if (grid_test(g, x, y, z)) st = (has_finer && grid_test(finer_parents, x, y, z)) ? 2 : 1;
+    out[i] = st;
+}
+
+// Trilinear interpolation of per-voxel rows at the points (_hash_backend.py:320-330): for each
+// point the 8 corners of its trilinear stencil at the grid's stride (the splat's corners and
+// weights), absent corners skipped; products data * weight in FP32, summed in corner order in FP64.
+__global__ void split_data_kernel(ngs_grid g, const float* __restrict__ xyz, int64_t n, float inv_vs,
+                                  const float* __restrict__ data, int nc, float* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = 1 << g.log2_stride;
+    const float inv_s = 1.0f / (float)s;
+    int dc[3], sg[3];
+    float loc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) trilinear_axis(xyz[3 * i + a] * inv_vs, s, inv_s, &dc[a], &sg[a], &loc[a]);
+    int id[8];
+    float w[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        const int nx = (o >> 2) & 1, ny = (o >> 1) & 1, nz = o & 1;
+        id[o] = grid_find(g, dc[0] + nx * sg[0] * s, dc[1] + ny * sg[1] * s, dc[2] + nz * sg[2] * s);
+        const float ax = nx ? loc[0] : 1.0f - loc[0];
+        const float ay = ny ? loc[1] : 1.0f - loc[1];
+        const float az = nz ? loc[2] : 1.0f - loc[2];
+        w[o] = ax * ay * az;
+    }
+    for (int c = 0; c < nc; ++c) {
+        double acc = 0.0;
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            if (id[o] >= 0) acc += (double)(data[(int64_t)id[o] * nc + c] * w[o]);
+        out[i * nc + c] = (float)acc;
+    }
+}
+
 // Float ordering as signed ints for atomic min/max.
 __device__ __forceinline__ int f2key(float f) {
     int b = __float_as_int(f);
@@ -993,6 +1053,36 @@ int ngs_grid_query(ngs_grid g, const int32_t* coords, int64_t n, const int32_t* 
 int ngs_grid_rank_rows(ngs_grid g, const int32_t* coords, int64_t n, int32_t* out, ngs_stream_t s) {
     if (n <= 0) return NGS_OK;
     grid_rank_rows_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, coords, n, out);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_point_cells(const float* q, int64_t n, float scale, int32_t stride, int32_t* cells, ngs_stream_t s) {
+    NGS_REQUIRE(stride >= 1 && (stride & (stride - 1)) == 0, "ngs_point_cells: stride must be a power of two");
+    if (n <= 0) return NGS_OK;
+    NGS_REQUIRE(q && cells, "ngs_point_cells: null argument");
+    point_cells_kernel<<<grid1d(3 * n, 256), 256, 0, S(s)>>>(q, 3 * n, scale, stride, 1.0f / (float)stride, cells);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_voxel_status(ngs_grid g, ngs_grid finer_parents, int32_t has_finer, const int32_t* coords, int64_t n,
+                     int64_t* status, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    NGS_REQUIRE(coords && status && g.bits && g.rank, "ngs_voxel_status: null argument");
+    NGS_REQUIRE(!has_finer || (finer_parents.bits && finer_parents.log2_stride == g.log2_stride),
+                "ngs_voxel_status: the finer level's parents must be a grid at the level's stride");
+    voxel_status_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, finer_parents, has_finer, coords, n, status);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_split_data(ngs_grid g, const float* xyz, int64_t n, float inv_voxel_size, const float* data, int32_t nc,
+                   float* out, ngs_stream_t s) {
+    NGS_REQUIRE(nc >= 1, "ngs_split_data: at least one component");
+    if (n <= 0) return NGS_OK;
+    NGS_REQUIRE(xyz && data && out && g.bits && g.rank, "ngs_split_data: null argument");
+    split_data_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(g, xyz, n, inv_voxel_size, data, nc, out);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

@@ -102,7 +102,7 @@ def reconstruct(xyz, normal, voxel_size: float, depth: int = 4, min_density: flo
         recon.operator = operator
         recon.sample_weight = sample_weight
         recon.solve_multigrid(depth - 1, 0, normal_data, screen_alpha=screen_alpha, screen_xyz=xyz_t, solver=solver)
-        v, f = recon.extract_multiscale_mesh(n_upsample=n_upsample)
+        v, f = recon.extract_multiscale_mesh(n_upsample=n_upsample, build_o3d_mesh=False)
     if return_result:
         return ReconstructionResult(tree, recon, v, f)
     if is_numpy:

@@ -181,6 +181,75 @@ class Hierarchy:
             self.set_coords(d, quantize_coords(np.asarray(xyz)[xyz_depth <= d], self.voxel_size, self.strides[d]))
         return xyz_depth
 
+    # _hash_backend.py:358-378
+    def build_dense(self, xyz, expand_range: int = 0):
+        if expand_range == 2:
+            c0 = quantize_coords(xyz, self.voxel_size, 1)
+        else:
+            c0 = unique_rows(np.floor(scaled_positions(xyz, self.voxel_size)).astype(np.int64))
+            if expand_range > 0:
+                off = range_kernel(expand_range).astype(np.int64)
+                c0 = unique_rows((c0.astype(np.int64)[:, None, :] + off[None]).reshape(-1, 3))
+        self.set_coords(0, c0)
+        for d in range(1, self.depth):
+            s = self.strides[d]
+            self.set_coords(d, unique_rows(np.floor_divide(self.coords[d - 1].astype(np.int64), s) * s))
+        return np.zeros(np.asarray(xyz).shape[0], np.int32)
+
+    # _hash_backend.py:380-412
+    def build_subdivide(self, xyz, policy, expand=False, limit_adaptive_depth=100, **kw):
+        """policy(current_pts (float32, voxel units), inverse map to the coarser level's unique cells,
+        **kw) -> bool mask of the points that go one level finer."""
+        xyz = np.asarray(xyz, np.float32)
+        cur = scaled_positions(xyz, self.voxel_size)
+        n = xyz.shape[0]
+        xyz_depth = np.full(n, self.depth - 1, np.int32)
+        inds = np.arange(n)
+        inv = None
+        for d in range(self.depth - 1, -1, -1):
+            if d != self.depth - 1:
+                m = np.asarray(policy(cur, inv, **kw), bool)
+                cur, inds = cur[m], inds[m]
+                kw = {k: (v[m] if isinstance(v, np.ndarray) else v) for k, v in kw.items()}
+                xyz_depth[inds] -= 1
+            s = self.strides[d]
+            cells = (np.floor(cur * (np.float32(1) / np.float32(s))) * np.float32(s)).astype(np.int64)
+            uc, inv = np.unique(cells, axis=0, return_inverse=True) if cells.shape[0] else (np.zeros((0, 3)), None)
+            if inv is not None:
+                inv = inv.reshape(-1)
+            self.set_coords(d, uc)
+        xyz_depth = np.minimum(xyz_depth, limit_adaptive_depth - 1)
+        if expand:
+            for d in range(self.depth):
+                self.set_coords(d, quantize_coords(xyz[xyz_depth <= d], self.voxel_size, self.strides[d]))
+        return xyz_depth
+
+    # _hash_backend.py:292-318: 0 absent, 1 present and no child found, 2 some conformal child present
+    def voxel_status(self, coords, d):
+        coords = np.asarray(coords, np.int64).reshape(-1, 3)
+        st = np.zeros(coords.shape[0], np.int64)
+        sidx = np.nonzero(self.index[d].query(coords) >= 0)[0]
+        st[sidx] = 1
+        if d > 0 and sidx.shape[0]:
+            ch = (coords[sidx][:, None, :] + CONFORM_OFFSETS[None] * self.strides[d - 1]).reshape(-1, 3)
+            q = np.nonzero(self.index[d - 1].query(ch) >= 0)[0] // 8
+            st[sidx[q]] = 2
+        return st
+
+    # _hash_backend.py:320-330: trilinear interpolation of per-voxel rows at the points
+    def split_data(self, xyz, d, data):
+        data = np.asarray(data, np.float64)
+        ac, aw = trilinear_weights(xyz, self.voxel_size, self.strides[d])
+        tid = self.index[d].query(ac)
+        n = np.asarray(xyz).shape[0]
+        src = np.arange(ac.shape[0]) % n
+        ok = tid >= 0
+        d2 = data.reshape(data.shape[0], -1)
+        out = np.zeros((n, d2.shape[1]))
+        for c in range(d2.shape[1]):
+            out[:, c] = np.bincount(src[ok], weights=d2[tid[ok], c] * aw[ok], minlength=n)
+        return out if data.ndim == 2 else out[:, 0]
+
     # _hash_backend.py:230-290 (same- and cross-level neighbours, range kernel)
     def self_neighbours(self, source_depth, target_depth, target_range):
         inv = source_depth > target_depth

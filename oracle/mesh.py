@@ -6,7 +6,7 @@ and csrc/marching_cubes/dual_mc.cu:20-189 (cube classification, edge interpolati
 """
 import numpy as np
 
-from .hierarchy import CONFORM_OFFSETS, Hierarchy, unique_rows
+from .hierarchy import CONFORM_OFFSETS, Hierarchy, range_kernel, unique_rows
 from .mc_tables import EDGE_CORNERS, case_edges
 
 
@@ -55,18 +55,31 @@ def mean_chi(h, solutions, xyz, sample_weight):
     return float(np.sum(chi * w) / np.sum(w))
 
 
-def conformal_leaves(h: Hierarchy, max_depth=100):
-    """reconstructor.py:309-336 (trim=False, expand=0). Returns ({d: leaf coords}, max_depth)."""
+def level_coords_ext(h: Hierarchy, d, expand=0, conforming=False):
+    """_hash_backend.py:121-147: a level's voxels, dilated by the expand-cube (expand >= 3), and with
+    conforming replaced by the 8 children of every distinct parent at twice the stride."""
+    s = h.strides[d]
+    c = h.coords[d].astype(np.int64)
+    if expand >= 3:
+        off = range_kernel(expand).astype(np.int64) * s
+        c = unique_rows((c[:, None, :] + off[None]).reshape(-1, 3)).astype(np.int64)
+    if conforming:
+        par = unique_rows(np.floor_divide(c, 2 * s) * 2 * s).astype(np.int64)
+        c = (par[:, None, :] + CONFORM_OFFSETS[None] * s).reshape(-1, 3)
+    return c
+
+
+def conformal_leaves(h: Hierarchy, max_depth=100, expand=0, trim=False):
+    """reconstructor.py:309-336. Returns ({d: leaf coords}, max_depth)."""
     max_depth = min(max_depth, h.depth)
     base, mask = {}, {}
+    if trim:
+        for d in range(max_depth):
+            st = h.voxel_status(h.coords[d], d)
+            base[d] = h.coords[d].astype(np.int64)[st == 1]
+        return base, max_depth
     for d in range(max_depth):
-        s = h.strides[d]
-        coords = h.coords[d].astype(np.int64)
-        if d < max_depth - 1:  # hashtree.py:209-229 -> _hash_backend.py:139-145
-            par = unique_rows(np.floor_divide(coords, 2 * s) * 2 * s).astype(np.int64)
-            bc = (par[:, None, :] + CONFORM_OFFSETS[None] * s).reshape(-1, 3)
-        else:
-            bc = coords
+        bc = level_coords_ext(h, d, expand, conforming=d < max_depth - 1)
         if d > 0:
             ol = RowLookup(base[d - 1]).query(bc) == -1
             if not np.any(ol):
@@ -241,10 +254,10 @@ def dedup_vertices(tris, vids):
     return verts, inv.reshape(-1, 3).astype(np.int64)
 
 
-def extract_multiscale_mesh(h, solutions, sample_weight, xyz, n_upsample=2, chi_fn=None):
-    """reconstructor.py:296-397 (expand=0, trim=False, no Open3D). chi_fn(points) overrides the
-    field (used to feed identical samples to two implementations)."""
-    leaves, max_depth = conformal_leaves(h)
+def extract_multiscale_mesh(h, solutions, sample_weight, xyz, n_upsample=2, chi_fn=None, expand=0, trim=False):
+    """reconstructor.py:296-397 (no Open3D). chi_fn(points) overrides the field (used to feed
+    identical samples to two implementations)."""
+    leaves, max_depth = conformal_leaves(h, expand=expand, trim=trim)
     primal = expand_primal(h, leaves, max_depth, n_upsample)
     centres = dual_centres(h, primal, max_depth)
     acc, cnt = accumulate_dual_corners(h, primal, max_depth, centres)
