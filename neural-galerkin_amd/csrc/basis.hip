@@ -331,6 +331,37 @@ __global__ void additive_eval_backward_kernel(const float* __restrict__ xyz, con
 
 using namespace ngs;
 
+// Screening term of a level pair for per-(point, voxel) basis values (the generic basis path,
+// reconstructor.py:59-77 -> sparse_product.cu:100-137): for every point p, every entry (i, va) of p at
+// level a and every entry (j, vb) of p at level b, out[k] += (va vb) w_p where k is the slot of column
+// j in row i of the pair structure (rowptr, cols; columns ascending within a row), found by binary
+// search; pairs absent from the structure are dropped, as the reference's hash lookup drops them.
+// One thread per entry of level a; entries of one point are contiguous (a_start, b_start: n + 1
+// offsets).
+__global__ void screen_pairs_kernel(int64_t na, const int32_t* __restrict__ a_pt, const int32_t* __restrict__ a_ids,
+                                    const float* __restrict__ a_vals, const int32_t* __restrict__ b_start,
+                                    const int32_t* __restrict__ b_ids, const float* __restrict__ b_vals,
+                                    const float* __restrict__ w, float w_scalar, const int32_t* __restrict__ rowptr,
+                                    const int32_t* __restrict__ cols, float* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na) return;
+    const int32_t p = a_pt[t], i = a_ids[t];
+    const float va = a_vals[t];
+    const float wp = w ? w[p] : w_scalar;
+    const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
+    for (int32_t k2 = b_start[p]; k2 < b_start[p + 1]; ++k2) {
+        const int32_t j = b_ids[k2];
+        int32_t lo = r0, hi = r1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (cols[mid] < j) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < r1 && cols[lo] == j) atomicAdd(&out[lo], (va * b_vals[k2]) * wp);
+    }
+}
+
 extern "C" {
 
 int ngs_ddi_forward(const float* source_expanded, const float* target_expanded, const int64_t* rel_pos, int64_t n,
@@ -393,6 +424,17 @@ int ngs_additive_eval_backward(const float* xyz, const int64_t* feat_ids, int64_
     NGS_REQUIRE(xyz && feat_ids && components && grad_funcs && grad_coeffs, "ngs_additive_eval_backward: null argument");
     additive_eval_backward_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(xyz, feat_ids, n, components, n_components, stride,
                                                                     grad_funcs, grad_dfuncs, grad_coeffs);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_screen_pairs(int64_t na, const int32_t* a_pt, const int32_t* a_ids, const float* a_vals,
+                     const int32_t* b_start, const int32_t* b_ids, const float* b_vals, const float* w,
+                     float w_scalar, const int32_t* rowptr, const int32_t* cols, float* out, ngs_stream_t s) {
+    if (na <= 0) return NGS_OK;
+    NGS_REQUIRE(a_pt && a_ids && a_vals && b_start && rowptr && cols && out, "ngs_screen_pairs: null argument");
+    screen_pairs_kernel<<<grid1d(na, 256), 256, 0, S(s)>>>(na, a_pt, a_ids, a_vals, b_start, b_ids, b_vals, w,
+                                                            w_scalar, rowptr, cols, out);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

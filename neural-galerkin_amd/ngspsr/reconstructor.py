@@ -24,7 +24,8 @@ class PointBinning:
         cell = torch.empty((n, 3), dtype=torch.int32, device=dev)
         b = torch.empty((n, 9), dtype=torch.float32, device=dev)
         call("ngs_point_basis", ptr(xyz), n, tree.inv_vs, s, ptr(cell), ptr(b), stream())
-        self.cells = SparseGrid.for_points(tree._qmin, tree._qmax, s, dev)
+        qmin, qmax = tree.point_bounds(xyz)
+        self.cells = SparseGrid.for_points(qmin, qmax, s, dev)
         self.cells.mark_coords(cell)
         self.ncells = self.cells.finalize(emit=False)
         rank = torch.empty(n, dtype=torch.int32, device=dev)
@@ -67,11 +68,23 @@ class PointBinning:
 
 class Reconstructor:
     def __init__(self, hash_tree: HashTree, basis=None, feat=None):
-        if basis is not None and not isinstance(basis, BezierTensorBasis):
-            raise NotImplementedError("only the degree-2 Bezier tensor basis is supported on this path")
+        """reconstructor.py:26-49. The degree-2 B-spline basis (the default) runs the specialised
+        kernels below; any other basis (AdditiveBasis, AdditiveHermite) runs the reference's generic
+        path with per-voxel basis features (`feat`: {depth: (M_d, basis.get_feature_size())}, rows in
+        the level's row order; initialised by the basis when omitted)."""
         self.hash_tree = hash_tree
-        self.basis = basis or BezierTensorBasis()
+        self.basis = basis if basis is not None else BezierTensorBasis()
+        self.generic = not isinstance(self.basis, BezierTensorBasis)
+        if self.generic and feat is None:
+            feat = {}
+            for d in range(hash_tree.depth):
+                f = torch.zeros((hash_tree.get_coords_size(HashTree.DECODER, d), self.basis.get_feature_size()),
+                                device=hash_tree.device)
+                self.basis.initialize_feature_value(f)
+                feat[d] = f
+        self.grid_features = feat if feat is not None else {}
         self.solutions = {}
+        self._f32 = {}
         self._points_chi = None
         self.systems = {}
         self.solve_info = {}
@@ -83,6 +96,22 @@ class Reconstructor:
 
     def set_fixed_level_set(self, value):
         self.fixed_level_set = value
+
+    def field(self, d):
+        """Level d's coefficients as the FP32 vector the field kernels read, in rank order: the
+        solution itself, or its FP32 rounding when the solver returned FP64 (iterative refinement),
+        reordered when the level's rows are not in rank order (cached per tensor and in-place
+        version). The kernels sum in FP32, so the rounding of the coefficients is of the same order as
+        the evaluation's own."""
+        sol = self.solutions[d]
+        lv = self.hash_tree._branch(self.branch)[d]
+        if sol.dtype == torch.float32 and (lv is None or lv.row_of_rank is None):
+            return sol
+        c = self._f32.get(d)
+        if c is None or c[0] is not sol or c[1] != sol._version:
+            c = (sol, sol._version, self.hash_tree.to_rank(d, sol.float()).contiguous())
+            self._f32[d] = c
+        return c[2]
 
     # Per-application time model of the operator forms (picoseconds, fitted to MI355X kernel timings
     # on 0.05M-5.6M-row levels): the run form ~115 ps per row (600 bytes per row at ~5.3 TB/s); CSR
@@ -166,6 +195,9 @@ class Reconstructor:
         """reconstructor.py:79-195: coarse-to-fine per-level solves; each level's RHS is corrected by
         the coarser solutions (cascadic). solve_fn(system, rhs) -> (x, info) replaces the solver
         (the serving engine routes solves through its batched pool)."""
+        if self.generic:
+            return self._solve_multigrid_generic(start_depth, end_depth, normal_data, screen_alpha, screen_xyz,
+                                                 screen_delta, solver, verbose, solve_fn)
         tree = self.hash_tree
         if isinstance(screen_alpha, torch.Tensor):
             assert screen_xyz is not None and screen_alpha.shape[0] == screen_xyz.shape[0]
@@ -174,8 +206,12 @@ class Reconstructor:
         else:
             should_screen = screen_alpha > 0.0
             alpha = float(screen_alpha)
+        delta_t = None
         if isinstance(screen_delta, torch.Tensor):
-            raise NotImplementedError("per-point screen_delta is not supported on this path")
+            # per-point targets (reconstructor.py:143-147): the screening RHS sf sum_p a_p (d_p - X_p) B_i(p)
+            # is the kernels' -sf sum_p a_p X'_p B_i(p) with X' = X - d and a scalar target of 0
+            assert screen_xyz is not None and screen_delta.shape[0] == screen_xyz.shape[0]
+            delta_t = screen_delta.to(device=tree.device, dtype=torch.float32).contiguous()
         self.solutions = {}
         self._points_chi = None
         running, added = None, []
@@ -191,7 +227,8 @@ class Reconstructor:
             sf = 0.25 ** d
             rhs = torch.zeros(M, dtype=torch.float64, device=tree.device)
             for dd, nd in normal_data.items():
-                call("ngs_rhs_normals", tree.level(d), tree.level(dd), ptr(nd.float().contiguous()), ptr(rhs), stream())
+                nd = tree.to_rank(dd, nd.float()).contiguous()
+                call("ngs_rhs_normals", tree.level(d), tree.level(dd), ptr(nd), ptr(rhs), stream())
             coarser = [dd for dd in range(start_depth, d, -1) if tree.get_coords_size(None, dd) > 0]
             coarse = None
             if should_screen and coarser:
@@ -201,15 +238,23 @@ class Reconstructor:
                     running = torch.zeros(xyz.shape[0], dtype=torch.float32, device=tree.device)
                 for dd in coarser:
                     if dd not in added:
-                        self._add_field_at_points(dd, self.solutions[dd], xyz, bins[dd], running)
+                        self._add_field_at_points(dd, self.field(dd), xyz, bins[dd], running)
                         added.append(dd)
                 coarse = running
+            if delta_t is not None and should_screen:
+                coarse = (coarse - delta_t) if coarse is not None else -delta_t
             for dd in coarser:
-                call("ngs_rhs_cross_stiffness", tree.level(d), tree.level(dd), ptr(self.solutions[dd]), ptr(rhs),
+                call("ngs_rhs_cross_stiffness", tree.level(d), tree.level(dd), ptr(self.field(dd)), ptr(rhs),
                      stream())
-            system = self.assemble_level(d, bins[d], coarse, sf, float(screen_delta), rhs)
-            x, info = solve_fn(system, rhs) if solve_fn is not None else _solver.solve(system, rhs, solver)
-            self.solutions[d] = x.contiguous()
+            system = self.assemble_level(d, bins[d], coarse, sf, 0.0 if delta_t is not None else float(screen_delta),
+                                         rhs)
+            # "mixed" tries PCG only on the finest level; the coarser levels are solved directly
+            # (reconstructor.py:186-189)
+            level_solver = solver
+            if solver.split("@")[0] == "mixed" and d != end_depth:
+                level_solver = "cholmod"
+            x, info = solve_fn(system, rhs) if solve_fn is not None else _solver.solve(system, rhs, level_solver)
+            self.solutions[d] = tree.to_rows(d, x).contiguous()
             self.systems[d] = (system, rhs)
             self.solve_info[d] = info
             if verbose:
@@ -219,9 +264,161 @@ class Reconstructor:
             # completed with the remaining levels, the running sum is chi at the input points
             for dd, sol in self.solutions.items():
                 if sol.numel() and dd not in added:
-                    self._add_field_at_points(dd, sol, xyz, bins[dd], running)
+                    self._add_field_at_points(dd, self.field(dd), xyz, bins[dd], running)
             self._points_chi = (self._solutions_key(), running)
         return self.solutions
+
+    # -------------------------------------------------------------------------- generic bases
+    def _screen_data(self, d, base):
+        """reconstructor.py:111-120: (point ids, voxel ids, basis values, entries per point) of the
+        points around every voxel of level d, point-major."""
+        tree = self.hash_tree
+        pts, vx, off, nbs = tree.get_neighbours_data(base, 1, d, tree.get_range_kernel(self.basis.get_domain_range()),
+                                                     self.branch, transposed=True)
+        qc = -off / tree.get_stride(self.branch, d)
+        vals = self.basis.evaluate(feat=self.grid_features[d], xyz=qc, feat_ids=vx)
+        return pts, vx, vals, nbs
+
+    def _screening_term(self, sa, sb, src, tgt, m_a, m_b, weight):
+        """reconstructor.py:59-77 (screened_multiplication): per COO pair (src of level a, tgt of level
+        b), sum over the points p around both of w_p B_src(p) B_tgt(p); ngs_screen_pairs locates each
+        pair in the row-sorted pair structure instead of the reference's pair hash table."""
+        dev = self.hash_tree.device
+        E = src.shape[0]
+        out = torch.zeros(E, dtype=torch.float32, device=dev)
+        if E == 0 or sa[0].numel() == 0 or sb[0].numel() == 0:
+            return out
+        order = torch.argsort(src.long() * m_b + tgt.long())
+        rowptr = torch.zeros(m_a + 1, dtype=torch.int32, device=dev)
+        rowptr[1:] = torch.cumsum(torch.bincount(src.long(), minlength=m_a), 0).int()
+        cols = tgt[order].int().contiguous()
+        b_start = torch.zeros(sb[3].shape[0] + 1, dtype=torch.int32, device=dev)
+        b_start[1:] = torch.cumsum(sb[3], 0).int()
+        w = weight.float().contiguous() if isinstance(weight, torch.Tensor) else None
+        o = torch.zeros(E, dtype=torch.float32, device=dev)
+        a_pt, a_ids, a_vals = sa[0].int().contiguous(), sa[1].int().contiguous(), sa[2].float().contiguous()
+        b_ids, b_vals = sb[1].int().contiguous(), sb[2].float().contiguous()
+        call("ngs_screen_pairs", a_pt.shape[0], ptr(a_pt), ptr(a_ids), ptr(a_vals), ptr(b_start), ptr(b_ids),
+             ptr(b_vals), ptr(w), 0.0 if w is not None else float(weight), ptr(rowptr), ptr(cols), ptr(o), stream())
+        out[order] = o
+        return out
+
+    @staticmethod
+    def coo_system(src, tgt, val, M):
+        """COO level matrix (unique pairs) as the solver's CSR: rows sorted, columns ascending, rows
+        padded to multiples of 4 entries with zero-valued diagonal entries (solver.py:40-68)."""
+        dev = val.device
+        order = torch.argsort(src.long() * M + tgt.long())
+        s, t, v = src[order].long(), tgt[order].int(), val[order].float()
+        cnt = torch.bincount(s, minlength=M)
+        pcnt = cnt + (-cnt) % 4
+        rowptr = torch.zeros(M + 1, dtype=torch.int64, device=dev)
+        rowptr[1:] = torch.cumsum(pcnt, 0)
+        cols = torch.repeat_interleave(torch.arange(M, dtype=torch.int32, device=dev), pcnt)
+        vals = torch.zeros(cols.shape[0], dtype=torch.float32, device=dev)
+        first = torch.cumsum(cnt, 0) - cnt
+        pos = rowptr[s] + (torch.arange(s.shape[0], device=dev) - first[s])
+        cols[pos] = t
+        vals[pos] = v
+        diag = torch.zeros(M, dtype=torch.float32, device=dev)
+        on = s == t.long()
+        diag[s[on]] = v[on]
+        return _solver.CSRSystem(rowptr.int(), cols, vals, 1.0 / diag)
+
+    def _solve_multigrid_generic(self, start_depth, end_depth, normal_data, screen_alpha, screen_xyz, screen_delta,
+                                 solver, verbose, solve_fn):
+        """reconstructor.py:79-195 for bases with per-voxel features: COO neighbour pairs of the tree,
+        Galerkin integrals of the basis (DDI kernels), the screening pairs (ngs_screen_pairs), the
+        level system in CSR and the chosen solver. Every per-voxel array is in the level's row order."""
+        tree, basis, br = self.hash_tree, self.basis, self.branch
+        dev = tree.device
+        feats = self.grid_features
+        if isinstance(screen_alpha, torch.Tensor):
+            assert screen_xyz is not None and screen_alpha.shape[0] == screen_xyz.shape[0]
+            should_screen = True
+        else:
+            should_screen = screen_alpha > 0.0
+        self.solutions = {}
+        self._points_chi = None
+        nrange = 2 * basis.get_domain_range() - 1
+        screen = {}
+        if should_screen:
+            base = screen_xyz.float() * tree.inv_vs - 0.5  # screen_xyz / voxel_size - 0.5
+            for d in range(end_depth, start_depth + 1):
+                screen[d] = self._screen_data(d, base)
+        for d in range(start_depth, end_depth - 1, -1):
+            M = tree.get_coords_size(br, d)
+            if M == 0:
+                self.solutions[d] = torch.zeros(0, dtype=torch.float32, device=dev)
+                continue
+            sf = 0.25 ** d
+            s_d = tree.get_stride(br, d)
+            rhs = torch.zeros(M, dtype=torch.float64, device=dev)
+            for dd, nd in normal_data.items():
+                nids, tids, off, _ = tree.get_neighbours(dd, d, basis.get_domain_range(), br)
+                part = basis.integrate_const_deriv_product(data=-nd.float()[nids], target_feat=feats[d], rel_pos=off,
+                                                           data_stride=tree.get_stride(br, dd), target_stride=s_d,
+                                                           target_ids=tids)
+                rhs.index_add_(0, tids, part.double())
+            if should_screen:
+                pts, vx, vals, _ = screen[d]
+                if isinstance(screen_alpha, torch.Tensor) or isinstance(screen_delta, torch.Tensor):
+                    mult = (screen_delta * screen_alpha)[pts]
+                else:
+                    mult = screen_alpha * screen_delta
+                part = torch.zeros(M, dtype=torch.float64, device=dev).index_add_(0, vx, (vals * mult).double())
+                rhs += sf * part
+            for dd in range(start_depth, d, -1):
+                m_dd = tree.get_coords_size(br, dd)
+                if m_dd == 0:
+                    continue
+                src, tgt, rel, _ = tree.get_neighbours(d, dd, nrange, br)
+                a = basis.integrate_deriv_deriv_product(source_feat=feats[d], target_feat=feats[dd], rel_pos=rel,
+                                                        source_stride=s_d, target_stride=tree.get_stride(br, dd),
+                                                        source_ids=src, target_ids=tgt)
+                if should_screen:
+                    a = a + sf * self._screening_term(screen[d], screen[dd], src, tgt, M, m_dd, screen_alpha)
+                rhs.index_add_(0, src, -(self.solutions[dd].double()[tgt] * a.double()))
+            src, tgt, rel, _ = tree.get_neighbours(d, d, nrange, br)
+            lhs = basis.integrate_deriv_deriv_product(source_feat=feats[d], target_feat=feats[d], rel_pos=rel,
+                                                      source_stride=s_d, target_stride=s_d, source_ids=src,
+                                                      target_ids=tgt)
+            if should_screen:
+                lhs = lhs + sf * self._screening_term(screen[d], screen[d], src, tgt, M, M, screen_alpha)
+            system = self.coo_system(src, tgt, lhs, M)
+            level_solver = "cholmod" if (solver.split("@")[0] == "mixed" and d != end_depth) else solver
+            x, info = solve_fn(system, rhs) if solve_fn is not None else _solver.solve(system, rhs, level_solver)
+            self.solutions[d] = x.contiguous()
+            self.systems[d] = (system, rhs)
+            self.solve_info[d] = info
+            if verbose:
+                r = system.matvec(x).double() - rhs
+                print(f"Solving complete at level {d}, residual = {torch.linalg.norm(r).item()}.")
+        return self.solutions
+
+    def _evaluate_generic(self, q, compute_mask, compute_grad, depths):
+        """hashtree.py:157-192 with the basis and its features, summed over the solved levels."""
+        tree, basis = self.hash_tree, self.basis
+        n = q.shape[0]
+        base = q * tree.inv_vs - 0.5
+        out = torch.zeros((n, 3) if compute_grad else n, dtype=torch.float32, device=tree.device)
+        mask = torch.zeros(n, dtype=torch.bool, device=tree.device) if compute_mask else None
+        for d, sol in self.solutions.items():
+            if (depths is not None and d not in depths) or sol.numel() == 0:
+                continue
+            st = tree.get_stride(self.branch, d)
+            bids, tids, off, _ = tree.get_neighbours_data(base, 1, d, tree.get_range_kernel(basis.get_domain_range()),
+                                                          self.branch)
+            qc = -off / st
+            f = self.grid_features[d]
+            if compute_grad:
+                v = basis.evaluate_derivative(feat=f, xyz=qc, feat_ids=tids, stride=st) * sol.float()[tids, None]
+            else:
+                v = basis.evaluate(feat=f, xyz=qc, feat_ids=tids) * sol.float()[tids]
+            out.index_add_(0, bids, v.float())
+            if compute_mask:
+                mask[bids] = True
+        return (out, mask) if compute_mask else out
 
     def _same_points(self, xyz):
         q = self.hash_tree.xyz
@@ -245,6 +442,8 @@ class Reconstructor:
         bool mask of queries that some level's basis covers (a voxel among the 27 around the query's
         cell at any evaluated level; hashtree.py:186-190)."""
         assert len(self.solutions) > 0, "Please run solver before evaluation."
+        if self.generic:
+            return self._evaluate_generic(xyz.float().contiguous(), compute_mask, compute_grad, depths)
         tree = self.hash_tree
         q = xyz.float().contiguous()
         n = q.shape[0]
@@ -257,7 +456,8 @@ class Reconstructor:
             if sol.numel() == 0:
                 continue
             g = tree.grids[d].struct()
-            call("ngs_eval_field", g, ptr(sol), ptr(q), n, tree.inv_vs, 1.0, 1, ptr(out), ptr(grad), stream())
+            call("ngs_eval_field", g, ptr(self.field(d)), ptr(q), n, tree.inv_vs, 1.0, 1, ptr(out), ptr(grad),
+                 stream())
             if compute_mask:
                 call("ngs_field_support", g, ptr(q), n, tree.inv_vs, ptr(mask), stream())
         val = grad if compute_grad else out
@@ -323,30 +523,48 @@ class Reconstructor:
         return triangle_mesh(vertices, triangles, normals)
 
     def extract_multiscale_mesh(self, n_upsample: int = 1, max_depth: int = 100, expand: int = 0, trim: bool = False,
-                                build_o3d_mesh: bool = False, max_points: int = -1, return_info=False):
-        """reconstructor.py:296-397 (expand=0, trim=False): dual marching cubes over the conformal
-        leaves, each split into n_upsample^3 sub-cells. Returns (vertices (V,3) float32,
-        triangles (T,3) int64)."""
-        if expand != 0 or trim:
-            raise NotImplementedError("only expand=0, trim=False is supported on this path")
+                                build_o3d_mesh: bool = True, max_points: int = -1, return_info=False):
+        """reconstructor.py:296-397: dual marching cubes over the leaves of the tree, each split into
+        n_upsample^3 sub-cells. The leaves are those of the conformal tree (every voxel's parent
+        replaced by all 8 children, then the cells with finer children dropped), over each level's
+        voxels dilated by the expand-cube when expand >= 3; with trim, the voxels without children
+        (evaluate_voxel_status == VS_EXIST_STOP). Returns (vertices (V,3) float32, triangles (T,3)
+        int64), or a mesh object with build_o3d_mesh."""
         tree = self.hash_tree
         dev = tree.device
         L = min(max_depth, tree.depth)
+        # per level: the voxels the leaves come from and their box (lattice units of the level's stride)
+        lev_coords, lev_box = {}, {}
+        for d in range(L):
+            s = tree.strides[d]
+            g = tree.grids[d]
+            lo, hi = list(g.origin), [o + dm - 1 for o, dm in zip(g.origin, g.dims)]
+            coords = g.coords
+            if expand >= 3 and not trim:
+                if expand % 2 == 0:
+                    raise ValueError("expand must be odd")
+                h = expand // 2
+                lo, hi = [v - h for v in lo], [v + h for v in hi]
+                dg = SparseGrid(lo, hi, s, dev)
+                if coords.shape[0]:
+                    call("ngs_grid_mark_dilated", dg.struct(), ptr(coords), coords.shape[0], h, stream())
+                dg.finalize()
+                coords = dg.coords
+            lev_coords[d], lev_box[d] = coords, (lo, hi)
         # parents (stride 2s) of every level but the last, sorted
         parents = {}
         for d in range(L - 1):
             s = tree.strides[d]
-            g = tree.grids[d]
-            pg = SparseGrid([(o * s) // (2 * s) - 1 for o in g.origin],
-                            [((o + dm) * s) // (2 * s) + 1 for o, dm in zip(g.origin, g.dims)], 2 * s, dev)
-            pg.mark_coords(g.coords)
+            lo, hi = lev_box[d]
+            pg = SparseGrid([(o * s) // (2 * s) - 1 for o in lo], [(o * s) // (2 * s) + 1 for o in hi], 2 * s, dev)
+            pg.mark_coords(lev_coords[d])
             parents[d] = pg
         SparseGrid.finalize_all(list(parents.values()))
         leaves = {}
         max_d = L
         for d in range(L):
             s = tree.strides[d]
-            if d < L - 1:
+            if d < L - 1 and not trim:
                 npar = parents[d].count
                 conf = torch.empty((npar * 8, 3), dtype=torch.int32, device=dev)
                 keep = torch.empty(npar * 8, dtype=torch.int32, device=dev)
@@ -354,9 +572,9 @@ class Reconstructor:
                 call("ngs_conformal_children", ptr(parents[d].coords), npar, s, fin.struct(), int(d > 0), ptr(conf),
                      ptr(keep), stream())
             else:
-                conf = tree.grids[d].coords
+                conf = lev_coords[d]
                 keep = torch.ones(conf.shape[0], dtype=torch.int32, device=dev)
-                if d > 0:
+                if d > 0 and conf.shape[0]:
                     call("ngs_unrefined_flags", ptr(conf), conf.shape[0], parents[d - 1].struct(), ptr(keep), stream())
             n = conf.shape[0]
             out = torch.empty((n, 3), dtype=torch.int32, device=dev)
@@ -367,7 +585,7 @@ class Reconstructor:
                 k = int(offs[n].item())
             else:
                 k = 0
-            if d > 0 and k == 0:
+            if d > 0 and k == 0 and not trim:
                 max_d = d
                 break
             leaves[d] = out[:k]
@@ -390,15 +608,14 @@ class Reconstructor:
             bases_[d] = nprim
             nprim += nsub[d]
         # dual centres: sorted unique corners of all primal sub-cells (upsampled units)
-        lo = [min(((o - 2) * tree.strides[d]) * nu for d in range(L) for o in [tree.grids[d].origin[a]]) for a in range(3)]
-        hi = [max(((o + dm + 2) * tree.strides[d]) * nu for d in range(L)
-                  for o, dm in [(tree.grids[d].origin[a], tree.grids[d].dims[a])]) for a in range(3)]
+        lo = [min((lev_box[d][0][a] - 2) * tree.strides[d] * nu for d in range(L)) for a in range(3)]
+        hi = [max((lev_box[d][1][a] + 3) * tree.strides[d] * nu for d in range(L)) for a in range(3)]
         cg = SparseGrid(lo, hi, 1, dev)
         first = 0
         lgrid = None
         if nu == 2 and tree.strides[0] == 1 and leaves[0].shape[0]:
             # finest leaves: centre words computed from a leaf bitmap (no atomics), before the others
-            lgrid = self._leaf_grid(0, leaves[0], parents)
+            lgrid = self._leaf_grid(0, leaves[0], parents, lev_box, trim)
             call("ngs_mark_dual_leaf_words", cg.struct(), lgrid.struct(), stream())
             first = 1
         for d in range(first, L):
@@ -413,7 +630,10 @@ class Reconstructor:
             if nsub[d]:
                 call("ngs_leaf_corner_positions", ptr(leaves[d]), leaves[d].shape[0], tree.strides[d], nu, scale,
                      ptr(pos[bases_[d]:]), stream())
-        values = self.leaf_corner_chi(leaves, bases_, nprim, nu, scale) - self.get_mean_chi()
+        if self.generic:  # the basis's own evaluation at the corner positions
+            values = self.evaluate_chi(pos, max_points=max_points)
+        else:
+            values = self.leaf_corner_chi(leaves, bases_, nprim, nu, scale) - self.get_mean_chi()
         # primal id per octant, read only for the kept centres (all eight slots written); the count
         # word also counts the registered cells with a negative value
         acc = torch.empty((DC, 8), dtype=torch.int32, device=dev)
@@ -427,7 +647,7 @@ class Reconstructor:
                 if d == 0 and lgrid is not None:
                     lg = lgrid
                 else:
-                    lg = self._leaf_grid(d, leaves[d], parents)
+                    lg = self._leaf_grid(d, leaves[d], parents, lev_box, trim)
                 lg.finalize(emit=False, count=False)  # the leaves are distinct: the rank of each is below n_l
                 n_l = leaves[d].shape[0]
                 ranks = torch.empty(n_l, dtype=torch.int32, device=dev)
@@ -466,19 +686,21 @@ class Reconstructor:
         if return_info:
             return v, f, dict(leaves=leaves, primal=primal, centres=cg.coords, acc=acc, count=cnt, cubes=cubes, pos=pos,
                               values=values)
+        if build_o3d_mesh:
+            from .io import triangle_mesh
+            return triangle_mesh(v, f)
         return v, f
 
-    def _leaf_grid(self, d, lf, parents):
-        """Bitmap of the level-d leaves (stride of level d): the children box of the level's parents,
-        or the level's own box for the last level."""
+    def _leaf_grid(self, d, lf, parents, lev_box, trim):
+        """Bitmap of the level-d leaves (stride of level d): the children box of the level's parents
+        (conformal leaves), or the level's own box (the last level, or trimmed leaves)."""
         tree = self.hash_tree
         s = tree.strides[d]
-        if d in parents:
+        if d in parents and not trim:
             po, pdm = parents[d].origin, parents[d].dims  # lattice units of stride 2s
             lo, hi = [2 * o for o in po], [2 * (o + dm - 1) + 1 for o, dm in zip(po, pdm)]
         else:
-            g = tree.grids[d]
-            lo, hi = list(g.origin), [o + dm - 1 for o, dm in zip(g.origin, g.dims)]
+            lo, hi = lev_box[d]
         lg = SparseGrid(lo, hi, s, tree.device)
         lg.mark_coords(lf)
         return lg
@@ -488,7 +710,7 @@ class Reconstructor:
         positions ngs_dual_corner_positions writes, level by level in the same order)."""
         tree = self.hash_tree
         out = torch.empty(nprim, dtype=torch.float32, device=tree.device)
-        levels = [(d, sol) for d, sol in self.solutions.items() if sol.numel()]
+        levels = [(d, self.field(d)) for d, sol in self.solutions.items() if sol.numel()]
         if not levels:
             return out.zero_()
         garr = (Grid * len(levels))(*[tree.grids[d].struct() for d, _ in levels])
