@@ -1,13 +1,18 @@
 """Benchmark of the screened-Poisson reconstruction path on one MI355X.
 
-Default workload (one GPU): 2M points on the union of 16 seeded random ellipsoids (semi-axes in
-[0.05, 0.25], sigma = 0.002 position noise, von Mises-Fisher kappa = 200 normal noise), voxel size
-1/512 (512^3 finest), 4 levels, screening 4.0, FP32 Jacobi-PCG with FP64 reductions inside FP64
-iterative refinement to relative residual 1e-8, dual marching cubes with 2x upsampling. One step
-is one full reconstruction (hierarchy, splat, assembly, solve, mesh) with the points already in
-HBM. Prints one JSON line.
+Default workload (config 3, the metric's): a stream of 256 requests, each a seeded scene of 16
+random ellipsoids with 1e4-1e6 points (log-uniform), depth 7-9 (voxel size 2^-depth), position
+noise 0-0.005, 4 levels, screening 4.0, solved to relative residual 1e-6. It is served plain (one
+request at a time, FP32 Jacobi-PCG) and by the continuous-batching server (mixed precision: FP32
+inner PCG in one shared batched pool, FP64 residual check and refinement). A step is one pass of
+the whole stream through the batched server with every request queued at t = 0; `value` is its
+solves/s. Latency is measured under Poisson arrivals at 0.8x the plain capacity. The line also
+carries the single-solve leg (config 2: 2M points on 16 noisy ellipsoids at 512^3, refinement to
+1e-8) whose finest-level SpMV gives the roofline. With --config c1|c2|c5 the single-solve workload
+is the headline (reconstructions/s). Points are in HBM when the timed region starts. Prints one
+JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c5] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3|c2|c1|c5] [--no-cpu]
 """
 import argparse
 import json
@@ -105,7 +110,7 @@ def stencil_roofline(system, reps=50):
     return t_pts, b_pts, t_rows, b_rows
 
 
-def assembly_roofline(res, xyz, reps=10):
+def assembly_roofline(res, xyz, name="c2", reps=10):
     """Average duration of the finest-level assembly kernel (ngs_lhs_assemble_runs, screening with a
     coarse field at the points) and its algorithmic bytes, the survey's per-level figure with the run
     form's writes: 4 nnz (values) + 4 R (non-empty run words) + 4 M (inv_diag) + 8 M (rhs read) +
@@ -137,7 +142,7 @@ def assembly_roofline(res, xyz, reps=10):
     algo = 4 * nnz + 4 * nruns + (4 + 16 + 12) * M + 52 * n
     return {"kernel": "lhs_assemble_kernel (finest level, run form, screening with a coarse field)", "bound": "latency",
             "achieved": round(algo / t / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4), "traffic": measured_traffic("lhs_assemble_kernel"),
+            "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4), "traffic": measured_traffic("lhs_assemble_kernel", name),
             "launch_us": round(1e6 * t, 2), "algorithmic_bytes": int(algo), "rows": int(M), "points": int(n)}
 
 
@@ -192,11 +197,23 @@ def fp64_error(csr, rhs, x, tol=1e-12, max_iter=3000):
             "stored_fp32_solution_rel_residual": float(opcg.relative_residual(a, xg, b))}
 
 
-def cpu_stages():
-    """Per-stage wall-clock of the oracle pipeline (numpy, with the C/OpenMP Jacobi-PCG in FP32 with
-    FP32 dots as the reference solves, to 1e-6) on the 100k-point torus workload (128^3, 4 levels),
-    one run per stage (the whole pipeline takes minutes), beside the GPU's end-to-end time for the same
-    reconstruction. Prints one JSON line."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_stages(runs=5):
+    """CPU baseline by the stated protocol (1 warm-up and `runs` timed runs, median per stage):
+    config 1 (100k-point torus, 128^3, 4 levels) through the whole oracle pipeline (numpy stages, the
+    C/OpenMP Jacobi-PCG in FP32 to 1e-6 as the reference solves, dual MC), beside the GPU's time for
+    the same reconstruction; config 2 (2M points, 512^3) as per-level PCG only: the GPU's assembled
+    level systems solved by the oracle's C/OpenMP Jacobi-PCG in FP64 to 1e-8. Prints one JSON line."""
     import numpy as np
     import torch
     import ngspsr
@@ -204,56 +221,84 @@ def cpu_stages():
     from oracle import assembly, mesh, pcg
     cfg = CONFIGS["c1"]
     xyz, nrm = make_scene(cfg)
-    t = {}
-    t0 = time.perf_counter()
-    h, lv, rho = assembly.build_tree(xyz, cfg["vs"], cfg["depth"], cfg["min_density"])
-    t["voxelise_density_levels"] = time.perf_counter() - t0
-    print(f"[cpu stage] voxelise_density_levels: {t['voxelise_density_levels']:.1f} s", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    nd = assembly.splat_normals(h, xyz, nrm, lv, 1.0 / rho)
-    t["splat"] = time.perf_counter() - t0
-    print(f"[cpu stage] splat: {t['splat']:.1f} s", file=sys.stderr, flush=True)
-    pcg_s = [0.0]
+    per_run = []
+    meta = {}
+    for r in range(runs + 1):
+        t = {}
+        t0 = time.perf_counter()
+        h, lv, rho = assembly.build_tree(xyz, cfg["vs"], cfg["depth"], cfg["min_density"])
+        t["voxelise_density_levels"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        nd = assembly.splat_normals(h, xyz, nrm, lv, 1.0 / rho)
+        t["splat"] = time.perf_counter() - t0
+        pcg_s = [0.0]
+        iters = {}
 
-    def solve(a, b):
-        t1 = time.perf_counter()
-        out = pcg.solve_pcg(a, b, tol=1e-6, max_iter=10000, dtype=np.float32)
-        pcg_s[0] += time.perf_counter() - t1
-        print(f"[cpu stage] level of {a.shape[0]} rows solved in {out[1]} iterations", file=sys.stderr, flush=True)
-        return out
+        def solve(a, b):
+            t1 = time.perf_counter()
+            out = pcg.solve_pcg(a, b, tol=1e-6, max_iter=10000, dtype=np.float32)
+            pcg_s[0] += time.perf_counter() - t1
+            iters[a.shape[0]] = int(out[1])
+            return out
 
-    t0 = time.perf_counter()
-    res = assembly.solve_multigrid(h, nd, xyz, 4.0, 0.1, solve=solve)
-    t["assembly_rhs"] = time.perf_counter() - t0 - pcg_s[0]
-    t["pcg"] = pcg_s[0]
-    print(f"[cpu stage] pcg: {t['pcg']:.1f} s", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    v, f, _ = mesh.extract_multiscale_mesh(h, res.solutions, 1.0 / rho, xyz)
-    t["mesh"] = time.perf_counter() - t0
-    print(f"[cpu stage] mesh: {t['mesh']:.1f} s", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        res = assembly.solve_multigrid(h, nd, xyz, 4.0, 0.1, solve=solve)
+        t["assembly_rhs"] = time.perf_counter() - t0 - pcg_s[0]
+        t["pcg"] = pcg_s[0]
+        t0 = time.perf_counter()
+        v, f, _ = mesh.extract_multiscale_mesh(h, res.solutions, 1.0 / rho, xyz)
+        t["mesh"] = time.perf_counter() - t0
+        t["total"] = sum(t.values())
+        log(f"cpu stages c1 run {r} ({'warm-up' if r == 0 else 'timed'}): " +
+            ", ".join(f"{k} {v:.2f} s" for k, v in t.items()))
+        if r > 0:
+            per_run.append(t)
+        meta = {"voxels_by_level": [int(h.num_voxels(d)) for d in range(h.depth)],
+                "cpu_pcg_iterations": {int(d): int(i) for d, i in res.iters.items()},
+                "cpu_mesh_triangles": int(f.shape[0])}
+    med = {k: round(float(np.median([t[k] for t in per_run])), 3) for k in per_run[0]}
     dev = torch.device("cuda:0")
     xt, nt = torch.from_numpy(xyz).to(dev), torch.from_numpy(nrm).to(dev)
 
-    def gpu():
-        return ngspsr.reconstruct(xt, nt, cfg["vs"], depth=cfg["depth"], min_density=cfg["min_density"],
-                                  screen_alpha=4.0, solver=cfg["solver"])
+    def gpu(c, x, n):
+        return ngspsr.reconstruct(x, n, c["vs"], depth=c["depth"], min_density=c["min_density"], screen_alpha=4.0,
+                                  solver=c["solver"], return_result=True)
     for _ in range(3):
-        gpu()
+        gpu(cfg, xt, nt)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(10):
-        vg, fg = gpu()
+        g1 = gpu(cfg, xt, nt)
     torch.cuda.synchronize()
     g_s = (time.perf_counter() - t0) / 10
-    total = sum(t.values())
-    print(json.dumps({"cpu_stages_s": {k: round(v, 3) for k, v in t.items()}, "cpu_total_s": round(total, 2),
-                      "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
-                      "workload": cfg["workload"], "voxels_by_level": [int(h.num_voxels(d)) for d in range(h.depth)],
-                      "cpu_pcg_iterations": {int(d): int(i) for d, i in res.iters.items()},
-                      "cpu_mesh_triangles": int(f.shape[0]), "gpu_ms": round(1e3 * g_s, 2),
-                      "gpu_triangles": int(fg.shape[0]), "cpu_over_gpu": round(total / g_s, 1),
-                      "note": "one run per CPU stage; numpy stages single-threaded, PCG on the OpenMP threads"}),
-          flush=True)
+    # config 2: per-level PCG of the GPU's assembled systems, FP64 to 1e-8, on the host
+    cfg2 = CONFIGS["c2"]
+    x2, n2 = make_scene(cfg2)
+    r2 = gpu(cfg2, torch.from_numpy(x2).to(dev), torch.from_numpy(n2).to(dev))
+    c2 = {}
+    for d, (system, rhs) in sorted(r2.reconstructor.systems.items()):
+        a = system.to_scipy()
+        b = rhs.double().cpu().numpy()
+        ts, its = [], 0
+        for r in range(runs + 1):
+            t0 = time.perf_counter()
+            _, its = pcg.solve_pcg(a, b, tol=1e-8, max_iter=20000, dtype=np.float64)
+            if r > 0:
+                ts.append(time.perf_counter() - t0)
+        c2[int(d)] = {"rows": int(a.shape[0]), "nnz": int(a.nnz), "iterations": int(its),
+                      "median_s": round(float(np.median(ts)), 3)}
+        log(f"cpu stages c2 level {d}: {c2[int(d)]}")
+    c2_total = sum(v["median_s"] for v in c2.values())
+    print(json.dumps({
+        "protocol": f"1 warm-up + {runs} timed runs, median per stage",
+        "cpu_model": _cpu_model(), "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+        "c1": {"workload": cfg["workload"], "cpu_stages_median_s": med, **meta, "gpu_ms": round(1e3 * g_s, 2),
+               "gpu_triangles": int(g1.triangles.shape[0]), "cpu_over_gpu": round(med["total"] / g_s, 1),
+               "note": "numpy stages single-threaded; the PCG on the OpenMP threads"},
+        "c2_pcg_by_level": {"workload": cfg2["workload"], "levels": c2, "cpu_pcg_total_median_s": round(c2_total, 3),
+                            "solver": "the oracle's C/OpenMP Jacobi-PCG in FP64 to relative residual 1e-8 on the "
+                                      "GPU's assembled level systems (PCG only; assembly and mesh not run on the CPU)"},
+    }), flush=True)
 
 
 COPY_BYTES = 1 << 30
@@ -283,11 +328,16 @@ def pmc_probe(res):
     print(json.dumps({"pmc_probe": True, "M": csr.M, "nnz": csr.nnz(), "copy_bytes": COPY_BYTES}), flush=True)
 
 
-def measured_traffic(kernel):
+TRAFFIC_FILES = {"c2": "traffic_ellipsoids2M_512.json", "c5": "traffic_room10M_1024.json"}
+
+
+def measured_traffic(kernel, name="c2"):
     """HBM bytes per launch of a kernel ("spmv_kernel", "spmv_runs_kernel", or "lhs_assemble_kernel" for
-    the finest-level assembly) at the default workload, from the committed counter summary
+    the finest-level assembly) at a single-solve workload, from the committed counter summary
     (profiles/), if present."""
-    path = os.path.join(ROOT, "profiles", "traffic_ellipsoids2M_512.json")
+    if name not in TRAFFIC_FILES:
+        return None
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILES[name])
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -295,94 +345,133 @@ def measured_traffic(kernel):
     return t.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
 
 
-def serving_scenes(n_req, seed0=1000):
-    """Requests of the serving stream: random-ellipsoid scenes, points log-uniform in [1e4, 1e6],
-    depth uniform in {7, 8, 9} (voxel size 2^-depth), sigma uniform in [0, 0.005]."""
+def _serving_scene(i, seed0=1000):
     import numpy as np
     from ngspsr import synthetic
-    out = []
-    for i in range(n_req):
-        rng = np.random.default_rng(seed0 + i)
-        n = synthetic.log_uniform_int(rng, 1e4, 1e6)
-        depth = int(rng.integers(7, 10))
-        sigma = float(rng.uniform(0, 0.005))
-        xyz, nrm = synthetic.ellipsoids(n, k=16, sigma=sigma, kappa=200.0, seed=seed0 + i)
-        out.append((xyz, nrm, 2.0 ** -depth))
-    return out
+    rng = np.random.default_rng(seed0 + i)
+    n = synthetic.log_uniform_int(rng, 1e4, 1e6)
+    depth = int(rng.integers(7, 10))
+    sigma = float(rng.uniform(0, 0.005))
+    xyz, nrm = synthetic.ellipsoids(n, k=16, sigma=sigma, kappa=200.0, seed=seed0 + i)
+    return xyz, nrm, 2.0 ** -depth
 
 
-def run_serving(n_req=64):
-    """The same request stream served three ways, at relative residual 1e-6:
-    plain (one request at a time, `reconstruct` with FP32 PCG), batched FP32 (the continuous-batching
-    server, FP32 PCG solves batched in the pool), and plain vs batched with mixed precision (FP64
-    refinement; residual verified in FP64). Then the batched FP32 server under Poisson arrivals at
-    0.8x its measured capacity for latency."""
+def serving_scenes(n_req, seed0=1000):
+    """Requests of the serving stream: random-ellipsoid scenes (16 ellipsoids, kappa 200), points
+    log-uniform in [1e4, 1e6], depth uniform in {7, 8, 9} (voxel size 2^-depth), sigma uniform in
+    [0, 0.005], seeds seed0 + i. Generated on up to 16 host processes."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_serving_scene, range(n_req), [seed0] * n_req, chunksize=4))
+
+
+def run_serving(n_req=256, steps=20, warmup=5, workers=16, chunk=32, log=None):
+    """Config 3: the same stream of n_req requests served plain and batched, at relative residual
+    1e-6.
+    - plain: one request at a time, the reference's path (`reconstruct`, FP32 Jacobi-PCG to 1e-6);
+      one pass, all requests queued at t = 0: its capacity (solves/s) and service latencies.
+    - batched: the continuous-batching server with mixed precision (FP32 inner PCG in the shared
+      batched pool, FP64 residual check and refinement to 1e-6 per level); `warmup` untimed passes,
+      then `steps` timed passes of the whole stream, all requests queued at t = 0 (a step is one pass).
+    - latency: Poisson arrivals at 0.8x the plain capacity, served by the batched server and by the
+      plain path (first come, first served, one at a time): p50 / p99 of arrival-to-result time.
+    Returns (result dict, timed seconds)."""
     import numpy as np
     import torch
     import ngspsr
     from ngspsr import serving
     dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
     scenes = serving_scenes(n_req)
+    gen_s = time.perf_counter() - t0
     data = [(torch.from_numpy(x).to(dev), torch.from_numpy(n).to(dev), vs) for x, n, vs in scenes]
+    del scenes
     pts = int(sum(x.shape[0] for x, _, _ in data))
+    say = log or (lambda m: None)
+    say(f"serving: {n_req} requests, {pts} points, generated in {gen_s:.1f} s")
 
-    def plain(solver):
-        for x, n, vs in data[:2]:
-            ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver=solver)
+    def plain_one(x, n, vs):
+        ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
         torch.cuda.synchronize()
-        lat = []
-        t0 = time.perf_counter()
-        for x, n, vs in data:
-            t1 = time.perf_counter()
-            ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver=solver)
-            torch.cuda.synchronize()
-            lat.append(time.perf_counter() - t1)
-        return n_req / (time.perf_counter() - t0), lat
 
-    def batched(mixed, workers, chunk):
-        srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, refine=mixed, chunk=chunk)
-        srv.serve([serving.Request(-1 - i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data[:workers])])
-        reqs = [serving.Request(i, x, n, vs, arrival=0.0) for i, (x, n, vs) in enumerate(data)]
-        t0 = time.perf_counter()
-        srv.serve(reqs, t0=t0)
+    for x, n, vs in data[:4]:
+        plain_one(x, n, vs)
+    lat = []
+    tp = time.perf_counter()
+    for x, n, vs in data:
+        t1 = time.perf_counter()
+        plain_one(x, n, vs)
+        lat.append(time.perf_counter() - t1)
+    plain_s = time.perf_counter() - tp
+    plain_rate = n_req / plain_s
+    say(f"serving: plain {plain_rate:.2f} solves/s")
+
+    srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, refine=True, chunk=chunk)
+
+    def one_pass(arrivals=None):
+        reqs = [serving.Request(i, x, n, vs, arrival=0.0 if arrivals is None else float(arrivals[i]))
+                for i, (x, n, vs) in enumerate(data)]
+        ts = time.perf_counter()
+        srv.serve(reqs, t0=ts)
         torch.cuda.synchronize()
         errs = [r for r in reqs if r.error is not None]
         if errs:
             srv.close()
             raise errs[0].error
-        return srv, n_req / (max(r.finish for r in reqs) - t0), reqs
+        return reqs, time.perf_counter() - ts
 
-    plain_rate, plain_lat = plain("pcg@tol=1e-6@maxiter=20000")
-    plain_mixed_rate, _ = plain("refine@tol=1e-6@inner_tol=1e-4")
-    srv_m, mixed_rate, reqs_m = batched(True, 8, 8)
-    mixed_res = max(r.max_rel_residual for r in reqs_m)
-    srv_m.close()
-    srv, batched_rate, reqs = batched(False, 16, 32)
-    # latency at a Poisson arrival rate of 0.8x the measured batched capacity
-    lam = 0.8 * batched_rate
-    rng = np.random.default_rng(7)
-    arr = np.cumsum(rng.exponential(1.0 / lam, n_req))
-    reqs2 = [serving.Request(i, x, n, vs, arrival=float(a)) for i, ((x, n, vs), a) in enumerate(zip(data, arr))]
-    t0 = time.perf_counter()
-    srv.serve(reqs2, t0=t0)
-    lat = serving.latency_stats(reqs2)
+    for w in range(warmup):
+        one_pass()
+        say(f"serving: batched warm-up pass {w + 1}/{warmup}")
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    res_max = 0.0
+    for k in range(steps):
+        reqs, _ = one_pass()
+        res_max = max(res_max, max(r.max_rel_residual for r in reqs))
+        say(f"serving: batched pass {k + 1}/{steps}")
+    torch.cuda.synchronize()
+    timed_s = time.perf_counter() - tb
+    batched_rate = steps * n_req / timed_s
+    # latency under Poisson arrivals at 0.8x the plain capacity
+    lam = 0.8 * plain_rate
+    arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / lam, n_req))
+    reqs, _ = one_pass(arr)
+    lat_b = serving.latency_stats(reqs)
     pool = srv.pool
     srv.close()
-    return {
-        "requests": n_req, "points_total": pts,
+    # the plain path under the same arrivals: first come, first served
+    tq = time.perf_counter()
+    done = []
+    for (x, n, vs), a in zip(data, arr):
+        now = time.perf_counter() - tq
+        if now < a:
+            time.sleep(a - now)
+        plain_one(x, n, vs)
+        done.append(time.perf_counter() - tq - a)
+    lat_p = np.array(done)
+    say("serving: latency passes done")
+    out = {
+        "workload": "serving stream: 256 requests, ellipsoid scenes 1e4-1e6 points (log-uniform), depth 7-9, "
+                    "sigma 0-0.005, 4 levels, relative residual 1e-6",
+        "requests": n_req, "points_total": pts, "passes_timed": steps, "passes_warmup": warmup,
+        "batched_mixed_solves_per_s": round(batched_rate, 3),
+        "batched_config": {"workers": workers, "pool_chunk": chunk, "precision": "FP32 inner PCG, FP64 residual "
+                           "check and refinement to 1e-6"},
+        "batched_max_fp64_rel_residual": res_max,
         "plain_fp32_solves_per_s": round(plain_rate, 3),
-        "plain_latency_ms": {"p50": round(1e3 * float(np.percentile(plain_lat, 50)), 2),
-                             "p99": round(1e3 * float(np.percentile(plain_lat, 99)), 2)},
-        "batched_fp32_solves_per_s": round(batched_rate, 3), "batched_fp32_config": {"workers": 16, "chunk": 32},
-        "batched_vs_plain": round(batched_rate / plain_rate, 3),
-        "plain_mixed_solves_per_s": round(plain_mixed_rate, 3),
-        "batched_mixed_solves_per_s": round(mixed_rate, 3), "batched_mixed_config": {"workers": 8, "chunk": 8},
-        "batched_vs_plain_mixed": round(mixed_rate / plain_mixed_rate, 3),
-        "batched_mixed_max_fp64_rel_residual": mixed_res,
-        "poisson_rate_per_s": round(lam, 3),
-        "poisson_latency_ms": {"p50": round(lat["p50_ms"], 2), "p99": round(lat["p99_ms"], 2)},
+        "plain_service_latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
+                                     "p99": round(1e3 * float(np.percentile(lat, 99)), 2)},
+        "north_star_ratio_batched_mixed_vs_plain_fp32": round(batched_rate / plain_rate, 3),
+        "poisson_rate_per_s": round(lam, 3), "poisson_rate_basis": "0.8 x plain FP32 capacity",
+        "poisson_latency_ms_batched": {"p50": round(lat_b["p50_ms"], 2), "p99": round(lat_b["p99_ms"], 2)},
+        "poisson_latency_ms_plain": {"p50": round(1e3 * float(np.percentile(lat_p, 50)), 2),
+                                     "p99": round(1e3 * float(np.percentile(lat_p, 99)), 2)},
         "pool_launches": pool.launches,
     }
+    return out, timed_s
 
 
 def extensions(res, xyz):
@@ -434,31 +523,18 @@ def extensions(res, xyz):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extensions", action="store_true",
-                    help="skip the field-query, primal marching cubes and DDI timings")
-    ap.add_argument("--serving", type=int, default=64, help="requests of the serving comparison (0: skip)")
-    ap.add_argument("--cpu-stages", action="store_true",
-                    help="per-stage CPU timing of the oracle pipeline on the 100k-point torus workload (minutes)")
-    ap.add_argument("--pmc-probe", action="store_true",
-                    help="counter probe: one reconstruction, then 3 device copies of 1 GiB (calibration) and 5 "
-                         "launches of the finest-level SpMV; run under rocprofv3 --pmc")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the path's multi-GPU sharding (by request for serving, "
-                                   "by spatial slabs for one large solve) is not launched", "n_gpus": args.gpus}))
-        sys.exit(2)
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def single_solve(name, steps, warmup, cpu=True, ext=True, fp64=True):
+    """One workload solved end to end `steps` times (after `warmup`), points in HBM: the record of a
+    single-solve configuration (c1, c2, c5), with the finest-level SpMV roofline, the assembly, the
+    FP64 error, host-I/O latency and, with cpu, the oracle's C/OpenMP PCG on the finest level."""
     import numpy as np
     import torch
     import ngspsr
-
-    cfg = CONFIGS[args.config]
+    cfg = CONFIGS[name]
     dev = torch.device("cuda:0")
     xyz_np, nrm_np = make_scene(cfg)
     xyz = torch.from_numpy(xyz_np).to(dev)
@@ -468,30 +544,37 @@ def main():
         return ngspsr.reconstruct(xyz, nrm, cfg["vs"], depth=cfg["depth"], min_density=cfg["min_density"],
                                   screen_alpha=4.0, solver=cfg["solver"], return_result=True)
 
-    if args.cpu_stages:
-        cpu_stages()
-        return
-    if args.pmc_probe:
-        pmc_probe(step())
-        return
-
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         res = step()
     torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
     lat = []
     t_start = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         t0 = time.perf_counter()
         res = step()
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
     total = time.perf_counter() - t_start
+    peak = torch.cuda.max_memory_allocated(dev)
     lat = np.array(lat)
+    log(f"{name}: {steps / total:.3f} reconstructions/s")
+    # numpy in, numpy out: the H2D copy of the points and the D2H copy of the mesh included
+    io = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ngspsr.reconstruct(xyz_np, nrm_np, cfg["vs"], depth=cfg["depth"], min_density=cfg["min_density"],
+                           screen_alpha=4.0, solver=cfg["solver"])
+        io.append(time.perf_counter() - t0)
 
     info = res.solve_info
     total_iters = int(sum(i.iters for i in info.values()))
-    rel = max(float(i.rel_resid) for i in info.values())
     from ngspsr.solver import RunSystem, StencilSystem
+    # the FP64 residual of every level's returned solution, recomputed from the assembled system
+    rel_ret = {}
+    for d, (s_, b_) in sorted(res.reconstructor.systems.items()):
+        _, rr, bb = s_.residual_f64(b_.double(), res.reconstructor.solutions[d].double())
+        rel_ret[int(d)] = (rr / bb) ** 0.5 if bb > 0 else 0.0
     system, rhs = res.reconstructor.systems[0]
     forms = {int(d): ("stencil" if isinstance(s_, StencilSystem) else "runs" if isinstance(s_, RunSystem) else "csr")
              for d, (s_, _) in sorted(res.reconstructor.systems.items())}
@@ -500,7 +583,7 @@ def main():
     t_iter = pcg_iteration_rate(system)
     spmv_obj = {"kernel": "spmv_kernel<8,0,1> (finest-level CSR SpMV, int32 columns, the solve_pcg_diag layout)",
                 "bound": "hbm", "achieved": round(algo / t_spmv / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": measured_traffic("spmv_kernel"),
+                "frac": round(algo / t_spmv / 1e9 / HBM_PEAK_GBS, 4), "traffic": measured_traffic("spmv_kernel", name),
                 "launch_us": round(1e6 * t_spmv, 2), "algorithmic_bytes": int(algo)}
     if isinstance(system, StencilSystem):
         t_pts, b_pts, t_rows, b_rows = stencil_roofline(system)
@@ -519,56 +602,244 @@ def main():
         roof = {"kernel": "spmv_runs_kernel (finest-level SpMV, run form: one run word per bitmap column, "
                           "values plane-major)", "bound": "hbm", "achieved": round(algo_run / t_run / 1e9, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(algo_run / t_run / 1e9 / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic("spmv_runs_kernel"), "launch_us": round(1e6 * t_run, 2),
+                "traffic": measured_traffic("spmv_runs_kernel", name), "launch_us": round(1e6 * t_run, 2),
                 "algorithmic_bytes": int(algo_run), **extra, "csr_spmv_us_same_matrix": round(1e6 * t_spmv, 2)}
     else:
         iter_bytes = algo + 44 * system.M
         roof = dict(spmv_obj)
+    roof["workload"] = cfg["workload"]
     out = {
-        "metric": "solves/s + p99 latency at 1e-6 rel. residual, batched vs plain; PCG HBM GB/s",
-        "value": round(args.steps / total, 4),
-        "unit": "reconstructions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * total / args.steps, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "precision": "FP32 level operators (assembled, run form) and PCG vectors; FP64 dot products, residual "
-                     "and refinement",
-        "data": "synthetic",
+        "value": round(steps / total, 4), "unit": "reconstructions/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * total / steps, 3),
         "config": {"workload": cfg["workload"], "points": cfg["n"], "voxel_size": cfg["vs"], "levels": cfg["depth"],
                    "solver": cfg["solver"], "finest_voxels": int(system.M), "finest_nnz": int(nnz),
+                   "voxels_by_level": [int(res.tree.get_coords_size(None, d)) for d in range(cfg["depth"])],
                    "operator_forms": forms},
+        "timed_region": "points already in HBM; the mesh stays on the device (no host copies)",
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
-        "max_level_rel_residual": rel,
+        "host_io_latency_ms": {"median": round(1e3 * float(np.median(io)), 3),
+                               "note": "numpy points in, numpy mesh out: host-to-device copy of the points and "
+                                       "device-to-host copy of the mesh included"},
+        "peak_hbm_GiB": round(peak / 2 ** 30, 3),
+        "max_level_rel_residual": max(rel_ret.values()),
+        "rel_residual_by_level": rel_ret,
+        "residual_note": "FP64 relative residual of the solution each level returns (FP64 for refine, FP32 for pcg)",
         "pcg_iterations_total": total_iters,
         "pcg_iterations_by_level": {int(d): int(i.iters) for d, i in sorted(info.items())},
         "pcg_iteration": {"ms": round(1e3 * t_iter, 4), "algorithmic_bytes": int(iter_bytes),
                           "GB/s": round(iter_bytes / t_iter / 1e9, 1)},
         "roofline": roof,
         "csr_spmv": spmv_obj,
-        "assembly": assembly_roofline(res, xyz),
+        "assembly": assembly_roofline(res, xyz, name),
         "mesh": {"vertices": int(res.vertices.shape[0]), "triangles": int(res.triangles.shape[0])},
     }
-    if not args.no_extensions:
+    if ext:
         out["extensions"] = extensions(res, xyz)
-    if not args.no_cpu:
+    if fp64:
         out["solution_error_vs_fp64_oracle"] = fp64_error(csr, rhs, res.reconstructor.solutions[0])
+    if cpu:
         per_iter, it = cpu_baseline(csr, rhs)
-        cpu_rate = 1.0 / (per_iter * total_iters)
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
-        out["cpu_baseline"] = {
-            "value": round(cpu_rate, 5), "unit": "reconstructions/s", "cores": cores, "kind": "port",
-            "sample": f"{it} iterations of the oracle's C/OpenMP FP32 Jacobi-PCG (FP64 dots) on the finest-level "
-                      f"system (M={system.M}, nnz={nnz}), {per_iter * 1e3:.1f} ms/iteration, scaled by the GPU run's "
-                      f"{total_iters} PCG iterations over all levels; PCG only (upper bound on CPU solves/s)"}
-    if args.serving > 0:
-        out["serving"] = run_serving(args.serving)
-    print(json.dumps(out), flush=True)
+        out["cpu_pcg_finest"] = {
+            "ms_per_iteration": round(per_iter * 1e3, 3), "iterations_timed": it,
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+            "note": "the oracle's C/OpenMP FP32 Jacobi-PCG (FP64 dots) on the finest-level system"}
+    return out
+
+
+def cpu_serving_sample(budget_s=30.0):
+    """CPU baseline of the serving stream: the oracle pipeline (numpy stages and the C/OpenMP
+    Jacobi-PCG in FP32 to 1e-6, the reference's plain method) end to end on the stream's smallest
+    requests, smallest first, until budget_s is spent (at least one). Returns (record, the indices
+    of the requests run)."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import reconstruct as orec
+    sizes = []
+    for i in range(256):
+        rng = np.random.default_rng(1000 + i)
+        from ngspsr import synthetic
+        n = synthetic.log_uniform_int(rng, 1e4, 1e6)
+        depth = int(rng.integers(7, 10))
+        sizes.append((n * 2 ** (depth - 7), i, n, depth))
+    sizes.sort()
+    done, t_all = [], 0.0
+    for _, i, n, depth in sizes:
+        xyz, nrm, vs = _serving_scene(i)
+        t0 = time.perf_counter()
+        orec.reconstruct(xyz, nrm, vs, depth=4, min_density=1.0, screen_alpha=4.0, tol=1e-6, dtype=np.float32)
+        t_all += time.perf_counter() - t0
+        done.append((i, n, depth))
+        log(f"cpu sample: request {i} ({n} points, depth {depth}) done, {t_all:.1f} s so far")
+        if t_all >= budget_s:
+            break
+    rec = {"value": round(len(done) / t_all, 5), "unit": "solves/s", "cores": int(os.environ.get("OMP_NUM_THREADS",
+                                                                                                   os.cpu_count())),
+           "kind": "port",
+           "sample": f"the oracle pipeline (numpy stages, C/OpenMP FP32 Jacobi-PCG to 1e-6, dual MC) end to end on the "
+                     f"{len(done)} smallest requests of the stream ({', '.join(f'{n} points at depth {d}' for _, n, d in done)}), "
+                     f"one run each, {t_all:.1f} s; the stream's requests average 8x more points than these",
+           "note": "numpy stages run on one core; the PCG uses the OpenMP threads"}
+    return rec, [i for i, _, _ in done]
+
+
+def plain_pass(n_req=256):
+    """The plain path once over the serving stream (after 4 warm-up requests), for a kernel trace:
+    prints the wall time of the pass. Run under rocprofv3 --kernel-trace; --trace-busy summarises."""
+    import torch
+    import ngspsr
+    dev = torch.device("cuda:0")
+    data = [(torch.from_numpy(x).to(dev), torch.from_numpy(n).to(dev), vs) for x, n, vs in serving_scenes(n_req)]
+    for x, n, vs in data[:4]:
+        ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t0_ns = time.monotonic_ns()
+    for x, n, vs in data:
+        ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
+        torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print(json.dumps({"plain_pass": True, "requests": n_req, "wall_s": round(wall, 4),
+                      "solves_per_s": round(n_req / wall, 3), "start_monotonic_ns": t0_ns,
+                      "end_monotonic_ns": time.monotonic_ns()}), flush=True)
+
+
+def trace_busy(path, skip_first=0):
+    """GPU-busy summary of a rocprofv3 kernel trace (CSV): the union of kernel execution intervals
+    over the span from the first to the last kernel (after skipping the first `skip_first` kernels,
+    the warm-up), the sum of kernel durations, and the largest kernels by total time."""
+    import csv
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    rows = rows[skip_first:]
+    busy, cur_s, cur_e, tot = 0, None, None, 0
+    per = {}
+    for s_, e_, name in rows:
+        tot += e_ - s_
+        key = name.split("(")[0]
+        per[key] = per.get(key, 0) + (e_ - s_)
+        if cur_e is None or s_ > cur_e:
+            if cur_e is not None:
+                busy += cur_e - cur_s
+            cur_s, cur_e = s_, e_
+        else:
+            cur_e = max(cur_e, e_)
+    if cur_e is not None:
+        busy += cur_e - cur_s
+    span = rows[-1][1] - rows[0][0] if rows else 0
+    top = sorted(per.items(), key=lambda kv: -kv[1])[:15]
+    print(json.dumps({"trace": os.path.basename(path), "kernels": len(rows), "span_s": span / 1e9,
+                      "busy_union_s": busy / 1e9, "busy_fraction": busy / span if span else 0.0,
+                      "kernel_time_sum_s": tot / 1e9,
+                      "top_kernels_s": {k: round(v / 1e9, 4) for k, v in top}}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps: passes over the serving stream (c3) or "
+                                                          "reconstructions (c1, c2, c5)")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c3"])
+    ap.add_argument("--requests", type=int, default=256, help="requests of the serving stream (c3)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extensions", action="store_true",
+                    help="skip the field-query, primal marching cubes and DDI timings")
+    ap.add_argument("--cpu-stages", action="store_true",
+                    help="per-stage CPU timing of the oracle pipeline (100k-point torus in full, 2M-point ellipsoids "
+                         "per-level PCG), 1 warm-up and 5 timed runs (minutes)")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="counter probe: one reconstruction, then 3 device copies of 1 GiB (calibration) and 5 "
+                         "launches of the finest-level SpMV; run under rocprofv3 --pmc")
+    ap.add_argument("--probe-config", default="c2", choices=sorted(CONFIGS), help="workload of --pmc-probe")
+    ap.add_argument("--plain-pass", action="store_true",
+                    help="the plain path once over the serving stream, for a kernel trace (rocprofv3)")
+    ap.add_argument("--trace-busy", default=None, help="summarise a rocprofv3 kernel-trace CSV (no GPU)")
+    ap.add_argument("--trace-skip", type=int, default=0, help="kernels to skip at the start of --trace-busy")
+    args = ap.parse_args()
+    if args.trace_busy:
+        trace_busy(args.trace_busy, args.trace_skip)
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the path's multi-GPU sharding (by request for serving, "
+                                   "by spatial slabs for one large solve) is not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    import ngspsr
+
+    if args.cpu_stages:
+        cpu_stages()
+        return
+    if args.plain_pass:
+        plain_pass(args.requests)
+        return
+    if args.pmc_probe:
+        cfg = CONFIGS[args.probe_config]
+        xyz_np, nrm_np = make_scene(cfg)
+        dev = torch.device("cuda:0")
+        res = ngspsr.reconstruct(torch.from_numpy(xyz_np).to(dev), torch.from_numpy(nrm_np).to(dev), cfg["vs"],
+                                 depth=cfg["depth"], min_density=cfg["min_density"], screen_alpha=4.0,
+                                 solver=cfg["solver"], return_result=True)
+        pmc_probe(res)
+        return
+
+    common = {"metric": "solves/s + p99 latency at 1e-6 rel. residual, batched vs plain; PCG HBM GB/s",
+              "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+              "precision": "FP32 level operators (assembled, run form) and PCG vectors; FP64 dot products, residual "
+                           "and refinement", "data": "synthetic"}
+    if args.config != "c3":
+        out = single_solve(args.config, args.steps, args.warmup, cpu=not args.no_cpu, ext=not args.no_extensions)
+        line = {**common, **{k: out[k] for k in ("value", "unit", "steps", "warmup", "ms_per_step")},
+                "config": out["config"]}
+        line["unit"] = "reconstructions/s"
+        line["measure"] = (f"single-solve workload {CONFIGS[args.config]['workload']}: reconstructions/s end to end "
+                           "(the serving metric's batched/plain ratio is the c3 workload's)")
+        line.update({k: v for k, v in out.items() if k not in line})
+        if not args.no_cpu:
+            line["cpu_baseline"] = cpu_single_baseline(args.config, out)
+        print(json.dumps(line), flush=True)
+        return
+
+    # config 3, the metric's workload: the serving stream
+    srv, timed_s = run_serving(args.requests, args.steps, args.warmup, log=log)
+    log("single-solve leg (2M points, 512^3, refinement to 1e-8)")
+    single = single_solve("c2", 10, 2, cpu=False, ext=not args.no_extensions)
+    line = {**common,
+            "value": srv["batched_mixed_solves_per_s"], "unit": "solves/s",
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * timed_s / args.steps, 3),
+            "step": f"one pass over the {args.requests}-request stream, all requests queued at t = 0",
+            "config": {"workload": "serving stream of 256 random-ellipsoid requests (1e4-1e6 points, depth 7-9, "
+                                   "4 levels) at relative residual 1e-6", "requests": args.requests,
+                       "points_total": srv["points_total"], "server": srv["batched_config"]},
+            "latency_ms": srv["poisson_latency_ms_batched"],
+            "plain_fp32_solves_per_s": srv["plain_fp32_solves_per_s"],
+            "batched_vs_plain": srv["north_star_ratio_batched_mixed_vs_plain_fp32"],
+            "timed_region": "request points already in HBM; meshes stay on the device",
+            "roofline": single["roofline"],
+            "pcg_iteration": single["pcg_iteration"],
+            "serving": srv,
+            "single_solve": {k: v for k, v in single.items() if k not in ("roofline", "pcg_iteration")}}
+    if not args.no_cpu:
+        rec, _ = cpu_serving_sample()
+        line["cpu_baseline"] = rec
+    print(json.dumps(line), flush=True)
+
+
+def cpu_single_baseline(name, out):
+    """CPU baseline of a single-solve workload: the oracle's C/OpenMP PCG on the finest level,
+    scaled by the GPU run's iterations over all levels (PCG only)."""
+    c = out.get("cpu_pcg_finest")
+    if c is None:
+        return None
+    total_iters = out["pcg_iterations_total"]
+    rate = 1.0 / (c["ms_per_iteration"] * 1e-3 * total_iters)
+    return {"value": round(rate, 5), "unit": "reconstructions/s", "cores": c["cores"], "kind": "port",
+            "sample": f"{c['iterations_timed']} iterations of the oracle's C/OpenMP FP32 Jacobi-PCG (FP64 dots) on the "
+                      f"finest-level system, {c['ms_per_iteration']} ms/iteration, scaled by the GPU run's "
+                      f"{total_iters} PCG iterations over all levels; PCG only (an upper bound on CPU solves/s)"}
 
 
 if __name__ == "__main__":

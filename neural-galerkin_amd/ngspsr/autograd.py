@@ -81,7 +81,7 @@ class DifferentiableScreenedPoisson:
             system = self.recon.assemble_level(d, bins, X if coarser else None, sf, self.delta, rhs)
             x, info = _solver.solve(system, rhs, self.solver)
             self.systems[d] = system
-            self.x[d] = x.contiguous()
+            self.x[d] = x.float().contiguous()  # the FP32 kernels below read it ("refine" returns FP64)
             self.coarse[d] = X
             self.infos[d] = info
         return torch.cat([self.x[d] for d in range(self.L)])

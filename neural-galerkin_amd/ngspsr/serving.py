@@ -9,6 +9,7 @@ a device work table, replayed as a captured hipGraph), admitting new systems and
 ones between chunks of iterations. Mixed precision (FP64 refinement around FP32 inner solves)
 runs per request around the pool's solves.
 """
+import concurrent.futures as cf
 import math
 import queue
 import threading
@@ -234,6 +235,10 @@ class ReconstructionServer:
         self.tol, self.refine, self.inner_tol = tol, refine, inner_tol
         self.pool = BatchedPCG(self.device, capacity_blocks=capacity_blocks, chunk=chunk, use_graph=use_graph)
         self.workers = workers
+        # persistent worker threads: each keeps its thread-local scratch and mark buffers (ngspsr._lib
+        # Scratch, Marks) from one request to the next, so steady-state requests allocate only their
+        # own tree, matrices and Krylov state
+        self.executor = cf.ThreadPoolExecutor(max_workers=workers, thread_name_prefix="ngspsr-request")
         self.in_use = 0
         self.lock = threading.Condition()
         self.completed = []
@@ -250,14 +255,15 @@ class ReconstructionServer:
         rel = math.sqrt(rr) / nb if nb > 0 else 0.0
         info = _solver.SolveInfo()
         while rel > self.tol and info.outer < 20:
-            e, inner = self.pool.solve(system, r64.float(), tol=self.inner_tol)
+            inner_tol = _solver.inner_tolerance(self.tol, self.inner_tol, info.outer)
+            e, inner = self.pool.solve(system, r64.float(), tol=inner_tol)
             x64 += e.double()
             info.iters += inner.iters
             info.outer += 1
             r64, rr, _ = system.residual_f64(b64, x64)
             rel = math.sqrt(rr) / nb
         info.rel_resid = info.true_rel_resid = rel
-        return x64.float(), info
+        return x64, info
 
     def run_request(self, req: Request):
         s = torch.cuda.Stream(self.device)
@@ -272,6 +278,7 @@ class ReconstructionServer:
                     return
                 tree = HashTree(req.xyz, req.voxel_size, req.depth)
                 tree.build_encoder_hierarchy_adaptive(min_density=req.min_density)
+                tree.reflect_decoder_coords()
                 w = 1.0 / tree.xyz_density
                 counts = tree.level_point_counts()
                 nd = {}
@@ -283,7 +290,7 @@ class ReconstructionServer:
                 rec.sample_weight = w
                 rec.solve_multigrid(req.depth - 1, 0, nd, screen_alpha=req.screen_alpha, screen_xyz=req.xyz,
                                     solve_fn=self._solve)
-                v, f = rec.extract_multiscale_mesh(n_upsample=2)
+                v, f = rec.extract_multiscale_mesh(n_upsample=2, build_o3d_mesh=False)
                 s.synchronize()
                 req.result = (v, f)
                 infos = rec.solve_info.values()
@@ -292,7 +299,7 @@ class ReconstructionServer:
             req.error = e
         finally:
             if req.error is not None:
-                # the thread's scratch and mark buffers serve its next request on another stream
+                # the worker's scratch and mark buffers serve its next request on another stream
                 try:
                     s.synchronize()
                 except Exception:  # the request's error is already recorded
@@ -310,7 +317,7 @@ class ReconstructionServer:
         """Runs the requests honouring their arrival offsets (seconds after t0); returns them."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter() if t0 is None else t0
-        threads = []
+        futures = []
         for req in reqs:
             delay = t0 + req.arrival - time.perf_counter()
             if delay > 0:
@@ -321,14 +328,13 @@ class ReconstructionServer:
                     self.lock.wait()
                 self.workers_busy += 1
                 self.in_use += req.footprint
-            th = threading.Thread(target=self.run_request, args=(req,), daemon=True)
-            th.start()
-            threads.append(th)
-        for th in threads:
-            th.join()
+            futures.append(self.executor.submit(self.run_request, req))
+        for fu in futures:
+            fu.result()
         return reqs
 
     def close(self):
+        self.executor.shutdown(wait=True)
         self.pool.close()
 
 

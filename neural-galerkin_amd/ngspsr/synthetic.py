@@ -80,8 +80,10 @@ def vmf_perturb(rng, normals, kappa):
     return out / np.linalg.norm(out, axis=1, keepdims=True)
 
 
-def ellipsoids(n, k=16, axis_range=(0.05, 0.25), center_range=(0.2, 0.8), sigma=0.0, kappa=None, seed=0):
-    """Union of k random ellipsoids, points allocated by (approximate) area."""
+def ellipsoids(n, k=16, axis_range=(0.05, 0.25), center_range=(0.2, 0.8), sigma=0.0, kappa=None, seed=0,
+               return_params=False):
+    """Union of k random ellipsoids, points allocated by (approximate) area. With return_params, also
+    (semi-axes (k, 3), rotations (k, 3, 3), centres (k, 3))."""
     rng = np.random.default_rng(seed)
     axes = rng.uniform(*axis_range, (k, 3))
     rots = [random_rotation(rng) for _ in range(k)]
@@ -99,7 +101,25 @@ def ellipsoids(n, k=16, axis_range=(0.05, 0.25), center_range=(0.2, 0.8), sigma=
         xyz = xyz + rng.normal(0, sigma, xyz.shape)
     if kappa is not None:
         nrm = vmf_perturb(rng, nrm, kappa)
+    if return_params:
+        return xyz.astype(np.float32), nrm.astype(np.float32), (axes, np.stack(rots), cents)
     return xyz.astype(np.float32), nrm.astype(np.float32)
+
+
+def ellipsoids_sdf(p, params):
+    """Signed distance (approximate: the normalised radial distance times the smallest semi-axis,
+    exact on spheres) to the union of the ellipsoids of ellipsoids(..., return_params=True)."""
+    axes, rots, cents = params
+    d = np.full(p.shape[0], np.inf)
+    for a, R, c in zip(axes, rots, cents):
+        local = (np.asarray(p, np.float64) - c) @ R  # rotated into the ellipsoid's frame
+        d = np.minimum(d, (np.linalg.norm(local / a, axis=1) - 1.0) * a.min())
+    return d
+
+
+def torus_sdf(p, R=0.35, r=0.15, center=(0.5, 0.5, 0.5)):
+    q = np.asarray(p, np.float64) - np.asarray(center)
+    return np.sqrt((np.sqrt(q[:, 0] ** 2 + q[:, 1] ** 2) - R) ** 2 + q[:, 2] ** 2) - r
 
 
 def room(n, size=(8.0, 6.0, 3.0), n_boxes=40, box_range=(0.2, 1.5), sigma=0.0, seed=0, scale=1 / 8.0):

@@ -29,27 +29,45 @@ def _check_tree(tree, xyz_t):
         assert torch.equal(tree.hash_table(d).query(c[idx]), idx)
 
 
-@pytest.mark.parametrize("cfg", ["c1", "c2"])
+@pytest.mark.parametrize("cfg", ["c1", "c2", "c5"])
 def test_full_size_properties(cuda, cfg):
+    """Config 1 (100k-point torus, 128^3, FP32 PCG to 1e-6), config 2 (2M points on 16 noisy
+    ellipsoids, 512^3, FP64 refinement to 1e-8) and config 5 (10M points on a room with 40 boxes,
+    1024^3, FP64 refinement to 1e-7): tree invariants, the FP64 residual of every level's returned
+    solution against its tolerance, and a closed mesh."""
     import ngspsr
     from ngspsr import synthetic
     print(f"[case] full size {cfg}", flush=True)
     if cfg == "c1":
         xyz, nrm = synthetic.torus(100000, seed=0)
         vs, mind, solver, tol = 1 / 128, 8.0, "pcg@tol=1e-6@maxiter=10000", 1e-6
-    else:
+        # the FP32 recurrence residual stops the solve; the true residual of the FP32 iterate can
+        # drift slightly above it
+        slack = 1.5
+    elif cfg == "c2":
         xyz, nrm = synthetic.ellipsoids(2000000, k=16, sigma=0.002, kappa=200.0, seed=0)
-        vs, mind, solver, tol = 1 / 512, 1.0, "refine@tol=1e-8@inner_tol=1e-4", 1e-8
+        vs, mind, solver, tol, slack = 1 / 512, 1.0, "refine@tol=1e-8@inner_tol=1e-4", 1e-8, 1.0
+    else:
+        xyz, nrm = synthetic.room(10_000_000, sigma=0.003, seed=0)
+        vs, mind, solver, tol, slack = 1 / 1024, 1.0, "refine@tol=1e-7@inner_tol=1e-4", 1e-7, 1.0
     xyz_t, nrm_t = torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda)
+    del xyz, nrm
+    torch.cuda.reset_peak_memory_stats(cuda)
     res = ngspsr.reconstruct(xyz_t, nrm_t, vs, depth=4, min_density=mind, solver=solver, return_result=True)
+    torch.cuda.synchronize()
+    peak = torch.cuda.max_memory_allocated(cuda)
     tree = res.tree
+    print(f"[result] {cfg}: voxels {[tree.get_coords_size(None, d) for d in range(tree.depth)]}, "
+          f"peak HBM {peak / 2 ** 30:.2f} GiB", flush=True)
     _check_tree(tree, xyz_t)
     for d, (system, rhs) in res.reconstructor.systems.items():
-        x = res.reconstructor.solutions[d].double()
-        _, rr, bb = system.residual_f64(rhs.double(), x)
+        x = res.reconstructor.solutions[d]
+        if solver.startswith("refine"):
+            assert x.dtype == torch.float64  # the returned solution is the FP64 iterate
+        _, rr, bb = system.residual_f64(rhs.double(), x.double())
         rel = (rr / bb) ** 0.5
-        # FP32 storage of x adds ~1e-7 relative on top of the solver's residual
-        assert rel <= max(tol, 3e-7) * 1.01, (d, rel)
+        assert rel <= tol * slack, (d, rel)
+        assert abs(rel - res.reconstructor.solve_info[d].rel_resid) <= 0.5 * tol or not solver.startswith("refine")
     v, f = res.vertices, res.triangles
     assert f.shape[0] > 1000
     assert int(f.max()) < v.shape[0] and int(f.min()) >= 0
