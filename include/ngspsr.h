@@ -433,8 +433,10 @@ int ngs_residual_f64_runs(const int32_t *runs, const float *rvals, const double 
 
 /* Sparse direct solve (replaces _sparse_solve.solve_cusparse, csrc/sparse_solve/solve_cusparse.cu:15-42,
  * the sparse Cholesky behind the "cholmod" solver, jspsr/core/solver.py:147-157): a supernodal
- * multifrontal Cholesky factorization in FP64 of a symmetric positive definite CSR matrix (the
- * lower triangle is read; padded rows repeating their diagonal with value 0 are accepted).
+ * multifrontal Cholesky factorization in FP64 of a symmetric positive definite CSR matrix (its
+ * lower triangle in the caller's numbering is read, as cuSOLVER reads it, so an upper triangle
+ * that differs by rounding is ignored; padded rows repeating their diagonal with value 0 are
+ * accepted).
  * ngs_chol_analyze reads the pattern (device arrays, read on stream s, which it synchronises) and
  * computes on the host a
  * nested-dissection order, the supernodes and the front structures; the opaque handle owns only host

@@ -667,7 +667,8 @@ static int chol_analyze(std::vector<int32_t> rp, std::vector<int32_t> cl, int64_
             h->cmap.push_back(pos);
         }
     }
-    // A's lower entries -> pool positions
+    // A's lower entries in the caller's numbering (the triangle cuSOLVER reads; the other one may
+    // differ from it by rounding) -> pool positions of the permuted lower triangle
     std::vector<int32_t> owner(n);
     for (int32_t id = 0; id < nn; ++id)
         for (int32_t j = h->nodes[id].start; j < h->nodes[id].end; ++j) owner[j] = id;
@@ -675,9 +676,8 @@ static int chol_analyze(std::vector<int32_t> rp, std::vector<int32_t> cl, int64_
     for (int32_t io = 0; io < n; ++io)
         for (int32_t k = rp[io]; k < rp[io + 1]; ++k) {
             const int32_t jo = cl[k];
-            if (jo < 0 || jo >= n) continue;
-            const int32_t i = h->iperm[io], j = h->iperm[jo];
-            if (i < j) continue;
+            if (jo < 0 || jo >= n || jo > io) continue;
+            const int32_t i = std::max(h->iperm[io], h->iperm[jo]), j = std::min(h->iperm[io], h->iperm[jo]);
             const int32_t id = owner[j];
             const HostNode& nd = h->nodes[id];
             const NodeDesc& D = h->desc[id];

@@ -4,11 +4,14 @@ A numpy restatement of the multifrontal factorization driven only by the structu
 exports (the CSR-to-front map, the child-to-parent maps, the front index lists) must reproduce the
 dense FP64 solution of the system: a wrong front row, a missing fill row or a bad map changes it.
 Systems: the 125-point stencil pattern on sparse voxel sets (a sphere shell and two disjoint blobs,
-so the dissection also meets disconnected parts), with padded rows repeating their diagonal."""
+so the dissection also meets disconnected parts), with padded rows repeating their diagonal and an
+upper triangle that differs from the lower one by rounding-sized perturbations."""
 import ctypes
 
 import numpy as np
 import pytest
+
+from oracle import direct
 
 
 def _stencil_csr(coords, seed, pad=True):
@@ -148,13 +151,19 @@ def test_multifrontal_structure_reproduces_dense_solve(kind, pad):
     if kind == "blobs":
         assert int((nodes[:, 3] == -1).sum()) >= 2  # one tree per connected part
     assert st[2] < n  # the dissection split the system
-    # the values as the matrix's entries in the CSR order: A's own values (FP32), padding 0
+    # the values as the matrix's entries in the CSR order: A's own values (FP32), padding 0; the
+    # upper triangle is perturbed (an assembled matrix is symmetric only to rounding): the factor
+    # is of the lower triangle in the caller's numbering, as cuSOLVER reads it
+    vv = vv.copy()
+    upper = cl > np.repeat(np.arange(n), np.diff(rp))
+    vv[upper] *= np.float32(1.0) + np.float32(1e-3) * np.random.default_rng(3).uniform(-1, 1, int(upper.sum())).astype(np.float32)
     A32 = np.zeros_like(A)
     for i in range(n):
         for k in range(rp[i], rp[i + 1]):
             A32[i, cl[k]] += float(vv[k])
     b = np.random.default_rng(2).standard_normal(n)
-    x_ref = np.linalg.solve(A32, b)
+    x_ref = np.linalg.solve(direct.lower_symmetric(A32), b)
+    assert np.linalg.norm(np.linalg.solve(A32, b) - x_ref) > 1e-8 * np.linalg.norm(x_ref)  # the triangles differ
     y = _multifrontal_solve(st, nodes, fidx, cmap, amap, vv, b[perm])
     x = np.empty(n)
     x[perm] = y

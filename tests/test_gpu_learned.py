@@ -57,8 +57,12 @@ def test_bspline_as_additive_equals_bspline_path(cuda):
         # the cross-level products cancel against the normal term: FP32 rounding of entries of
         # size sum|a||x| shows at ~1e-5 of the largest entry
         np.testing.assert_allclose(ba_, bb_, rtol=0, atol=3e-5 * np.abs(bb_).max())
+        # each FP64 solution satisfies the other path's system to the size of the systems'
+        # difference; the solutions themselves differ by the condition number times that
         xa, xb = ra.solutions[d].double(), rb.solutions[d].double()
-        assert float((xa - xb).norm() / xb.norm()) < 1e-5, d
+        _, rr, nb = sb.residual_f64(bb.double(), xa)
+        assert (rr / nb) ** 0.5 < 2e-5, (d, (rr / nb) ** 0.5)
+        assert float((xa - xb).norm() / xb.norm()) < 1e-4, d
     q = xt[::5].contiguous()
     ca, cb = ra.evaluate_chi(q), rb.evaluate_chi(q)
     assert float((ca - cb).abs().max()) <= 1e-5 * float(cb.abs().max())
