@@ -38,10 +38,14 @@ __global__ void lhs_count_kernel(ngs_level lv, int32_t* __restrict__ row_nnz) {
     row_nnz[i] = PAD ? (cnt + 3) & ~3 : cnt;
 }
 
-// One thread per row: the 125 entries accumulate in registers over the points of the 27 cells
-// around the voxel (points in bin order, basis values and weights stored in bin order), then the
-// row is written in increasing column order. Cells and neighbours are found one bitmap column
-// (3 or 5 consecutive lattice points along z) at a time.
+// One thread per row, one x-plane of the row at a time: the 25 entries of plane dx accumulate in
+// registers over the points of the cells that reach it (cell offsets ox with |dx - ox| <= 1; points
+// in bin order, basis values and weights stored in bin order), then the plane is written. Every
+// entry receives its point terms in the same order as a one-pass loop over all 27 cells would give
+// them (cells in (ox, oy, oz) order, points in bin order), so the values do not depend on the plane
+// split; the split keeps 25 accumulators live instead of 125, which fit the register file without
+// spilling. Cells and neighbours are found one bitmap column (3 or 5 consecutive lattice points
+// along z) at a time.
 // RUNS: the row goes to the run form instead of CSR. The present neighbours of voxel i in the
 // bitmap column (dx, dy) (its five z-neighbours) have consecutive ranks start .. start + n - 1;
 // runs[p P + i] = start | n << 28 (n = 0: start = i) for p = 5 (dx + 2) + (dy + 2), and
@@ -73,13 +77,13 @@ __device__ __forceinline__ float group_sum(float v) {
 }
 
 template <bool HAS_ALPHA, bool HAS_COARSE, bool RUNS, int G>
-__global__ void __launch_bounds__(64, 3) lhs_assemble_kernel(ngs_level lv, ngs_point_bins pts,
-                                                          const float* __restrict__ coarse, float sf, float sf_delta,
-                                                          const float* __restrict__ F1, const float* __restrict__ G1,
-                                                          const int32_t* __restrict__ rowptr, int32_t* __restrict__ cols,
-                                                          float* __restrict__ vals, float* __restrict__ inv_diag,
-                                                          double* __restrict__ rhs, int32_t* __restrict__ runs,
-                                                          float* __restrict__ rvals) {
+__global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_point_bins pts,
+                                                       const float* __restrict__ coarse, float sf, float sf_delta,
+                                                       const float* __restrict__ F1, const float* __restrict__ G1,
+                                                       const int32_t* __restrict__ rowptr, int32_t* __restrict__ cols,
+                                                       float* __restrict__ vals, float* __restrict__ inv_diag,
+                                                       double* __restrict__ rhs, int32_t* __restrict__ runs,
+                                                       float* __restrict__ rvals) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t i = t / G;  // the G lanes of a row are consecutive lanes of one wave
     const int gl = (int)(t % G);
@@ -89,69 +93,10 @@ __global__ void __launch_bounds__(64, 3) lhs_assemble_kernel(ngs_level lv, ngs_p
     const float* __restrict__ pb_all = pts.b;
     const float* __restrict__ pa_all = pts.alpha;
     const int32_t* __restrict__ cstart = pts.cell_start;
-    float acc[125];
-#pragma unroll
-    for (int k = 0; k < 125; ++k) acc[k] = 0.f;
-    float rs = 0.f, rc = 0.f;
     const ngs_grid& cg = pts.cells;
     const int c2 = cg.log2_stride;
     const int mx = (cx >> c2) - cg.origin[0], my = (cy >> c2) - cg.origin[1], mz = (cz >> c2) - cg.origin[2];
-#pragma unroll
-    for (int ox = -1; ox <= 1; ++ox) {
-#pragma unroll
-        for (int oy = -1; oy <= 1; ++oy) {
-            int32_t cid;
-            const uint32_t cm = grid_column(cg, mx + ox, my + oy, mz - 1, 3, &cid);
-#pragma unroll
-            for (int oz = -1; oz <= 1; ++oz) {
-                if (!((cm >> (oz + 1)) & 1u)) continue;
-                const int k0 = cstart[cid], k1 = cstart[cid + 1];
-                ++cid;
-                for (int k = k0 + gl; k < k1; k += G) {
-                    const float* bp = pb_all + 9 * (int64_t)k;
-                    float bx[3] = {bp[0], bp[1], bp[2]};
-                    float by[3] = {bp[3], bp[4], bp[5]};
-                    float bz[3] = {bp[6], bp[7], bp[8]};
-                    const float a = HAS_ALPHA ? pa_all[k] : pts.alpha_scalar;
-                    // products rounded on their own, sums by explicit FMA: the same arithmetic as
-                    // stencil_diag_rhs_kernel, whatever the contraction choices around it
-                    float w;
-                    {
-#pragma clang fp contract(off)
-                        w = a * (bx[1 - ox] * by[1 - oy] * bz[1 - oz]);
-                    }
-                    rs += w;
-                    // coarse is in bin order when the bins carry no permutation (a contiguous stream
-                    // instead of one random gather per visit; 27 rows visit every point)
-                    if (HAS_COARSE) rc = fmaf(w, coarse[pts.perm ? pts.perm[k] : k], rc);
-#pragma unroll
-                    for (int vx = -1; vx <= 1; ++vx) {
-#pragma unroll
-                        for (int vy = -1; vy <= 1; ++vy) {
-                            float wxy;
-                            {
-#pragma clang fp contract(off)
-                                wxy = w * (bx[vx + 1] * by[vy + 1]);
-                            }
-#pragma unroll
-                            for (int vz = -1; vz <= 1; ++vz) {
-                                float& e = acc[(ox + vx + 2) * 25 + (oy + vy + 2) * 5 + (oz + vz + 2)];
-                                e = fmaf(wxy, bz[vz + 1], e);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-    }
-    if (G > 1) {
-#pragma unroll
-        for (int q = 0; q < 125; ++q) acc[q] = group_sum<G>(acc[q]);
-        rs = group_sum<G>(rs);
-        if (HAS_COARSE) rc = group_sum<G>(rc);
-        if (gl != 0) return;
-    }
-    // Emit the row in increasing column order (lexicographic offsets over sorted voxels).
+    float rs = 0.f, rc = 0.f;
     int cnt = 0;
     const float fs = (float)s, inv_s = 1.0f / (float)s;
     const ngs_grid& g = lv.grid;
@@ -175,33 +120,100 @@ __global__ void __launch_bounds__(64, 3) lhs_assemble_kernel(ngs_level lv, ngs_p
         ++cnt;
     };
     const int64_t P = run_plane_stride(lv.M);
-#pragma unroll
+#pragma nounroll
     for (int dx = -2; dx <= 2; ++dx) {
+        float acc[25];
 #pragma unroll
-        for (int dy = -2; dy <= 2; ++dy) {
-            int32_t j;
-            const uint32_t m = grid_column(g, lx + dx, ly + dy, lz - 2, 5, &j);
-            const int pl = (dx + 2) * 5 + (dy + 2);
-            if (RUNS) runs[pl * P + i] = (m ? j : (int32_t)i) | (__popc(m) << RUN_COUNT_SHIFT);
-            int q = 0;
+        for (int k = 0; k < 25; ++k) acc[k] = 0.f;
+        for (int ox = max(-1, dx - 1); ox <= min(1, dx + 1); ++ox) {
+            const int vx = dx - ox;
 #pragma unroll
-            for (int dz = -2; dz <= 2; ++dz) {
-                if (!((m >> (dz + 2)) & 1u)) continue;
-                float v;
-                {
+            for (int oy = -1; oy <= 1; ++oy) {
+                int32_t cid;
+                const uint32_t cm = grid_column(cg, mx + ox, my + oy, mz - 1, 3, &cid);
+#pragma unroll
+                for (int oz = -1; oz <= 1; ++oz) {
+                    if (!((cm >> (oz + 1)) & 1u)) continue;
+                    const int k0 = cstart[cid], k1 = cstart[cid + 1];
+                    ++cid;
+                    for (int k = k0 + gl; k < k1; k += G) {
+                        const float* bp = pb_all + 9 * (int64_t)k;
+                        const float bx0 = bp[0], bx1 = bp[1], bx2 = bp[2];
+                        const float by[3] = {bp[3], bp[4], bp[5]};
+                        const float bz[3] = {bp[6], bp[7], bp[8]};
+                        const float a = HAS_ALPHA ? pa_all[k] : pts.alpha_scalar;
+                        // the x factors of the row's basis (cell offset ox) and of the column's
+                        // (plane offset vx): selects, not indexing, keep them in registers
+                        const float bxo = ox < 0 ? bx2 : (ox == 0 ? bx1 : bx0);
+                        const float bxv = vx < 0 ? bx0 : (vx == 0 ? bx1 : bx2);
+                        // products rounded on their own, sums by explicit FMA: the same arithmetic
+                        // as stencil_diag_rhs_kernel, whatever the contraction choices around it
+                        float w;
+                        {
 #pragma clang fp contract(off)
-                    const float c0 = F1[dx + 2] * fs, c1 = F1[dy + 2] * fs, c2 = F1[dz + 2] * fs;
-                    const float g0 = G1[dx + 2] * inv_s, g1 = G1[dy + 2] * inv_s, g2 = G1[dz + 2] * inv_s;
-                    const float K = g0 * c1 * c2 + c0 * g1 * c2 + c0 * c1 * g2;
-                    v = K + sf * acc[(dx + 2) * 25 + (dy + 2) * 5 + (dz + 2)];
+                            w = a * (bxo * by[1 - oy] * bz[1 - oz]);
+                        }
+                        if (dx == 0) {  // the plane whose pass visits all 27 cells
+                            rs += w;
+                            // coarse is in bin order when the bins carry no permutation (a
+                            // contiguous stream instead of one random gather per visit)
+                            if (HAS_COARSE) rc = fmaf(w, coarse[pts.perm ? pts.perm[k] : k], rc);
+                        }
+#pragma unroll
+                        for (int vy = -1; vy <= 1; ++vy) {
+                            float wxy;
+                            {
+#pragma clang fp contract(off)
+                                wxy = w * (bxv * by[vy + 1]);
+                            }
+#pragma unroll
+                            for (int vz = -1; vz <= 1; ++vz) {
+                                float& e = acc[(oy + vy + 2) * 5 + (oz + vz + 2)];
+                                e = fmaf(wxy, bz[vz + 1], e);
+                            }
+                        }
+                    }
                 }
-                if (dx == 0 && dy == 0 && dz == 0) inv_diag[i] = 1.0f / v;
-                if (RUNS) rvals[(5 * pl + q++) * P + i] = v;
-                else emit(v, j++);
             }
-            if (RUNS)
-                for (; q < 5; ++q) rvals[(5 * pl + q) * P + i] = 0.f;
         }
+        if (G > 1) {
+#pragma unroll
+            for (int q = 0; q < 25; ++q) acc[q] = group_sum<G>(acc[q]);
+        }
+        if (G == 1 || gl == 0) {
+            // the plane's entries in increasing column order (lexicographic offsets over sorted voxels)
+            const float c0 = F1[dx + 2] * fs, g0 = G1[dx + 2] * inv_s;
+#pragma unroll
+            for (int dy = -2; dy <= 2; ++dy) {
+                int32_t j;
+                const uint32_t m = grid_column(g, lx + dx, ly + dy, lz - 2, 5, &j);
+                const int pl = (dx + 2) * 5 + (dy + 2);
+                if (RUNS) runs[pl * P + i] = (m ? j : (int32_t)i) | (__popc(m) << RUN_COUNT_SHIFT);
+                int q = 0;
+#pragma unroll
+                for (int dz = -2; dz <= 2; ++dz) {
+                    if (!((m >> (dz + 2)) & 1u)) continue;
+                    float v;
+                    {
+#pragma clang fp contract(off)
+                        const float c1 = F1[dy + 2] * fs, c2f = F1[dz + 2] * fs;
+                        const float g1 = G1[dy + 2] * inv_s, g2 = G1[dz + 2] * inv_s;
+                        const float K = g0 * c1 * c2f + c0 * g1 * c2f + c0 * c1 * g2;
+                        v = K + sf * acc[(dy + 2) * 5 + (dz + 2)];
+                    }
+                    if (dx == 0 && dy == 0 && dz == 0) inv_diag[i] = 1.0f / v;
+                    if (RUNS) rvals[(5 * pl + q++) * P + i] = v;
+                    else emit(v, j++);
+                }
+                if (RUNS)
+                    for (; q < 5; ++q) rvals[(5 * pl + q) * P + i] = 0.f;
+            }
+        }
+    }
+    if (G > 1) {
+        rs = group_sum<G>(rs);
+        if (HAS_COARSE) rc = group_sum<G>(rc);
+        if (gl != 0) return;
     }
     if (!RUNS)
         while (cnt & 3) emit(0.f, (int32_t)i);  // padding: the row itself with a zero value
