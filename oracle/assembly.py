@@ -14,36 +14,36 @@ from .hierarchy import Hierarchy, compute_density, range_kernel  # noqa: F401
 
 def screening_term(sd_a, sd_b, src, tgt, m_b, pts_weight):
     """reconstructor.py:59-77 + sparse_product.cu:100-137: for each COO pair (src, tgt) the sum over
-    points p of w_p * a(p, src) * b(p, tgt); pairs absent from the COO are dropped (silent miss)."""
+    points p of w_p * a(p, src) * b(p, tgt); pairs absent from the COO are dropped (silent miss).
+    The point sums are the sparse product A^T W B of the (point x voxel) evaluation matrices; a
+    pair listed twice in the COO receives the sum once, at its first occurrence (the reference's
+    map from pair to COO index keeps one index per pair)."""
     pa, va, xa = sd_a
     pb, vb, xb = sd_b
     E = src.shape[0]
     if E == 0 or pa.shape[0] == 0 or pb.shape[0] == 0:
         return np.zeros(E)
-    coo_key = src.astype(np.int64) * m_b + tgt
-    order = np.argsort(coo_key, kind="stable")
-    skey = coo_key[order]
     n = int(max(pa.max(), pb.max())) + 1
     w = np.broadcast_to(np.asarray(pts_weight, np.float64), (n,)) if np.ndim(pts_weight) == 0 else np.asarray(pts_weight, np.float64)
-    # group b entries by point
-    bstart = np.searchsorted(pb, np.arange(n + 1))
+    m_a = int(max(int(np.max(src)), int(va.max()))) + 1
+    at = sp.csr_matrix((w[pa] * np.asarray(xa, np.float64), (va, pa)), shape=(m_a, n))
+    b = sp.csr_matrix((np.asarray(xb, np.float64), (pb, vb)), shape=(n, m_b))
+    s_ = (at @ b).tocsr()
+    s_.sum_duplicates()
+    s_.sort_indices()
+    rows = np.repeat(np.arange(m_a, dtype=np.int64), np.diff(s_.indptr))
+    skey = rows * m_b + s_.indices.astype(np.int64)
+    key = src.astype(np.int64) * m_b + tgt
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    first = np.ones(E, bool)
+    first[1:] = ks[1:] != ks[:-1]
     out = np.zeros(E)
-    chunk = 20000
-    for lo in range(0, n, chunk):
-        hi = min(n, lo + chunk)
-        sel = (pa >= lo) & (pa < hi)
-        ia, ii, iv = pa[sel], va[sel], xa[sel]
-        # expand each a entry against all b entries of the same point
-        cnt = bstart[ia + 1] - bstart[ia]
-        rep = np.repeat(np.arange(ia.shape[0]), cnt)
-        boff = np.arange(rep.shape[0]) - np.repeat(np.cumsum(cnt) - cnt, cnt)
-        bidx = bstart[ia[rep]] + boff
-        key = ii[rep].astype(np.int64) * m_b + vb[bidx]
-        val = w[ia[rep]] * iv[rep] * xb[bidx]
-        pos = np.searchsorted(skey, key)
-        pos_c = np.minimum(pos, E - 1)
-        hit = skey[pos_c] == key
-        out += np.bincount(order[pos_c[hit]], weights=val[hit], minlength=E)
+    if skey.shape[0] == 0:
+        return out
+    pos = np.minimum(np.searchsorted(skey, ks), skey.shape[0] - 1)
+    hit = (skey[pos] == ks) & first
+    out[order[hit]] = s_.data[pos[hit]]
     return out
 
 

@@ -1,4 +1,5 @@
-"""GPU parity of every stage of the reconstruct path against the CPU oracle (small scenes)."""
+"""GPU parity of every stage of the reconstruct path against the CPU oracle: small scenes in every
+level operator form, and the 100k-point torus at 128^3 in full."""
 import numpy as np
 import pytest
 import torch
@@ -15,9 +16,12 @@ def _scene(kind, n, seed):
     return synthetic.sphere(n, seed=seed)
 
 
-# (scene, points, voxel size, depth, min_density, level operator form)
+# (scene, points, voxel size, depth, min_density, level operator form); "auto" resolves to the run
+# form. The last case is the 100k-point torus at 128^3 with min_density 8 in full.
 CASES = [("torus", 4000, 1 / 32, 3, 8.0, "csr"), ("torus", 4000, 1 / 32, 3, 8.0, "stencil"),
-         ("ellipsoids", 6000, 1 / 64, 4, 8.0, "stencil"), ("sphere", 3000, 1 / 32, 3, 32.0, "auto")]
+         ("torus", 4000, 1 / 32, 3, 8.0, "runs"), ("ellipsoids", 6000, 1 / 64, 4, 8.0, "stencil"),
+         ("ellipsoids", 6000, 1 / 64, 4, 8.0, "auto"), ("sphere", 3000, 1 / 32, 3, 32.0, "auto"),
+         ("torus", 100000, 1 / 128, 4, 8.0, "auto")]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=lambda c: f"{c[0]}-{c[1]}-d{c[3]}-{c[5]}")

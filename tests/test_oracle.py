@@ -277,6 +277,21 @@ def test_evaluate_interpolated_gradient_and_mask_brute_force():
 
 
 # ---------------------------------------------------------------------------------- assembly
+def test_screening_term_duplicate_and_absent_pairs():
+    """Hand-computed: point 0 has a(0) = 1, a(1) = 2, b(0) = 5; point 1 has a(1) = 3, b(1) = 7;
+    weight 2. Sums: S[0,0] = 10, S[1,0] = 20, S[1,1] = 42, S[0,1] = 0. A pair listed twice gets
+    its sum at the first occurrence only; a pair with no points gets 0; a nonzero pair missing
+    from the COO is dropped."""
+    sda = (np.array([0, 0, 1]), np.array([0, 1, 1]), np.array([1.0, 2.0, 3.0]))
+    sdb = (np.array([0, 1]), np.array([0, 1]), np.array([5.0, 7.0]))
+    src = np.array([1, 0, 1, 0])
+    tgt = np.array([0, 1, 0, 0])
+    got = assembly.screening_term(sda, sdb, src, tgt, 2, 2.0)
+    np.testing.assert_array_equal(got, [20.0, 0.0, 0.0, 10.0])
+    got = assembly.screening_term(sda, sdb, np.array([1, 1]), np.array([1, 0]), 2, np.array([2.0, 0.5]))
+    np.testing.assert_array_equal(got, [10.5, 20.0])
+
+
 def test_screening_term_against_dense_brute_force():
     rng = np.random.default_rng(9)
     xyz = rng.uniform(0.4, 0.6, (30, 3)).astype(np.float32)
