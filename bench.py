@@ -367,7 +367,7 @@ def serving_scenes(n_req, seed0=1000):
         return list(ex.map(_serving_scene, range(n_req), [seed0] * n_req, chunksize=4))
 
 
-def run_serving(n_req=256, steps=20, warmup=5, workers=16, chunk=32, log=None):
+def run_serving(n_req=256, steps=20, warmup=5, workers=8, chunk=8, log=None):
     """Config 3: the same stream of n_req requests served plain and batched, at relative residual
     1e-6.
     - plain: one request at a time, the reference's path (`reconstruct`, FP32 Jacobi-PCG to 1e-6);
@@ -393,20 +393,33 @@ def run_serving(n_req=256, steps=20, warmup=5, workers=16, chunk=32, log=None):
     say(f"serving: {n_req} requests, {pts} points, generated in {gen_s:.1f} s")
 
     def plain_one(x, n, vs):
-        ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
+        res = ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000",
+                                 return_result=True)
         torch.cuda.synchronize()
+        return [res.tree.get_coords_size(None, d) for d in range(4)]
 
     for x, n, vs in data[:4]:
         plain_one(x, n, vs)
     lat = []
+    peaks = []  # (peak bytes allocated by the request, its footprint estimate)
     tp = time.perf_counter()
     for x, n, vs in data:
         t1 = time.perf_counter()
-        plain_one(x, n, vs)
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        sizes = plain_one(x, n, vs)
         lat.append(time.perf_counter() - t1)
+        peaks.append((torch.cuda.max_memory_allocated(dev) - base, serving.level_footprint(sizes, x.shape[0])))
     plain_s = time.perf_counter() - tp
+    ratio = max(p / e for p, e in peaks)
+    say(f"serving: largest request peak {max(p for p, _ in peaks) / 2 ** 30:.2f} GiB allocated; "
+        f"peak / level footprint at most {ratio:.2f}")
     plain_rate = n_req / plain_s
-    say(f"serving: plain {plain_rate:.2f} solves/s")
+    plain_peak = torch.cuda.max_memory_reserved(dev)
+    say(f"serving: plain {plain_rate:.2f} solves/s, peak HBM reserved {plain_peak / 2 ** 30:.1f} GiB")
+    # the plain path's cached blocks live on the default stream, which the server does not use
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats(dev)
 
     srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, refine=True, chunk=chunk)
 
@@ -424,7 +437,8 @@ def run_serving(n_req=256, steps=20, warmup=5, workers=16, chunk=32, log=None):
 
     for w in range(warmup):
         one_pass()
-        say(f"serving: batched warm-up pass {w + 1}/{warmup}")
+        say(f"serving: batched warm-up pass {w + 1}/{warmup}, peak HBM reserved "
+            f"{torch.cuda.max_memory_reserved(dev) / 2 ** 30:.1f} GiB")
     torch.cuda.synchronize()
     tb = time.perf_counter()
     res_max = 0.0
@@ -442,6 +456,8 @@ def run_serving(n_req=256, steps=20, warmup=5, workers=16, chunk=32, log=None):
     lat_b = serving.latency_stats(reqs)
     pool = srv.pool
     srv.close()
+    batched_peak = torch.cuda.max_memory_reserved(dev)
+    torch.cuda.empty_cache()
     # the plain path under the same arrivals: first come, first served
     tq = time.perf_counter()
     done = []
@@ -470,6 +486,9 @@ def run_serving(n_req=256, steps=20, warmup=5, workers=16, chunk=32, log=None):
         "poisson_latency_ms_plain": {"p50": round(1e3 * float(np.percentile(lat_p, 50)), 2),
                                      "p99": round(1e3 * float(np.percentile(lat_p, 99)), 2)},
         "pool_launches": pool.launches,
+        "hbm_peak_reserved_gib": {"plain": round(plain_peak / 2 ** 30, 2), "batched": round(batched_peak / 2 ** 30, 2)},
+        "request_peak_allocated_gib_max": round(max(p for p, _ in peaks) / 2 ** 30, 3),
+        "request_peak_over_level_footprint_max": round(ratio, 3),
     }
     return out, timed_s
 
@@ -692,6 +711,8 @@ def plain_pass(n_req=256):
     data = [(torch.from_numpy(x).to(dev), torch.from_numpy(n).to(dev), vs) for x, n, vs in serving_scenes(n_req)]
     for x, n, vs in data[:4]:
         ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
+    # a short spin kernel on each side of the pass marks its window in a kernel trace
+    torch.cuda._sleep(1000)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     t0_ns = time.monotonic_ns()
@@ -699,22 +720,30 @@ def plain_pass(n_req=256):
         ngspsr.reconstruct(x, n, vs, depth=4, min_density=1.0, solver="pcg@tol=1e-6@maxiter=20000")
         torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    torch.cuda._sleep(1000)
+    torch.cuda.synchronize()
     print(json.dumps({"plain_pass": True, "requests": n_req, "wall_s": round(wall, 4),
                       "solves_per_s": round(n_req / wall, 3), "start_monotonic_ns": t0_ns,
                       "end_monotonic_ns": time.monotonic_ns()}), flush=True)
 
 
-def trace_busy(path, skip_first=0):
+def trace_busy(path, skip_first=0, marker="spin_kernel"):
     """GPU-busy summary of a rocprofv3 kernel trace (CSV): the union of kernel execution intervals
-    over the span from the first to the last kernel (after skipping the first `skip_first` kernels,
-    the warm-up), the sum of kernel durations, and the largest kernels by total time."""
+    over the span from the first to the last kernel, the sum of kernel durations, and the largest
+    kernels by total time. The window is the kernels between the first and the last `marker` kernel
+    when the trace holds two (the plain pass brackets itself with them), else all kernels after the
+    first `skip_first`."""
     import csv
     rows = []
     with open(path) as f:
         for r in csv.DictReader(f):
             rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
     rows.sort()
-    rows = rows[skip_first:]
+    marks = [k for k, r in enumerate(rows) if marker and marker in r[2]]
+    if len(marks) >= 2:
+        rows = rows[marks[0] + 1:marks[-1]]
+    else:
+        rows = rows[skip_first:]
     busy, cur_s, cur_e, tot = 0, None, None, 0
     per = {}
     for s_, e_, name in rows:
@@ -731,7 +760,8 @@ def trace_busy(path, skip_first=0):
         busy += cur_e - cur_s
     span = rows[-1][1] - rows[0][0] if rows else 0
     top = sorted(per.items(), key=lambda kv: -kv[1])[:15]
-    print(json.dumps({"trace": os.path.basename(path), "kernels": len(rows), "span_s": span / 1e9,
+    print(json.dumps({"trace": os.path.basename(path), "window": "between markers" if len(marks) >= 2 else
+                      f"after {skip_first} kernels", "kernels": len(rows), "span_s": span / 1e9,
                       "busy_union_s": busy / 1e9, "busy_fraction": busy / span if span else 0.0,
                       "kernel_time_sum_s": tot / 1e9,
                       "top_kernels_s": {k: round(v / 1e9, 4) for k, v in top}}), flush=True)

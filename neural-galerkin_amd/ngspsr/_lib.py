@@ -224,40 +224,60 @@ def stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+class _ThreadBuffers:
+    """Per-thread, per-device device buffers reused from call to call in stream order. A buffer
+    that this thread last used on another stream is handed over on first use from the new stream:
+    the new stream waits for an event recorded on the old one (everything this thread enqueued
+    there, which includes every earlier use of the buffer), and the tensor is recorded on the new
+    stream, so the caching allocator does not hand its memory out before the new stream's work is
+    done when it is replaced."""
+
+    def __init__(self, zero):
+        self._tls = threading.local()
+        self._zero = zero
+
+    def get(self, key, nbytes, device):
+        ents = getattr(self._tls, "ents", None)
+        if ents is None:
+            ents = self._tls.ents = {}
+        cur_ptr = _raw_stream(_cur_device()) if _raw_stream is not None else None
+        ent = ents.get(key)
+        if ent is not None and (cur_ptr is None or ent[1] != cur_ptr):
+            cur = torch.cuda.current_stream()
+            if ent[2] != cur:
+                ev = torch.cuda.Event()
+                ev.record(ent[2])
+                cur.wait_event(ev)
+                ent[0].record_stream(cur)
+            ent[1], ent[2] = cur.cuda_stream, cur
+        if ent is None or ent[0].numel() < nbytes:
+            # grow geometrically: fewer replacements, so fewer odd-sized blocks left in the cache
+            n = max(int(nbytes), 256, 0 if ent is None else int(1.5 * ent[0].numel()))
+            buf = torch.zeros(n, dtype=torch.uint8, device=device) if self._zero else \
+                torch.empty(n + 256, dtype=torch.uint8, device=device)
+            cur = torch.cuda.current_stream()
+            ent = ents[key] = [buf, cur.cuda_stream, cur]
+        return ent[0]
+
+
 class Marks:
     """Per-thread, per-device all-zero byte maps for the byte-staged grid marking calls, which leave
     the map all zero again (stream-ordered use, as Scratch)."""
-    _tls = threading.local()
+    _bufs = _ThreadBuffers(zero=True)
     CAP = 1 << 31  # bytes; larger boxes mark with atomics
 
     @classmethod
     def get(cls, nbytes, device):
-        maps = getattr(cls._tls, "maps", None)
-        if maps is None:
-            maps = cls._tls.maps = {}
-        key = str(device)
-        buf = maps.get(key)
-        if buf is None or buf.numel() < nbytes:
-            buf = torch.zeros(max(int(nbytes), 256), dtype=torch.uint8, device=device)
-            maps[key] = buf
-        return buf
+        return cls._bufs.get(str(device), nbytes, device)
 
 
 class Scratch:
     """Per-thread, per-device growable scratch buffers for stream-ordered temporary workspaces.
 
     Buffers are private to the calling thread, so concurrent requests on different streams never
-    share one."""
-    _tls = threading.local()
+    share one; a thread that moves to another stream hands its buffers over in stream order."""
+    _bufs = _ThreadBuffers(zero=False)
 
     @classmethod
     def get(cls, nbytes, device, slot=0):
-        bufs = getattr(cls._tls, "bufs", None)
-        if bufs is None:
-            bufs = cls._tls.bufs = {}
-        key = (str(device), slot)
-        buf = bufs.get(key)
-        if buf is None or buf.numel() < nbytes:
-            buf = torch.empty(max(int(nbytes), 256) + 256, dtype=torch.uint8, device=device)
-            bufs[key] = buf
-        return buf
+        return cls._bufs.get((str(device), slot), nbytes, device)

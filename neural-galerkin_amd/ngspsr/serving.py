@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from . import solver as _solver
-from ._lib import NgsError, call, lib, ptr, stream
+from ._lib import Marks, NgsError, call, lib, ptr, stream
 from .bases import ensure_device_tables
 from .hashtree import HashTree
 from .reconstructor import Reconstructor
@@ -202,11 +202,20 @@ class BatchedPCG:
     _dirty = False
 
 
-def estimate_footprint(n_points: int, voxel_size: float, area: float = 1.5) -> int:
-    """HBM bytes of one request (finest CSR dominates: ~1 KB per finest voxel, voxels bounded by the
-    8 trilinear corners of every point and by a 3-voxel band over the surface area)."""
-    m0 = min(8.0 * n_points, 3.5 * area / (voxel_size * voxel_size))
-    return int(1.3 * (1100.0 * m0 + 400.0 * n_points) + 64e6)
+def estimate_footprint(n_points: int, voxel_size: float = None) -> int:
+    """HBM bytes a request reserves at admission, before its tree exists: the tree and splat stages
+    (measured under 250 bytes per point at 1e4-1e6 points) plus a margin. The request reserves its
+    full footprint (level_footprint) once its tree gives the voxel counts."""
+    return int(250.0 * n_points + 64 * 2 ** 20)
+
+
+def level_footprint(level_sizes, n_points: int) -> int:
+    """Peak HBM bytes of a request from its tree's voxel counts: the dual-mesh stage dominates (its
+    dual cubes, corner positions and values, vertex keys and their sort), measured at 2.7-5.7 KB
+    per finest voxel and below 3.5 KB per voxel of all levels plus 200 bytes per point on the
+    largest requests of the serving stream (1e4-1e6 points, depth 7-9); solve state and the
+    returned mesh are included."""
+    return int(3500.0 * sum(int(m) for m in level_sizes) + 200.0 * n_points + 256 * 2 ** 20)
 
 
 class Request:
@@ -226,9 +235,16 @@ class ReconstructionServer:
     one shared batched PCG pool."""
 
     def __init__(self, device="cuda:0", workers=8, hbm_budget=None, tol=1e-6, refine=True, inner_tol=1e-4,
-                 chunk=8, capacity_blocks=4096, use_graph=True):
+                 chunk=8, capacity_blocks=4096, use_graph=True, hbm_fraction=0.85):
+        """hbm_fraction (None: leave as is) caps this process's caching allocator on the device at
+        that fraction of its memory (torch.cuda.set_per_process_memory_fraction, process-wide):
+        every worker stream keeps its own cached blocks, and without a cap the cache can fill the
+        device and leave the HIP runtime no room for its own allocations (queue scratch); at the
+        cap the allocator releases cached blocks and retries instead."""
         self.device = torch.device(device)
         torch.cuda.set_device(self.device)
+        if hbm_fraction is not None:
+            torch.cuda.set_per_process_memory_fraction(float(hbm_fraction), self.device)
         ensure_device_tables(self.device)
         free, total = torch.cuda.mem_get_info(self.device)
         self.budget = int(hbm_budget if hbm_budget is not None else 0.6 * free)
@@ -239,6 +255,10 @@ class ReconstructionServer:
         # Scratch, Marks) from one request to the next, so steady-state requests allocate only their
         # own tree, matrices and Krylov state
         self.executor = cf.ThreadPoolExecutor(max_workers=workers, thread_name_prefix="ngspsr-request")
+        # one stream per worker for its lifetime: the caching allocator reuses freed blocks only on
+        # the stream that allocated them, so a fresh stream per request would keep every request's
+        # peak cached on its own stream
+        self._tls = threading.local()
         self.in_use = 0
         self.lock = threading.Condition()
         self.completed = []
@@ -265,8 +285,14 @@ class ReconstructionServer:
         info.rel_resid = info.true_rel_resid = rel
         return x64, info
 
+    def _worker_stream(self):
+        s = getattr(self._tls, "stream", None)
+        if s is None:
+            s = self._tls.stream = torch.cuda.Stream(self.device)
+        return s
+
     def run_request(self, req: Request):
-        s = torch.cuda.Stream(self.device)
+        s = self._worker_stream()
         try:
             with torch.cuda.stream(s):
                 req.start = time.perf_counter()
@@ -278,6 +304,9 @@ class ReconstructionServer:
                     return
                 tree = HashTree(req.xyz, req.voxel_size, req.depth)
                 tree.build_encoder_hierarchy_adaptive(min_density=req.min_density)
+                # the voxel counts are known now: reserve the request's full footprint
+                self._reserve(req, level_footprint([tree.get_coords_size(None, d) for d in range(req.depth)],
+                                                   req.xyz.shape[0]))
                 tree.reflect_decoder_coords()
                 w = 1.0 / tree.xyz_density
                 counts = tree.level_point_counts()
@@ -299,7 +328,7 @@ class ReconstructionServer:
             req.error = e
         finally:
             if req.error is not None:
-                # the worker's scratch and mark buffers serve its next request on another stream
+                # let the failed request's launches drain before the worker takes the next one
                 try:
                     s.synchronize()
                 except Exception:  # the request's error is already recorded
@@ -312,6 +341,17 @@ class ReconstructionServer:
                 self.lock.notify_all()
 
     workers_busy = 0
+
+    def _reserve(self, req, need):
+        """Raises the request's reservation to `need` bytes, waiting while it would exceed the
+        budget and other requests hold memory (a request alone is always let through)."""
+        with self.lock:
+            while need > req.footprint and self.in_use - req.footprint + need > self.budget \
+                    and self.in_use > req.footprint:
+                self.lock.wait()
+            if need > req.footprint:
+                self.in_use += need - req.footprint
+                req.footprint = need
 
     def serve(self, requests, t0=None):
         """Runs the requests honouring their arrival offsets (seconds after t0); returns them."""
