@@ -61,9 +61,10 @@ def _event_time(fn, reps):
 def spmv_roofline(system, reps=50):
     """Average duration of the SpMV kernel of an assembled level system and its bytes per launch.
     CSR (spmv_kernel): algorithmic 8 nnz + 4 (M+1) + 4 M (x) + 4 M (y).
-    Run form (spmv_runs_kernel): algorithmic 4 nnz (values) + 4 R (run words of the R non-empty
-    runs) + 4 M (x) + 4 M (y), zero slots and empty runs excluded; the layout itself streams all
-    125 value slots and 25 run words per row, 600 M + 8 M bytes ("layout_bytes").
+    Run form: algorithmic 4 nnz (values) + 4 R (run words of the R non-empty runs) + 4 M (x) +
+    4 M (y), zero slots and empty runs excluded. Packed (spmv_packed_kernel, the solver's form):
+    the layout streams 100 M (all run words) + 4 nnz + 8 M + 8 per 64-row slice; the slot form
+    (spmv_runs_kernel) streams 600 M + 8 M (all 125 value slots), timed beside it.
     Returns (seconds, algorithmic bytes, nnz, extra dict)."""
     import torch
     from ngspsr._lib import call, ptr, stream
@@ -81,7 +82,20 @@ def spmv_roofline(system, reps=50):
                  "layout_GB/s": round((600 * M + 8 * M) / t / 1e9, 1),
                  "csr_equivalent_bytes": 8 * nnz + 4 * (M + 1) + 8 * M,
                  "csr_equivalent_GB/s": round((8 * nnz + 4 * (M + 1) + 8 * M) / t / 1e9, 1)}
-        return t, algo, nnz, extra
+        if not system.is_packed:
+            return t, algo, nnz, extra
+        # the packed form the solver runs: every run word, the values without the zero slots
+        pvals, soff = system._packed
+        tp = _event_time(lambda: call("ngs_spmv_packed", ptr(system.runs), ptr(pvals), ptr(soff), ptr(x), ptr(y),
+                                      M, stream()), reps)
+        equal = bool(torch.equal(system.matvec(x), system.matvec_slots(x)))
+        packed_layout = 100 * M + 4 * nnz + 8 * M + 8 * (M // 64 + 1)
+        slot = {"kernel": "spmv_runs_kernel (slot form: 5 value slots per run word, zeros included)",
+                "launch_us": round(1e6 * t, 2), "achieved": round(algo / t / 1e9, 1),
+                "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4), **extra}
+        extra = {"layout_bytes": int(packed_layout), "layout_GB/s": round(packed_layout / tp / 1e9, 1),
+                 "nonempty_runs": runs, "equal_to_slot_form": equal, "slot_form_same_matrix": slot}
+        return tp, algo, nnz, extra
     t = _event_time(lambda: call("ngs_spmv", ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(x),
                                  ptr(y), M, stream()), reps)
     return t, 8 * nnz + 4 * (M + 1) + 8 * M, nnz, {}
@@ -89,7 +103,8 @@ def spmv_roofline(system, reps=50):
 
 def stencil_roofline(system, reps=50):
     """Average durations of the two passes of the stencil operator (stencil_points_kernel,
-    stencil_rows_kernel) and their algorithmic bytes, each array counted once:
+    stencil_rows_kernel) and their algorithmic bytes, each array counted once. With the bitmap
+    lookups:
       points: cell coords 12 n + basis 36 n (+ alpha 4 n) + x 4 M + level bitmap 8 W + u 4 n
       rows:   coords 12 M + x 4 M + y 4 M + level bitmap 8 W + cell bitmap 8 Wc + cell_start 4 (C+1)
               + basis 36 n + u 4 n
@@ -108,6 +123,23 @@ def stencil_roofline(system, reps=50):
     b_pts = n * (12 + 36 + 4 + (4 if system.alpha is not None else 0)) + 4 * M + 8 * W
     b_rows = 12 * M + 8 * M + 8 * W + 8 * Wc + 4 * (C + 1) + 40 * n
     return t_pts, b_pts, t_rows, b_rows
+
+
+def stencil_probe(res, xyz, level=0):
+    """The stencil form of one level of a finished reconstruction (rebuilt from its tree and the
+    points), timed: the operator form that "auto" weighs against the run form."""
+    import torch
+    from ngspsr.reconstructor import PointBinning
+    rec = res.reconstructor
+    tree = res.tree
+    M = tree.get_coords_size(None, level)
+    rhs = torch.zeros(M, dtype=torch.float64, device=xyz.device)
+    bins = PointBinning(tree, xyz, level, 4.0)
+    S = rec.assemble_level(level, bins, None, 0.25 ** level, 0.0, rhs, form="stencil")
+    t_pts, b_pts, t_rows, b_rows = stencil_roofline(S)
+    return {"level": level, "rows": M, "points": int(S.n), "points_us": round(1e6 * t_pts, 2),
+            "rows_us": round(1e6 * t_rows, 2), "operator_us": round(1e6 * (t_pts + t_rows), 2),
+            "rows_bytes": int(b_rows), "points_bytes": int(b_pts), "rows_GB/s": round(b_rows / t_rows / 1e9, 1)}
 
 
 def assembly_roofline(res, xyz, name="c2", reps=10):
@@ -134,16 +166,32 @@ def assembly_roofline(res, xyz, name="c2", reps=10):
     cb = torch.empty_like(coarse)
     call("ngs_gather_f32", ptr(coarse), ptr(bins.perm), bins.n, ptr(cb), stream())
     pb.perm = None
-    t = _event_time(lambda: call("ngs_lhs_assemble_runs", lv, pb, ptr(cb), 1.0, 0.1, ptr(runs), ptr(rvals),
-                                 ptr(inv_diag), ptr(rhs), stream()), reps)
-    R = RunSystem(runs, rvals, inv_diag)
+    t_slot = _event_time(lambda: call("ngs_lhs_assemble_runs", lv, pb, ptr(cb), 1.0, 0.1, ptr(runs), ptr(rvals),
+                                      ptr(inv_diag), ptr(rhs), stream()), reps)
+    R = RunSystem(runs, rvals, inv_diag, packed=False)
     nnz, nruns = R.nnz(), int((R.counts() > 0).sum().item())
     n = xyz.shape[0]
     algo = 4 * nnz + 4 * nruns + (4 + 16 + 12) * M + 52 * n
-    return {"kernel": "lhs_assemble_kernel (finest level, run form, screening with a coarse field)", "bound": "latency",
-            "achieved": round(algo / t / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4), "traffic": measured_traffic("lhs_assemble_kernel", name),
-            "launch_us": round(1e6 * t, 2), "algorithmic_bytes": int(algo), "rows": int(M), "points": int(n)}
+    out = {"kernel": "lhs_assemble_kernel (finest level, run form, screening with a coarse field)", "bound": "latency",
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_bytes": int(algo), "rows": int(M), "points": int(n)}
+    from ngspsr._lib import lib
+    if RunSystem.PACKED and lib().ngs_assembly_lanes(int(pb.n), M) == 1:
+        # the solver's form: written straight into the packed layout (slice offsets from the row counts)
+        nsl = (M + 63) // 64
+        row_nnz = torch.zeros(nsl * 64, dtype=torch.int32, device=xyz.device)
+        call("ngs_stencil_row_counts", lv, ptr(row_nnz), stream())
+        soff = torch.zeros(nsl + 1, dtype=torch.int64, device=xyz.device)
+        torch.cumsum(row_nnz.view(nsl, 64).sum(1, dtype=torch.int64), 0, out=soff[1:])
+        pvals = torch.empty(int(soff[-1].item()), dtype=torch.float32, device=xyz.device)
+        t = _event_time(lambda: call("ngs_lhs_assemble_packed", lv, pb, ptr(cb), 1.0, 0.1, ptr(runs), ptr(pvals),
+                                     ptr(soff), ptr(inv_diag), ptr(rhs), stream()), reps)
+        out["kernel"] = "lhs_assemble_kernel (finest level, into the packed run form, screening with a coarse field)"
+        out["slot_form_launch_us"] = round(1e6 * t_slot, 2)
+    else:
+        t = t_slot
+    out.update({"achieved": round(algo / t / 1e9, 1), "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic("lhs_assemble_kernel", name), "launch_us": round(1e6 * t, 2)})
+    return out
 
 
 def pcg_iteration_rate(system, iters=200):
@@ -306,8 +354,8 @@ COPY_BYTES = 1 << 30
 
 def pmc_probe(res):
     """Kernels for the HBM counter runs: a 1 GiB device copy (its FETCH_SIZE / WRITE_SIZE readings
-    calibrate the counters against 1 GiB read + 1 GiB written), then the finest-level SpMV in run
-    form and in CSR form (the same matrix), 5 launches each."""
+    calibrate the counters against 1 GiB read + 1 GiB written), then the finest-level SpMV in the
+    slot run form, the packed run form and CSR form (the same matrix), 5 launches each."""
     import torch
     from ngspsr._lib import call, ptr, stream
     from ngspsr.solver import RunSystem, StencilSystem
@@ -322,6 +370,10 @@ def pmc_probe(res):
     if isinstance(system, RunSystem):
         for _ in range(5):
             call("ngs_spmv_runs", ptr(system.runs), ptr(system.rvals), ptr(x), ptr(y), system.M, stream())
+        if system.is_packed:
+            pvals, soff = system._packed
+            for _ in range(5):
+                call("ngs_spmv_packed", ptr(system.runs), ptr(pvals), ptr(soff), ptr(x), ptr(y), system.M, stream())
     for _ in range(5):
         call("ngs_spmv", ptr(csr.rowptr), ptr(csr.cols), ptr(csr.vals), ptr(x), ptr(y), csr.M, stream())
     torch.cuda.synchronize()
@@ -423,11 +475,17 @@ def run_serving(n_req=256, steps=20, warmup=5, workers=8, chunk=8, log=None):
 
     srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, refine=True, chunk=chunk)
 
+    def hand_off(req):
+        # a client takes the mesh as each request finishes; the bench keeps only its size
+        if req.result is not None:
+            req.mesh_size = (int(req.result[0].shape[0]), int(req.result[1].shape[0]))
+            req.result = None
+
     def one_pass(arrivals=None):
         reqs = [serving.Request(i, x, n, vs, arrival=0.0 if arrivals is None else float(arrivals[i]))
                 for i, (x, n, vs) in enumerate(data)]
         ts = time.perf_counter()
-        srv.serve(reqs, t0=ts)
+        srv.serve(reqs, t0=ts, on_complete=hand_off)
         torch.cuda.synchronize()
         errs = [r for r in reqs if r.error is not None]
         if errs:
@@ -617,11 +675,16 @@ def single_solve(name, steps, warmup, cpu=True, ext=True, fp64=True):
                 "csr_spmv_us_same_system": round(1e6 * t_spmv, 2)}
     elif isinstance(system, RunSystem):
         t_run, algo_run, _, extra = spmv_roofline(system)
+        extra["stencil_form_same_level"] = stencil_probe(res, xyz)
         iter_bytes = algo_run + 44 * system.M
-        roof = {"kernel": "spmv_runs_kernel (finest-level SpMV, run form: one run word per bitmap column, "
-                          "values plane-major)", "bound": "hbm", "achieved": round(algo_run / t_run / 1e9, 1),
+        roof = {"kernel": ("spmv_packed_kernel (finest-level SpMV, packed run form: one run word per bitmap "
+                           "column, the values of 64-row slices packed and staged through LDS; the PCG's "
+                           "bpcg_packed_dot_kernel is the same row loop plus the p.q partials)") if system.is_packed
+                else "spmv_runs_kernel (finest-level SpMV, run form: one run word per bitmap column, values "
+                     "plane-major)", "bound": "hbm", "achieved": round(algo_run / t_run / 1e9, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(algo_run / t_run / 1e9 / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic("spmv_runs_kernel", name), "launch_us": round(1e6 * t_run, 2),
+                "traffic": measured_traffic("spmv_packed_kernel" if system.is_packed else "spmv_runs_kernel", name),
+                "launch_us": round(1e6 * t_run, 2),
                 "algorithmic_bytes": int(algo_run), **extra, "csr_spmv_us_same_matrix": round(1e6 * t_spmv, 2)}
     else:
         iter_bytes = algo + 44 * system.M
@@ -767,6 +830,34 @@ def trace_busy(path, skip_first=0, marker="spin_kernel"):
                       "top_kernels_s": {k: round(v / 1e9, 4) for k, v in top}}), flush=True)
 
 
+def trace_levels(path, kernels=("bpcg_runs_dot_kernel", "spmv_runs_kernel", "residual_f64_runs_kernel",
+                                  "bpcg_update_kernel", "bpcg_direction_kernel", "lhs_assemble_kernel",
+                                  "stencil_rows_kernel", "stencil_points_kernel", "bpcg_stencil")):
+    """Per-launch-shape summary of a rocprofv3 kernel trace (CSV): for each kernel whose name holds
+    one of `kernels`, launches grouped by grid size (one group per level system, since the grid
+    follows the row count), with count, total and average duration. Prints a markdown table."""
+    import csv
+    groups = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"]
+            hit = next((k for k in kernels if k in name), None)
+            if hit is None:
+                continue
+            grid = r.get("Grid_Size") or r.get("Grid_Size_X") or "?"
+            key = (hit, name.split("(")[0] if "<" in name else hit, grid)
+            groups.setdefault(key, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    print("| kernel | grid size | launches | total ms | average us | working launches | their average us | "
+          "min us | max us |")
+    print("|---|---|---|---|---|---|---|---|---|")
+    for (hit, full, grid), ds in sorted(groups.items(), key=lambda kv: (kv[0][0], -sum(kv[1]))):
+        n, tot, hi = len(ds), sum(ds), max(ds)
+        # launches of a converged system return at entry: "working" launches take >= 10% of the longest
+        work = [d for d in ds if d >= 0.1 * hi]
+        print(f"| `{full}` | {grid} | {n} | {tot / 1e6:.3f} | {tot / n / 1e3:.2f} | {len(work)} | "
+              f"{sum(work) / len(work) / 1e3:.2f} | {min(ds) / 1e3:.2f} | {hi / 1e3:.2f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -789,9 +880,15 @@ def main():
                     help="the plain path once over the serving stream, for a kernel trace (rocprofv3)")
     ap.add_argument("--trace-busy", default=None, help="summarise a rocprofv3 kernel-trace CSV (no GPU)")
     ap.add_argument("--trace-skip", type=int, default=0, help="kernels to skip at the start of --trace-busy")
+    ap.add_argument("--trace-levels", default=None,
+                    help="per-grid-size table of the PCG, residual and assembly kernels of a rocprofv3 kernel-trace "
+                         "CSV (no GPU)")
     args = ap.parse_args()
     if args.trace_busy:
         trace_busy(args.trace_busy, args.trace_skip)
+        return
+    if args.trace_levels:
+        trace_levels(args.trace_levels)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the path's multi-GPU sharding (by request for serving, "

@@ -287,6 +287,14 @@ int ngs_assembly_config(int32_t lanes_per_row);
 int ngs_lhs_assemble_runs(ngs_level lv, ngs_point_bins pts, const float *coarse, float sf, float delta,
                           int32_t *runs, float *rvals, float *inv_diag, double *rhs, ngs_stream_t s);
 
+/* ngs_lhs_assemble_runs writing the values straight into the packed run form (ngs_runs_pack's
+ * layout; soff from the slice sums of ngs_stencil_row_counts), for levels that assemble with one
+ * lane per row: ngs_assembly_lanes(pts.n, M) == 1 (NGS_ERR_ARG otherwise). Same run words,
+ * inv_diag, rhs and values, bit for bit. */
+int32_t ngs_assembly_lanes(int64_t n_points, int64_t M);
+int ngs_lhs_assemble_packed(ngs_level lv, ngs_point_bins pts, const float *coarse, float sf, float delta, int32_t *runs,
+                            float *pvals, const int64_t *soff, float *inv_diag, double *rhs, ngs_stream_t s);
+
 /* Stiffness part of the cross-level correction (reconstructor.py:153-168, COO of
  * get_neighbours(d, dd, 5), integrate_deriv_deriv_product bezier_tensor.py:75-96):
  * rhs_i -= sum_j K_{d,dd}(i, j) x_dd[j] over the 125 dd-voxels around the dd-cell of i. */
@@ -373,6 +381,12 @@ int ngs_pcg_solve_runs_noalloc(const int32_t *runs, const float *rvals, const fl
                                float *x, int64_t M, double tol, int32_t max_iter, int32_t chunk, ngs_stream_t s,
                                int32_t *iters, double *rel_resid);
 
+/* ngs_pcg_solve_runs on the packed run form (ngs_runs_pack): the same iteration and results, bit
+ * for bit; the SpMV streams the values without the run form's zero slots. */
+int ngs_pcg_solve_packed(const int32_t *runs, const float *pvals, const int64_t *soff, const float *inv_diag,
+                         const float *b, float *x, int64_t M, double tol, int32_t max_iter, int32_t chunk,
+                         void *workspace, ngs_stream_t s, int32_t *iters, double *rel_resid);
+
 /* ngs_pcg_solve on the stencil form of the system (same iteration, same stop rule; the SpMV is the
  * point pass + row pass of ngs_stencil_apply). */
 int ngs_pcg_solve_stencil(ngs_stencil op, const float *inv_diag, const float *b, float *x, double tol,
@@ -402,6 +416,10 @@ int ngs_pcg_batch_place(void *entry, int32_t blk0);
 int ngs_pcg_batch_prepare_runs(const int32_t *runs, const float *rvals, const float *inv_diag, const float *b,
                                float *x, int64_t M, double tol, int32_t max_iter, void *workspace, int32_t blk0,
                                int32_t nblk, void *entry, ngs_stream_t s);
+/* The same as ngs_pcg_batch_prepare for a system in packed run form. */
+int ngs_pcg_batch_prepare_packed(const int32_t *runs, const float *pvals, const int64_t *soff, const float *inv_diag,
+                                 const float *b, float *x, int64_t M, double tol, int32_t max_iter, void *workspace,
+                                 int32_t blk0, int32_t nblk, void *entry, ngs_stream_t s);
 /* The same as ngs_pcg_batch_prepare for a system in stencil form. */
 int ngs_pcg_batch_prepare_stencil(ngs_stencil op, const float *inv_diag, const float *b, float *x, double tol,
                                   int32_t max_iter, void *workspace, int32_t blk0, int32_t nblk, void *entry,
@@ -409,7 +427,7 @@ int ngs_pcg_batch_prepare_stencil(ngs_stencil op, const float *inv_diag, const f
 /* `iters` (even) iterations of every active system in one launch per kernel (graph-replayed when
  * use_graph). flags: bit 0 the table may hold stencil systems (adds their point and row passes),
  * bit 1 it may hold run-form systems (adds their SpMV), bit 2 it holds no CSR systems (skips
- * that SpMV). */
+ * that SpMV), bit 3 it may hold packed-form systems (adds their SpMV). */
 int ngs_pcg_batch_iterate(const void *table, const int32_t *blk2sys, int32_t total_blocks, int32_t iters,
                           int32_t use_graph, int32_t flags, ngs_stream_t s);
 /* (iter, done) of each system of the host copy of the table into states (2 nsys int32, host);
@@ -418,6 +436,20 @@ int ngs_pcg_batch_status(const void *host_table, int32_t nsys, int32_t *states, 
 
 /* y = A x (FP32) on the run form (terms in increasing column order), exposed for measurement. */
 int ngs_spmv_runs(const int32_t *runs, const float *rvals, const float *x, float *y, int64_t M, ngs_stream_t s);
+/* Packed run form (no reference counterpart): the run form without its zero value slots. Rows are
+ * cut into slices of 64; slice s keeps, plane by plane (p = 0..24), the values of its rows' runs
+ * back to back in row order from pvals[soff[s]], with the run words of the run form.
+ * ngs_runs_slice_counts writes the number of values of every slice ((M + 63) / 64 int64); soff is
+ * its exclusive scan (one more entry, the total); ngs_runs_pack copies the run form's values
+ * there and ngs_runs_unpack back (slots past a run get 0; rows M..P-1 are not written);
+ * ngs_spmv_packed is y = A x with the same sums as ngs_spmv_runs, bit for bit. */
+int ngs_runs_slice_counts(const int32_t *runs, int64_t M, int64_t *counts, ngs_stream_t s);
+int ngs_runs_pack(const int32_t *runs, const float *rvals, int64_t M, const int64_t *soff, float *pvals,
+                  ngs_stream_t s);
+int ngs_runs_unpack(const int32_t *runs, const float *pvals, int64_t M, const int64_t *soff, float *rvals,
+                    ngs_stream_t s);
+int ngs_spmv_packed(const int32_t *runs, const float *pvals, const int64_t *soff, const float *x, float *y, int64_t M,
+                    ngs_stream_t s);
 
 /* y = A x (FP32), the SpMV of the solver, exposed for measurement. */
 int ngs_spmv(const int32_t *rowptr, const int32_t *cols, const float *vals, const float *x, float *y,
@@ -430,6 +462,9 @@ int ngs_residual_f64(const int32_t *rowptr, const int32_t *cols, const float *va
 /* The same on the run form. */
 int ngs_residual_f64_runs(const int32_t *runs, const float *rvals, const double *b, const double *x, double *r,
                           int64_t M, double *norms, void *workspace, ngs_stream_t s);
+/* The same on the packed run form (the sums of ngs_residual_f64_runs, bit for bit). */
+int ngs_residual_f64_packed(const int32_t *runs, const float *pvals, const int64_t *soff, const double *b,
+                            const double *x, double *r, int64_t M, double *norms, void *workspace, ngs_stream_t s);
 
 /* Sparse direct solve (replaces _sparse_solve.solve_cusparse, csrc/sparse_solve/solve_cusparse.cu:15-42,
  * the sparse Cholesky behind the "cholmod" solver, jspsr/core/solver.py:147-157): a supernodal

@@ -38,14 +38,15 @@ __global__ void lhs_count_kernel(ngs_level lv, int32_t* __restrict__ row_nnz) {
     row_nnz[i] = PAD ? (cnt + 3) & ~3 : cnt;
 }
 
-// One thread per row, one x-plane of the row at a time: the 25 entries of plane dx accumulate in
-// registers over the points of the cells that reach it (cell offsets ox with |dx - ox| <= 1; points
-// in bin order, basis values and weights stored in bin order), then the plane is written. Every
-// entry receives its point terms in the same order as a one-pass loop over all 27 cells would give
-// them (cells in (ox, oy, oz) order, points in bin order), so the values do not depend on the plane
-// split; the split keeps 25 accumulators live instead of 125, which fit the register file without
-// spilling. Cells and neighbours are found one bitmap column (3 or 5 consecutive lattice points
-// along z) at a time.
+// One thread per row, in two passes over the points of the cells around the voxel (points in bin
+// order, basis values and weights stored in bin order): the first pass visits all 27 cells and
+// accumulates the 75 entries of the x-planes dx = -2..0, the second visits the 18 cells with
+// ox >= 0 and accumulates the 50 entries of dx = 1, 2; each pass then writes its planes. Every
+// entry receives its point terms in the order of a one-pass loop over all 27 cells (cells in
+// (ox, oy, oz) order, points in bin order), so the values do not depend on the split; the split
+// keeps at most 75 accumulators live (125 spilled to scratch) for 1.67x the point visits.
+// Cells and neighbours are found one bitmap column (3 or 5 consecutive lattice points along z) at
+// a time.
 // RUNS: the row goes to the run form instead of CSR. The present neighbours of voxel i in the
 // bitmap column (dx, dy) (its five z-neighbours) have consecutive ranks start .. start + n - 1;
 // runs[p P + i] = start | n << 28 (n = 0: start = i) for p = 5 (dx + 2) + (dy + 2), and
@@ -76,23 +77,94 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
-template <bool HAS_ALPHA, bool HAS_COARSE, bool RUNS, int G>
+// Accumulates the NPL x-planes DX0 .. DX0 + NPL - 1 of the row over the cells that reach them
+// (|dx - ox| <= 1); the pass holding plane 0 visits all 27 cells and also sums rs and rc.
+template <bool HAS_ALPHA, bool HAS_COARSE, int G, int DX0, int NPL>
+__device__ __forceinline__ void assemble_pass(const ngs_point_bins& pts, int mx, int my, int mz, int gl,
+                                              const float* __restrict__ coarse, float (&acc)[NPL * 25], float& rs,
+                                              float& rc) {
+    constexpr int OX0 = DX0 - 1 < -1 ? -1 : DX0 - 1;
+    constexpr int OX1 = DX0 + NPL > 1 ? 1 : DX0 + NPL;
+    constexpr bool CENTRE = DX0 <= 0 && 0 < DX0 + NPL;
+    const ngs_grid& cg = pts.cells;
+#pragma unroll
+    for (int k = 0; k < NPL * 25; ++k) acc[k] = 0.f;
+#pragma unroll
+    for (int ox = OX0; ox <= OX1; ++ox) {
+#pragma unroll
+        for (int oy = -1; oy <= 1; ++oy) {
+            int32_t cid;
+            const uint32_t cm = grid_column(cg, mx + ox, my + oy, mz - 1, 3, &cid);
+#pragma unroll
+            for (int oz = -1; oz <= 1; ++oz) {
+                if (!((cm >> (oz + 1)) & 1u)) continue;
+                const int k0 = pts.cell_start[cid], k1 = pts.cell_start[cid + 1];
+                ++cid;
+                for (int k = k0 + gl; k < k1; k += G) {
+                    const float* bp = pts.b + 9 * (int64_t)k;
+                    const float bx[3] = {bp[0], bp[1], bp[2]};
+                    const float by[3] = {bp[3], bp[4], bp[5]};
+                    const float bz[3] = {bp[6], bp[7], bp[8]};
+                    const float a = HAS_ALPHA ? pts.alpha[k] : pts.alpha_scalar;
+                    // products rounded on their own, sums by explicit FMA: the same arithmetic as
+                    // stencil_diag_rhs_kernel, whatever the contraction choices around it
+                    float w;
+                    {
+#pragma clang fp contract(off)
+                        w = a * (bx[1 - ox] * by[1 - oy] * bz[1 - oz]);
+                    }
+                    if (CENTRE) {
+                        rs += w;
+                        // coarse is in bin order when the bins carry no permutation (a contiguous
+                        // stream instead of one random gather per visit; 27 rows visit every point)
+                        if (HAS_COARSE) rc = fmaf(w, coarse[pts.perm ? pts.perm[k] : k], rc);
+                    }
+#pragma unroll
+                    for (int pl = 0; pl < NPL; ++pl) {
+                        const int vx = DX0 + pl - ox;
+                        if (vx < -1 || vx > 1) continue;  // resolved at compile time
+#pragma unroll
+                        for (int vy = -1; vy <= 1; ++vy) {
+                            float wxy;
+                            {
+#pragma clang fp contract(off)
+                                wxy = w * (bx[vx + 1] * by[vy + 1]);
+                            }
+#pragma unroll
+                            for (int vz = -1; vz <= 1; ++vz) {
+                                float& e = acc[pl * 25 + (oy + vy + 2) * 5 + (oz + vz + 2)];
+                                e = fmaf(wxy, bz[vz + 1], e);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (G > 1) {
+#pragma unroll
+        for (int q = 0; q < NPL * 25; ++q) acc[q] = group_sum<G>(acc[q]);
+    }
+}
+
+// PK (with RUNS and G = 1): the values go straight to the packed run form (pvals from soff[slice],
+// plane by plane, the rows' runs at the wave prefix of their lengths; blocks of 64 threads are
+// exactly the 64-row slices) instead of the value slots.
+template <bool HAS_ALPHA, bool HAS_COARSE, bool RUNS, int G, bool PK = false>
 __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_point_bins pts,
                                                        const float* __restrict__ coarse, float sf, float sf_delta,
                                                        const float* __restrict__ F1, const float* __restrict__ G1,
                                                        const int32_t* __restrict__ rowptr, int32_t* __restrict__ cols,
                                                        float* __restrict__ vals, float* __restrict__ inv_diag,
                                                        double* __restrict__ rhs, int32_t* __restrict__ runs,
-                                                       float* __restrict__ rvals) {
+                                                       float* __restrict__ rvals, const int64_t* __restrict__ soff = nullptr) {
+    static_assert(!PK || (RUNS && G == 1), "the packed form is written by one lane per row");
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t i = t / G;  // the G lanes of a row are consecutive lanes of one wave
     const int gl = (int)(t % G);
     if (i >= lv.M) return;
     const int s = lv.stride;
     const int cx = lv.coords[3 * i], cy = lv.coords[3 * i + 1], cz = lv.coords[3 * i + 2];
-    const float* __restrict__ pb_all = pts.b;
-    const float* __restrict__ pa_all = pts.alpha;
-    const int32_t* __restrict__ cstart = pts.cell_start;
     const ngs_grid& cg = pts.cells;
     const int c2 = cg.log2_stride;
     const int mx = (cx >> c2) - cg.origin[0], my = (cy >> c2) - cg.origin[1], mz = (cz >> c2) - cg.origin[2];
@@ -120,95 +192,54 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
         ++cnt;
     };
     const int64_t P = run_plane_stride(lv.M);
-#pragma nounroll
-    for (int dx = -2; dx <= 2; ++dx) {
-        float acc[25];
+    int64_t ppos = PK ? soff[i >> 6] : 0;  // running value position of the slice (wave-uniform)
+    // the planes' entries in increasing column order (lexicographic offsets over sorted voxels)
+    auto emit_plane = [&](int dx, const float* acc) {
+        const float c0 = F1[dx + 2] * fs, g0 = G1[dx + 2] * inv_s;
 #pragma unroll
-        for (int k = 0; k < 25; ++k) acc[k] = 0.f;
-        for (int ox = max(-1, dx - 1); ox <= min(1, dx + 1); ++ox) {
-            const int vx = dx - ox;
+        for (int dy = -2; dy <= 2; ++dy) {
+            int32_t j;
+            const uint32_t m = grid_column(g, lx + dx, ly + dy, lz - 2, 5, &j);
+            const int pl = (dx + 2) * 5 + (dy + 2);
+            if (RUNS) runs[pl * P + i] = (m ? j : (int32_t)i) | (__popc(m) << RUN_COUNT_SHIFT);
+            int q = 0;
+            int pex = 0, ptot = 0;
+            if (PK) wave_prefix3(__popc(m), pex, ptot);
 #pragma unroll
-            for (int oy = -1; oy <= 1; ++oy) {
-                int32_t cid;
-                const uint32_t cm = grid_column(cg, mx + ox, my + oy, mz - 1, 3, &cid);
-#pragma unroll
-                for (int oz = -1; oz <= 1; ++oz) {
-                    if (!((cm >> (oz + 1)) & 1u)) continue;
-                    const int k0 = cstart[cid], k1 = cstart[cid + 1];
-                    ++cid;
-                    for (int k = k0 + gl; k < k1; k += G) {
-                        const float* bp = pb_all + 9 * (int64_t)k;
-                        const float bx0 = bp[0], bx1 = bp[1], bx2 = bp[2];
-                        const float by[3] = {bp[3], bp[4], bp[5]};
-                        const float bz[3] = {bp[6], bp[7], bp[8]};
-                        const float a = HAS_ALPHA ? pa_all[k] : pts.alpha_scalar;
-                        // the x factors of the row's basis (cell offset ox) and of the column's
-                        // (plane offset vx): selects, not indexing, keep them in registers
-                        const float bxo = ox < 0 ? bx2 : (ox == 0 ? bx1 : bx0);
-                        const float bxv = vx < 0 ? bx0 : (vx == 0 ? bx1 : bx2);
-                        // products rounded on their own, sums by explicit FMA: the same arithmetic
-                        // as stencil_diag_rhs_kernel, whatever the contraction choices around it
-                        float w;
-                        {
+            for (int dz = -2; dz <= 2; ++dz) {
+                if (!((m >> (dz + 2)) & 1u)) continue;
+                float v;
+                {
 #pragma clang fp contract(off)
-                            w = a * (bxo * by[1 - oy] * bz[1 - oz]);
-                        }
-                        if (dx == 0) {  // the plane whose pass visits all 27 cells
-                            rs += w;
-                            // coarse is in bin order when the bins carry no permutation (a
-                            // contiguous stream instead of one random gather per visit)
-                            if (HAS_COARSE) rc = fmaf(w, coarse[pts.perm ? pts.perm[k] : k], rc);
-                        }
-#pragma unroll
-                        for (int vy = -1; vy <= 1; ++vy) {
-                            float wxy;
-                            {
-#pragma clang fp contract(off)
-                                wxy = w * (bxv * by[vy + 1]);
-                            }
-#pragma unroll
-                            for (int vz = -1; vz <= 1; ++vz) {
-                                float& e = acc[(oy + vy + 2) * 5 + (oz + vz + 2)];
-                                e = fmaf(wxy, bz[vz + 1], e);
-                            }
-                        }
-                    }
+                    const float c1 = F1[dy + 2] * fs, c2f = F1[dz + 2] * fs;
+                    const float g1 = G1[dy + 2] * inv_s, g2 = G1[dz + 2] * inv_s;
+                    const float K = g0 * c1 * c2f + c0 * g1 * c2f + c0 * c1 * g2;
+                    v = K + sf * acc[(dy + 2) * 5 + (dz + 2)];
                 }
+                if (dx == 0 && dy == 0 && dz == 0) inv_diag[i] = 1.0f / v;
+                if (PK) rvals[ppos + pex + q++] = v;
+                else if (RUNS) rvals[(5 * pl + q++) * P + i] = v;
+                else emit(v, j++);
             }
+            if (PK) ppos += ptot;
+            else if (RUNS)
+                for (; q < 5; ++q) rvals[(5 * pl + q) * P + i] = 0.f;
         }
-        if (G > 1) {
+    };
+    {
+        float acc[75];
+        assemble_pass<HAS_ALPHA, HAS_COARSE, G, -2, 3>(pts, mx, my, mz, gl, coarse, acc, rs, rc);
+        if (G == 1 || gl == 0)
 #pragma unroll
-            for (int q = 0; q < 25; ++q) acc[q] = group_sum<G>(acc[q]);
-        }
-        if (G == 1 || gl == 0) {
-            // the plane's entries in increasing column order (lexicographic offsets over sorted voxels)
-            const float c0 = F1[dx + 2] * fs, g0 = G1[dx + 2] * inv_s;
+            for (int pl = 0; pl < 3; ++pl) emit_plane(pl - 2, acc + 25 * pl);
+    }
+    {
+        float acc[50];
+        float unused_s = 0.f, unused_c = 0.f;
+        assemble_pass<HAS_ALPHA, HAS_COARSE, G, 1, 2>(pts, mx, my, mz, gl, coarse, acc, unused_s, unused_c);
+        if (G == 1 || gl == 0)
 #pragma unroll
-            for (int dy = -2; dy <= 2; ++dy) {
-                int32_t j;
-                const uint32_t m = grid_column(g, lx + dx, ly + dy, lz - 2, 5, &j);
-                const int pl = (dx + 2) * 5 + (dy + 2);
-                if (RUNS) runs[pl * P + i] = (m ? j : (int32_t)i) | (__popc(m) << RUN_COUNT_SHIFT);
-                int q = 0;
-#pragma unroll
-                for (int dz = -2; dz <= 2; ++dz) {
-                    if (!((m >> (dz + 2)) & 1u)) continue;
-                    float v;
-                    {
-#pragma clang fp contract(off)
-                        const float c1 = F1[dy + 2] * fs, c2f = F1[dz + 2] * fs;
-                        const float g1 = G1[dy + 2] * inv_s, g2 = G1[dz + 2] * inv_s;
-                        const float K = g0 * c1 * c2f + c0 * g1 * c2f + c0 * c1 * g2;
-                        v = K + sf * acc[(dy + 2) * 5 + (dz + 2)];
-                    }
-                    if (dx == 0 && dy == 0 && dz == 0) inv_diag[i] = 1.0f / v;
-                    if (RUNS) rvals[(5 * pl + q++) * P + i] = v;
-                    else emit(v, j++);
-                }
-                if (RUNS)
-                    for (; q < 5; ++q) rvals[(5 * pl + q) * P + i] = 0.f;
-            }
-        }
+            for (int pl = 0; pl < 2; ++pl) emit_plane(pl + 1, acc + 25 * pl);
     }
     if (G > 1) {
         rs = group_sum<G>(rs);
@@ -685,7 +716,7 @@ int ngs_stencil_row_counts(ngs_level lv, int32_t* row_nnz, ngs_stream_t s) {
 
 static int lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, float sf, float delta,
                         const int32_t* rowptr, int32_t* cols, float* vals, float* inv_diag, double* rhs, int32_t* runs,
-                        float* rvals, ngs_stream_t s) {
+                        float* rvals, ngs_stream_t s, const int64_t* soff = nullptr) {
     if (lv.M <= 0) return NGS_OK;
     Tables T;
     NGS_REQUIRE(tables_for(1, &T), "ngs_lhs_assemble: integral tables not registered");
@@ -693,6 +724,23 @@ static int lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, f
     const int lanes = assembly_lanes(pts.n, lv.M);
     dim3 grid(grid1d(lv.M * lanes, 64)), block(64);
     hipStream_t st = S(s);
+    if (soff) {
+        NGS_REQUIRE(lanes == 1, "ngs_lhs_assemble_packed: this level assembles with several lanes per row "
+                                "(ngs_assembly_lanes); assemble the run form and pack it");
+#define NGS_ASM_P(A, C)                                                                                          \
+    lhs_assemble_kernel<A, C, true, 1, true><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, nullptr, \
+                                                                     nullptr, nullptr, inv_diag, rhs, runs, rvals, soff)
+        if (pts.alpha) {
+            if (coarse) NGS_ASM_P(true, true);
+            else NGS_ASM_P(true, false);
+        } else {
+            if (coarse) NGS_ASM_P(false, true);
+            else NGS_ASM_P(false, false);
+        }
+#undef NGS_ASM_P
+        NGS_CHECK_LAUNCH();
+        return NGS_OK;
+    }
 #define NGS_ASM_G(A, C, R, LN)                                                                                      \
     lhs_assemble_kernel<A, C, R, LN><<<grid, block, 0, st>>>(lv, pts, coarse, sf, sfd, T.F, T.G, rowptr, cols, vals, \
                                                              inv_diag, rhs, runs, rvals)
@@ -719,6 +767,15 @@ static int lhs_assemble(ngs_level lv, ngs_point_bins pts, const float* coarse, f
 #undef NGS_ASM_G
     NGS_CHECK_LAUNCH();
     return NGS_OK;
+}
+
+int32_t ngs_assembly_lanes(int64_t n_points, int64_t M) { return assembly_lanes(n_points, M); }
+
+int ngs_lhs_assemble_packed(ngs_level lv, ngs_point_bins pts, const float* coarse, float sf, float delta, int32_t* runs,
+                            float* pvals, const int64_t* soff, float* inv_diag, double* rhs, ngs_stream_t s) {
+    NGS_REQUIRE(lv.M <= 0 || (runs && pvals && soff && inv_diag && rhs), "ngs_lhs_assemble_packed: null array");
+    NGS_REQUIRE(lv.M <= ((int64_t)1 << RUN_COUNT_SHIFT), "ngs_lhs_assemble_packed: too many rows for run words");
+    return lhs_assemble(lv, pts, coarse, sf, delta, nullptr, nullptr, nullptr, inv_diag, rhs, runs, pvals, s, soff);
 }
 
 int ngs_assembly_config(int32_t lanes_per_row) {

@@ -133,6 +133,17 @@ __device__ __forceinline__ uint32_t grid_column(const ngs_grid& g, int lx, int l
     return m << (zlo - lz0);
 }
 
+__device__ __forceinline__ int lane64() { return (int)(threadIdx.x & 63); }
+
+// Exclusive prefix over the wave's active lanes and the wave total of n in [0, 7] (inactive lanes
+// count as 0): the positions of the rows' runs in a slice of the packed run form.
+__device__ __forceinline__ void wave_prefix3(int n, int& excl, int& total) {
+    const uint64_t below = (1ull << lane64()) - 1ull;
+    const uint64_t b0 = __ballot(n & 1), b1 = __ballot(n & 2), b2 = __ballot(n & 4);
+    excl = __popcll(b0 & below) + 2 * __popcll(b1 & below) + 4 * __popcll(b2 & below);
+    total = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+}
+
 // Plane stride of the run form (runs (25, P) int32, rvals (125, P) float32): M rounded up to 64
 // rows, so every plane starts on a 256-byte boundary and the loads of a wave's 64 consecutive rows
 // are single aligned 256-byte segments.

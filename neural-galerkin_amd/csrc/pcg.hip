@@ -49,7 +49,8 @@ struct PcgArgs {
     const int32_t* rowptr;
     const int32_t* cols;
     const int32_t* runs;    // run form (kind SYS_RUNS): 25 plane-major run words start | n << 28
-    const float* rvals;     // run form: 125 plane-major value slots per row
+    const float* rvals;     // run form: 125 plane-major value slots per row; packed form: the values
+    const int64_t* soff;    // packed form (kind SYS_PACKED): value offset of every 64-row slice
     const float* vals;
     const float* dinv;
     const float* b;
@@ -171,6 +172,15 @@ __device__ __forceinline__ float row_dot(int32_t k0, int32_t k1, const NGS_GLOBA
     return sum;
 }
 
+// a * b + c with one rounding, in the sum's type: the row sums of the run and packed forms are
+// written with it so that their rounding does not depend on the compiler's contraction choices.
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c);
+template <>
+__device__ __forceinline__ float fma_t<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double fma_t<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Row `row` of the run form times x: the 25 bitmap columns (dx, dy) in order, each a run of n <= 5
 // consecutive columns start .. start + n - 1 with its values in slots 0 .. n - 1 (the rest hold 0),
 // so the terms are summed in increasing column order. One lane per row: the run words and value
@@ -203,7 +213,7 @@ __device__ __forceinline__ T run_row_dot(const NGS_GLOBAL int32_t* __restrict__ 
 #pragma unroll
         for (int b = 0; b < B; ++b)
 #pragma unroll
-            for (int j = 0; j < 5; ++j) sum += (T)v[b][j] * (T)xv[b][j];
+            for (int j = 0; j < 5; ++j) sum = fma_t<T>((T)v[b][j], (T)xv[b][j], sum);
     }
     return sum;
 }
@@ -311,6 +321,159 @@ __global__ void __launch_bounds__(PCG_THREADS) spmv_runs_kernel(const int32_t* _
         y[row] = run_row_dot<float, float>(as_global(runs), as_global(rvals), as_global(x), run_plane_stride(M), row);
 }
 
+// ------------------------------------------------------------------------ packed run form
+// The run form without its zero value slots: the rows are cut into slices of 64 (one wave each);
+// slice s stores, plane by plane (p = 0..24), the values of its rows' runs back to back in lane
+// order (row 64 s + l's run of plane p at the exclusive prefix of the run lengths of lanes < l),
+// starting at soff[s]. The run words are those of the run form.
+constexpr int PACK_BATCH = 5;                    // planes staged together
+constexpr int PACK_STAGE = PACK_BATCH * 5 * 64;  // floats of LDS per wave
+
+// Values per slice (the sum of its rows' run lengths). One wave per slice.
+__global__ void __launch_bounds__(PCG_THREADS) runs_slice_counts_kernel(const int32_t* __restrict__ runs, int64_t M,
+                                                                        int64_t* __restrict__ counts) {
+    const int64_t nsl = (M + 63) / 64;
+    const int64_t sl = ((int64_t)blockIdx.x * PCG_THREADS + threadIdx.x) >> 6;
+    if (sl >= nsl) return;  // uniform over the wave
+    const int64_t row = sl * 64 + lane64();
+    const int64_t P = run_plane_stride(M);
+    int tot = 0;
+    for (int p = 0; p < 25; ++p) {
+        const int n = row < M ? (int)((uint32_t)runs[p * P + row] >> RUN_COUNT_SHIFT) : 0;
+        int ex, t;
+        wave_prefix3(n, ex, t);
+        tot += t;
+    }
+    if (lane64() == 0) counts[sl] = tot;
+}
+
+// Slot form -> packed form. One wave per slice.
+__global__ void __launch_bounds__(PCG_THREADS) runs_pack_kernel(const int32_t* __restrict__ runs,
+                                                                const float* __restrict__ rvals, int64_t M,
+                                                                const int64_t* __restrict__ soff,
+                                                                float* __restrict__ pvals) {
+    const int64_t nsl = (M + 63) / 64;
+    const int64_t sl = ((int64_t)blockIdx.x * PCG_THREADS + threadIdx.x) >> 6;
+    if (sl >= nsl) return;
+    const int64_t row = sl * 64 + lane64();
+    const int64_t P = run_plane_stride(M);
+    int64_t pos = soff[sl];
+    for (int p = 0; p < 25; ++p) {
+        const int n = row < M ? (int)((uint32_t)runs[p * P + row] >> RUN_COUNT_SHIFT) : 0;
+        int ex, t;
+        wave_prefix3(n, ex, t);
+        for (int j = 0; j < n; ++j) pvals[pos + ex + j] = rvals[(5 * p + j) * P + row];
+        pos += t;
+    }
+}
+
+// Packed form -> slot form (the inverse of runs_pack_kernel; slots past a run hold 0).
+__global__ void __launch_bounds__(PCG_THREADS) runs_unpack_kernel(const int32_t* __restrict__ runs,
+                                                                  const float* __restrict__ pvals, int64_t M,
+                                                                  const int64_t* __restrict__ soff,
+                                                                  float* __restrict__ rvals) {
+    const int64_t nsl = (M + 63) / 64;
+    const int64_t sl = ((int64_t)blockIdx.x * PCG_THREADS + threadIdx.x) >> 6;
+    if (sl >= nsl) return;
+    const int64_t row = sl * 64 + lane64();
+    const int64_t P = run_plane_stride(M);
+    int64_t pos = soff[sl];
+    for (int p = 0; p < 25; ++p) {
+        const int n = row < M ? (int)((uint32_t)runs[p * P + row] >> RUN_COUNT_SHIFT) : 0;
+        int ex, t;
+        wave_prefix3(n, ex, t);
+        if (row < M)
+            for (int j = 0; j < 5; ++j) rvals[(5 * p + j) * P + row] = j < n ? pvals[pos + ex + j] : 0.f;
+        pos += t;
+    }
+}
+
+// Row `row` (lane of the wave of slice sl) of the packed form times x, the terms in the order of
+// run_row_dot without its zero slots (so the sums are the same, bit for bit). Per batch of 5 planes
+// the wave stages the batch's values (contiguous in pvals) into its LDS region with coalesced loads,
+// then every lane reads its runs there and gathers x. All waves of the block call it in step (the
+// block barrier orders the LDS writes and reads); lanes past M carry empty runs.
+template <typename T, typename XT>
+__device__ __forceinline__ T packed_row_dot(const NGS_GLOBAL int32_t* __restrict__ runs,
+                                            const NGS_GLOBAL float* __restrict__ pvals,
+                                            const NGS_GLOBAL XT* __restrict__ x, int64_t P, int64_t row, bool ok,
+                                            int64_t pos, float* stage) {
+    T sum = 0;
+    const int lane = lane64();
+#pragma unroll 1
+    for (int p0 = 0; p0 < 25; p0 += PACK_BATCH) {
+        int32_t w[PACK_BATCH];
+        int ex[PACK_BATCH];
+        int tot = 0;
+#pragma unroll
+        for (int b = 0; b < PACK_BATCH; ++b) w[b] = ok ? runs[(p0 + b) * P + row] : 0;
+#pragma unroll
+        for (int b = 0; b < PACK_BATCH; ++b) {
+            int e, t;
+            wave_prefix3((int)((uint32_t)w[b] >> RUN_COUNT_SHIFT), e, t);
+            ex[b] = tot + e;
+            tot += t;
+        }
+        {
+            // all of the batch's loads issued together (one memory round trip per batch): lane l takes
+            // values l, l + 64, ... of the batch's segment (tot <= PACK_STAGE)
+            float t[PACK_STAGE / 64];
+#pragma unroll
+            for (int u = 0; u < PACK_STAGE / 64; ++u) {
+                const int e = u * 64 + lane;
+                t[u] = (u * 64 < tot && e < tot) ? pvals[pos + e] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < PACK_STAGE / 64; ++u) {
+                const int e = u * 64 + lane;
+                if (u * 64 < tot && e < tot) stage[e] = t[u];
+            }
+        }
+        __syncthreads();
+        float v[PACK_BATCH][5];
+        XT xv[PACK_BATCH][5];
+#pragma unroll
+        for (int b = 0; b < PACK_BATCH; ++b) {
+            const int32_t start = w[b] & RUN_START_MASK;
+            const int n = (int)((uint32_t)w[b] >> RUN_COUNT_SHIFT);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                v[b][j] = j < n ? stage[ex[b] + j] : 0.f;
+                xv[b][j] = j < n ? x[start + j] : (XT)0;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < PACK_BATCH; ++b)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) sum = fma_t<T>((T)v[b][j], (T)xv[b][j], sum);
+        pos += tot;
+        __syncthreads();
+    }
+    return sum;
+}
+
+// y = A x for the packed form: blocks of 4 waves take groups of 4 slices, grid-stride.
+__global__ void __launch_bounds__(PCG_THREADS) spmv_packed_kernel(const int32_t* __restrict__ runs,
+                                                                  const float* __restrict__ pvals,
+                                                                  const int64_t* __restrict__ soff,
+                                                                  const float* __restrict__ x, float* __restrict__ y,
+                                                                  int64_t M) {
+    __shared__ float stage_all[PCG_THREADS / 64][PACK_STAGE];
+    float* stage = stage_all[threadIdx.x >> 6];
+    const int64_t nsl = (M + 63) / 64;
+    const int64_t groups = (nsl + PCG_THREADS / 64 - 1) / (PCG_THREADS / 64);
+    const int64_t P = run_plane_stride(M);
+    for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int64_t sl = g * (PCG_THREADS / 64) + (threadIdx.x >> 6);
+        const int64_t row = sl * 64 + lane64();
+        const bool ok = row < M;
+        const int64_t pos = sl < nsl ? soff[sl] : 0;
+        const float sum = packed_row_dot<float, float>(as_global(runs), as_global(pvals), as_global(x), P, row, ok,
+                                                       pos, stage);
+        if (ok) y[row] = sum;
+    }
+}
+
 // r = b - A x in FP64 (FP32 entries, FP64 x and products), 8 lanes per row with 16-byte loads of
 // entries and columns. x == nullptr means x = 0 (r = b, no matrix pass).
 __global__ void __launch_bounds__(PCG_THREADS) residual_f64_kernel(const int32_t* __restrict__ rowptr,
@@ -394,6 +557,40 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_runs_kernel(const in
         r[row] = ri;
         rr += ri * ri;
         bb += bi * bi;
+    }
+    store_partials2(rr, bb, partials, partials + gridDim.x, blockIdx.x);
+}
+
+// r = b - A x in FP64 for the packed form (the sums of residual_f64_runs_kernel, bit for bit): blocks
+// take 256-row groups (4 slices), grid-stride.
+__global__ void __launch_bounds__(PCG_THREADS) residual_f64_packed_kernel(const int32_t* __restrict__ runs,
+                                                                          const float* __restrict__ pvals,
+                                                                          const int64_t* __restrict__ soff,
+                                                                          const double* __restrict__ b,
+                                                                          const double* __restrict__ x,
+                                                                          double* __restrict__ r, int64_t M,
+                                                                          double* partials) {
+    __shared__ float stage_all[PCG_THREADS / 64][PACK_STAGE];
+    float* stage = stage_all[threadIdx.x >> 6];
+    const int64_t step = (int64_t)gridDim.x * PCG_THREADS;
+    const int64_t nsl = (M + 63) / 64;
+    const int64_t P = run_plane_stride(M);
+    double rr = 0, bb = 0;
+    for (int64_t base = (int64_t)blockIdx.x * PCG_THREADS; base < M; base += step) {
+        const int64_t row = base + threadIdx.x;
+        const bool ok = row < M;
+        const int64_t sl = row >> 6;
+        const int64_t pos = sl < nsl ? soff[sl] : 0;
+        const double ax = x ? packed_row_dot<double, double>(as_global(runs), as_global(pvals), as_global(x), P, row,
+                                                             ok, pos, stage)
+                            : 0.0;
+        if (ok) {
+            const double bi = b[row];
+            const double ri = x ? bi - ax : bi;
+            r[row] = ri;
+            rr += ri * ri;
+            bb += bi * bi;
+        }
     }
     store_partials2(rr, bb, partials, partials + gridDim.x, blockIdx.x);
 }
@@ -505,10 +702,10 @@ static HostState& host_state() {
 // contiguous range of blocks each one owns; every kernel looks up its system from its block index,
 // so one launch (and one captured graph, whose arguments are the fixed table pointers) serves any
 // mix of systems. Systems join and leave between launches by rewriting the table.
-enum : int32_t { SYS_CSR = 0, SYS_STENCIL = 1, SYS_RUNS = 2 };
+enum : int32_t { SYS_CSR = 0, SYS_STENCIL = 1, SYS_RUNS = 2, SYS_PACKED = 3 };
 
 // Flags of a launch sequence (which kernels the table's systems need).
-enum : int32_t { FLAG_STENCIL = 1, FLAG_RUNS = 2, FLAG_NO_CSR = 4 };
+enum : int32_t { FLAG_STENCIL = 1, FLAG_RUNS = 2, FLAG_NO_CSR = 4, FLAG_PACKED = 8 };
 
 struct BatchSys {
     PcgArgs a;
@@ -585,6 +782,39 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_runs_dot_kernel(const BatchS
         const float sum = run_row_dot<float, float>(runs, rvals, p, run_plane_stride(A.M), row);
         q[row] = sum;
         pq += (double)prow * sum;
+    }
+    store_partials2(pq, 0.0, A.partials, nullptr, lb);
+}
+
+// The same for the packed-form systems: each step of a block takes 4 whole slices (256 rows from a
+// multiple of 256), its 4 waves staging their slices' values through LDS in step.
+__global__ void __launch_bounds__(PCG_THREADS) bpcg_packed_dot_kernel(const BatchSys* tab, const int32_t* blk2sys,
+                                                                      int ph) {
+    __shared__ float stage_all[PCG_THREADS / 64][PACK_STAGE];
+    int lb;
+    const BatchSys* B = batch_lookup(tab, blk2sys, &lb);
+    if (!B || B->kind != SYS_PACKED) return;
+    const PcgArgs& A = B->a;
+    if (A.st->slot[ph].done) return;
+    float* stage = stage_all[threadIdx.x >> 6];
+    const int64_t step = (int64_t)B->nblk * PCG_THREADS;
+    const int64_t nsl = (A.M + 63) / 64;
+    const int64_t P = run_plane_stride(A.M);
+    const NGS_GLOBAL float* __restrict__ p = as_global(A.p);
+    NGS_GLOBAL float* __restrict__ q = as_global(A.q);
+    double pq = 0;
+    for (int64_t base = (int64_t)(blockIdx.x - B->blk0) * PCG_THREADS; base < A.M; base += step) {
+        const int64_t row = base + threadIdx.x;
+        const bool ok = row < A.M;
+        const int64_t sl = row >> 6;
+        const int64_t pos = sl < nsl ? A.soff[sl] : 0;
+        const float prow = ok ? p[row] : 0.f;
+        const float sum = packed_row_dot<float, float>(as_global(A.runs), as_global(A.rvals), p, P, row, ok, pos,
+                                                       stage);
+        if (ok) {
+            q[row] = sum;
+            pq += (double)prow * sum;
+        }
     }
     store_partials2(pq, 0.0, A.partials, nullptr, lb);
 }
@@ -791,6 +1021,7 @@ static void launch_batch_iteration(const BatchSys* tab, const int32_t* blk2sys, 
         });
     }
     if (flags & FLAG_RUNS) bpcg_runs_dot_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
+    if (flags & FLAG_PACKED) bpcg_packed_dot_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
     if (flags & FLAG_STENCIL) bpcg_stencil_dot_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
     bpcg_update_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
     bpcg_direction_kernel<<<grid, block, 0, st>>>(tab, blk2sys, ph);
@@ -852,6 +1083,7 @@ static int pcg_solve_entry(BatchSys E, int64_t M, double tol, int32_t max_iter, 
     NGS_CHECK_LAUNCH();
     const int flags = E.kind == SYS_STENCIL ? (FLAG_STENCIL | FLAG_NO_CSR)
                       : E.kind == SYS_RUNS ? (FLAG_RUNS | FLAG_NO_CSR)
+                      : E.kind == SYS_PACKED ? (FLAG_PACKED | FLAG_NO_CSR)
                                              : 0;
     int status = NGS_OK;
     int32_t it = 0, done = 0;
@@ -1073,6 +1305,86 @@ int ngs_spmv_runs(const int32_t* runs, const float* rvals, const float* x, float
     return NGS_OK;
 }
 
+int ngs_runs_slice_counts(const int32_t* runs, int64_t M, int64_t* counts, ngs_stream_t s) {
+    if (M <= 0) return NGS_OK;
+    NGS_REQUIRE(runs && counts, "ngs_runs_slice_counts: null array");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_runs_slice_counts: too many rows for run words");
+    runs_slice_counts_kernel<<<grid1d((M + 63) / 64 * 64, PCG_THREADS), PCG_THREADS, 0, S(s)>>>(runs, M, counts);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_runs_pack(const int32_t* runs, const float* rvals, int64_t M, const int64_t* soff, float* pvals,
+                  ngs_stream_t s) {
+    if (M <= 0) return NGS_OK;
+    NGS_REQUIRE(runs && rvals && soff && pvals, "ngs_runs_pack: null array");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_runs_pack: too many rows for run words");
+    runs_pack_kernel<<<grid1d((M + 63) / 64 * 64, PCG_THREADS), PCG_THREADS, 0, S(s)>>>(runs, rvals, M, soff, pvals);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_runs_unpack(const int32_t* runs, const float* pvals, int64_t M, const int64_t* soff, float* rvals,
+                    ngs_stream_t s) {
+    if (M <= 0) return NGS_OK;
+    NGS_REQUIRE(runs && pvals && soff && rvals, "ngs_runs_unpack: null array");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_runs_unpack: too many rows for run words");
+    runs_unpack_kernel<<<grid1d((M + 63) / 64 * 64, PCG_THREADS), PCG_THREADS, 0, S(s)>>>(runs, pvals, M, soff, rvals);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_spmv_packed(const int32_t* runs, const float* pvals, const int64_t* soff, const float* x, float* y, int64_t M,
+                    ngs_stream_t s) {
+    if (M <= 0) return NGS_OK;
+    NGS_REQUIRE(runs && pvals && soff && x && y, "ngs_spmv_packed: null array");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_spmv_packed: too many rows for run words");
+    const int64_t groups = ((M + 63) / 64 + 3) / 4;
+    const int nb = (int)std::min<int64_t>(groups, 8192);
+    spmv_packed_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(runs, pvals, soff, x, y, M);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_pcg_solve_packed(const int32_t* runs, const float* pvals, const int64_t* soff, const float* inv_diag,
+                         const float* b, float* x, int64_t M, double tol, int32_t max_iter, int32_t chunk,
+                         void* workspace, ngs_stream_t s, int32_t* iters, double* rel_resid) {
+    NGS_REQUIRE(M >= 0 && workspace, "ngs_pcg_solve_packed: bad arguments");
+    NGS_REQUIRE(tol >= 0, "ngs_pcg_solve_packed: negative tolerance");
+    if (M == 0) {
+        if (iters) *iters = 0;
+        if (rel_resid) *rel_resid = 0;
+        return NGS_OK;
+    }
+    NGS_REQUIRE(runs && pvals && soff && inv_diag && b && x, "ngs_pcg_solve_packed: null array");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_pcg_solve_packed: too many rows for run words");
+    BatchSys E{};
+    E.kind = SYS_PACKED;
+    E.a.runs = runs;
+    E.a.rvals = pvals;
+    E.a.soff = soff;
+    E.a.dinv = inv_diag;
+    E.a.b = b;
+    E.a.x = x;
+    return pcg_solve_entry(E, M, tol, max_iter, chunk, workspace, s, iters, rel_resid);
+}
+
+int ngs_residual_f64_packed(const int32_t* runs, const float* pvals, const int64_t* soff, const double* b,
+                            const double* x, double* r, int64_t M, double* norms, void* workspace, ngs_stream_t s) {
+    NGS_REQUIRE(workspace && b && r && norms && (!x || (runs && pvals && soff)),
+                "ngs_residual_f64_packed: bad arguments");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_residual_f64_packed: too many rows for run words");
+    if (M <= 0) return NGS_OK;
+    int64_t chunk;
+    int nb = pcg_blocks(M, &chunk);
+    char* w = reinterpret_cast<char*>(workspace);
+    double* partials = reinterpret_cast<double*>(w + al256(sizeof(PcgState)));
+    residual_f64_packed_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(runs, pvals, soff, b, x, r, M, partials);
+    norms_finish_kernel<<<1, PCG_THREADS, 0, S(s)>>>(partials, nb, norms);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
 int ngs_residual_f64(const int32_t* rowptr, const int32_t* cols, const float* vals, const double* b, const double* x,
                      double* r, int64_t M, double* norms, void* workspace, ngs_stream_t s) {
     NGS_REQUIRE(workspace && b && r && norms && (!x || (rowptr && cols && vals)), "ngs_residual_f64: bad arguments");
@@ -1189,6 +1501,26 @@ int ngs_pcg_batch_prepare_runs(const int32_t* runs, const float* rvals, const fl
     E.kind = SYS_RUNS;
     E.a.runs = runs;
     E.a.rvals = rvals;
+    E.a.dinv = inv_diag;
+    E.a.b = b;
+    E.a.x = x;
+    int r = batch_prepare(&E, M, tol, max_iter, workspace, blk0, nblk, s);
+    if (r == NGS_OK) *reinterpret_cast<BatchSys*>(entry) = E;
+    return r;
+}
+
+int ngs_pcg_batch_prepare_packed(const int32_t* runs, const float* pvals, const int64_t* soff, const float* inv_diag,
+                                 const float* b, float* x, int64_t M, double tol, int32_t max_iter, void* workspace,
+                                 int32_t blk0, int32_t nblk, void* entry, ngs_stream_t s) {
+    NGS_REQUIRE(M > 0 && workspace && entry && runs && pvals && soff && inv_diag && b && x,
+                "ngs_pcg_batch_prepare_packed: bad arguments");
+    NGS_REQUIRE(M <= RUN_START_MASK + 1, "ngs_pcg_batch_prepare_packed: too many rows for run words");
+    NGS_REQUIRE(nblk == ngs_pcg_batch_blocks(M), "ngs_pcg_batch_prepare_packed: nblk must be ngs_pcg_batch_blocks(M)");
+    BatchSys E{};
+    E.kind = SYS_PACKED;
+    E.a.runs = runs;
+    E.a.rvals = pvals;
+    E.a.soff = soff;
     E.a.dinv = inv_diag;
     E.a.b = b;
     E.a.x = x;

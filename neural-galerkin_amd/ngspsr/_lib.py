@@ -98,6 +98,8 @@ _SIGNATURES = {
     "ngs_lhs_assemble": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P, P], c_int32),
     "ngs_assembly_config": ([c_int32], c_int32),
     "ngs_lhs_assemble_runs": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P], c_int32),
+    "ngs_assembly_lanes": ([c_int64, c_int64], c_int32),
+    "ngs_lhs_assemble_packed": ([Level, PointBins, P, c_float, c_float, P, P, P, P, P, P], c_int32),
     "ngs_pcg_solve_runs": ([P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
                             ctypes.POINTER(c_double)], c_int32),
     "ngs_pcg_solve_runs_noalloc": ([P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, ctypes.POINTER(c_int32),
@@ -125,6 +127,15 @@ _SIGNATURES = {
     "ngs_bin_points": ([P, c_int64, P, P, P, P, P, P, P], c_int32),
     "ngs_stencil_diag_rhs": ([Stencil, P, c_float, P, P, P], c_int32),
     "ngs_stencil_apply": ([Stencil, P, P, P], c_int32),
+    "ngs_runs_slice_counts": ([P, c_int64, P, P], c_int32),
+    "ngs_runs_pack": ([P, P, c_int64, P, P, P], c_int32),
+    "ngs_spmv_packed": ([P, P, P, P, P, c_int64, P], c_int32),
+    "ngs_runs_unpack": ([P, P, c_int64, P, P, P], c_int32),
+    "ngs_pcg_solve_packed": ([P, P, P, P, P, P, c_int64, c_double, c_int32, c_int32, P, P, ctypes.POINTER(c_int32),
+                              ctypes.POINTER(c_double)], c_int32),
+    "ngs_pcg_batch_prepare_packed": ([P, P, P, P, P, P, c_int64, c_double, c_int32, P, c_int32, c_int32, P, P],
+                                     c_int32),
+    "ngs_residual_f64_packed": ([P, P, P, P, P, P, c_int64, P, P, P], c_int32),
     "ngs_stencil_pass": ([Stencil, P, P, c_int32, P], c_int32),
     "ngs_stencil_row_counts": ([Level, P, P], c_int32),
     "ngs_stencil_residual_f64": ([Stencil, P, P, P, P, P, P, P], c_int32),

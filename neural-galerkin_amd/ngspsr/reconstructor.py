@@ -161,7 +161,23 @@ class Reconstructor:
         if form == "runs":
             P = _solver.RunSystem.plane_stride(M)
             runs = torch.empty((25, P), dtype=torch.int32, device=dev)
-            rvals = torch.empty((125, P), dtype=torch.float32, device=dev)
+            if _solver.RunSystem.PACKED and lib().ngs_assembly_lanes(int(pb.n), M) == 1:
+                # one lane per row: the assembly writes the packed form itself, at slice offsets from
+                # the row counts (64-row slice sums, scanned)
+                nsl = (M + 63) // 64
+                row_nnz = torch.zeros(nsl * 64, dtype=torch.int32, device=dev)
+                call("ngs_stencil_row_counts", lv, ptr(row_nnz), stream())
+                soff = torch.zeros(nsl + 1, dtype=torch.int64, device=dev)
+                torch.cumsum(row_nnz.view(nsl, 64).sum(1, dtype=torch.int64), 0, out=soff[1:])
+                pvals = torch.empty(max(int(soff[-1].item()), 1), dtype=torch.float32, device=dev)
+                call("ngs_lhs_assemble_packed", lv, pb, ptr(coarse), float(sf), float(delta), ptr(runs), ptr(pvals),
+                     ptr(soff), ptr(inv_diag), ptr(rhs), stream())
+                return _solver.RunSystem.from_packed(runs, pvals, soff, inv_diag)
+            if _solver.RunSystem.PACKED:
+                # the slot values only feed the packing: a thread scratch buffer, reused level to level
+                rvals = Scratch.get(125 * P * 4, dev, slot=7)[:125 * P * 4].view(torch.float32).view(125, P)
+            else:
+                rvals = torch.empty((125, P), dtype=torch.float32, device=dev)
             call("ngs_lhs_assemble_runs", lv, pb, ptr(coarse), float(sf), float(delta), ptr(runs),
                  ptr(rvals), ptr(inv_diag), ptr(rhs), stream())
             return _solver.RunSystem(runs, rvals, inv_diag)

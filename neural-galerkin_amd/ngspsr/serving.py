@@ -45,15 +45,18 @@ class _Job:
 
 def launch_flags(systems):
     """ngs_pcg_batch_iterate flags for a set of systems: which SpMV kernels they need."""
-    stencil = runs = plain = False
+    stencil = runs = packed = plain = False
     for s in systems:
         if isinstance(s, _solver.StencilSystem):
             stencil = True
         elif isinstance(s, _solver.RunSystem):
-            runs = True
+            if s.is_packed:
+                packed = True
+            else:
+                runs = True
         else:
             plain = True
-    return (1 if stencil else 0) | (2 if runs else 0) | (0 if plain else 4)
+    return (1 if stencil else 0) | (2 if runs else 0) | (0 if plain else 4) | (8 if packed else 0)
 
 
 class BatchedPCG:
@@ -135,6 +138,11 @@ class BatchedPCG:
                 call("ngs_pcg_batch_prepare_stencil", s.struct(), ptr(s.inv_diag), ptr(job.b), ptr(job.x),
                      float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
                      self.stream.cuda_stream)
+            elif isinstance(s, _solver.RunSystem) and s.is_packed:
+                pvals, soff = s._packed
+                call("ngs_pcg_batch_prepare_packed", ptr(s.runs), ptr(pvals), ptr(soff), ptr(s.inv_diag), ptr(job.b),
+                     ptr(job.x), s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
+                     self.stream.cuda_stream)
             elif isinstance(s, _solver.RunSystem):
                 call("ngs_pcg_batch_prepare_runs", ptr(s.runs), ptr(s.rvals), ptr(s.inv_diag), ptr(job.b), ptr(job.x),
                      s.M, float(job.tol), int(job.max_iter), ptr(job.ws), 0, job.nblk, entry.data_ptr(),
@@ -212,10 +220,10 @@ def estimate_footprint(n_points: int, voxel_size: float = None) -> int:
 def level_footprint(level_sizes, n_points: int) -> int:
     """Peak HBM bytes of a request from its tree's voxel counts: the dual-mesh stage dominates (its
     dual cubes, corner positions and values, vertex keys and their sort), measured at 2.7-5.7 KB
-    per finest voxel and below 3.5 KB per voxel of all levels plus 200 bytes per point on the
-    largest requests of the serving stream (1e4-1e6 points, depth 7-9); solve state and the
-    returned mesh are included."""
-    return int(3500.0 * sum(int(m) for m in level_sizes) + 200.0 * n_points + 256 * 2 ** 20)
+    per finest voxel and up to 3.9 KB per voxel of all levels plus 200 bytes per point on the
+    requests of the serving stream (1e4-1e6 points, depth 7-9); solve state and the returned mesh
+    are included."""
+    return int(4000.0 * sum(int(m) for m in level_sizes) + 200.0 * n_points + 256 * 2 ** 20)
 
 
 class Request:
@@ -334,6 +342,12 @@ class ReconstructionServer:
                 except Exception:  # the request's error is already recorded
                     pass
             req.finish = time.perf_counter()
+            cb = getattr(self, "_on_complete", None)
+            if cb is not None:
+                try:
+                    cb(req)
+                except Exception as e:  # reported on the request
+                    req.error = req.error or e
             with self.lock:
                 self.in_use -= req.footprint
                 self.workers_busy -= 1
@@ -353,8 +367,12 @@ class ReconstructionServer:
                 self.in_use += need - req.footprint
                 req.footprint = need
 
-    def serve(self, requests, t0=None):
-        """Runs the requests honouring their arrival offsets (seconds after t0); returns them."""
+    def serve(self, requests, t0=None, on_complete=None):
+        """Runs the requests honouring their arrival offsets (seconds after t0); returns them.
+        on_complete(req), when given, is called on the worker thread as each request finishes
+        (result or error set, its stream synchronised): the place to hand the mesh on and drop it,
+        since results held by the caller are device memory that admission does not count."""
+        self._on_complete = on_complete
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter() if t0 is None else t0
         futures = []

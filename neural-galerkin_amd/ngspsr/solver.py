@@ -95,16 +95,65 @@ class RunSystem:
     reads 600 bytes per row with every load an aligned contiguous segment across consecutive rows."""
 
     COUNT_SHIFT = 28
+    # keep the matrix packed (values without the zero slots, ngs_runs_pack): the solver's SpMV and
+    # FP64 residual read it; the slot values are regenerated on demand (rvals) for the entry points
+    # that take them. False keeps the slot form (same results, bit for bit).
+    PACKED = True
 
     @staticmethod
     def plane_stride(M):
         return (M + 63) // 64 * 64
 
-    def __init__(self, runs, rvals, inv_diag):
-        self.runs, self.rvals, self.inv_diag = runs, rvals, inv_diag
+    def __init__(self, runs, rvals, inv_diag, packed=None):
+        """rvals may be a caller's temporary when the system packs (it is read once, on the
+        current stream)."""
+        self.runs, self.inv_diag = runs, inv_diag
         self.M = inv_diag.shape[0]
         self.device = inv_diag.device
         assert runs.shape == (25, self.plane_stride(self.M)) and rvals.shape == (125, self.plane_stride(self.M))
+        self._rvals = rvals
+        self._packed = None
+        if (self.PACKED if packed is None else packed) and self.M:
+            self._pack(rvals)
+            self._rvals = None
+
+    @classmethod
+    def from_packed(cls, runs, pvals, soff, inv_diag):
+        """A system assembled straight into the packed form (ngs_lhs_assemble_packed)."""
+        self = cls.__new__(cls)
+        self.runs, self.inv_diag = runs, inv_diag
+        self.M = inv_diag.shape[0]
+        self.device = inv_diag.device
+        assert runs.shape == (25, cls.plane_stride(self.M)) and soff.shape == ((self.M + 63) // 64 + 1,)
+        self._rvals = None
+        self._packed = (pvals, soff)
+        return self
+
+    def _pack(self, rvals):
+        nsl = (self.M + 63) // 64
+        counts = torch.empty(nsl, dtype=torch.int64, device=self.device)
+        call("ngs_runs_slice_counts", ptr(self.runs), self.M, ptr(counts), stream())
+        soff = torch.zeros(nsl + 1, dtype=torch.int64, device=self.device)
+        torch.cumsum(counts, 0, out=soff[1:])
+        pvals = torch.empty(max(int(soff[-1].item()), 1), dtype=torch.float32, device=self.device)
+        call("ngs_runs_pack", ptr(self.runs), ptr(rvals), self.M, ptr(soff), ptr(pvals), stream())
+        self._packed = (pvals, soff)
+
+    @property
+    def is_packed(self):
+        return self._packed is not None
+
+    @property
+    def rvals(self):
+        """The slot values (125, P) float32, regenerated from the packed form when the system packs."""
+        if self._rvals is None:
+            P = self.plane_stride(self.M)
+            rv = torch.zeros((125, P), dtype=torch.float32, device=self.device)
+            if self.M:
+                pvals, soff = self._packed
+                call("ngs_runs_unpack", ptr(self.runs), ptr(pvals), self.M, ptr(soff), ptr(rv), stream())
+            self._rvals = rv
+        return self._rvals
 
     def counts(self):
         """(25, M) run lengths."""
@@ -117,6 +166,18 @@ class RunSystem:
     def matvec(self, x):
         y = torch.empty(self.M, dtype=torch.float32, device=self.device)
         if self.M:
+            xc = x.float().contiguous()
+            if self._packed is not None:
+                pvals, soff = self._packed
+                call("ngs_spmv_packed", ptr(self.runs), ptr(pvals), ptr(soff), ptr(xc), ptr(y), self.M, stream())
+            else:
+                call("ngs_spmv_runs", ptr(self.runs), ptr(self.rvals), ptr(xc), ptr(y), self.M, stream())
+        return y
+
+    def matvec_slots(self, x):
+        """y = A x through the slot form (the same sums as matvec, bit for bit)."""
+        y = torch.empty(self.M, dtype=torch.float32, device=self.device)
+        if self.M:
             call("ngs_spmv_runs", ptr(self.runs), ptr(self.rvals), ptr(x.float().contiguous()), ptr(y), self.M,
                  stream())
         return y
@@ -126,8 +187,14 @@ class RunSystem:
         r = torch.empty(self.M, dtype=torch.float64, device=self.device)
         norms = torch.empty(2, dtype=torch.float64, device=self.device)
         ws = _lib.Scratch.get(lib().ngs_pcg_workspace_bytes(self.M), self.device, slot=1)
-        call("ngs_residual_f64_runs", ptr(self.runs), ptr(self.rvals), ptr(b64.contiguous()),
-             None if x64 is None else ptr(x64.contiguous()), ptr(r), self.M, ptr(norms), ptr(ws), stream())
+        xp = None if x64 is None else ptr(x64.contiguous())
+        if self._packed is not None:
+            pvals, soff = self._packed
+            call("ngs_residual_f64_packed", ptr(self.runs), ptr(pvals), ptr(soff), ptr(b64.contiguous()), xp, ptr(r),
+                 self.M, ptr(norms), ptr(ws), stream())
+        else:
+            call("ngs_residual_f64_runs", ptr(self.runs), ptr(self.rvals), ptr(b64.contiguous()), xp, ptr(r), self.M,
+                 ptr(norms), ptr(ws), stream())
         nv = norms.cpu()
         return r, float(nv[0]), float(nv[1])
 
@@ -304,9 +371,15 @@ def pcg(system, b, tol=1e-5, maxiter=-1, chunk=32, x_out=None, raise_on_cap=True
                                          int(maxiter), int(chunk), ptr(ws), stream(), ctypes.byref(it),
                                          ctypes.byref(rr))
     elif isinstance(system, RunSystem):
-        st = lib().ngs_pcg_solve_runs(ptr(system.runs), ptr(system.rvals), ptr(system.inv_diag), ptr(b), ptr(x), M,
-                                      float(tol), int(maxiter), int(chunk), ptr(ws), stream(), ctypes.byref(it),
-                                      ctypes.byref(rr))
+        if system.is_packed:
+            pvals, soff = system._packed
+            st = lib().ngs_pcg_solve_packed(ptr(system.runs), ptr(pvals), ptr(soff), ptr(system.inv_diag), ptr(b),
+                                            ptr(x), M, float(tol), int(maxiter), int(chunk), ptr(ws), stream(),
+                                            ctypes.byref(it), ctypes.byref(rr))
+        else:
+            st = lib().ngs_pcg_solve_runs(ptr(system.runs), ptr(system.rvals), ptr(system.inv_diag), ptr(b), ptr(x),
+                                          M, float(tol), int(maxiter), int(chunk), ptr(ws), stream(),
+                                          ctypes.byref(it), ctypes.byref(rr))
     else:
         st = lib().ngs_pcg_solve(ptr(system.rowptr), ptr(system.cols), ptr(system.vals), ptr(system.inv_diag),
                                  ptr(b), ptr(x), M, float(tol), int(maxiter), int(chunk), ptr(ws), stream(),

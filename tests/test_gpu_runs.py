@@ -128,7 +128,8 @@ def test_runs_solve_matches_csr_solve(cuda):
 
 
 def test_pool_mixes_runs_csr_and_stencil(cuda):
-    """Batched pool with run, CSR and stencil systems together: each result equals its single solve."""
+    """Batched pool with packed run, slot run, CSR and stencil systems together: each result equals
+    its single solve."""
     import threading
     from ngspsr import serving, solver
     print("[case] pool runs + csr + stencil", flush=True)
@@ -139,9 +140,15 @@ def test_pool_mixes_runs_csr_and_stencil(cuda):
         for form in ("runs", "csr", "stencil"):
             rhs = torch.zeros(M, dtype=torch.float64, device=cuda)
             systems.append((recon.assemble_level(d, bins[d], None, 0.25 ** d, 0.1, rhs, form=form), rhs.float()))
+        # the same run-form matrix kept in slot form
+        packed = systems[-3][0]
+        systems.append((solver.RunSystem(packed.runs, packed.rvals.clone(), packed.inv_diag, packed=False),
+                        systems[-3][1]))
     torch.cuda.synchronize()
-    assert serving.launch_flags([s for s, _ in systems]) == 3
-    assert serving.launch_flags([s for s, _ in systems[:1]]) == 2 | 4
+    assert systems[0][0].is_packed and not systems[3][0].is_packed
+    assert serving.launch_flags([s for s, _ in systems]) == 1 | 2 | 8
+    assert serving.launch_flags([s for s, _ in systems[:1]]) == 8 | 4
+    assert serving.launch_flags([systems[3][0]]) == 2 | 4
     pool = serving.BatchedPCG(cuda, chunk=4)
     outs = [None] * len(systems)
     try:
@@ -184,3 +191,77 @@ def test_reference_solver_modes(cuda):
     xc, infoc = solver.solve(A, large, "cholmod")
     assert infoc.direct and infoc.true_rel_resid <= 1e-12
     assert torch.equal(xc.float(), x)  # the same factor (kept on the system), deterministic solves
+
+
+def test_packed_form_equals_slot_form(cuda):
+    """The packed run form (values without zero slots, slices of 64 rows staged through LDS), which
+    the run-form systems keep by default, gives the slot form's products, FP64 residuals and PCG
+    solves bit for bit, on every level of a scene whose level sizes are not multiples of 64; the
+    slot values regenerated from it equal the assembled ones."""
+    import ngspsr
+    from ngspsr import solver, synthetic
+    from ngspsr.solver import RunSystem
+    print("[case] packed run form", flush=True)
+    xyz, nrm = synthetic.ellipsoids(30000, k=5, sigma=0.002, seed=17)
+    res = ngspsr.reconstruct(torch.from_numpy(xyz).to(cuda), torch.from_numpy(nrm).to(cuda), 1 / 128, depth=4,
+                             min_density=2.0, solver="pcg@tol=1e-6@maxiter=20000", operator="runs",
+                             return_result=True)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    sizes = []
+    for d, (system, rhs) in sorted(res.reconstructor.systems.items()):
+        assert isinstance(system, RunSystem) and system.is_packed
+        sizes.append(system.M)
+        pvals, soff = system._packed
+        assert int(soff[-1]) == system.nnz()
+        slot = RunSystem(system.runs, system.rvals.clone(), system.inv_diag, packed=False)
+        assert not slot.is_packed
+        # the regenerated slots hold the assembled values: the CSR of both forms is the same
+        a, b = system.to_csr(), slot.to_csr()
+        assert torch.equal(a.vals, b.vals) and torch.equal(a.cols, b.cols)
+        for _ in range(2):
+            x = torch.randn(system.M, generator=g).to(cuda)
+            assert torch.equal(system.matvec(x), slot.matvec(x)), d
+            assert torch.equal(system.matvec(x), system.matvec_slots(x)), d
+        b64 = torch.randn(system.M, generator=g, dtype=torch.float64).to(cuda)
+        x64 = torch.randn(system.M, generator=g, dtype=torch.float64).to(cuda)
+        r1, rr1, bb1 = system.residual_f64(b64, x64)
+        r2, rr2, bb2 = slot.residual_f64(b64, x64)
+        assert torch.equal(r1, r2), d
+        x1, i1 = solver.pcg(system, rhs.float(), tol=1e-6, maxiter=20000)
+        x2, i2 = solver.pcg(slot, rhs.float(), tol=1e-6, maxiter=20000)
+        assert i1.iters == i2.iters and torch.equal(x1, x2), d
+    assert any(m % 64 for m in sizes), sizes
+
+
+@pytest.mark.parametrize("per_point_alpha", [False, True])
+def test_assembly_into_packed_form(cuda, per_point_alpha):
+    """Levels that assemble with one lane per row write the packed form directly: run words, slice
+    offsets, packed values, inv_diag and RHS equal the slot-form assembly packed afterwards, bit for
+    bit (with and without the coarse field); levels with several lanes per row pack afterwards."""
+    from ngspsr import solver
+    from ngspsr._lib import lib
+    print(f"[case] packed assembly alpha={'point' if per_point_alpha else 'scalar'}", flush=True)
+    tree, recon, bins, xyz_t = _setup(cuda, per_point_alpha, seed=19)
+    g = torch.Generator(device="cpu").manual_seed(4)
+    direct = 0
+    for d in range(tree.depth):
+        M = tree.get_coords_size(None, d)
+        if M == 0:
+            continue
+        coarse = (torch.randn(xyz_t.shape[0], generator=g) * 0.1).to(cuda) if d < tree.depth - 1 else None
+        base = torch.randn(M, generator=g, dtype=torch.float64).to(cuda)
+        rhs_p, rhs_s = base.clone(), base.clone()
+        A = recon.assemble_level(d, bins[d], coarse, 0.25 ** d, 0.1, rhs_p, form="runs")
+        solver.RunSystem.PACKED = False
+        try:
+            S = recon.assemble_level(d, bins[d], coarse, 0.25 ** d, 0.1, rhs_s, form="runs")
+        finally:
+            solver.RunSystem.PACKED = True
+        torch.cuda.synchronize()
+        assert A.is_packed and not S.is_packed
+        direct += int(lib().ngs_assembly_lanes(int(bins[d].n), M) == 1)
+        Sp = solver.RunSystem(S.runs, S.rvals, S.inv_diag, packed=True)
+        assert torch.equal(A.runs[:, :M], S.runs[:, :M]), d
+        assert torch.equal(A._packed[1], Sp._packed[1]) and torch.equal(A._packed[0], Sp._packed[0]), d
+        assert torch.equal(A.inv_diag, S.inv_diag) and torch.equal(rhs_p, rhs_s), d
+    assert direct >= 1
