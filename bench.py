@@ -377,7 +377,77 @@ def pmc_probe(res):
     for _ in range(5):
         call("ngs_spmv", ptr(csr.rowptr), ptr(csr.cols), ptr(csr.vals), ptr(x), ptr(y), csr.M, stream())
     torch.cuda.synchronize()
-    print(json.dumps({"pmc_probe": True, "M": csr.M, "nnz": csr.nnz(), "copy_bytes": COPY_BYTES}), flush=True)
+    M, nnz = csr.M, csr.nnz()
+    algo = {"spmv_kernel": 8 * nnz + 4 * (M + 1) + 8 * M}
+    layout = {"spmv_kernel": 8 * csr.vals.shape[0] + 4 * (M + 1) + 8 * M}
+    if isinstance(system, RunSystem):
+        runs = int((system.counts() > 0).sum().item())
+        n = res.tree.xyz.shape[0]
+        algo["spmv_runs_kernel"] = algo["spmv_packed_kernel"] = 4 * nnz + 4 * runs + 8 * M
+        algo["lhs_assemble_kernel"] = 4 * nnz + 4 * runs + 32 * M + 52 * n
+        layout["spmv_runs_kernel"] = 608 * M
+        layout["spmv_packed_kernel"] = 100 * M + 4 * nnz + 8 * M + 8 * ((M + 63) // 64 + 1)
+    print(json.dumps({"pmc_probe": True, "M": M, "nnz": nnz, "copy_bytes": COPY_BYTES, "algorithmic_bytes": algo,
+                      "layout_bytes": layout}), flush=True)
+
+
+def pmc_summary(probe_json, fetch_csv, write_csv):
+    """HBM traffic per launch from two rocprofv3 --pmc runs of --pmc-probe (FETCH_SIZE, then
+    WRITE_SIZE; counter_collection CSVs) and the probe's printed line: the 1 GiB copies calibrate
+    both counters (bytes = factor x KiB x 1024, factor = copy bytes / the copy's reading); per
+    kernel, the readings of its launches (the SpMVs' 5 launches; the finest-level assembly is the
+    assembly launch with the largest grid)."""
+    import csv
+    with open(probe_json) as f:
+        probe = json.loads([l for l in f.read().splitlines() if l.startswith("{")][-1])
+
+    def load(path, counter):
+        per = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                name = r["Kernel_Name"]
+                key = next((k for k in ("spmv_packed_kernel", "spmv_runs_kernel", "spmv_kernel", "lhs_assemble_kernel",
+                                        "__amd_rocclr_copyBuffer") if k in name), None)
+                if key is None or (key == "spmv_kernel" and "bpcg" in name):
+                    continue
+                dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 if r.get("End_Timestamp") else None
+                per.setdefault(key, []).append((int(r.get("Grid_Size") or 0), float(r["Counter_Value"]), dur))
+        return per
+
+    fe, wr = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    copy_b = probe["copy_bytes"]
+    f_copy = max(v for _, v, _ in fe["__amd_rocclr_copyBuffer"])
+    w_copy = max(v for _, v, _ in wr["__amd_rocclr_copyBuffer"])
+    ff, wf = copy_b / (f_copy * 1024), copy_b / (w_copy * 1024)
+    out = {"calibration": f"device copies of {copy_b} bytes read FETCH_SIZE = {f_copy} KiB and WRITE_SIZE = {w_copy} KiB "
+                          f"per dispatch: read bytes = {ff:.3f} x FETCH_SIZE x 1024, written = {wf:.3f} x WRITE_SIZE x 1024",
+           "fetch_factor": round(ff, 4), "write_factor": round(wf, 4), "M": probe["M"], "nnz": probe["nnz"],
+           "kernels": {}}
+    for k in ("spmv_packed_kernel", "spmv_runs_kernel", "spmv_kernel", "lhs_assemble_kernel"):
+        if k not in fe or k not in wr:
+            continue
+        if k == "lhs_assemble_kernel":
+            g = max(x[0] for x in fe[k])
+            fk, wk = [x for x in fe[k] if x[0] == g][:1], [x for x in wr[k] if x[0] == g][:1]
+        else:
+            fk, wk = fe[k], wr[k]
+        rd = ff * 1024 * sorted(v for _, v, _ in fk)[len(fk) // 2]
+        wt = wf * 1024 * sorted(v for _, v, _ in wk)[len(wk) // 2]
+        e = {"fetch_size_kib_per_launch": [round(v, 1) for _, v, _ in fk],
+             "write_size_kib_per_launch": [round(v, 1) for _, v, _ in wk],
+             "hbm_read_bytes_per_launch": int(rd), "hbm_write_bytes_per_launch": int(wt),
+             "hbm_bytes_per_launch": int(rd + wt),
+             "launch_us_under_counters": [round(d, 1) for _, _, d in fk if d is not None]}
+        if k in probe.get("algorithmic_bytes", {}):
+            a = probe["algorithmic_bytes"][k]
+            e["algorithmic_bytes_per_launch"] = a
+            e["traffic_over_algorithmic"] = round((rd + wt) / a, 3)
+        if k in probe.get("layout_bytes", {}):
+            e["layout_bytes_per_launch"] = probe["layout_bytes"][k]
+        out["kernels"][k] = e
+    print(json.dumps(out, indent=1))
 
 
 TRAFFIC_FILES = {"c2": "traffic_ellipsoids2M_512.json", "c5": "traffic_room10M_1024.json"}
@@ -880,6 +950,9 @@ def main():
                     help="the plain path once over the serving stream, for a kernel trace (rocprofv3)")
     ap.add_argument("--trace-busy", default=None, help="summarise a rocprofv3 kernel-trace CSV (no GPU)")
     ap.add_argument("--trace-skip", type=int, default=0, help="kernels to skip at the start of --trace-busy")
+    ap.add_argument("--pmc-summary", nargs=3, default=None, metavar=("PROBE_OUT", "FETCH_CSV", "WRITE_CSV"),
+                    help="traffic per launch from the --pmc-probe line and its FETCH_SIZE and WRITE_SIZE counter CSVs "
+                         "(no GPU)")
     ap.add_argument("--trace-levels", default=None,
                     help="per-grid-size table of the PCG, residual and assembly kernels of a rocprofv3 kernel-trace "
                          "CSV (no GPU)")
@@ -889,6 +962,9 @@ def main():
         return
     if args.trace_levels:
         trace_levels(args.trace_levels)
+        return
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the path's multi-GPU sharding (by request for serving, "

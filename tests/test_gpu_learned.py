@@ -63,11 +63,12 @@ def test_bspline_as_additive_equals_bspline_path(cuda):
         _, rr, nb = sb.residual_f64(bb.double(), xa)
         assert (rr / nb) ** 0.5 < 2e-5, (d, (rr / nb) ** 0.5)
         assert float((xa - xb).norm() / xb.norm()) < 1e-4, d
+    # the fields carry the solutions' difference (below 1e-4 relative, above)
     q = xt[::5].contiguous()
     ca, cb = ra.evaluate_chi(q), rb.evaluate_chi(q)
-    assert float((ca - cb).abs().max()) <= 1e-5 * float(cb.abs().max())
+    assert float((ca - cb).abs().max()) <= 1e-4 * float(cb.abs().max())
     ga, gb = ra.evaluate_raw_chi(q, compute_grad=True), rb.evaluate_raw_chi(q, compute_grad=True)
-    assert float((ga - gb).abs().max()) <= 1e-5 * float(gb.abs().max())
+    assert float((ga - gb).abs().max()) <= 1e-4 * float(gb.abs().max())
     va, fa = ra.extract_multiscale_mesh(n_upsample=2, build_o3d_mesh=False)
     vb, fb = rb.extract_multiscale_mesh(n_upsample=2, build_o3d_mesh=False)
     assert abs(fa.shape[0] - fb.shape[0]) <= 0.01 * fb.shape[0] and fb.shape[0] > 1000
