@@ -900,9 +900,10 @@ def trace_busy(path, skip_first=0, marker="spin_kernel"):
                       "top_kernels_s": {k: round(v / 1e9, 4) for k, v in top}}), flush=True)
 
 
-def trace_levels(path, kernels=("bpcg_runs_dot_kernel", "spmv_runs_kernel", "residual_f64_runs_kernel",
+def trace_levels(path, kernels=("bpcg_packed_dot_kernel", "spmv_packed_kernel", "residual_f64_packed_kernel",
+                                  "bpcg_runs_dot_kernel", "spmv_runs_kernel", "residual_f64_runs_kernel",
                                   "bpcg_update_kernel", "bpcg_direction_kernel", "lhs_assemble_kernel",
-                                  "stencil_rows_kernel", "stencil_points_kernel", "bpcg_stencil")):
+                                  "runs_pack_kernel", "stencil_rows_kernel", "stencil_points_kernel", "bpcg_stencil")):
     """Per-launch-shape summary of a rocprofv3 kernel trace (CSV): for each kernel whose name holds
     one of `kernels`, launches grouped by grid size (one group per level system, since the grid
     follows the row count), with count, total and average duration. Prints a markdown table."""
@@ -921,11 +922,23 @@ def trace_levels(path, kernels=("bpcg_runs_dot_kernel", "spmv_runs_kernel", "res
           "min us | max us |")
     print("|---|---|---|---|---|---|---|---|---|")
     for (hit, full, grid), ds in sorted(groups.items(), key=lambda kv: (kv[0][0], -sum(kv[1]))):
-        n, tot, hi = len(ds), sum(ds), max(ds)
         # launches of a converged system return at entry: "working" launches take >= 10% of the longest
-        work = [d for d in ds if d >= 0.1 * hi]
-        print(f"| `{full}` | {grid} | {n} | {tot / 1e6:.3f} | {tot / n / 1e3:.2f} | {len(work)} | "
-              f"{sum(work) / len(work) / 1e3:.2f} | {min(ds) / 1e3:.2f} | {hi / 1e3:.2f} |")
+        hi = max(ds)
+        work = sorted(d for d in ds if d >= 0.1 * hi)
+        # systems of different sizes can share a grid (the block count is capped): split the working
+        # launches at their largest duration gap when it is a factor above 1.5, one row per cluster
+        parts = [work]
+        if len(work) > 1:
+            k = max(range(1, len(work)), key=lambda j: work[j] / max(work[j - 1], 1))
+            if work[k] / max(work[k - 1], 1) > 1.5:
+                parts = [work[k:], work[:k]]
+        idle = [d for d in ds if d < 0.1 * hi]
+        for c, part in enumerate(parts):
+            rows = part + (idle if c == len(parts) - 1 else [])
+            tot = sum(rows)
+            print(f"| `{full}`{' (longer)' if len(parts) > 1 and c == 0 else ' (shorter)' if len(parts) > 1 else ''} | "
+                  f"{grid} | {len(rows)} | {tot / 1e6:.3f} | {tot / len(rows) / 1e3:.2f} | {len(part)} | "
+                  f"{sum(part) / len(part) / 1e3:.2f} | {min(rows) / 1e3:.2f} | {max(rows) / 1e3:.2f} |")
 
 
 def main():

@@ -388,11 +388,19 @@ __global__ void __launch_bounds__(PCG_THREADS) runs_unpack_kernel(const int32_t*
     }
 }
 
+// Orders a wave's LDS writes before its own later reads (each wave stages into its own region, so
+// the block's waves need not wait for each other).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Row `row` (lane of the wave of slice sl) of the packed form times x, the terms in the order of
 // run_row_dot without its zero slots (so the sums are the same, bit for bit). Per batch of 5 planes
 // the wave stages the batch's values (contiguous in pvals) into its LDS region with coalesced loads,
-// then every lane reads its runs there and gathers x. All waves of the block call it in step (the
-// block barrier orders the LDS writes and reads); lanes past M carry empty runs.
+// then every lane reads its runs there and gathers x. Each wave has its own LDS region, ordered by
+// wave-scope fences; lanes past M carry empty runs.
 template <typename T, typename XT>
 __device__ __forceinline__ T packed_row_dot(const NGS_GLOBAL int32_t* __restrict__ runs,
                                             const NGS_GLOBAL float* __restrict__ pvals,
@@ -429,7 +437,7 @@ __device__ __forceinline__ T packed_row_dot(const NGS_GLOBAL int32_t* __restrict
                 if (u * 64 < tot && e < tot) stage[e] = t[u];
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         float v[PACK_BATCH][5];
         XT xv[PACK_BATCH][5];
 #pragma unroll
@@ -447,7 +455,7 @@ __device__ __forceinline__ T packed_row_dot(const NGS_GLOBAL int32_t* __restrict
 #pragma unroll
             for (int j = 0; j < 5; ++j) sum = fma_t<T>((T)v[b][j], (T)xv[b][j], sum);
         pos += tot;
-        __syncthreads();
+        wave_lds_sync();
     }
     return sum;
 }
