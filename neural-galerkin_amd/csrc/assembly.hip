@@ -159,7 +159,8 @@ __global__ void __launch_bounds__(64) lhs_assemble_kernel(ngs_level lv, ngs_poin
                                                        double* __restrict__ rhs, int32_t* __restrict__ runs,
                                                        float* __restrict__ rvals, const int64_t* __restrict__ soff = nullptr) {
     static_assert(!PK || (RUNS && G == 1), "the packed form is written by one lane per row");
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // neighbouring rows share most of their points: each XCD takes a contiguous stretch of rows
+    const int64_t t = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     const int64_t i = t / G;  // the G lanes of a row are consecutive lanes of one wave
     const int gl = (int)(t % G);
     if (i >= lv.M) return;
@@ -292,7 +293,7 @@ template <int G>
 __global__ void __launch_bounds__(256) stencil_diag_rhs_kernel(ngs_stencil op, const float* __restrict__ coarse,
                                                                float sf_delta,
                                                                float* __restrict__ inv_diag, double* __restrict__ rhs) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     const int64_t i = t / G;
     const int gl = (int)(t % G);
     if (i >= op.level.M) return;
@@ -346,7 +347,7 @@ __global__ void __launch_bounds__(256) stencil_diag_rhs_kernel(ngs_stencil op, c
 // coarse voxel at Q + k s_c): rel = k m + (Q - c)/s_f + (m - 1)/2, row = rel + 1.5 m + 0.5.
 __global__ void cross_stiffness_kernel(ngs_level fine, ngs_level coarse, const float* __restrict__ xc, Tables T,
                                        double* __restrict__ rhs) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     if (i >= fine.M) return;
     const int sf_ = fine.stride, sc = coarse.stride, m = T.m;
     int c[3], Q[3];
@@ -461,7 +462,7 @@ __global__ void rhs_normals_scatter_kernel(ngs_level tgt, ngs_level dat, const f
 // order (no atomics).
 __global__ void rhs_normals_same_kernel(ngs_level tgt, ngs_level dat, const float* __restrict__ nrm, Tables T,
                                         double* __restrict__ rhs) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     if (i >= tgt.M) return;
     const int s = tgt.stride;
     const int cx = tgt.coords[3 * i], cy = tgt.coords[3 * i + 1], cz = tgt.coords[3 * i + 2];
@@ -509,7 +510,7 @@ __global__ void rhs_normals_same_kernel(ngs_level tgt, ngs_level dat, const floa
 template <int G>
 __global__ void rhs_normals_fine_gather_kernel(ngs_level tgt, ngs_level dat, const float* __restrict__ nrm, Tables T,
                                                double* __restrict__ rhs) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     const int64_t i = t / G;  // the G lanes of a target are consecutive lanes of one wave
     const int gl = (int)(t % G);
     if (i >= tgt.M) return;

@@ -135,6 +135,15 @@ __device__ __forceinline__ uint32_t grid_column(const ngs_grid& g, int lx, int l
 
 __device__ __forceinline__ int lane64() { return (int)(threadIdx.x & 63); }
 
+// Logical index of this block such that the blocks of each XCD form one contiguous range. Blocks are
+// dispatched round-robin over the 8 XCDs (block b to XCD b % 8), so consecutive blocks land on
+// different XCDs and data shared by neighbouring blocks is fetched into every XCD's L2; with this
+// map each XCD sweeps its own stretch of consecutive blocks. A bijection of [0, gridDim.x).
+__device__ __forceinline__ int64_t xcd_contiguous_block() {
+    const int64_t nb = gridDim.x, b = blockIdx.x, x = b % 8;
+    return x * (nb / 8) + min(x, nb % 8) + b / 8;
+}
+
 // Exclusive prefix over the wave's active lanes and the wave total of n in [0, 7] (inactive lanes
 // count as 0): the positions of the rows' runs in a slice of the packed run form.
 __device__ __forceinline__ void wave_prefix3(int n, int& excl, int& total) {

@@ -596,7 +596,7 @@ __global__ void splat_gather_kernel(ngs_grid g, const int32_t* __restrict__ node
                                     const float* __restrict__ bxyz, const float* __restrict__ brow, double post,
                                     double* __restrict__ acc) {
 #pragma clang fp contract(off)
-    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t node = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     if (node >= count) return;
     const int s = 1 << g.log2_stride;
     const float inv_s = 1.0f / (float)s;

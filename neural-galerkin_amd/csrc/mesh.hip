@@ -384,7 +384,7 @@ struct FieldLevels {
 __global__ void eval_leaf_corners_levels_kernel(FieldLevels fl, const int32_t* __restrict__ leaves, int64_t n,
                                                 int s_leaf, int nu, float scale, float inv_vs, int accumulate,
                                                 float* __restrict__ out) {
-    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t l = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     if (l >= n) return;
     for (int k = 0; k < fl.n; ++k)
         eval_leaf_corners_one(fl.g[k], fl.field[k], leaves, l, s_leaf, nu, scale, inv_vs, accumulate || k > 0, out);
@@ -573,7 +573,7 @@ struct LeafLevels {
 __global__ void dual_gather_cubes_kernel(const int32_t* __restrict__ centres, int64_t dcount, LeafLevels lv,
                                          const float* __restrict__ values, int32_t* __restrict__ acc,
                                          int32_t* __restrict__ cnt) {
-    const int64_t dc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t dc = xcd_contiguous_block() * blockDim.x + threadIdx.x;
     if (dc >= dcount) return;
     const int q[3] = {centres[3 * dc], centres[3 * dc + 1], centres[3 * dc + 2]};
     int32_t ids[8];
