@@ -66,3 +66,30 @@ def test_splat_gather_density_all_points():
          ptr(got), ptr(ws), wsb, stream())
     # the trilinear weights of a point sum to 1
     assert abs(float(got.sum()) - n) < 1e-6 * n
+
+
+def test_splat_gather_duplicated_cloud():
+    """A cloud where most points repeat a few positions: the nodes around those positions gather
+    long bins (one thread walks all of their points), and the sums still equal the scatter's."""
+    from ngspsr._lib import call, lib, ptr, stream
+    from ngspsr.grid import SparseGrid
+    n = 40000
+    xyz, data, scale, _ = _points(n, 21)
+    xyz[: n // 2] = torch.tensor([3.25, 4.5, 5.75], device=xyz.device)   # half the cloud on one point
+    xyz[n // 2: 3 * n // 4] = torch.tensor([7.0, 7.0, 7.0], device=xyz.device)  # a quarter on a lattice node
+    dev = xyz.device
+    g = SparseGrid([-14] * 3, [44] * 3, 1, dev)
+    g.mark_corners(xyz, 1.0)
+    g.finalize()
+    for ncomp in (0, 3):
+        d = data[:, :ncomp].contiguous() if ncomp else None
+        shape = (g.count, ncomp) if ncomp else (g.count,)
+        ref = torch.zeros(shape, dtype=torch.float64, device=dev)
+        got = torch.zeros(shape, dtype=torch.float64, device=dev)
+        call("ngs_splat", g.struct(), ptr(xyz), n, 1.0, ptr(d), ncomp, ptr(scale), None, 0, 1.0, ptr(ref), stream())
+        wsb = lib().ngs_splat_gather_workspace_bytes(n, g.count)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        call("ngs_splat_gather", g.struct(), ptr(g.coords), g.count, ptr(xyz), n, 1.0, ptr(d), ncomp, ptr(scale),
+             None, 0, 1.0, ptr(got), ptr(ws), wsb, stream())
+        r, o = ref.cpu().numpy(), got.cpu().numpy()
+        np.testing.assert_allclose(o, r, rtol=1e-12, atol=1e-14 * np.abs(r).max())
