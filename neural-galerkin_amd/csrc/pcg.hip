@@ -21,6 +21,8 @@
 #include "common.h"
 #include "stencil.h"
 
+#include <cstddef>
+#include <cstring>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -44,6 +46,20 @@ struct PcgState {
     int32_t max_iter, pad;
     PcgSlot slot[2];
 };
+
+// Bytes of the state read back after a launch sequence: rr and bb, then slot 0, one contiguous span.
+constexpr size_t kStatusBytes = offsetof(PcgState, slot) + sizeof(PcgSlot) - offsetof(PcgState, rr);
+static_assert(offsetof(PcgState, bb) == offsetof(PcgState, rr) + sizeof(double), "rr, bb adjacent");
+static_assert(kStatusBytes <= 64, "the status span fits the 64-byte pinned staging buffer");
+
+inline void read_status(const void* bytes, double sc[2], int32_t* iter, int32_t* done) {
+    const char* c = static_cast<const char*>(bytes);
+    std::memcpy(sc, c, sizeof(double) * 2);
+    PcgSlot slot;
+    std::memcpy(&slot, c + (offsetof(PcgState, slot) - offsetof(PcgState, rr)), sizeof(PcgSlot));
+    *iter = slot.iter;
+    *done = slot.done;
+}
 
 struct PcgArgs {
     const int32_t* rowptr;
@@ -1095,18 +1111,17 @@ static int pcg_solve_entry(BatchSys E, int64_t M, double tol, int32_t max_iter, 
                                              : 0;
     int status = NGS_OK;
     int32_t it = 0, done = 0;
+    double sc[2];
     while (true) {
         int r = ngs_pcg_batch_iterate(tab, map, A.nblocks, chunk, g_pcg_use_graph, flags,
                                       reinterpret_cast<ngs_stream_t>(work));
         if (r != NGS_OK) return r;
-        NGS_CHECK_HIP(hipMemcpyAsync(hs.pinned, &A.st->slot[0], sizeof(PcgSlot), hipMemcpyDeviceToHost, work));
+        // one read per chunk: rr, bb and slot 0 (iteration count, done flag)
+        NGS_CHECK_HIP(hipMemcpyAsync(hs.pinned, &A.st->rr, kStatusBytes, hipMemcpyDeviceToHost, work));
         NGS_CHECK_HIP(hipStreamSynchronize(work));
-        it = hs.pinned[0];
-        done = hs.pinned[1];
+        read_status(hs.pinned, sc, &it, &done);
         if (done) break;
     }
-    double sc[2];
-    NGS_CHECK_HIP(hipMemcpy(sc, &A.st->rr, sizeof(double) * 2, hipMemcpyDeviceToHost));  // rr, bb
     (void)hipEventDestroy(ev);
     if (iters) *iters = it;
     if (rel_resid) *rel_resid = sc[1] > 0 ? std::sqrt(sc[0] / sc[1]) : 0.0;
@@ -1248,15 +1263,16 @@ int ngs_pcg_solve_runs_noalloc(const int32_t* runs, const float* rvals, const fl
     fill_map_kernel<<<grid1d(A.nblocks, 256), 256, 0, st>>>(map, A.nblocks);
     NGS_CHECK_LAUNCH();
     PcgSlot slot{};
+    double sc[2];
+    alignas(8) char status_bytes[kStatusBytes];
     while (true) {
         if (int r = ngs_pcg_batch_iterate(tab, map, A.nblocks, chunk, 0, FLAG_RUNS | FLAG_NO_CSR, s)) return r;
-        NGS_CHECK_HIP(hipMemcpyAsync(&slot, &A.st->slot[0], sizeof(PcgSlot), hipMemcpyDeviceToHost, st));
+        // one read per chunk: rr, bb and slot 0 (iteration count, done flag)
+        NGS_CHECK_HIP(hipMemcpyAsync(status_bytes, &A.st->rr, kStatusBytes, hipMemcpyDeviceToHost, st));
         NGS_CHECK_HIP(hipStreamSynchronize(st));
+        read_status(status_bytes, sc, &slot.iter, &slot.done);
         if (slot.done) break;
     }
-    double sc[2];
-    NGS_CHECK_HIP(hipMemcpyAsync(sc, &A.st->rr, sizeof(double) * 2, hipMemcpyDeviceToHost, st));  // rr, bb
-    NGS_CHECK_HIP(hipStreamSynchronize(st));
     if (iters) *iters = slot.iter;
     if (rel_resid) *rel_resid = sc[1] > 0 ? std::sqrt(sc[0] / sc[1]) : 0.0;
     if (slot.done == 2) {
