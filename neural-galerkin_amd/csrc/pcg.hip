@@ -586,20 +586,18 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_runs_kernel(const in
 }
 
 // r = b - A x in FP64 for the packed form (the sums of residual_f64_runs_kernel, bit for bit): blocks
-// take 256-row groups (4 slices), grid-stride.
-__global__ void __launch_bounds__(PCG_THREADS) residual_f64_packed_kernel(const int32_t* __restrict__ runs,
-                                                                          const float* __restrict__ pvals,
-                                                                          const int64_t* __restrict__ soff,
-                                                                          const double* __restrict__ b,
-                                                                          const double* __restrict__ x,
-                                                                          double* __restrict__ r, int64_t M,
-                                                                          double* partials) {
+// take 256-row groups (4 slices), grid-stride. The norms are summed by norms2_kernel afterwards: with
+// its FP64 accumulators the row loop here would hold 2 waves per SIMD instead of 3 (172 VGPRs), and a
+// separate pass over r and b (16 bytes per row) costs less than the lost occupancy.
+__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+residual_f64_packed_kernel(const int32_t* __restrict__ runs, const float* __restrict__ pvals,
+                           const int64_t* __restrict__ soff, const double* __restrict__ b,
+                           const double* __restrict__ x, double* __restrict__ r, int64_t M) {
     __shared__ float stage_all[PCG_THREADS / 64][PACK_STAGE];
     float* stage = stage_all[threadIdx.x >> 6];
     const int64_t step = (int64_t)gridDim.x * PCG_THREADS;
     const int64_t nsl = (M + 63) / 64;
     const int64_t P = run_plane_stride(M);
-    double rr = 0, bb = 0;
     for (int64_t base = (int64_t)blockIdx.x * PCG_THREADS; base < M; base += step) {
         const int64_t row = base + threadIdx.x;
         const bool ok = row < M;
@@ -608,13 +606,20 @@ __global__ void __launch_bounds__(PCG_THREADS) residual_f64_packed_kernel(const 
         const double ax = x ? packed_row_dot<double, double>(as_global(runs), as_global(pvals), as_global(x), P, row,
                                                              ok, pos, stage)
                             : 0.0;
-        if (ok) {
-            const double bi = b[row];
-            const double ri = x ? bi - ax : bi;
-            r[row] = ri;
-            rr += ri * ri;
-            bb += bi * bi;
-        }
+        if (ok) r[row] = x ? b[row] - ax : b[row];
+    }
+}
+
+// Per-block partials of r.r and b.b over contiguous chunks of rows.
+__global__ void __launch_bounds__(PCG_THREADS) norms2_kernel(const double* __restrict__ r, const double* __restrict__ b,
+                                                             int64_t M, int64_t chunk, double* partials) {
+    const int64_t r0 = (int64_t)blockIdx.x * chunk;
+    const int64_t r1 = min(M, r0 + chunk);
+    double rr = 0, bb = 0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_THREADS) {
+        const double ri = r[i], bi = b[i];
+        rr += ri * ri;
+        bb += bi * bi;
     }
     store_partials2(rr, bb, partials, partials + gridDim.x, blockIdx.x);
 }
@@ -1403,7 +1408,10 @@ int ngs_residual_f64_packed(const int32_t* runs, const float* pvals, const int64
     int nb = pcg_blocks(M, &chunk);
     char* w = reinterpret_cast<char*>(workspace);
     double* partials = reinterpret_cast<double*>(w + al256(sizeof(PcgState)));
-    residual_f64_packed_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(runs, pvals, soff, b, x, r, M, partials);
+    const int64_t groups = ((M + 63) / 64 + 3) / 4;
+    residual_f64_packed_kernel<<<(int)std::min<int64_t>(groups, 8192), PCG_THREADS, 0, S(s)>>>(runs, pvals, soff, b, x,
+                                                                                               r, M);
+    norms2_kernel<<<nb, PCG_THREADS, 0, S(s)>>>(r, b, M, chunk, partials);
     norms_finish_kernel<<<1, PCG_THREADS, 0, S(s)>>>(partials, nb, norms);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
