@@ -900,6 +900,129 @@ def trace_busy(path, skip_first=0, marker="spin_kernel"):
                       "top_kernels_s": {k: round(v / 1e9, 4) for k, v in top}}), flush=True)
 
 
+def trace_runs(name, runs=3, warmup=2):
+    """Reconstructions of a single-solve workload for a kernel trace: `warmup` untimed ones, then
+    `runs` with a spin_kernel (torch.cuda._sleep) before and after each, so --trace-stages can cut
+    the trace into reconstructions."""
+    import torch
+    import ngspsr
+    cfg = CONFIGS[name]
+    dev = torch.device("cuda:0")
+    xyz_np, nrm_np = make_scene(cfg)
+    xyz, nrm = torch.from_numpy(xyz_np).to(dev), torch.from_numpy(nrm_np).to(dev)
+    for i in range(warmup + runs):
+        if i >= warmup:
+            torch.cuda._sleep(20000)
+            torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ngspsr.reconstruct(xyz, nrm, cfg["vs"], depth=cfg["depth"], min_density=cfg["min_density"], screen_alpha=4.0,
+                           solver=cfg["solver"])
+        torch.cuda.synchronize()
+        log(f"reconstruction {i}: {1e3 * (time.perf_counter() - t0):.2f} ms")
+    torch.cuda._sleep(20000)
+    torch.cuda.synchronize()
+
+
+TRACE_STAGES = [("PCG SpMV with p.q (bpcg_*_dot)", r"bpcg_.*dot"), ("PCG update and direction", r"bpcg_(update|direction)"),
+                ("PCG set-up", r"pcg_init|fill_map|write_entry"), ("FP64 residual and norms", r"residual_f64|norms"),
+                ("assembly", r"lhs_assemble|lhs_count|runs_pack|runs_slice"),
+                ("RHS and cross-level stiffness", r"rhs_normals|cross_stiff"),
+                ("field at the points, point bins", r"eval_field|gather_f32|bin_points|point_basis|cell_starts"),
+                ("dual mesh", r"dual_|mc_count|mc_emit|dedup|run_heads|eval_leaf|mark_dual|complete_flags|conformal|"
+                              r"unrefined|compact_|leaf_corner|weighted_sum"),
+                ("voxels, levels, splats", r"stage_corners|fold_marks|emit_kernel|grid_rank|levels_kernel|splat|bounds|"
+                                           r"density|mark_coords|clear_words|stage_coords"),
+                ("sorts and scans (rocPRIM)", r"rocprim"), ("torch operations and copies", r"at::|rocclr")]
+
+
+def trace_stages(path):
+    """Kernel time of each reconstruction of a --trace-runs kernel trace (CSV), by stage: the windows
+    between consecutive spin_kernel markers, with span, kernel time and idle time, and the largest
+    kernels of the last window."""
+    import csv
+    import re
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    marks = [i for i, r in enumerate(rows) if "spin_kernel" in r[2]]
+    if len(marks) < 2:
+        print("no spin_kernel markers: record the trace with bench.py --trace-runs")
+        return
+    per = {}
+    for a, b in zip(marks, marks[1:]):
+        w = rows[a + 1:b]
+        if not w:
+            continue
+        span = w[-1][1] - w[0][0]
+        busy, cs, ce = 0, None, None
+        for s_, e_, _ in w:
+            if ce is None or s_ > ce:
+                if ce is not None:
+                    busy += ce - cs
+                cs, ce = s_, e_
+            else:
+                ce = max(ce, e_)
+        busy += ce - cs
+        tot, per = {}, {}
+        for s_, e_, name in w:
+            per[name.split("(")[0][:80]] = per.get(name.split("(")[0][:80], 0) + (e_ - s_)
+            lab = next((lb for lb, pat in TRACE_STAGES if re.search(pat, name)), "other")
+            tot[lab] = tot.get(lab, 0) + (e_ - s_)
+        print(f"reconstruction: {len(w)} kernels, span {span / 1e6:.2f} ms, kernel time "
+              f"{sum(e - s for s, e, _ in w) / 1e6:.2f} ms, idle between kernels {(span - busy) / 1e6:.2f} ms")
+        for lab in [lb for lb, _ in TRACE_STAGES] + ["other"]:
+            print(f"  {lab:34s} {tot.get(lab, 0) / 1e6:7.2f} ms")
+    print("largest kernels of the last reconstruction:")
+    for k, v in sorted(per.items(), key=lambda kv: -kv[1])[:25]:
+        print(f"  {v / 1e6:7.3f} ms  {k}")
+
+
+def counter_summary(path, kernels=("lhs_assemble", "bpcg_packed_dot", "spmv_packed", "residual_f64_packed")):
+    """Per (kernel, grid size) averages of the SQ counters of a rocprofv3 --pmc CSV, per wave of 64
+    lanes, with the median dispatch duration and the VALU issue time per SIMD they imply (summed
+    instruction counts over the 1024 SIMDs, 4 cycles per wave64 VALU instruction, 2.4 GHz). Dispatches of
+    one grid size are split at every duration gap above a factor 1.5 (levels that share the PCG's block
+    cap, launches of a converged system)."""
+    import csv
+    groups = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"]
+            if not any(k in name for k in kernels):
+                continue
+            key = (name.split("(")[0], int(r["Grid_Size"]))
+            d = groups.setdefault(key, {}).setdefault(r["Dispatch_Id"], {"dur": 0, "c": {}})
+            d["dur"] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            d["c"][r["Counter_Name"]] = float(r["Counter_Value"])
+    print("| kernel | grid | dispatches | median us | VALU/wave | SALU/wave | VMEM rd/wave | VMEM wr/wave | LDS/wave | "
+          "wait/wave cycles | VALU issue us/SIMD |")
+    print("|---|---|---|---|---|---|---|---|---|---|---|")
+    for (name, grid), ds in sorted(groups.items(), key=lambda kv: (kv[0][0], -kv[0][1])):
+        ds = sorted(ds.values(), key=lambda d: d["dur"])
+        # split at every duration gap above a factor 1.5 (levels sharing the grid, no-op launches)
+        parts, cur = [], [ds[0]]
+        for a, b in zip(ds, ds[1:]):
+            if b["dur"] / max(a["dur"], 1) > 1.5:
+                parts.append(cur)
+                cur = []
+            cur.append(b)
+        parts.append(cur)
+        for part in reversed(parts):
+            c = {}
+            for d in part:
+                for k_, v in d["c"].items():
+                    c[k_] = c.get(k_, 0.0) + v / len(part)
+            waves = grid / 64
+            med = part[len(part) // 2]["dur"] / 1e3
+            per = lambda k_: f"{c[k_] / waves:.0f}" if k_ in c else "-"
+            wait = f"{c['SQ_WAIT_ANY'] / c['SQ_WAVE_CYCLES']:.2f}" if c.get("SQ_WAVE_CYCLES") and "SQ_WAIT_ANY" in c else "-"
+            issue = f"{c['SQ_INSTS_VALU'] / 1024 * 4 / 2.4e3:.0f}" if "SQ_INSTS_VALU" in c else "-"
+            print(f"| `{name}` | {grid} | {len(part)} | {med:.1f} | {per('SQ_INSTS_VALU')} | {per('SQ_INSTS_SALU')} | "
+                  f"{per('SQ_INSTS_VMEM_RD')} | {per('SQ_INSTS_VMEM_WR')} | {per('SQ_INSTS_LDS')} | {wait} | {issue} |")
+
+
 def trace_levels(path, kernels=("bpcg_packed_dot_kernel", "spmv_packed_kernel", "residual_f64_packed_kernel",
                                   "bpcg_runs_dot_kernel", "spmv_runs_kernel", "residual_f64_runs_kernel",
                                   "bpcg_update_kernel", "bpcg_direction_kernel", "lhs_assemble_kernel",
@@ -966,6 +1089,13 @@ def main():
     ap.add_argument("--pmc-summary", nargs=3, default=None, metavar=("PROBE_OUT", "FETCH_CSV", "WRITE_CSV"),
                     help="traffic per launch from the --pmc-probe line and its FETCH_SIZE and WRITE_SIZE counter CSVs "
                          "(no GPU)")
+    ap.add_argument("--trace-runs", action="store_true",
+                    help="reconstructions of --config (c1, c2, c5) between spin_kernel markers, for a kernel trace")
+    ap.add_argument("--trace-stages", default=None,
+                    help="per-stage kernel time of each reconstruction of a --trace-runs kernel-trace CSV (no GPU)")
+    ap.add_argument("--counter-summary", default=None,
+                    help="per-wave SQ instruction counters of the assembly and SpMV kernels in a rocprofv3 --pmc CSV "
+                         "(no GPU)")
     ap.add_argument("--trace-levels", default=None,
                     help="per-grid-size table of the PCG, residual and assembly kernels of a rocprofv3 kernel-trace "
                          "CSV (no GPU)")
@@ -975,6 +1105,12 @@ def main():
         return
     if args.trace_levels:
         trace_levels(args.trace_levels)
+        return
+    if args.trace_stages:
+        trace_stages(args.trace_stages)
+        return
+    if args.counter_summary:
+        counter_summary(args.counter_summary)
         return
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
@@ -991,6 +1127,9 @@ def main():
         return
     if args.plain_pass:
         plain_pass(args.requests)
+        return
+    if args.trace_runs:
+        trace_runs("c2" if args.config == "c3" else args.config)
         return
     if args.pmc_probe:
         cfg = CONFIGS[args.probe_config]
