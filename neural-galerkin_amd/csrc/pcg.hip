@@ -476,8 +476,10 @@ __device__ __forceinline__ T packed_row_dot(const NGS_GLOBAL int32_t* __restrict
     return sum;
 }
 
-// y = A x for the packed form: blocks of 4 waves take groups of 4 slices, grid-stride.
-__global__ void __launch_bounds__(PCG_THREADS) spmv_packed_kernel(const int32_t* __restrict__ runs,
+// y = A x for the packed form: blocks of 4 waves take groups of 4 slices, grid-stride. Capped at 5 waves
+// per SIMD (95 VGPRs, no spills): 535 against 550 us at 4 waves (107 VGPRs) on the 2M-point finest level;
+// at 6 (80 VGPRs) it spills and runs 568 us.
+__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(5))) spmv_packed_kernel(const int32_t* __restrict__ runs,
                                                                   const float* __restrict__ pvals,
                                                                   const int64_t* __restrict__ soff,
                                                                   const float* __restrict__ x, float* __restrict__ y,
@@ -487,8 +489,9 @@ __global__ void __launch_bounds__(PCG_THREADS) spmv_packed_kernel(const int32_t*
     const int64_t nsl = (M + 63) / 64;
     const int64_t groups = (nsl + PCG_THREADS / 64 - 1) / (PCG_THREADS / 64);
     const int64_t P = run_plane_stride(M);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar slice and offset
     for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
-        const int64_t sl = g * (PCG_THREADS / 64) + (threadIdx.x >> 6);
+        const int64_t sl = g * (PCG_THREADS / 64) + wv;
         const int64_t row = sl * 64 + lane64();
         const bool ok = row < M;
         const int64_t pos = sl < nsl ? soff[sl] : 0;
@@ -816,7 +819,9 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_runs_dot_kernel(const BatchS
 }
 
 // The same for the packed-form systems: each step of a block takes 4 whole slices (256 rows from a
-// multiple of 256), its 4 waves staging their slices' values through LDS in step.
+// multiple of 256), its 4 waves staging their slices' values through LDS in step. It stays at 4 waves
+// per SIMD (121 VGPRs): with spmv_packed_kernel's 5-wave cap it spills (the table's pointers and the
+// p.q accumulator) and ran 615 against 586 us per finest-level launch inside the PCG.
 __global__ void __launch_bounds__(PCG_THREADS) bpcg_packed_dot_kernel(const BatchSys* tab, const int32_t* blk2sys,
                                                                       int ph) {
     __shared__ float stage_all[PCG_THREADS / 64][PACK_STAGE];
