@@ -344,6 +344,7 @@ __global__ void __launch_bounds__(PCG_THREADS) spmv_runs_kernel(const int32_t* _
 // starting at soff[s]. The run words are those of the run form.
 constexpr int PACK_BATCH = 5;                    // planes staged together
 constexpr int PACK_STAGE = PACK_BATCH * 5 * 64;  // floats of LDS per wave
+static_assert(PACK_BATCH == 5, "batch_prefix5 packs the run lengths of 5 planes");
 
 // Values per slice (the sum of its rows' run lengths). One wave per slice.
 __global__ void __launch_bounds__(PCG_THREADS) runs_slice_counts_kernel(const int32_t* __restrict__ runs, int64_t M,
@@ -404,6 +405,41 @@ __global__ void __launch_bounds__(PCG_THREADS) runs_unpack_kernel(const int32_t*
     }
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (every lane active): shifts within rows of 16,
+// then the last lane of rows 0 and 2 into rows 1 and 3, and lane 31 into rows 2 and 3 (DPP).
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); // row_bcast:31
+    return v;
+}
+
+// Positions of a wave's runs in one batch of 5 planes of the packed form: ex[b] = the runs of planes
+// before b in the batch plus those of plane b in lanes below this one; returns the batch total. The
+// 3-bit run lengths go as 9-bit fields (sums <= 320) into two words, planes 0-2 and 3-4, and two wave
+// scans give every field's prefix at once: no ballot masks held in scalar registers, which left the
+// row loop short of them (spilled to VGPR lanes) in the table-driven PCG kernel. Every lane of the
+// wave must be active.
+__device__ __forceinline__ int batch_prefix5(const int32_t (&w)[5], int (&ex)[5]) {
+    int n[5];
+#pragma unroll
+    for (int b = 0; b < 5; ++b) n[b] = (int)((uint32_t)w[b] >> RUN_COUNT_SHIFT);
+    const int a0 = n[0] | (n[1] << 9) | (n[2] << 18), a1 = n[3] | (n[4] << 9);
+    const int i0 = wave_incl_scan(a0), i1 = wave_incl_scan(a1);
+    const int t0 = __builtin_amdgcn_readlane(i0, 63), t1 = __builtin_amdgcn_readlane(i1, 63);
+    const int e0 = i0 - a0, e1 = i1 - a1;
+    const int T0 = t0 & 511, T1 = (t0 >> 9) & 511, T2 = (t0 >> 18) & 511, T3 = t1 & 511, T4 = (t1 >> 9) & 511;
+    ex[0] = e0 & 511;
+    ex[1] = T0 + ((e0 >> 9) & 511);
+    ex[2] = T0 + T1 + ((e0 >> 18) & 511);
+    ex[3] = T0 + T1 + T2 + (e1 & 511);
+    ex[4] = T0 + T1 + T2 + T3 + ((e1 >> 9) & 511);
+    return T0 + T1 + T2 + T3 + T4;
+}
+
 // Orders a wave's LDS writes before its own later reads (each wave stages into its own region, so
 // the block's waves need not wait for each other).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -428,16 +464,9 @@ __device__ __forceinline__ T packed_row_dot(const NGS_GLOBAL int32_t* __restrict
     for (int p0 = 0; p0 < 25; p0 += PACK_BATCH) {
         int32_t w[PACK_BATCH];
         int ex[PACK_BATCH];
-        int tot = 0;
 #pragma unroll
         for (int b = 0; b < PACK_BATCH; ++b) w[b] = ok ? runs[(p0 + b) * P + row] : 0;
-#pragma unroll
-        for (int b = 0; b < PACK_BATCH; ++b) {
-            int e, t;
-            wave_prefix3((int)((uint32_t)w[b] >> RUN_COUNT_SHIFT), e, t);
-            ex[b] = tot + e;
-            tot += t;
-        }
+        const int tot = batch_prefix5(w, ex);
         {
             // all of the batch's loads issued together (one memory round trip per batch): lane l takes
             // values l, l + 64, ... of the batch's segment (tot <= PACK_STAGE)
@@ -477,8 +506,7 @@ __device__ __forceinline__ T packed_row_dot(const NGS_GLOBAL int32_t* __restrict
 }
 
 // y = A x for the packed form: blocks of 4 waves take groups of 4 slices, grid-stride. Capped at 5 waves
-// per SIMD (95 VGPRs, no spills): 535 against 550 us at 4 waves (107 VGPRs) on the 2M-point finest level;
-// at 6 (80 VGPRs) it spills and runs 568 us.
+// per SIMD; with the scan-based run prefixes it needs 80 VGPRs, so the LDS (6 blocks per CU) sets 6.
 __global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(5))) spmv_packed_kernel(const int32_t* __restrict__ runs,
                                                                   const float* __restrict__ pvals,
                                                                   const int64_t* __restrict__ soff,
@@ -819,9 +847,9 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_runs_dot_kernel(const BatchS
 }
 
 // The same for the packed-form systems: each step of a block takes 4 whole slices (256 rows from a
-// multiple of 256), its 4 waves staging their slices' values through LDS in step. It stays at 4 waves
-// per SIMD (121 VGPRs): with spmv_packed_kernel's 5-wave cap it spills (the table's pointers and the
-// p.q accumulator) and ran 615 against 586 us per finest-level launch inside the PCG.
+// multiple of 256), its 4 waves staging their slices' values through LDS in step. It runs at 4 waves per
+// SIMD (118 VGPRs): capped at 5 (96 VGPRs, no spills) it took 606 against 586 us per finest-level launch
+// inside the PCG, where the previous kernel's writes of p are still arriving.
 __global__ void __launch_bounds__(PCG_THREADS) bpcg_packed_dot_kernel(const BatchSys* tab, const int32_t* blk2sys,
                                                                       int ph) {
     __shared__ float stage_all[PCG_THREADS / 64][PACK_STAGE];
