@@ -848,8 +848,8 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_runs_dot_kernel(const BatchS
 
 // The same for the packed-form systems: each step of a block takes 4 whole slices (256 rows from a
 // multiple of 256), its 4 waves staging their slices' values through LDS in step. It runs at 4 waves per
-// SIMD (118 VGPRs): capped at 5 (96 VGPRs, no spills) it took 606 against 586 us per finest-level launch
-// inside the PCG, where the previous kernel's writes of p are still arriving.
+// SIMD (118 VGPRs): capped at 5 (96 VGPRs, no spills) it took 606 against 574 us per finest-level launch
+// inside the PCG, unlike the standalone SpMV, which gains from 6.
 __global__ void __launch_bounds__(PCG_THREADS) bpcg_packed_dot_kernel(const BatchSys* tab, const int32_t* blk2sys,
                                                                       int ph) {
     __shared__ float stage_all[PCG_THREADS / 64][PACK_STAGE];
