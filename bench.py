@@ -749,7 +749,7 @@ def single_solve(name, steps, warmup, cpu=True, ext=True, fp64=True):
         iter_bytes = algo_run + 44 * system.M
         roof = {"kernel": ("spmv_packed_kernel (finest-level SpMV, packed run form: one run word per bitmap "
                            "column, the values of 64-row slices packed and staged through LDS; the PCG's "
-                           "bpcg_packed_dot_kernel runs the same row loop with the p.q partials at 4 waves per SIMD "
+                           "bpcg_packed_dot_kernel runs the same row loop with the p.q partials at 5 waves per SIMD "
                            "instead of 6, about 8% slower per launch)") if system.is_packed
                 else "spmv_runs_kernel (finest-level SpMV, run form: one run word per bitmap column, values "
                      "plane-major)", "bound": "hbm", "achieved": round(algo_run / t_run / 1e9, 1),

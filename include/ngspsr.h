@@ -376,7 +376,7 @@ int ngs_pcg_solve_runs(const int32_t *runs, const float *rvals, const float *inv
  * array, so calls that overlap on one GPU (other threads or streams) race; serialise them. The
  * stream must belong to the current device. */
 #define NGS_PCG_NOALLOC_SCRATCH_BYTES (32ll << 20)
-#define NGS_PCG_NOALLOC_MAX_ROWS 2791360ll
+#define NGS_PCG_NOALLOC_MAX_ROWS 2790144ll
 int ngs_pcg_solve_runs_noalloc(const int32_t *runs, const float *rvals, const float *inv_diag, const float *b,
                                float *x, int64_t M, double tol, int32_t max_iter, int32_t chunk, ngs_stream_t s,
                                int32_t *iters, double *rel_resid);

@@ -704,9 +704,11 @@ __global__ void __launch_bounds__(PCG_THREADS) stencil_residual_kernel(ngs_stenc
 
 static int pcg_blocks(int64_t M, int64_t* chunk) {
     // ~256 rows per block (8 rows per 8-lane group): small levels get enough blocks that each
-    // group walks only a few rows, so their iterations are not bound by a long latency chain
+    // group walks only a few rows, so their iterations are not bound by a long latency chain. At most
+    // 2560 = 2 x 256 CUs x 5: the grid-stride SpMV of large systems runs in two full rounds of resident
+    // blocks (bpcg_packed_dot_kernel holds 5 blocks per CU)
     int64_t nb = (M + 255) / 256;
-    if (nb > 2048) nb = 2048;
+    if (nb > 2560) nb = 2560;
     if (nb >= NUM_XCD) nb = (nb / NUM_XCD) * NUM_XCD;
     if (nb < 1) nb = 1;
     // whole 64-row groups per block: the vector loads and stores of each wave are aligned 256-byte
@@ -847,10 +849,12 @@ __global__ void __launch_bounds__(PCG_THREADS) bpcg_runs_dot_kernel(const BatchS
 }
 
 // The same for the packed-form systems: each step of a block takes 4 whole slices (256 rows from a
-// multiple of 256), its 4 waves staging their slices' values through LDS in step. It runs at 4 waves per
-// SIMD (118 VGPRs): capped at 5 (96 VGPRs, no spills) it took 606 against 574 us per finest-level launch
-// inside the PCG, unlike the standalone SpMV, which gains from 6.
-__global__ void __launch_bounds__(PCG_THREADS) bpcg_packed_dot_kernel(const BatchSys* tab, const int32_t* blk2sys,
+// multiple of 256), its 4 waves staging their slices' values through LDS in step. Capped at 5 waves per
+// SIMD (96 VGPRs, no spills; 118 uncapped). The block count of large systems (pcg_blocks) is two full
+// rounds of 5 blocks per CU, so the cap does not leave a partial last round: the finest level runs as at
+// 4 waves (574 us), the coarse ones faster (55k rows: 31 -> 27 us per launch). With 2048 blocks the cap
+// had cost 606 against 574 us at the finest level.
+__global__ void __launch_bounds__(PCG_THREADS) __attribute__((amdgpu_waves_per_eu(5))) bpcg_packed_dot_kernel(const BatchSys* tab, const int32_t* blk2sys,
                                                                       int ph) {
     __shared__ float stage_all[PCG_THREADS / 64][PACK_STAGE];
     int lb;

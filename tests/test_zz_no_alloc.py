@@ -72,8 +72,8 @@ def test_noalloc_pcg_rejects_oversize(cuda):
     from ngspsr._lib import lib
     fake = ctypes.c_void_p(256)
     it, rr = ctypes.c_int32(), ctypes.c_double()
-    st = lib().ngs_pcg_solve_runs_noalloc(fake, fake, fake, fake, fake, 2791360 + 1, 1e-6, 10, 32, None,
+    st = lib().ngs_pcg_solve_runs_noalloc(fake, fake, fake, fake, fake, 2790144 + 1, 1e-6, 10, 32, None,
                                           ctypes.byref(it), ctypes.byref(rr))
     assert st == 6
     msg = lib().ngs_last_error().decode()
-    assert "2791360 rows" in msg and "scratch" in msg
+    assert "2790144 rows" in msg and "scratch" in msg
