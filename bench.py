@@ -861,6 +861,47 @@ def plain_pass(n_req=256):
                       "end_monotonic_ns": time.monotonic_ns()}), flush=True)
 
 
+def batched_pass(n_req=256, workers=8, chunk=8, warmup=2):
+    """The batched server once over the serving stream (all requests queued at t = 0, after `warmup`
+    passes), for a kernel trace: prints the wall time of the pass. Run under rocprofv3 --kernel-trace;
+    --trace-busy summarises the window between the two spin-kernel markers."""
+    import threading
+    import torch
+    from ngspsr import serving
+    dev = torch.device("cuda:0")
+    data = [(torch.from_numpy(x).to(dev), torch.from_numpy(n).to(dev), vs) for x, n, vs in serving_scenes(n_req)]
+    # the profiler's launch hooks run on the server's worker threads and overran their default 8 MiB
+    # stacks (a fault 1 MiB below a worker's stack); threads made from here on get 64 MiB
+    threading.stack_size(64 << 20)
+    srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, refine=True, chunk=chunk)
+
+    def one_pass():
+        reqs = [serving.Request(i, x, n, vs) for i, (x, n, vs) in enumerate(data)]
+        # as in the timed passes: the mesh is handed off (dropped) as each request finishes
+        srv.serve(reqs, t0=time.perf_counter(), on_complete=lambda r: setattr(r, "result", None))
+        torch.cuda.synchronize()
+        errs = [r for r in reqs if r.error is not None]
+        if errs:
+            srv.close()
+            raise errs[0].error
+
+    for _ in range(warmup):
+        one_pass()
+    torch.cuda._sleep(1000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t0_ns = time.monotonic_ns()
+    one_pass()
+    wall = time.perf_counter() - t0
+    torch.cuda._sleep(1000)
+    torch.cuda.synchronize()
+    launches = srv.pool.launches
+    srv.close()
+    print(json.dumps({"batched_pass": True, "requests": n_req, "workers": workers, "pool_chunk": chunk,
+                      "wall_s": round(wall, 4), "solves_per_s": round(n_req / wall, 3), "pool_launches": launches,
+                      "start_monotonic_ns": t0_ns, "end_monotonic_ns": time.monotonic_ns()}), flush=True)
+
+
 def trace_busy(path, skip_first=0, marker="spin_kernel"):
     """GPU-busy summary of a rocprofv3 kernel trace (CSV): the union of kernel execution intervals
     over the span from the first to the last kernel, the sum of kernel durations, and the largest
@@ -1085,6 +1126,8 @@ def main():
     ap.add_argument("--probe-config", default="c2", choices=sorted(CONFIGS), help="workload of --pmc-probe")
     ap.add_argument("--plain-pass", action="store_true",
                     help="the plain path once over the serving stream, for a kernel trace (rocprofv3)")
+    ap.add_argument("--batched-pass", action="store_true",
+                    help="one traced pass of the batched server over the serving stream (for --trace-busy)")
     ap.add_argument("--trace-busy", default=None, help="summarise a rocprofv3 kernel-trace CSV (no GPU)")
     ap.add_argument("--trace-skip", type=int, default=0, help="kernels to skip at the start of --trace-busy")
     ap.add_argument("--pmc-summary", nargs=3, default=None, metavar=("PROBE_OUT", "FETCH_CSV", "WRITE_CSV"),
@@ -1128,6 +1171,9 @@ def main():
         return
     if args.plain_pass:
         plain_pass(args.requests)
+        return
+    if args.batched_pass:
+        batched_pass(args.requests)
         return
     if args.trace_runs:
         trace_runs("c2" if args.config == "c3" else args.config)

@@ -430,9 +430,11 @@ int ngs_pcg_batch_prepare_stencil(ngs_stencil op, const float *inv_diag, const f
  * that SpMV), bit 3 it may hold packed-form systems (adds their SpMV). */
 int ngs_pcg_batch_iterate(const void *table, const int32_t *blk2sys, int32_t total_blocks, int32_t iters,
                           int32_t use_graph, int32_t flags, ngs_stream_t s);
-/* (iter, done) of each system of the host copy of the table into states (2 nsys int32, host);
+/* (iter, done) of each system of the host copy of the table into states (2 nsys int32, host),
+ * gathered on the device into dev_states (2 nsys int32, device scratch) and copied once;
  * synchronises s. done = 1 converged, 2 iteration cap. */
-int ngs_pcg_batch_status(const void *host_table, int32_t nsys, int32_t *states, ngs_stream_t s);
+int ngs_pcg_batch_status(const void *host_table, int32_t nsys, int32_t *dev_states, int32_t *states,
+                         ngs_stream_t s);
 
 /* y = A x (FP32) on the run form (terms in increasing column order), exposed for measurement. */
 int ngs_spmv_runs(const int32_t *runs, const float *rvals, const float *x, float *y, int64_t M, ngs_stream_t s);

@@ -789,6 +789,22 @@ __device__ __forceinline__ const BatchSys* batch_lookup(const BatchSys* tab, con
     return B;
 }
 
+// (iter, done) of slot 0 of up to kStatusGather systems, whose slot addresses come by value: one
+// launch and one copy to the host per status read instead of one copy per system.
+constexpr int kStatusGather = 128;
+struct StatusSlots {
+    const PcgSlot* p[kStatusGather];
+};
+
+__global__ void __launch_bounds__(kStatusGather) batch_status_kernel(StatusSlots slots, int32_t n, int32_t* out) {
+    const int i = threadIdx.x;
+    if (i < n) {
+        const PcgSlot sl = *slots.p[i];
+        out[2 * i] = sl.iter;
+        out[2 * i + 1] = sl.done;
+    }
+}
+
 // Kernels of one iteration step take the step parity `ph` (0, 1): the state slot they read.
 
 // q = A p and per-block p.q partials for the CSR systems of the table (stencil systems:
@@ -1672,12 +1688,24 @@ int ngs_pcg_batch_iterate(const void* table, const int32_t* blk2sys, int32_t tot
 }
 
 // Copy (iter, done) of each system in the table to host memory states[2*i], states[2*i+1]
-// through the device scratch `dev_states` (2*nsys int32); synchronises `s`.
-int ngs_pcg_batch_status(const void* host_table, int32_t nsys, int32_t* states, ngs_stream_t s) {
+// through the device scratch `dev_states` (2*nsys int32): one gather launch per 128 systems and
+// one copy; synchronises `s`.
+int ngs_pcg_batch_status(const void* host_table, int32_t nsys, int32_t* dev_states, int32_t* states,
+                         ngs_stream_t s) {
+    NGS_REQUIRE(nsys >= 0, "ngs_pcg_batch_status: negative system count");
+    if (nsys == 0) return NGS_OK;
+    NGS_REQUIRE(host_table && dev_states && states, "ngs_pcg_batch_status: null argument");
     const BatchSys* tab = reinterpret_cast<const BatchSys*>(host_table);
-    for (int i = 0; i < nsys; ++i)
-        NGS_CHECK_HIP(hipMemcpyAsync(states + 2 * i, &tab[i].a.st->slot[0], sizeof(PcgSlot), hipMemcpyDeviceToHost, S(s)));
-    NGS_CHECK_HIP(hipStreamSynchronize(S(s)));
+    hipStream_t st = S(s);
+    for (int i0 = 0; i0 < nsys; i0 += ngs::kStatusGather) {
+        const int m = std::min(ngs::kStatusGather, nsys - i0);
+        ngs::StatusSlots sl{};
+        for (int k = 0; k < m; ++k) sl.p[k] = &tab[i0 + k].a.st->slot[0];
+        ngs::batch_status_kernel<<<1, ngs::kStatusGather, 0, st>>>(sl, m, dev_states + 2 * (int64_t)i0);
+        NGS_CHECK_LAUNCH();
+    }
+    NGS_CHECK_HIP(hipMemcpyAsync(states, dev_states, 2 * sizeof(int32_t) * (size_t)nsys, hipMemcpyDeviceToHost, st));
+    NGS_CHECK_HIP(hipStreamSynchronize(st));
     return NGS_OK;
 }
 

@@ -5,13 +5,40 @@
 #include <rocprim/iterator/counting_iterator.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace ngs {
 
+namespace {
+// rocPRIM sorts up to 1M items by merge sort whatever the key width. For keys of at most 24 bits
+// (three 8-bit digit passes) its onesweep radix path is faster from 256k items on gfx950 (u32 keys
+// of 14-20 bits with int32 values: 300k items 115 -> 94 us, 1M items 160 -> 102 us; at 100k items
+// merge sort is faster, 50 against 90 us). Both sorts are stable, so the output is the same.
+using onesweep_only = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                 rocprim::default_config, 0>;
+constexpr int64_t kOnesweepMinItems = int64_t(1) << 18;
+constexpr int kOnesweepMaxBits = 24;
+}  // namespace
+
+// A size query (tmp == nullptr) returns enough for either algorithm at up to `bits` key bits, so a
+// caller may size its workspace for wider keys than it then sorts.
 int sort_pairs_u32(const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout, int64_t n, int bits,
                    void* tmp, size_t* tmp_bytes, hipStream_t st) {
-    hipError_t e = rocprim::radix_sort_pairs(tmp, *tmp_bytes, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+    hipError_t e;
+    if (tmp == nullptr) {
+        size_t a = 0, b = 0;
+        e = rocprim::radix_sort_pairs(nullptr, a, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+        if (e == hipSuccess && n >= kOnesweepMinItems)
+            e = rocprim::radix_sort_pairs<onesweep_only>(nullptr, b, kin, kout, vin, vout, (size_t)n, 0,
+                                                         std::min(bits, kOnesweepMaxBits), st);
+        *tmp_bytes = std::max(a, b);
+    } else if (n >= kOnesweepMinItems && bits <= kOnesweepMaxBits) {
+        e = rocprim::radix_sort_pairs<onesweep_only>(tmp, *tmp_bytes, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+    } else {
+        e = rocprim::radix_sort_pairs(tmp, *tmp_bytes, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+    }
     if (e != hipSuccess) {
         set_error(std::string("radix_sort_pairs(u32): ") + hipGetErrorString(e));
         return NGS_ERR_HIP;

@@ -139,7 +139,7 @@ _SIGNATURES = {
     "ngs_stencil_pass": ([Stencil, P, P, c_int32, P], c_int32),
     "ngs_stencil_row_counts": ([Level, P, P], c_int32),
     "ngs_stencil_residual_f64": ([Stencil, P, P, P, P, P, P, P], c_int32),
-    "ngs_pcg_batch_status": ([P, c_int32, P, P], c_int32),
+    "ngs_pcg_batch_status": ([P, c_int32, P, P, P], c_int32),
     "ngs_spmv": ([P, P, P, P, P, c_int64, P], c_int32),
     "ngs_residual_f64": ([P, P, P, P, P, P, c_int64, P, P, P], c_int32),
     "ngs_conformal_children": ([P, c_int64, c_int32, Grid, c_int32, P, P, P], c_int32),

@@ -74,6 +74,7 @@ class BatchedPCG:
         self.host_map = torch.full((self.cap,), -1, dtype=torch.int32).pin_memory()
         self.dev_map = torch.full((self.cap,), -1, dtype=torch.int32, device=self.device)
         self.states = torch.zeros(2 * self.max_sys, dtype=torch.int32).pin_memory()
+        self.dev_states = torch.zeros(2 * self.max_sys, dtype=torch.int32, device=self.device)
         self.stream = torch.cuda.Stream(self.device)
         self.pending = queue.Queue()
         self.active = []
@@ -178,8 +179,8 @@ class BatchedPCG:
                     self.launches += 1
                     self.iterations += self.chunk
                     n = len(self.active)
-                    call("ngs_pcg_batch_status", self.host_table.data_ptr(), n, self.states.data_ptr(),
-                         self.stream.cuda_stream)
+                    call("ngs_pcg_batch_status", self.host_table.data_ptr(), n, ptr(self.dev_states),
+                         self.states.data_ptr(), self.stream.cuda_stream)
                     self.busy_s += time.perf_counter() - t0
                     st = self.states.numpy()[:2 * n].reshape(n, 2)
                     keep = []

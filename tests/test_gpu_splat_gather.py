@@ -93,3 +93,24 @@ def test_splat_gather_duplicated_cloud():
              None, 0, 1.0, ptr(got), ptr(ws), wsb, stream())
         r, o = ref.cpu().numpy(), got.cpu().numpy()
         np.testing.assert_allclose(o, r, rtol=1e-12, atol=1e-14 * np.abs(r).max())
+
+
+def test_splat_gather_large_cloud():
+    """400k points (their bin sort takes the onesweep radix path): the sums equal the scatter's."""
+    from ngspsr._lib import call, lib, ptr, stream
+    from ngspsr.grid import SparseGrid
+    n = 400_000
+    xyz, data, scale, _ = _points(n, 33)
+    dev = xyz.device
+    g = SparseGrid([-14] * 3, [44] * 3, 1, dev)
+    g.mark_corners(xyz, 1.0)
+    g.finalize()
+    ref = torch.zeros((g.count, 3), dtype=torch.float64, device=dev)
+    got = torch.zeros((g.count, 3), dtype=torch.float64, device=dev)
+    call("ngs_splat", g.struct(), ptr(xyz), n, 1.0, ptr(data), 3, ptr(scale), None, 0, 1.0, ptr(ref), stream())
+    wsb = lib().ngs_splat_gather_workspace_bytes(n, g.count)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    call("ngs_splat_gather", g.struct(), ptr(g.coords), g.count, ptr(xyz), n, 1.0, ptr(data), 3, ptr(scale), None, 0,
+         1.0, ptr(got), ptr(ws), wsb, stream())
+    r, o = ref.cpu().numpy(), got.cpu().numpy()
+    np.testing.assert_allclose(o, r, rtol=1e-12, atol=1e-14 * np.abs(r).max())
