@@ -864,15 +864,14 @@ def plain_pass(n_req=256):
 def batched_pass(n_req=256, workers=8, chunk=8, warmup=2):
     """The batched server once over the serving stream (all requests queued at t = 0, after `warmup`
     passes), for a kernel trace: prints the wall time of the pass. Run under rocprofv3 --kernel-trace;
-    --trace-busy summarises the window between the two spin-kernel markers."""
-    import threading
+    --trace-busy summarises the window between the two spin-kernel markers. Under rocprofv3 7.2's
+    kernel trace this multi-threaded pass has ended in a host-side SIGSEGV inside the profiler's
+    launch hooks in two of three runs (in a request thread's rocPRIM launch, then in the pool's graph
+    launch); the unprofiled server runs the same launches in every bench pass."""
     import torch
     from ngspsr import serving
     dev = torch.device("cuda:0")
     data = [(torch.from_numpy(x).to(dev), torch.from_numpy(n).to(dev), vs) for x, n, vs in serving_scenes(n_req)]
-    # the profiler's launch hooks run on the server's worker threads and overran their default 8 MiB
-    # stacks (a fault 1 MiB below a worker's stack); threads made from here on get 64 MiB
-    threading.stack_size(64 << 20)
     srv = serving.ReconstructionServer(dev, workers=workers, tol=1e-6, refine=True, chunk=chunk)
 
     def one_pass():
