@@ -969,7 +969,7 @@ TRACE_STAGES = [("PCG SpMV with p.q (bpcg_*_dot)", r"bpcg_.*dot"), ("PCG update 
                 ("assembly", r"lhs_assemble|lhs_count|runs_pack|runs_slice"),
                 ("RHS and cross-level stiffness", r"rhs_normals|cross_stiff"),
                 ("field at the points, point bins", r"eval_field|gather_f32|bin_points|point_basis|cell_starts"),
-                ("dual mesh", r"dual_|mc_count|mc_emit|dedup|run_heads|eval_leaf|mark_dual|complete_flags|conformal|"
+                ("dual mesh", r"dual_|mc_count|mc_emit|remap_slots|dedup|run_heads|eval_leaf|mark_dual|complete_flags|conformal|"
                               r"unrefined|compact_|leaf_corner|weighted_sum"),
                 ("voxels, levels, splats", r"stage_corners|fold_marks|emit_kernel|grid_rank|levels_kernel|splat|bounds|"
                                            r"density|mark_coords|clear_words|stage_coords"),

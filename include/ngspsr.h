@@ -583,13 +583,21 @@ int ngs_eval_leaf_corners_levels(int32_t nlevels, const ngs_grid *grids, const f
 int ngs_eval_leaf_corners(ngs_grid g, const float *field, const int32_t *leaves, int64_t n, int32_t leaf_stride,
                           int32_t n_upsample, float scale, float inv_voxel_size, int32_t accumulate, float *out,
                           ngs_stream_t s);
-/* Dual marching cubes (csrc/marching_cubes/dual_mc.cu:33-189): per-cube vertex counts, then per
- * triangle vertex (T x 3, cube order then case-table order) its key, larger corner id << id_bits |
- * smaller id (ids < 2^id_bits), and a tag, vertex index | (the interpolation's first corner is the
- * larger id) << 31. */
-int ngs_mc_count(const int32_t *cubes, int64_t k, const float *values, int32_t *vcount, ngs_stream_t s);
-int ngs_mc_emit(const int32_t *cubes, int64_t k, const float *values, const int32_t *voffset, int32_t id_bits,
-                uint64_t *vkeys, uint32_t *vtags, ngs_stream_t s);
+/* Dual marching cubes (csrc/marching_cubes/dual_mc.cu:33-189): per cube its triangle-vertex count
+ * (vcount) and, when ecount is not NULL, the number of distinct cube edges its triangles use; then,
+ * from the exclusive scans voffset and eoffset, per distinct edge of each cube (in the order the
+ * case's triangles first use them) its key, larger corner id << id_bits | smaller id (ids <
+ * 2^id_bits), and a tag, slot index | (the interpolation's first corner is the larger id) << 31,
+ * and per triangle vertex (T x 3, cube order then case-table order) the slot of its edge (vslot).
+ * ngs_dedup_vertices on the slots gives the rank of each slot; ngs_mc_remap_faces turns vslot
+ * into the faces, in place. */
+int ngs_mc_count(const int32_t *cubes, int64_t k, const float *values, int32_t *vcount, int32_t *ecount,
+                 ngs_stream_t s);
+int ngs_mc_emit(const int32_t *cubes, int64_t k, const float *values, const int32_t *voffset, const int32_t *eoffset,
+                int32_t id_bits, uint64_t *ekeys, uint32_t *etags, int64_t *vslot, ngs_stream_t s);
+/* faces[v] = slot_rank[faces[v]] for v < n (no counterpart in the reference: the second half of its
+ * per-vertex de-duplication, reconstructor.py:379-383, once the keys are de-duplicated per slot). */
+int ngs_mc_remap_faces(int64_t *faces, int64_t n, const int64_t *slot_rank, ngs_stream_t s);
 /* ---- Primal sparse marching cubes (marching_cubes_sparse_cuda, primal_mc.cu:176-234) ----
  * Input: nleaf leaves, each with a base corner base[3l..3l+2] (integer-valued floats, in leaf units)
  * and an (r+1)^3 block of samples sdf[l (r+1)^3 + (x (r+1) + y)(r+1) + z]. Sub-cube (rx, ry, rz) of a
@@ -648,8 +656,8 @@ int ngs_additive_eval_backward(const float *xyz, const int64_t *feat_ids, int64_
 int ngs_screen_pairs(int64_t na, const int32_t *a_pt, const int32_t *a_ids, const float *a_vals,
                      const int32_t *b_start, const int32_t *b_ids, const float *b_vals, const float *w,
                      float w_scalar, const int32_t *rowptr, const int32_t *cols, float *out, ngs_stream_t s);
-/* Vertex de-duplication (reconstructor.py:379-383): sorted unique keys; faces (T*3) = rank of each
- * key; the vertex of a key is interpolated (dual_mc.cu:20-31 sdf_interp) on its edge, in the corner
+/* Vertex de-duplication (reconstructor.py:379-383): sorted unique keys; faces[tag index] = rank of
+ * the element's key (per triangle vertex, or per slot of ngs_mc_emit); the vertex of a key is interpolated (dual_mc.cu:20-31 sdf_interp) on its edge, in the corner
  * order of the key's first occurrence, from the corner positions pos (3 per id) and values — the
  * position the reference's per-triangle vertex has at that occurrence. *nverts (device int64).
  * The keys are radix-sorted on their 2 id_bits low bits. */

@@ -398,11 +398,25 @@ __device__ __forceinline__ int cube_case(const int32_t* ids, const float* __rest
     return c;
 }
 
+// Distinct cube edges that case c's triangles use.
+__device__ __forceinline__ int case_edge_count(int c) {
+    const uint64_t w = c_mc_cases[c];
+    uint32_t seen = 0;
+    for (int n = 0; n < 15; ++n) {
+        const int e = (int)((w >> (4 * n)) & 0xF);
+        if (e == 0xF) break;
+        seen |= 1u << e;
+    }
+    return __popc(seen);
+}
+
 __global__ void mc_count_kernel(const int32_t* __restrict__ cubes, int64_t k, const float* __restrict__ val,
-                                int32_t* __restrict__ vc) {
+                                int32_t* __restrict__ vc, int32_t* __restrict__ ec) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k) return;
-    vc[i] = case_count(cube_case(cubes + 8 * i, val));
+    const int c = cube_case(cubes + 8 * i, val);
+    vc[i] = case_count(c);
+    if (ec) ec[i] = case_edge_count(c);
 }
 
 // dual_mc.cu:20-31 semantics: snap to an endpoint whose value is within 1e-5 of zero, or to p1
@@ -419,26 +433,46 @@ __device__ __forceinline__ void sdf_interp(const float* p1, const float* p2, flo
     out[2] = p1[2] * w1 + p2[2] * w2;
 }
 
-// Per triangle vertex: key (larger id << id_bits | smaller id) and tag = vertex index | (ia > ib) << 31,
-// where (ia, ib) are the edge's corners in sdf_interp argument order. Positions are computed later,
-// once per distinct key, from its first occurrence (the tag's orientation bit restores the order).
+// Per distinct edge of a cube (slots from eoff[i], in the order the case's triangles first use the
+// edges): key (larger id << id_bits | smaller id) and tag = slot | (ia > ib) << 31, where (ia, ib) are
+// the edge's corners in sdf_interp argument order; per triangle vertex (from voff[i], cube order then
+// case-table order) the slot of its edge in vslot. The first slot of a key in slot order is the edge
+// of the key's first triangle vertex in vertex order (same cube, first-used edge), so positions
+// computed once per distinct key from its first slot are those of the per-vertex form.
 __global__ void mc_emit_kernel(const int32_t* __restrict__ cubes, int64_t k, const float* __restrict__ val,
-                               const int32_t* __restrict__ voff, int id_bits, uint64_t* __restrict__ keys,
-                               uint32_t* __restrict__ tags) {
+                               const int32_t* __restrict__ voff, const int32_t* __restrict__ eoff, int id_bits,
+                               uint64_t* __restrict__ keys, uint32_t* __restrict__ tags,
+                               int64_t* __restrict__ vslot) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k) return;
     const int32_t* ids = cubes + 8 * i;
     const int c = cube_case(ids, val);
     uint64_t w = c_mc_cases[c];
     int64_t v = voff[i];
+    const int64_t e0 = eoff[i];
+    uint64_t slot_of = 0;  // 4 bits per cube edge: its slot within the cube
+    uint32_t seen = 0;
+    int ns = 0;
     for (int n = 0; n < 15; ++n, ++v) {
         int e = (int)((w >> (4 * n)) & 0xF);
         if (e == 0xF) break;
-        const int ia = ids[c_edge[e][0]], ib = ids[c_edge[e][1]];
-        const uint32_t hi = (uint32_t)max(ia, ib), lo = (uint32_t)min(ia, ib);
-        keys[v] = ((uint64_t)hi << id_bits) | lo;
-        tags[v] = (uint32_t)v | (ia > ib ? 0x80000000u : 0u);
+        if (!((seen >> e) & 1u)) {
+            seen |= 1u << e;
+            slot_of |= (uint64_t)ns << (4 * e);
+            const int ia = ids[c_edge[e][0]], ib = ids[c_edge[e][1]];
+            const uint32_t hi = (uint32_t)max(ia, ib), lo = (uint32_t)min(ia, ib);
+            keys[e0 + ns] = ((uint64_t)hi << id_bits) | lo;
+            tags[e0 + ns] = (uint32_t)(e0 + ns) | (ia > ib ? 0x80000000u : 0u);
+            ++ns;
+        }
+        vslot[v] = e0 + (int64_t)((slot_of >> (4 * e)) & 0xFu);
     }
+}
+
+// faces[v] <- rank[faces[v]] (slot -> vertex rank), in place.
+__global__ void remap_slots_kernel(int64_t* __restrict__ faces, int64_t n, const int64_t* __restrict__ rank) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) faces[v] = rank[faces[v]];
 }
 
 __global__ void run_heads_kernel(const uint64_t* __restrict__ k, int64_t n, int32_t* __restrict__ head) {
@@ -954,18 +988,29 @@ int ngs_dual_corner_positions(const int32_t* primal, int64_t n, int32_t stride, 
     return NGS_OK;
 }
 
-int ngs_mc_count(const int32_t* cubes, int64_t k, const float* values, int32_t* vcount, ngs_stream_t s) {
+int ngs_mc_count(const int32_t* cubes, int64_t k, const float* values, int32_t* vcount, int32_t* ecount,
+                 ngs_stream_t s) {
     if (k <= 0) return NGS_OK;
-    mc_count_kernel<<<grid1d(k, 256), 256, 0, S(s)>>>(cubes, k, values, vcount);
+    NGS_REQUIRE(cubes && values && vcount, "ngs_mc_count: null argument");
+    mc_count_kernel<<<grid1d(k, 256), 256, 0, S(s)>>>(cubes, k, values, vcount, ecount);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }
 
-int ngs_mc_emit(const int32_t* cubes, int64_t k, const float* values, const int32_t* voffset, int32_t id_bits,
-                uint64_t* vkeys, uint32_t* vtags, ngs_stream_t s) {
+int ngs_mc_emit(const int32_t* cubes, int64_t k, const float* values, const int32_t* voffset, const int32_t* eoffset,
+                int32_t id_bits, uint64_t* ekeys, uint32_t* etags, int64_t* vslot, ngs_stream_t s) {
     NGS_REQUIRE(id_bits >= 1 && id_bits <= 31, "ngs_mc_emit: id_bits must be in [1, 31]");
     if (k <= 0) return NGS_OK;
-    mc_emit_kernel<<<grid1d(k, 128), 128, 0, S(s)>>>(cubes, k, values, voffset, id_bits, vkeys, vtags);
+    NGS_REQUIRE(cubes && values && voffset && eoffset && ekeys && etags && vslot, "ngs_mc_emit: null argument");
+    mc_emit_kernel<<<grid1d(k, 128), 128, 0, S(s)>>>(cubes, k, values, voffset, eoffset, id_bits, ekeys, etags, vslot);
+    NGS_CHECK_LAUNCH();
+    return NGS_OK;
+}
+
+int ngs_mc_remap_faces(int64_t* faces, int64_t n, const int64_t* slot_rank, ngs_stream_t s) {
+    if (n <= 0) return NGS_OK;
+    NGS_REQUIRE(faces && slot_rank, "ngs_mc_remap_faces: null argument");
+    remap_slots_kernel<<<grid1d(n, 256), 256, 0, S(s)>>>(faces, n, slot_rank);
     NGS_CHECK_LAUNCH();
     return NGS_OK;
 }

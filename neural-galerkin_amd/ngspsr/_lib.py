@@ -158,8 +158,9 @@ _SIGNATURES = {
     "ngs_eval_leaf_corners_levels": ([c_int32, P, P, P, c_int64, c_int32, c_int32, c_float, c_float, c_int32, P, P],
                                      c_int32),
     "ngs_eval_leaf_corners": ([Grid, P, P, c_int64, c_int32, c_int32, c_float, c_float, c_int32, P, P], c_int32),
-    "ngs_mc_count": ([P, c_int64, P, P, P], c_int32),
-    "ngs_mc_emit": ([P, c_int64, P, P, c_int32, P, P, P], c_int32),
+    "ngs_mc_count": ([P, c_int64, P, P, P, P], c_int32),
+    "ngs_mc_emit": ([P, c_int64, P, P, P, c_int32, P, P, P, P], c_int32),
+    "ngs_mc_remap_faces": ([P, c_int64, P, P], c_int32),
     "ngs_dedup_workspace_bytes": ([c_int64], c_int64),
     "ngs_dedup_vertices": ([P, P, c_int64, c_int32, P, P, P, P, P, P, c_int64, P], c_int32),
     "ngs_f64_to_f32": ([P, P, c_int64, P], c_int32),

@@ -748,25 +748,32 @@ class Reconstructor:
             return torch.zeros((0, 3), dtype=torch.float32, device=dev), torch.zeros((0, 3), dtype=torch.int64, device=dev)
         cubes = cubes.contiguous()
         values = values.float().contiguous()
-        vcount = torch.empty(K, dtype=torch.int32, device=dev)
-        call("ngs_mc_count", ptr(cubes), K, ptr(values), ptr(vcount), stream())
-        voff = torch.empty(K + 1, dtype=torch.int32, device=dev)
+        # per cube: triangle vertices and the distinct edges they lie on (one key per edge and cube
+        # goes to the sort instead of one per triangle vertex)
+        counts = torch.empty((2, K), dtype=torch.int32, device=dev)
+        call("ngs_mc_count", ptr(cubes), K, ptr(values), ptr(counts[0]), ptr(counts[1]), stream())
+        offs = torch.empty((2, K + 1), dtype=torch.int32, device=dev)
         ws = Scratch.get(lib().ngs_scan_workspace_bytes(K), dev)
-        call("ngs_scan_i32", ptr(vcount), K, ptr(voff), ptr(ws), stream())
-        nv = int(voff[K].item())
+        call("ngs_scan_i32", ptr(counts[0]), K, ptr(offs[0]), ptr(ws), stream())
+        call("ngs_scan_i32", ptr(counts[1]), K, ptr(offs[1]), ptr(ws), stream())
+        nv, ne = (int(v) for v in offs[:, K].cpu())  # one read for both totals
         if nv == 0:
             return torch.zeros((0, 3), dtype=torch.float32, device=dev), torch.zeros((0, 3), dtype=torch.int64, device=dev)
         # corner ids fit id_bits bits: keys are sorted on 2 id_bits bits
         id_bits = max(1, (values.shape[0] - 1).bit_length())
-        keys = torch.empty(nv, dtype=torch.int64, device=dev)
-        tags = torch.empty(nv, dtype=torch.int32, device=dev)
-        call("ngs_mc_emit", ptr(cubes), K, ptr(values), ptr(voff), id_bits, ptr(keys), ptr(tags), stream())
+        keys = torch.empty(ne, dtype=torch.int64, device=dev)
+        tags = torch.empty(ne, dtype=torch.int32, device=dev)
         faces = torch.empty(nv, dtype=torch.int64, device=dev)
-        verts = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+        call("ngs_mc_emit", ptr(cubes), K, ptr(values), ptr(offs[0]), ptr(offs[1]), id_bits, ptr(keys), ptr(tags),
+             ptr(faces), stream())
+        slot_rank = torch.empty(ne, dtype=torch.int64, device=dev)
+        verts = torch.empty((ne, 3), dtype=torch.float32, device=dev)
         nvert = torch.empty(1, dtype=torch.int64, device=dev)
-        wsb = lib().ngs_dedup_workspace_bytes(nv)
+        wsb = lib().ngs_dedup_workspace_bytes(ne)
         ws2 = Scratch.get(wsb, dev, slot=4)
-        call("ngs_dedup_vertices", ptr(keys), ptr(tags), nv, id_bits, ptr(pos.contiguous()), ptr(values), ptr(faces),
-             ptr(verts), ptr(nvert), ptr(ws2), wsb, stream())
+        call("ngs_dedup_vertices", ptr(keys), ptr(tags), ne, id_bits, ptr(pos.contiguous()), ptr(values),
+             ptr(slot_rank), ptr(verts), ptr(nvert), ptr(ws2), wsb, stream())
+        del keys, tags
+        call("ngs_mc_remap_faces", ptr(faces), nv, ptr(slot_rank), stream())
         V = int(nvert.item())
         return verts[:V], faces.view(-1, 3)
