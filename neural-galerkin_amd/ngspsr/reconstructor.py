@@ -16,7 +16,7 @@ class PointBinning:
     per-axis basis values of the 27 voxels around each point (reconstructor.py:111-120). Per-point
     arrays are stored in bin order: slot k holds point perm[k]."""
 
-    def __init__(self, tree: HashTree, xyz, d: int, alpha):
+    def __init__(self, tree: HashTree, xyz, d: int, alpha, _finalize=True):
         n = xyz.shape[0]
         s = tree.strides[d]
         dev = tree.device
@@ -27,7 +27,27 @@ class PointBinning:
         qmin, qmax = tree.point_bounds(xyz)
         self.cells = SparseGrid.for_points(qmin, qmax, s, dev)
         self.cells.mark_coords(cell)
-        self.ncells = self.cells.finalize(emit=False)
+        self._pending = (cell, b)
+        if _finalize:
+            self.cells.finalize(emit=False)
+            self._bucket(alpha)
+
+    @classmethod
+    def levels(cls, tree: HashTree, xyz, levels, alpha):
+        """{d: PointBinning} of several levels, their cell counts read in one device-to-host read."""
+        out = {d: cls(tree, xyz, d, alpha, _finalize=False) for d in levels}
+        SparseGrid.finalize_all([b.cells for b in out.values()], emit=False)
+        for b in out.values():
+            b._bucket(alpha)
+        return out
+
+    def _bucket(self, alpha):
+        """Points sorted by cell rank (the cells grid finalised), per-point arrays in bin order."""
+        cell, b = self._pending
+        self._pending = None
+        n = self.n
+        dev = cell.device
+        self.ncells = self.cells.count
         rank = torch.empty(n, dtype=torch.int32, device=dev)
         call("ngs_grid_rank_rows", self.cells.struct(), ptr(cell), n, ptr(rank), stream())
         self.perm = torch.empty(n, dtype=torch.int32, device=dev)
@@ -137,10 +157,38 @@ class Reconstructor:
             form = "csr"
         return form
 
-    def assemble_level(self, d, bins, coarse, sf, delta, rhs, form=None):
+    def packed_layouts(self, levels, bins):
+        """{d: (slice offsets, value count)} of the packed run form for the levels that take it: the
+        level grid alone fixes each row's entry count, so the offsets of all levels are computed
+        before any is assembled and their totals come back in one device-to-host read."""
+        if not _solver.RunSystem.PACKED:
+            return {}
+        tree = self.hash_tree
+        soffs = {d: self._slice_offsets(d) for d in levels
+                 if tree.level(d).M and self.operator_form(tree.level(d).M, bins[d].n,
+                                                           getattr(bins[d], "ncells", 0)) == "runs"}
+        if not soffs:
+            return {}
+        totals = torch.cat([s[-1:] for s in soffs.values()]).cpu().tolist()
+        return {d: (s, int(t)) for (d, s), t in zip(soffs.items(), totals)}
+
+    def _slice_offsets(self, d):
+        """Offsets of level d's 64-row slices in the packed form (slice sums of the row entry
+        counts, scanned; nsl + 1 entries)."""
+        lv = self.hash_tree.level(d)
+        dev = self.hash_tree.device
+        nsl = (lv.M + 63) // 64
+        row_nnz = torch.zeros(nsl * 64, dtype=torch.int32, device=dev)
+        call("ngs_stencil_row_counts", lv, ptr(row_nnz), stream())
+        soff = torch.zeros(nsl + 1, dtype=torch.int64, device=dev)
+        torch.cumsum(row_nnz.view(nsl, 64).sum(1, dtype=torch.int64), 0, out=soff[1:])
+        return soff
+
+    def assemble_level(self, d, bins, coarse, sf, delta, rhs, form=None, layout=None):
         """Level-d system (reconstructor.py:143-183); the screening RHS terms are added to rhs.
         form: "runs" (ngs_lhs_assemble_runs), "csr" (ngs_lhs_assemble), "stencil" (ngs_stencil_*),
-        None = operator_form."""
+        None = operator_form. layout: the level's packed-form offsets from packed_layouts (the run
+        form only; counted here when None)."""
         tree = self.hash_tree
         lv = tree.level(d)
         M = lv.M
@@ -164,12 +212,11 @@ class Reconstructor:
             if _solver.RunSystem.PACKED and lib().ngs_assembly_lanes(int(pb.n), M) == 1:
                 # one lane per row: the assembly writes the packed form itself, at slice offsets from
                 # the row counts (64-row slice sums, scanned)
-                nsl = (M + 63) // 64
-                row_nnz = torch.zeros(nsl * 64, dtype=torch.int32, device=dev)
-                call("ngs_stencil_row_counts", lv, ptr(row_nnz), stream())
-                soff = torch.zeros(nsl + 1, dtype=torch.int64, device=dev)
-                torch.cumsum(row_nnz.view(nsl, 64).sum(1, dtype=torch.int64), 0, out=soff[1:])
-                pvals = torch.empty(max(int(soff[-1].item()), 1), dtype=torch.float32, device=dev)
+                if layout is None:
+                    soff = self._slice_offsets(d)
+                    layout = (soff, int(soff[-1].item()))
+                soff, nvals = layout
+                pvals = torch.empty(max(nvals, 1), dtype=torch.float32, device=dev)
                 call("ngs_lhs_assemble_packed", lv, pb, ptr(coarse), float(sf), float(delta), ptr(runs), ptr(pvals),
                      ptr(soff), ptr(inv_diag), ptr(rhs), stream())
                 return _solver.RunSystem.from_packed(runs, pvals, soff, inv_diag)
@@ -180,7 +227,7 @@ class Reconstructor:
                 rvals = torch.empty((125, P), dtype=torch.float32, device=dev)
             call("ngs_lhs_assemble_runs", lv, pb, ptr(coarse), float(sf), float(delta), ptr(runs),
                  ptr(rvals), ptr(inv_diag), ptr(rhs), stream())
-            return _solver.RunSystem(runs, rvals, inv_diag)
+            return _solver.RunSystem(runs, rvals, inv_diag, layout=layout)
         if form != "csr":
             raise ValueError(f"unknown operator form {form!r}")
         row_nnz = torch.empty(M, dtype=torch.int32, device=dev)
@@ -232,9 +279,10 @@ class Reconstructor:
         self._points_chi = None
         running, added = None, []
         xyz = screen_xyz.float().contiguous() if should_screen else None
-        bins = {}
-        for d in range(end_depth, start_depth + 1):
-            bins[d] = PointBinning(tree, xyz, d, alpha) if should_screen else PointBinning.empty(tree)
+        levels = list(range(end_depth, start_depth + 1))
+        bins = PointBinning.levels(tree, xyz, levels, alpha) if should_screen else \
+            {d: PointBinning.empty(tree) for d in levels}
+        layouts = self.packed_layouts(levels, bins)
         for d in range(start_depth, end_depth - 1, -1):
             M = tree.get_coords_size(None, d)
             if M == 0:
@@ -263,7 +311,7 @@ class Reconstructor:
                 call("ngs_rhs_cross_stiffness", tree.level(d), tree.level(dd), ptr(self.field(dd)), ptr(rhs),
                      stream())
             system = self.assemble_level(d, bins[d], coarse, sf, 0.0 if delta_t is not None else float(screen_delta),
-                                         rhs)
+                                         rhs, layout=layouts.get(d))
             # "mixed" tries PCG only on the finest level; the coarser levels are solved directly
             # (reconstructor.py:186-189)
             level_solver = solver

@@ -104,9 +104,10 @@ class RunSystem:
     def plane_stride(M):
         return (M + 63) // 64 * 64
 
-    def __init__(self, runs, rvals, inv_diag, packed=None):
+    def __init__(self, runs, rvals, inv_diag, packed=None, layout=None):
         """rvals may be a caller's temporary when the system packs (it is read once, on the
-        current stream)."""
+        current stream). layout: (slice offsets, value count) of the packed form when the caller
+        has them (from the level's row counts), else they are counted from the run words."""
         self.runs, self.inv_diag = runs, inv_diag
         self.M = inv_diag.shape[0]
         self.device = inv_diag.device
@@ -114,7 +115,7 @@ class RunSystem:
         self._rvals = rvals
         self._packed = None
         if (self.PACKED if packed is None else packed) and self.M:
-            self._pack(rvals)
+            self._pack(rvals, layout)
             self._rvals = None
 
     @classmethod
@@ -129,13 +130,18 @@ class RunSystem:
         self._packed = (pvals, soff)
         return self
 
-    def _pack(self, rvals):
+    def _pack(self, rvals, layout=None):
         nsl = (self.M + 63) // 64
-        counts = torch.empty(nsl, dtype=torch.int64, device=self.device)
-        call("ngs_runs_slice_counts", ptr(self.runs), self.M, ptr(counts), stream())
-        soff = torch.zeros(nsl + 1, dtype=torch.int64, device=self.device)
-        torch.cumsum(counts, 0, out=soff[1:])
-        pvals = torch.empty(max(int(soff[-1].item()), 1), dtype=torch.float32, device=self.device)
+        if layout is None:
+            counts = torch.empty(nsl, dtype=torch.int64, device=self.device)
+            call("ngs_runs_slice_counts", ptr(self.runs), self.M, ptr(counts), stream())
+            soff = torch.zeros(nsl + 1, dtype=torch.int64, device=self.device)
+            torch.cumsum(counts, 0, out=soff[1:])
+            nvals = int(soff[-1].item())
+        else:
+            soff, nvals = layout
+            assert soff.shape == (nsl + 1,)
+        pvals = torch.empty(max(nvals, 1), dtype=torch.float32, device=self.device)
         call("ngs_runs_pack", ptr(self.runs), ptr(rvals), self.M, ptr(soff), ptr(pvals), stream())
         self._packed = (pvals, soff)
 
