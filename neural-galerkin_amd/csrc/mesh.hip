@@ -640,22 +640,48 @@ __global__ void dual_gather_cubes_kernel(const int32_t* __restrict__ centres, in
                 }
             }
     }
-    // octants left: coarser levels, finest first
-    if (n < 8 && l0 < lv.n) {
+    // octants left: coarser levels, finest first. Per level the probes of all missing octants go
+    // out together (bitmap words, then rank words, then leaf indices, each phase's loads issued
+    // before any is used): one chain of three loads per level instead of one per octant and level.
+    for (int l = l0; l < lv.n && n < 8; ++l) {
+        const ngs_grid& g = lv.g[l];
+        const int k = g.log2_stride;
+        int64_t lin[8];
+        bool ok[8];
+        int e[8];
+#pragma unroll
         for (int o = 0; o < 8; ++o) {
             const int sx = (o >> 2) & 1, sy = (o >> 1) & 1, sz = o & 1;
-            const int sl = octant_slot(sx, sy, sz);
-            if (ids[sl] >= 0) continue;
-            for (int l = l0; l < lv.n; ++l) {
-                const int k = lv.g[l].log2_stride;
-                const int c[3] = {(q[0] - 1 + sx) >> k, (q[1] - 1 + sy) >> k, (q[2] - 1 + sz) >> k};  // in units of s
-                const int32_t rank = grid_find(lv.g[l], (c[0] >> 1) << k, (c[1] >> 1) << k, (c[2] >> 1) << k);
-                if (rank < 0) continue;
-                const int e = ((c[0] & 1) * 2 + (c[1] & 1)) * 2 + (c[2] & 1);
-                ids[sl] = lv.base[l] + lv.index[l][rank] * 8 + e;
-                ++n;
-                break;
-            }
+            const int c0 = (q[0] - 1 + sx) >> k, c1 = (q[1] - 1 + sy) >> k, c2 = (q[2] - 1 + sz) >> k;  // units of s
+            e[o] = ((c0 & 1) * 2 + (c1 & 1)) * 2 + (c2 & 1);
+            // the leaf's coordinates ((c >> 1) << k) are multiples of its stride 2^k, as grid_find requires
+            lin[o] = 0;
+            ok[o] = ids[octant_slot(sx, sy, sz)] < 0 &&
+                    grid_cell(g, (c0 >> 1) << k, (c1 >> 1) << k, (c2 >> 1) << k, &lin[o]);
+        }
+        // loads without branches: a probe that is not needed reads element 0, which every level has
+        // (levels without leaves are not passed), and its result is discarded
+        const NGS_GLOBAL uint32_t* bits = as_global(g.bits);
+        const NGS_GLOBAL int32_t* rnk = as_global(g.rank);
+        const NGS_GLOBAL int32_t* index = as_global(lv.index[l]);
+        uint32_t w[8];
+#pragma unroll
+        for (int o = 0; o < 8; ++o) w[o] = bits[ok[o] ? lin[o] >> 5 : 0];
+#pragma unroll
+        for (int o = 0; o < 8; ++o) ok[o] = ok[o] && ((w[o] >> (lin[o] & 31)) & 1u);
+        int32_t rw[8];
+#pragma unroll
+        for (int o = 0; o < 8; ++o) rw[o] = rnk[ok[o] ? lin[o] >> 5 : 0];
+        int32_t li[8];
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            li[o] = index[ok[o] ? rw[o] + __popc(w[o] & ((1u << (lin[o] & 31)) - 1u)) : 0];
+        const int32_t base = lv.base[l];
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {  // selects, not branches: the index loads above stay grouped
+            int32_t& id = ids[octant_slot((o >> 2) & 1, (o >> 1) & 1, o & 1)];
+            id = ok[o] ? base + li[o] * 8 + e[o] : id;
+            n += ok[o] ? 1 : 0;
         }
     }
     // only complete centres become cubes: the others write no slots and read no values
